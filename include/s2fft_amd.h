@@ -1,0 +1,126 @@
+/*
+ * s2fft_amd: C ABI of the MI355X (gfx950) spin spherical-harmonic transforms.
+ *
+ * Every entry point takes plain device pointers (FP64 / complex128 data laid
+ * out exactly as the reference's NumPy arrays), a batch count and a HIP
+ * stream.  Nothing on a launch path allocates or synchronises: the caller owns
+ * every input, output and workspace buffer; a plan owns only its constant
+ * tables.  Plans are immutable after creation and may be shared by threads.
+ * Every function returns an s2h status (0 = ok, negative = error; see
+ * s2h_status_string); none of them calls exit().
+ *
+ * Reference interfaces replaced (paths relative to the s2fft repository):
+ *   s2h_forward / s2h_inverse      s2fft/transforms/spherical.py:352-464 and
+ *                                  21-109 (forward_numpy 467-588,
+ *                                  inverse_numpy 112-208) for one spin
+ *   s2h_lat_forward / s2h_lat_inverse
+ *                                  s2fft/transforms/otf_recursions.py:432-588
+ *                                  and 15-171 (latitudinal Wigner-d steps)
+ *   s2h_healpix_fft / s2h_healpix_ifft
+ *                                  s2fft/utils/healpix_ffts.py:150-226 and
+ *                                  297-371 (healpix_fft / healpix_ifft)
+ *   healpix_fft_hip                lib/src/extensions.cc:26-100 (the XLA
+ *                                  custom call healpix_fft_cuda) with the
+ *                                  descriptor of lib/include/s2fft.h:29-59
+ *   s2h_gamma_dft                  the FFT over gamma of
+ *                                  s2fft/transforms/wigner.py:183-195,495-502
+ */
+#ifndef S2FFT_AMD_H
+#define S2FFT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct s2h_plan s2h_plan;
+typedef void *s2h_stream; /* a hipStream_t; NULL = the default stream */
+
+enum s2h_sampling { S2H_MW = 0, S2H_MWSS = 1, S2H_DH = 2, S2H_GL = 3, S2H_HEALPIX = 4 };
+
+enum s2h_status {
+  S2H_OK = 0,
+  S2H_ERR_INVALID = -1,     /* bad argument (the reference raises ValueError) */
+  S2H_ERR_UNSUPPORTED = -2, /* valid but outside what this build supports */
+  S2H_ERR_HIP = -3,         /* a HIP runtime call failed */
+  S2H_ERR_NOMEM = -4,       /* device or host allocation failed at plan creation */
+  S2H_ERR_SPIN = -5,        /* |spin| >= 8 (the reference raises Warning) */
+  S2H_ERR_BANDLIMIT = -6    /* L < 2 nside for HEALPix (reference AssertionError) */
+};
+
+/* Normalisation folded into a plan's Wigner-d recursion. */
+enum s2h_norm {
+  S2H_NORM_SPIN = 0,    /* sqrt((2l+1)/4pi) and (-1)^spin: spherical.py */
+  S2H_NORM_WIGNER = 1,  /* forward 1, inverse (2l+1)/(8 pi^2 sqrt(pi))... wigner.py */
+  S2H_NORM_NONE = 2     /* bare d^l_{m,-spin}: the latitudinal steps */
+};
+
+typedef struct s2h_desc {
+  int32_t sampling;   /* enum s2h_sampling */
+  int32_t L;          /* harmonic band-limit */
+  int32_t nside;      /* HEALPix resolution (ignored otherwise) */
+  int32_t spin;       /* |spin| < 8 */
+  int32_t L_lower;    /* transform only L_lower <= l < L */
+  int32_t reality;    /* real signal: m >= 0 computed, conjugate symmetry */
+  int32_t norm;       /* enum s2h_norm */
+  int32_t extra_sign; /* +1 or -1, multiplies every output (Wigner (-1)^n) */
+} s2h_desc;
+
+const char *s2h_status_string(int status);
+int s2h_version(void);
+
+int s2h_plan_create(s2h_plan **plan, const s2h_desc *desc);
+int s2h_plan_destroy(s2h_plan *plan);
+
+/* Bytes of device workspace s2h_forward / s2h_inverse need for `batch` maps. */
+size_t s2h_workspace_bytes(const s2h_plan *plan, int batch);
+/* 1 if s2h_inverse writes complex128 pixels, 0 if float64. */
+int s2h_inverse_output_is_complex(const s2h_plan *plan);
+
+/* f: batch maps, float64 (f_is_complex = 0) or complex128 (1), layout of
+ * s2fft.sampling.s2_samples.f_shape.  flm: batch x (L, 2L-1) complex128.  */
+int s2h_forward(const s2h_plan *plan, const void *f, int f_is_complex, void *flm,
+                void *work, int batch, s2h_stream stream);
+int s2h_inverse(const s2h_plan *plan, const void *flm, void *f, void *work, int batch,
+                s2h_stream stream);
+
+/* Latitudinal steps on the plan's theta grid (plan created with
+ * S2H_NORM_NONE).  ftm: batch x (ntheta, 2L or 2L-1) complex128 in the
+ * reference ftm layout; flm: batch x (L, 2L-1) complex128.  */
+int s2h_lat_forward(const s2h_plan *plan, const void *ftm, void *flm, int batch,
+                    s2h_stream stream);
+int s2h_lat_inverse(const s2h_plan *plan, const void *flm, void *ftm, int batch,
+                    s2h_stream stream);
+
+/* HEALPix ring FFTs with the reference layouts: f (12 nside^2,) complex128,
+ * ftm (4 nside - 1, 2L) complex128.  No plan needed (tables are cached). */
+int s2h_healpix_fft(int nside, int L, int reality, const void *f, void *ftm, int batch,
+                    s2h_stream stream);
+int s2h_healpix_ifft(int nside, int L, int reality, const void *ftm, void *f, int batch,
+                     s2h_stream stream);
+
+/* XLA custom-call compatible entry: buffers[0] = input, buffers[1] = output,
+ * opaque = packed s2h_healpix_fft_descriptor. The input is never modified. */
+typedef struct s2h_healpix_fft_descriptor {
+  int32_t nside, L, reality, forward, batch, reserved;
+} s2h_healpix_fft_descriptor;
+void healpix_fft_hip(void *stream, void **buffers, const char *opaque, size_t opaque_len);
+/* Status of the last healpix_fft_hip call on this thread (XLA's signature has
+ * no return value). */
+int healpix_fft_hip_last_status(void);
+
+/* DFT over the first (gamma) axis of an SO(3) signal with 2N-1 gamma samples:
+ * fban[n] = scale * sum_g f[g] exp(-2 pi i n g / (2N-1)) for n in
+ * [-(N-1), N-1] (forward != 0), or the unnormalised inverse
+ * f[g] = sum_n fban[n] exp(+2 pi i n g / (2N-1)) (forward == 0).
+ * npix = number of (beta, alpha) samples; complex128 in and out. */
+int s2h_gamma_dft(int N, int64_t npix, int forward, double scale, const void *in,
+                  void *out, s2h_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* S2FFT_AMD_H */

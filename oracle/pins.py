@@ -1,0 +1,119 @@
+"""Independent mathematics that pins the oracle (no reference code involved).
+
+Test infrastructure only -- see ``oracle/__init__.py``.
+
+* ``wigner_d``       -- Wigner's explicit sum for d^l_{mn}(beta), evaluated in
+                        mpmath at 40 digits (standard convention
+                        d^l_{mn}(beta) = <lm| exp(-i beta J_y) |ln>).
+* ``sylm``           -- spin-weighted harmonics in the convention the transforms
+                        implement: sY_lm = (-1)^s sqrt((2l+1)/4pi) e^{im phi}
+                        d^l_{m,-s}(theta).
+* ``ylm_scipy``      -- spin-0 harmonics from scipy.special.sph_harm_y
+                        (Condon-Shortley phase), an independent library pin of
+                        the spin-0 case of ``sylm``.
+* ``healpix_pix2ang``-- the HEALPix ring-scheme closed form (Gorski et al. 2005,
+                        eqs. 4-9), independent of the ring-index formulas.
+* ``dft``            -- brute-force discrete Fourier sums.
+"""
+
+import math
+
+import mpmath
+import numpy as np
+import scipy.special as sps
+
+mpmath.mp.dps = 40
+
+
+def wigner_d(l, m, n, beta):
+    """d^l_{mn}(beta) by Wigner's explicit formula in multiprecision."""
+    if abs(m) > l or abs(n) > l:
+        return 0.0
+    b = mpmath.mpf(beta) / 2
+    cb, sb = mpmath.cos(b), mpmath.sin(b)
+    pref = mpmath.sqrt(math.factorial(l + m) * math.factorial(l - m)
+                       * math.factorial(l + n) * math.factorial(l - n))
+    total = mpmath.mpf(0)
+    for k in range(max(0, n - m), min(l + n, l - m) + 1):
+        den = (math.factorial(l + n - k) * math.factorial(k)
+               * math.factorial(m - n + k) * math.factorial(l - m - k))
+        term = (-1) ** (m - n + k) * pref / den
+        total += term * cb ** (2 * l + n - m - 2 * k) * sb ** (m - n + 2 * k)
+    return float(total)
+
+
+def wigner_d_table(L, m, n, betas):
+    """[t, l] table of d^l_{mn}(beta_t) for l in [0, L)."""
+    out = np.zeros((len(betas), L))
+    for l in range(L):
+        for t, b in enumerate(betas):
+            out[t, l] = wigner_d(l, m, n, b)
+    return out
+
+
+def sylm(l, m, s, theta, phi):
+    """Spin-s harmonic sY_lm(theta, phi) = (-1)^s sqrt((2l+1)/4pi) e^{im phi} d^l_{m,-s}(theta)."""
+    d = np.array([wigner_d(l, m, -s, th) for th in np.ravel(theta)]).reshape(np.shape(theta))
+    return (-1.0) ** s * math.sqrt((2 * l + 1) / (4 * math.pi)) * d * np.exp(1j * m * np.asarray(phi))
+
+
+def ylm_scipy(l, m, theta, phi):
+    """scipy's Y_lm (polar angle theta, azimuth phi)."""
+    return sps.sph_harm_y(l, m, theta, phi)
+
+
+def synthesis_bruteforce(flm, L, spin, theta, phi, use_scipy=False):
+    """f(theta_i, phi_i) = sum_{l,m} flm[l, L-1+m] sY_lm(theta_i, phi_i) at given points."""
+    theta = np.asarray(theta, dtype=np.float64)
+    phi = np.asarray(phi, dtype=np.float64)
+    out = np.zeros(theta.shape, dtype=np.complex128)
+    for l in range(abs(spin), L):
+        for m in range(-l, l + 1):
+            c = flm[l, L - 1 + m]
+            if c == 0:
+                continue
+            if use_scipy and spin == 0:
+                out += c * ylm_scipy(l, m, theta, phi)
+            else:
+                out += c * sylm(l, m, spin, theta, phi)
+    return out
+
+
+def healpix_pix2ang(nside, p):
+    """(theta, phi) of ring-ordered HEALPix pixel p (Gorski et al. 2005)."""
+    npix = 12 * nside * nside
+    ncap = 2 * nside * (nside - 1)
+    if p < ncap:                               # north polar cap
+        i = int((1 + math.isqrt(1 + 2 * p)) // 2)
+        while 2 * i * (i - 1) > p:
+            i -= 1
+        while 2 * (i + 1) * i <= p:
+            i += 1
+        j = p - 2 * i * (i - 1) + 1
+        z = 1.0 - i * i / (3.0 * nside * nside)
+        phi = math.pi / (2 * i) * (j - 0.5)
+    elif p < npix - ncap:                      # equatorial belt
+        q = p - ncap
+        i = q // (4 * nside) + nside
+        j = q % (4 * nside) + 1
+        fodd = 1.0 if (i + nside) % 2 == 1 else 0.5
+        z = 4.0 / 3.0 - 2.0 * i / (3.0 * nside)
+        phi = math.pi / (2 * nside) * (j - fodd)
+    else:                                      # south polar cap
+        q = npix - p - 1
+        i = int((1 + math.isqrt(1 + 2 * q)) // 2)
+        while 2 * i * (i - 1) > q:
+            i -= 1
+        while 2 * (i + 1) * i <= q:
+            i += 1
+        j = 4 * i - (q - 2 * i * (i - 1))
+        z = -1.0 + i * i / (3.0 * nside * nside)
+        phi = math.pi / (2 * i) * (j - 0.5)
+    return math.acos(z), phi
+
+
+def dft(x):
+    """X[k] = sum_p x[p] exp(-2 pi i k p / n)."""
+    n = len(x)
+    p = np.arange(n)
+    return np.exp(-2j * np.pi * np.outer(p, p) / n) @ x

@@ -1,0 +1,130 @@
+"""ctypes binding of ``libs2fft_amd.so`` (the C ABI declared in include/s2fft_amd.h).
+
+The library is built in-tree by ``python -m s2fft_amd.build``.  There is no
+fallback: if it is missing every transform raises ``RuntimeError``.
+"""
+
+import ctypes
+import threading
+from pathlib import Path
+
+_LIB_PATH = Path(__file__).resolve().parent / "libs2fft_amd.so"
+_lock = threading.Lock()
+_lib = None
+
+S2H_MW, S2H_MWSS, S2H_DH, S2H_GL, S2H_HEALPIX = range(5)
+SAMPLING_CODES = {"mw": S2H_MW, "mwss": S2H_MWSS, "dh": S2H_DH, "gl": S2H_GL,
+                  "healpix": S2H_HEALPIX}
+NORM_SPIN, NORM_WIGNER, NORM_NONE = 0, 1, 2
+
+S2H_OK = 0
+S2H_ERR_INVALID = -1
+S2H_ERR_UNSUPPORTED = -2
+S2H_ERR_HIP = -3
+S2H_ERR_NOMEM = -4
+S2H_ERR_SPIN = -5
+S2H_ERR_BANDLIMIT = -6
+
+
+class Desc(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_int32) for name in
+                ("sampling", "L", "nside", "spin", "L_lower", "reality", "norm", "extra_sign")]
+
+
+class HealpixFftDescriptor(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_int32) for name in
+                ("nside", "L", "reality", "forward", "batch", "reserved")]
+
+
+EXPORTED = (
+    "s2h_status_string", "s2h_version", "s2h_plan_create", "s2h_plan_destroy",
+    "s2h_workspace_bytes", "s2h_inverse_output_is_complex", "s2h_forward", "s2h_inverse",
+    "s2h_lat_forward", "s2h_lat_inverse", "s2h_healpix_fft", "s2h_healpix_ifft",
+    "healpix_fft_hip", "healpix_fft_hip_last_status", "s2h_gamma_dft",
+)
+
+
+def _declare(lib):
+    vp, i32, i64, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_size_t
+    sig = {
+        "s2h_status_string": (ctypes.c_char_p, [i32]),
+        "s2h_version": (i32, []),
+        "s2h_plan_create": (i32, [ctypes.POINTER(vp), ctypes.POINTER(Desc)]),
+        "s2h_plan_destroy": (i32, [vp]),
+        "s2h_workspace_bytes": (sz, [vp, i32]),
+        "s2h_inverse_output_is_complex": (i32, [vp]),
+        "s2h_forward": (i32, [vp, vp, i32, vp, vp, i32, vp]),
+        "s2h_inverse": (i32, [vp, vp, vp, vp, i32, vp]),
+        "s2h_lat_forward": (i32, [vp, vp, vp, i32, vp]),
+        "s2h_lat_inverse": (i32, [vp, vp, vp, i32, vp]),
+        "s2h_healpix_fft": (i32, [i32, i32, i32, vp, vp, i32, vp]),
+        "s2h_healpix_ifft": (i32, [i32, i32, i32, vp, vp, i32, vp]),
+        "healpix_fft_hip": (None, [vp, ctypes.POINTER(vp), ctypes.c_char_p, sz]),
+        "healpix_fft_hip_last_status": (i32, []),
+        "s2h_gamma_dft": (i32, [i32, i64, i32, ctypes.c_double, vp, vp, vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def lib():
+    """Load the shared library once (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        with _lock:
+            if _lib is None:
+                if not _LIB_PATH.exists():
+                    raise RuntimeError(
+                        f"s2fft_amd native library not found at {_LIB_PATH}; build it with "
+                        "`python -m s2fft_amd.build` (there is no CPU fallback)")
+                handle = ctypes.CDLL(str(_LIB_PATH))
+                _declare(handle)
+                _lib = handle
+    return _lib
+
+
+def library_path():
+    return _LIB_PATH
+
+
+def check(status, what="s2fft_amd"):
+    """Map an s2h status to the reference's Python exception types."""
+    if status == S2H_OK:
+        return
+    msg = lib().s2h_status_string(status).decode()
+    if status == S2H_ERR_SPIN:
+        raise Warning(msg)
+    if status == S2H_ERR_BANDLIMIT:
+        raise AssertionError(msg)
+    if status == S2H_ERR_INVALID:
+        raise ValueError(f"{what}: {msg}")
+    if status == S2H_ERR_UNSUPPORTED:
+        raise NotImplementedError(f"{what}: {msg}")
+    raise RuntimeError(f"{what}: {msg} (status {status})")
+
+
+class Plan:
+    """Owner of one native plan (constant tables on the current device)."""
+
+    def __init__(self, sampling, L, nside=0, spin=0, L_lower=0, reality=False,
+                 norm=NORM_SPIN, extra_sign=1):
+        d = Desc(SAMPLING_CODES[sampling], L, nside or 0, spin, L_lower, int(bool(reality)),
+                 norm, extra_sign)
+        handle = ctypes.c_void_p()
+        check(lib().s2h_plan_create(ctypes.byref(handle), ctypes.byref(d)), "plan")
+        self.handle = handle
+        self.desc = d
+
+    def workspace_bytes(self, batch):
+        return int(lib().s2h_workspace_bytes(self.handle, batch))
+
+    def inverse_output_is_complex(self):
+        return bool(lib().s2h_inverse_output_is_complex(self.handle))
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.s2h_plan_destroy(h)
+            self.handle = None

@@ -1,0 +1,299 @@
+// extern "C" entry points: argument checks, kernel sequencing, status codes.
+#include <cstring>
+
+#include "s2h_plan.h"
+
+using namespace s2h;
+
+namespace {
+
+thread_local int g_last_xla_status = S2H_OK;
+
+int hip_status(hipError_t e) { return e == hipSuccess ? S2H_OK : S2H_ERR_HIP; }
+
+int check_device(const s2h_plan* p) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess) return S2H_ERR_HIP;
+  return dev == p->device ? S2H_OK : S2H_ERR_INVALID;
+}
+
+int64_t flm_stride(const s2h_plan* p) { return (int64_t)p->d.L * (2 * p->d.L - 1); }
+
+LatArgs lat_args(const s2h_plan* p, bool forward, int batch) {
+  LatArgs a{};
+  a.rings = forward ? p->fwd_rings : p->inv_rings;
+  a.nrings = forward ? p->n_fwd_rings : p->n_inv_rings;
+  a.mstart = forward ? p->fwd_mstart : p->inv_mstart;
+  a.coef = forward ? p->fwd_coef : p->inv_coef;
+  a.L = p->d.L;
+  a.m_lo = p->m_lo;
+  a.nm = p->nm;
+  a.lmin_out = p->d.L_lower;
+  a.batch = batch;
+  a.flm_map_stride = flm_stride(p);
+  a.groups = 1;
+  return a;
+}
+
+}  // namespace
+
+extern "C" const char* s2h_status_string(int s) {
+  switch (s) {
+    case S2H_OK: return "ok";
+    case S2H_ERR_INVALID: return "invalid argument";
+    case S2H_ERR_UNSUPPORTED: return "unsupported configuration";
+    case S2H_ERR_HIP: return "HIP runtime error";
+    case S2H_ERR_NOMEM: return "out of memory";
+    case S2H_ERR_SPIN: return "Recursive transform may provide lower precision beyond spin ~ 8";
+    case S2H_ERR_BANDLIMIT: return "HEALPix requires L >= 2 nside";
+    default: return "unknown status";
+  }
+}
+
+extern "C" int s2h_version(void) { return 1; }
+
+extern "C" size_t s2h_workspace_bytes(const s2h_plan* p, int batch) {
+  if (!p || batch < 0) return 0;
+  size_t rows = (size_t)std::max(p->fwd_rows, p->inv_rows);
+  if (p->fwd_kind != 0) rows += (size_t)p->fwd_in_rows;
+  return rows * (size_t)p->Mc * sizeof(cplx) * (size_t)batch;
+}
+
+extern "C" int s2h_inverse_output_is_complex(const s2h_plan* p) {
+  return p && p->inv_out_complex ? 1 : 0;
+}
+
+extern "C" int s2h_forward(const s2h_plan* p, const void* f, int f_is_complex, void* flm,
+                           void* work, int batch, s2h_stream stream) {
+  if (!p || !f || !flm || batch < 0 || (batch > 0 && !work)) return S2H_ERR_INVALID;
+  if (batch == 0) return S2H_OK;
+  int st = check_device(p);
+  if (st) return st;
+  if (p->d.norm == S2H_NORM_NONE) return S2H_ERR_INVALID;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int L = p->d.L;
+  const bool real = p->d.reality != 0;
+  cplx* ftm = static_cast<cplx*>(work);
+  const int64_t fs = flm_stride(p);
+
+  if ((st = hip_status(launch_zero(flm, 2 * fs * batch, s)))) return st;
+  if (p->fwd_kind != 0) return S2H_ERR_UNSUPPORTED;
+
+  RingFwdArgs r{};
+  r.jobs = p->fwd_jobs;
+  r.f = f;
+  r.f_complex = f_is_complex;
+  r.real_only_2L = 0;
+  r.real_only_all = 0;
+  if (real && f_is_complex) {
+    // reference reality semantics: HEALPix rings of length 2L and every
+    // equiangular ring use Re(f) (healpix_ffts.py:211-215, spherical.py:528-534)
+    if (p->d.sampling == S2H_HEALPIX)
+      r.real_only_2L = 1;
+    else
+      r.real_only_all = 1;
+  }
+  r.map_stride = p->npix;
+  r.out = ftm;
+  r.out_map_stride = (int64_t)p->fwd_rows * p->Mc;
+  r.ncol = p->Mc;
+  r.m_first = p->m_lo;
+  r.apply_weight = 1;
+  r.apply_phase = p->d.sampling == S2H_HEALPIX;
+  r.zero_neg_2L = 0;
+  r.L = L;
+  r.chirp_pool = p->chirp;
+  if ((st = hip_status(launch_ring_fft_forward(r, p->n_fwd_jobs, batch, p->fwd_lds, s)))) return st;
+
+  LatArgs a = lat_args(p, true, batch);
+  a.flm = static_cast<cplx*>(flm);
+  a.ftm = ftm;
+  a.ftm_map_stride = r.out_map_stride;
+  a.ftm_ncol = p->Mc;
+  a.ftm_m_first = p->m_lo;
+  a.lmin_out = std::max(std::abs(p->d.spin), p->d.L_lower);
+  if ((st = hip_status(launch_lat_forward(a, p->fwd_paired, s)))) return st;
+
+  if (real) {
+    if ((st = hip_status(launch_reality_fill(static_cast<cplx*>(flm), fs, L, batch, s)))) return st;
+  }
+  return S2H_OK;
+}
+
+extern "C" int s2h_inverse(const s2h_plan* p, const void* flm, void* f, void* work, int batch,
+                           s2h_stream stream) {
+  if (!p || !flm || !f || batch < 0 || (batch > 0 && !work)) return S2H_ERR_INVALID;
+  if (batch == 0) return S2H_OK;
+  int st = check_device(p);
+  if (st) return st;
+  if (p->d.norm == S2H_NORM_NONE) return S2H_ERR_INVALID;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int L = p->d.L;
+  cplx* ftm = static_cast<cplx*>(work);
+  const int64_t ftm_stride = (int64_t)p->inv_rows * p->Mc;
+
+  LatArgs a = lat_args(p, false, batch);
+  a.flm = const_cast<cplx*>(static_cast<const cplx*>(flm));
+  a.ftm = ftm;
+  a.ftm_map_stride = ftm_stride;
+  a.ftm_ncol = p->Mc;
+  a.ftm_m_first = p->m_lo;
+  if ((st = hip_status(launch_lat_inverse(a, p->inv_paired, s)))) return st;
+
+  if (p->inv_row_south >= 0 || p->inv_row_north >= 0) {
+    PoleArgs pa{};
+    pa.flm = a.flm;
+    pa.flm_map_stride = flm_stride(p);
+    pa.ftm = ftm;
+    pa.ftm_map_stride = ftm_stride;
+    pa.ftm_ncol = p->Mc;
+    pa.ftm_m_first = p->m_lo;
+    pa.L = L;
+    pa.spin = p->d.spin;
+    pa.lmin = p->d.L_lower;
+    pa.row_south = p->inv_row_south;
+    pa.row_north = p->inv_row_north;
+    pa.norm = p->inv_norm;
+    pa.batch = batch;
+    if ((st = hip_status(launch_pole_inverse(pa, s)))) return st;
+  }
+
+  RingInvArgs r{};
+  r.jobs = p->inv_jobs;
+  r.in = ftm;
+  r.in_map_stride = ftm_stride;
+  r.ncol = p->Mc;
+  r.m_first = p->m_lo;
+  r.m_lo = -(L - 1);
+  r.m_hi = L - 1;
+  r.herm_mode = p->d.reality ? 1 : 0;
+  r.apply_phase = p->d.sampling == S2H_HEALPIX;
+  r.apply_weight = 0;
+  r.f = f;
+  r.f_complex = p->inv_out_complex ? 1 : 0;
+  r.map_stride = p->npix;
+  r.chirp_pool = p->chirp;
+  r.L = L;
+  return hip_status(launch_ring_fft_inverse(r, p->n_inv_jobs, batch, p->inv_lds, s));
+}
+
+extern "C" int s2h_lat_forward(const s2h_plan* p, const void* ftm, void* flm, int batch,
+                               s2h_stream stream) {
+  if (!p || !ftm || !flm || batch < 0) return S2H_ERR_INVALID;
+  if (batch == 0) return S2H_OK;
+  int st = check_device(p);
+  if (st) return st;
+  if (p->d.norm != S2H_NORM_NONE) return S2H_ERR_INVALID;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int L = p->d.L;
+  const bool wide = p->d.sampling == S2H_MWSS || p->d.sampling == S2H_HEALPIX;
+  const int ncol = wide ? 2 * L : 2 * L - 1;
+  if ((st = hip_status(launch_zero(flm, 2 * flm_stride(p) * batch, s)))) return st;
+  LatArgs a = lat_args(p, true, batch);
+  a.rings = p->inv_rings;
+  a.nrings = p->n_inv_rings;
+  a.flm = static_cast<cplx*>(flm);
+  a.ftm = const_cast<cplx*>(static_cast<const cplx*>(ftm));
+  a.ftm_map_stride = (int64_t)p->inv_rows * ncol;
+  a.ftm_ncol = ncol;
+  a.ftm_m_first = wide ? -L : -(L - 1);
+  return hip_status(launch_lat_forward(a, p->inv_paired, s));
+}
+
+extern "C" int s2h_lat_inverse(const s2h_plan* p, const void* flm, void* ftm, int batch,
+                               s2h_stream stream) {
+  if (!p || !ftm || !flm || batch < 0) return S2H_ERR_INVALID;
+  if (batch == 0) return S2H_OK;
+  int st = check_device(p);
+  if (st) return st;
+  if (p->d.norm != S2H_NORM_NONE) return S2H_ERR_INVALID;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int L = p->d.L;
+  const bool wide = p->d.sampling == S2H_MWSS || p->d.sampling == S2H_HEALPIX;
+  const int ncol = wide ? 2 * L : 2 * L - 1;
+  const int64_t stride = (int64_t)p->inv_rows * ncol;
+  if ((st = hip_status(launch_zero(ftm, 2 * stride * batch, s)))) return st;
+  LatArgs a = lat_args(p, false, batch);
+  a.flm = const_cast<cplx*>(static_cast<const cplx*>(flm));
+  a.ftm = static_cast<cplx*>(ftm);
+  a.ftm_map_stride = stride;
+  a.ftm_ncol = ncol;
+  a.ftm_m_first = wide ? -L : -(L - 1);
+  return hip_status(launch_lat_inverse(a, p->inv_paired, s));
+}
+
+extern "C" int s2h_healpix_fft(int nside, int L, int reality, const void* f, void* ftm, int batch,
+                               s2h_stream stream) {
+  if (nside < 1 || !f || !ftm || batch < 0) return S2H_ERR_INVALID;
+  if (L < 2 * nside) return S2H_ERR_BANDLIMIT;
+  if (4 * nside > kMaxRingLen) return S2H_ERR_UNSUPPORTED;
+  if (batch == 0) return S2H_OK;
+  HpTables tb;
+  int st = healpix_tables(nside, L, &tb);
+  if (st) return st;
+  RingFwdArgs r{};
+  r.jobs = tb.jobs;
+  r.f = f;
+  r.f_complex = 1;
+  r.real_only_2L = reality ? 1 : 0;
+  r.map_stride = 12LL * nside * nside;
+  r.out = static_cast<cplx*>(ftm);
+  r.out_map_stride = (int64_t)(4 * nside - 1) * 2 * L;
+  r.ncol = 2 * L;
+  r.m_first = -L;
+  r.zero_neg_2L = reality ? 1 : 0;
+  r.L = L;
+  r.chirp_pool = tb.chirp;
+  return hip_status(
+      launch_ring_fft_forward(r, tb.njobs, batch, tb.lds, static_cast<hipStream_t>(stream)));
+}
+
+extern "C" int s2h_healpix_ifft(int nside, int L, int reality, const void* ftm, void* f,
+                                int batch, s2h_stream stream) {
+  if (nside < 1 || !f || !ftm || batch < 0) return S2H_ERR_INVALID;
+  if (L < 2 * nside) return S2H_ERR_BANDLIMIT;
+  if (4 * nside > kMaxRingLen) return S2H_ERR_UNSUPPORTED;
+  if (batch == 0) return S2H_OK;
+  HpTables tb;
+  int st = healpix_tables(nside, L, &tb);
+  if (st) return st;
+  RingInvArgs r{};
+  r.jobs = tb.jobs;
+  r.in = static_cast<const cplx*>(ftm);
+  r.in_map_stride = (int64_t)(4 * nside - 1) * 2 * L;
+  r.ncol = 2 * L;
+  r.m_first = -L;
+  r.m_lo = -L;
+  r.m_hi = L - 1;
+  r.herm_mode = reality ? 2 : 0;
+  r.f = f;
+  r.f_complex = 1;
+  r.map_stride = 12LL * nside * nside;
+  r.chirp_pool = tb.chirp;
+  r.L = L;
+  return hip_status(
+      launch_ring_fft_inverse(r, tb.njobs, batch, tb.lds, static_cast<hipStream_t>(stream)));
+}
+
+extern "C" void healpix_fft_hip(void* stream, void** buffers, const char* opaque,
+                                size_t opaque_len) {
+  if (!buffers || !opaque || opaque_len != sizeof(s2h_healpix_fft_descriptor)) {
+    g_last_xla_status = S2H_ERR_INVALID;
+    return;
+  }
+  s2h_healpix_fft_descriptor d;
+  std::memcpy(&d, opaque, sizeof(d));
+  const int batch = d.batch > 0 ? d.batch : 1;
+  g_last_xla_status =
+      d.forward ? s2h_healpix_fft(d.nside, d.L, d.reality, buffers[0], buffers[1], batch, stream)
+                : s2h_healpix_ifft(d.nside, d.L, d.reality, buffers[0], buffers[1], batch, stream);
+}
+
+extern "C" int healpix_fft_hip_last_status(void) { return g_last_xla_status; }
+
+extern "C" int s2h_gamma_dft(int N, int64_t npix, int forward, double scale, const void* in,
+                             void* out, s2h_stream stream) {
+  if (N < 1 || N > 8 || npix < 0 || !in || !out || in == out) return S2H_ERR_INVALID;
+  return hip_status(launch_gamma_dft(N, npix, forward, scale, static_cast<const cplx*>(in),
+                                     static_cast<cplx*>(out), static_cast<hipStream_t>(stream)));
+}

@@ -1,0 +1,101 @@
+// Kernel argument blocks and launcher declarations.
+#pragma once
+
+#include "s2h_internal.h"
+
+namespace s2h {
+
+struct RingFwdArgs {
+  const RingJob* jobs;
+  const void* f;            // float64 or complex128 maps
+  int f_complex;
+  int real_only_2L;         // reference reality: rings with nphi == 2L use Re(f)
+  int real_only_all;        // every ring uses Re(f)
+  int64_t map_stride;       // pixels per map
+  cplx* out;
+  int64_t out_map_stride;   // complex elements per map of the output
+  int ncol;                 // output columns per row
+  int m_first;              // m of column 0
+  int apply_weight;
+  int apply_phase;
+  int zero_neg_2L;          // reference reality: m < 0 zero on rings with nphi == 2L
+  int L;
+  const cplx* chirp_pool;
+};
+
+struct RingInvArgs {
+  const RingJob* jobs;
+  const cplx* in;
+  int64_t in_map_stride;
+  int ncol;
+  int m_first;              // m of input column 0
+  int m_lo, m_hi;           // folded m range (non-Hermitian rings)
+  int herm_mode;            // 0 none, 1 every ring, 2 rings with nphi == 2L
+  int apply_phase;
+  int apply_weight;
+  void* f;
+  int f_complex;
+  int64_t map_stride;
+  const cplx* chirp_pool;
+  int L;
+};
+
+struct LatArgs {
+  const LatRing* rings;
+  int nrings;               // entries of rings (pairs count as one)
+  const MStart* mstart;     // indexed by m - m_lo
+  const Coef* coef;         // [|m|][l]
+  int L;
+  int m_lo;                 // first m handled (0 or -(L-1))
+  int nm;                   // number of m values handled
+  int lmin_out;             // first l written / read (L_lower)
+  int batch;
+  // harmonic side: flm[b][l][L-1+m]
+  cplx* flm;
+  int64_t flm_map_stride;
+  // ring side: ftm[b][row][m - ftm_m_first]
+  cplx* ftm;
+  int64_t ftm_map_stride;
+  int ftm_ncol;
+  int ftm_m_first;
+  int groups;               // ring groups per m (forward); >1 accumulates atomically
+};
+
+hipError_t launch_ring_fft_forward(const RingFwdArgs& a, int njobs, int batch, int lds_elems,
+                                   hipStream_t s);
+hipError_t launch_ring_fft_inverse(const RingInvArgs& a, int njobs, int batch, int lds_elems,
+                                   hipStream_t s);
+hipError_t launch_chirp_setup(const int* lengths, const int* conv_lens, const int64_t* offs,
+                              cplx* pool, int count, int max_conv, hipStream_t s);
+hipError_t launch_gamma_dft(int N, int64_t npix, int forward, double scale, const cplx* in,
+                            cplx* out, hipStream_t s);
+hipError_t allow_large_lds();
+
+hipError_t launch_lat_inverse(const LatArgs& a, bool paired, hipStream_t s);
+hipError_t launch_lat_forward(const LatArgs& a, bool paired, hipStream_t s);
+
+// Small fix-up kernels (s2h_misc.hip).
+struct PoleArgs {
+  // inverse: set ftm rows of the poles; forward: add pole terms into flm
+  cplx* flm;
+  int64_t flm_map_stride;
+  cplx* ftm;
+  int64_t ftm_map_stride;
+  int ftm_ncol;
+  int ftm_m_first;
+  int L;
+  int spin;
+  int lmin;                 // L_lower (inverse) or max(|spin|, L_lower) (forward)
+  int row_south;            // -1 if absent
+  int row_north;            // -1 if absent
+  const double* norm;       // per-l normalisation (times the global sign)
+  int batch;
+};
+hipError_t launch_pole_inverse(const PoleArgs& a, hipStream_t s);
+hipError_t launch_pole_forward(const PoleArgs& a, hipStream_t s);
+hipError_t launch_zero(void* p, int64_t ndoubles, hipStream_t s);
+hipError_t launch_reality_fill(cplx* flm, int64_t map_stride, int L, int batch, hipStream_t s);
+hipError_t launch_mw_resample(const cplx* g, cplx* out, const double* weights, int L, int spin,
+                              int mw, int m_lo, int nm, int batch, hipStream_t s);
+
+}  // namespace s2h

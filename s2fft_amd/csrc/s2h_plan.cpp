@@ -1,0 +1,574 @@
+// Plan construction: geometry, quadrature, recursion tables, FFT job tables.
+//
+// Geometry and weights follow s2fft/sampling/s2_samples.py:203-308 (thetas),
+// s2fft/utils/quadrature.py:99-288 (weights) and
+// s2fft/utils/healpix_ffts.py:429-452 (first-pixel azimuths).
+#include "s2h_plan.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
+
+namespace s2h {
+
+namespace {
+
+constexpr long double kPi = 3.141592653589793238462643383279502884L;
+
+template <typename T>
+int upload(s2h_plan* p, const std::vector<T>& v, T** out) {
+  *out = nullptr;
+  if (v.empty()) return S2H_OK;
+  void* ptr = nullptr;
+  if (hipMalloc(&ptr, sizeof(T) * v.size()) != hipSuccess) return S2H_ERR_NOMEM;
+  if (p) p->allocations.push_back(ptr);
+  if (hipMemcpy(ptr, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return S2H_ERR_HIP;
+  *out = static_cast<T*>(ptr);
+  return S2H_OK;
+}
+
+// ---------------------------------------------------------------- geometry
+double healpix_z(int t, int nside) {
+  const double ns = nside;
+  if (t < nside - 1) return 1.0 - (t + 1.0) * (t + 1.0) / (3.0 * ns * ns);
+  if (t <= 3 * nside - 1) return 4.0 / 3.0 - 2.0 * (t + 1.0) / (3.0 * ns);
+  const double u = 4.0 * ns - 1.0 - t;
+  return u * u / (3.0 * ns * ns) - 1.0;
+}
+
+int healpix_nphi(int t, int nside) {
+  if (t < nside - 1) return 4 * (t + 1);
+  if (t <= 3 * nside - 1) return 4 * nside;
+  return 4 * (4 * nside - t - 1);
+}
+
+double healpix_phi0_over_pi(int t, int nside) {
+  if (t + 1 < nside) return 0.25 / (t + 1);
+  if (t + 1 <= 3 * nside) return 0.25 * ((t - nside + 2) % 2) / nside;
+  return 0.25 / (4 * nside - t - 1);
+}
+
+// Gauss-Legendre nodes (cos theta, descending) and weights, Newton on P_L.
+void gauss_legendre(int L, std::vector<long double>& z, std::vector<long double>& w) {
+  z.assign(L, 0.0L);
+  w.assign(L, 0.0L);
+  const int half = (L + 1) / 2;
+  for (int i = 1; i <= half; ++i) {
+    long double x = std::cos(kPi * (i - 0.25L) / (L + 0.5L));
+    long double pp = 1.0L;
+    for (int it = 0; it < 100; ++it) {
+      long double p1 = 1.0L, p2 = 0.0L;
+      for (int j = 1; j <= L; ++j) {
+        const long double p3 = p2;
+        p2 = p1;
+        p1 = ((2.0L * j - 1.0L) * x * p2 - (j - 1.0L) * p3) / j;
+      }
+      pp = L * (x * p1 - p2) / (x * x - 1.0L);
+      const long double dx = p1 / pp;
+      x -= dx;
+      if (std::fabs(dx) < 1e-19L) break;
+    }
+    {
+      long double p1 = 1.0L, p2 = 0.0L;
+      for (int j = 1; j <= L; ++j) {
+        const long double p3 = p2;
+        p2 = p1;
+        p1 = ((2.0L * j - 1.0L) * x * p2 - (j - 1.0L) * p3) / j;
+      }
+      pp = L * (x * p1 - p2) / (x * x - 1.0L);
+    }
+    z[i - 1] = x;
+    z[L - i] = -x;
+    w[i - 1] = 2.0L / ((1.0L - x * x) * pp * pp);
+    w[L - i] = w[i - 1];
+  }
+}
+
+std::vector<double> thetas(int sampling, int L, int nside) {
+  std::vector<double> th;
+  switch (sampling) {
+    case S2H_MW:
+      for (int t = 0; t < L; ++t) th.push_back((2.0 * t + 1.0) * M_PI / (2.0 * L - 1.0));
+      break;
+    case S2H_MWSS:
+      for (int t = 0; t <= L; ++t) th.push_back(2.0 * t * M_PI / (2.0 * L));
+      break;
+    case S2H_DH:
+      for (int t = 0; t < 2 * L; ++t) th.push_back((2.0 * t + 1.0) * M_PI / (4.0 * L));
+      break;
+    case S2H_GL: {
+      std::vector<long double> z, w;
+      gauss_legendre(L, z, w);
+      for (int t = 0; t < L; ++t) th.push_back((double)std::acos((double)z[t]));
+      break;
+    }
+    case S2H_HEALPIX:
+      for (int t = 0; t < 4 * nside - 1; ++t) th.push_back(std::acos(healpix_z(t, nside)));
+      break;
+  }
+  return th;
+}
+
+// Weights the forward transform applies per ring (quadrature.py:7-50).
+std::vector<double> forward_weights(int sampling, int L, int nside) {
+  std::vector<double> w;
+  if (sampling == S2H_HEALPIX) {
+    w.assign(4 * nside - 1, (double)(4.0L * kPi / (12.0L * nside * nside)));
+  } else if (sampling == S2H_GL) {
+    std::vector<long double> z, ww;
+    gauss_legendre(L, z, ww);
+    for (int t = 0; t < L; ++t) w.push_back((double)(ww[t] * 2.0L * kPi / (2.0L * L - 1.0L)));
+  } else if (sampling == S2H_DH) {
+    for (int t = 0; t < 2 * L; ++t) {
+      const long double th = (2.0L * t + 1.0L) * kPi / (4.0L * L);
+      long double s = 0.0L;
+      for (int k = 0; k < L; ++k) s += std::sin((2.0L * k + 1.0L) * th) / (2.0L * k + 1.0L);
+      s *= 2.0L / L * std::sin(th);
+      w.push_back((double)(s * 2.0L * kPi / (2.0L * L - 1.0L)));
+    }
+  } else {
+    // MWSS at 2L (used after the resampling of MW / MWSS): quadrature.py:238-262
+    const int Lp = 2 * L;
+    std::vector<long double> wr(2 * Lp);
+    for (int t = 0; t < 2 * Lp; ++t) {
+      const long double a = kPi * t / Lp;
+      long double s = 2.0L + kPi * std::sin(a);
+      for (int p = 2; p <= Lp - 1; p += 2) s += 4.0L * std::cos(a * p) / (1.0L - (long double)p * p);
+      wr[t] = s / (2.0L * Lp);
+    }
+    for (int t = 0; t <= Lp; ++t) {
+      long double q = wr[t];
+      if (t >= 1 && t <= Lp - 1) q += wr[2 * Lp - t];
+      w.push_back((double)(q * 2.0L * kPi / (2.0L * L)));
+    }
+  }
+  return w;
+}
+
+// ---------------------------------------------------------------- recursion
+long double ell_norm(int kind, int l, bool inverse) {
+  const long double n = std::sqrt((2.0L * l + 1.0L) / (4.0L * kPi));
+  if (kind == S2H_NORM_NONE) return 1.0L;
+  if (kind == S2H_NORM_SPIN) return n;
+  // Wigner: wigner.py:637-644 (forward), 259-266 (inverse)
+  if (!inverse) return 1.0L;
+  return n * std::sqrt((2.0L * l + 1.0L) / (16.0L * kPi * kPi * kPi));
+}
+
+// sqrt(binomial(2j, j+k)) as (mantissa in [0.5,1), exponent).
+void sqrt_binom(int j, int k, long double& mant, int& ex) {
+  long double v = 1.0L;
+  int e = 0;
+  const int lo = j + k, cnt = j - k;
+  for (int i = 1; i <= cnt; ++i) {
+    v *= (long double)(lo + i) / (long double)i;
+    int ee;
+    v = std::frexp(v, &ee);
+    e += ee;
+  }
+  // sqrt(v 2^e)
+  if (e & 1) {
+    v *= 2.0L;
+    e -= 1;
+  }
+  v = std::sqrt(v);
+  int ee;
+  v = std::frexp(v, &ee);
+  mant = v;
+  ex = e / 2 + ee;
+}
+
+// Start value data for signed m; n = -spin.  See d^j formulas (Wigner's sum
+// with one index at +/-j).
+MStart make_mstart(int m, int n, int kind, bool inverse, double sign) {
+  MStart st{};
+  const int am = std::abs(m), an = std::abs(n);
+  const int j = std::max(am, an);
+  st.ell0 = j;
+  int k, sgn = 1;
+  if (am >= an) {
+    k = n;
+    if (m >= 0) {
+      st.ecos = j + n;
+      st.esin = j - n;
+      sgn = ((j - n) & 1) ? -1 : 1;
+    } else {
+      st.ecos = j - n;
+      st.esin = j + n;
+    }
+  } else {
+    k = m;
+    if (n > 0) {
+      st.ecos = j + m;
+      st.esin = j - m;
+    } else {
+      st.ecos = j - m;
+      st.esin = j + m;
+      sgn = ((j + m) & 1) ? -1 : 1;
+    }
+  }
+  long double mant;
+  int ex;
+  sqrt_binom(j, std::abs(k), mant, ex);
+  mant *= ell_norm(kind, j, inverse) * sgn * sign;
+  int ee;
+  const double mm = (double)std::frexp(mant, &ee);
+  st.kmant = mm;
+  st.kexp = ex + ee;
+  return st;
+}
+
+// Normalised three-term coefficients for |m| (b for m = +|m|).
+void make_coefs(int L, int n, int kind, bool inverse, std::vector<Coef>& out) {
+  out.assign((size_t)L * L, Coef{0, 0, 0, 0});
+  std::vector<long double> N(L + 1);
+  for (int l = 0; l <= L; ++l) N[l] = ell_norm(kind, l, inverse);
+  const long double nn = n;
+  for (int am = 0; am < L; ++am) {
+    const long double mm = am;
+    const int l0 = std::max(am, std::abs(n));
+    for (int l = l0; l < L; ++l) {
+      Coef c{0, 0, 0, 0};
+      if (l == 0) {
+        c.a = (double)(N[1] / N[0]);
+      } else {
+        const long double L1 = l + 1.0L, Ll = l;
+        const long double D =
+            Ll * std::sqrt((L1 * L1 - mm * mm) * (L1 * L1 - nn * nn));
+        const long double A = (2.0L * Ll + 1.0L) * Ll * L1 / D;
+        const long double B = (2.0L * Ll + 1.0L) * mm * nn / D;
+        const long double C =
+            L1 * std::sqrt(std::max(0.0L, (Ll * Ll - mm * mm) * (Ll * Ll - nn * nn))) / D;
+        c.a = (double)(A * N[l + 1] / N[l]);
+        c.b = (double)(B * N[l + 1] / N[l]);
+        c.c = (double)(C * N[l + 1] / N[l - 1]);
+      }
+      out[(size_t)am * L + l] = c;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- FFT jobs
+int next_pow2(int v) {
+  int p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+struct ChirpBuilder {
+  std::map<int, std::pair<int, int64_t>> table;  // n -> (M, offset)
+  int64_t total = 0;
+  int max_conv = 0;
+  // Returns conv length (0 = mixed radix) and sets the chirp offset.
+  int add(int n, int64_t* off) {
+    *off = 0;
+    if (largest_prime_factor(n) <= 13) return 0;
+    auto it = table.find(n);
+    if (it != table.end()) {
+      *off = it->second.second;
+      return it->second.first;
+    }
+    const int M = next_pow2(2 * n - 1);
+    table[n] = {M, total};
+    *off = total;
+    total += M;
+    max_conv = std::max(max_conv, M);
+    return M;
+  }
+  int build(cplx** pool, std::vector<void*>* allocs) {
+    *pool = nullptr;
+    if (table.empty()) return S2H_OK;
+    void* ptr = nullptr;
+    if (hipMalloc(&ptr, sizeof(cplx) * total) != hipSuccess) return S2H_ERR_NOMEM;
+    if (allocs) allocs->push_back(ptr);
+    *pool = static_cast<cplx*>(ptr);
+    std::vector<int> ns, Ms;
+    std::vector<int64_t> offs;
+    for (auto& kv : table) {
+      ns.push_back(kv.first);
+      Ms.push_back(kv.second.first);
+      offs.push_back(kv.second.second);
+    }
+    int* dn = nullptr;
+    int* dM = nullptr;
+    int64_t* doff = nullptr;
+    if (hipMalloc(&dn, sizeof(int) * ns.size()) != hipSuccess) return S2H_ERR_NOMEM;
+    if (hipMalloc(&dM, sizeof(int) * ns.size()) != hipSuccess) return S2H_ERR_NOMEM;
+    if (hipMalloc(&doff, sizeof(int64_t) * ns.size()) != hipSuccess) return S2H_ERR_NOMEM;
+    (void)hipMemcpy(dn, ns.data(), sizeof(int) * ns.size(), hipMemcpyHostToDevice);
+    (void)hipMemcpy(dM, Ms.data(), sizeof(int) * ns.size(), hipMemcpyHostToDevice);
+    (void)hipMemcpy(doff, offs.data(), sizeof(int64_t) * ns.size(), hipMemcpyHostToDevice);
+    hipError_t e = launch_chirp_setup(dn, dM, doff, *pool, (int)ns.size(), max_conv, 0);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    (void)hipFree(dn);
+    (void)hipFree(dM);
+    (void)hipFree(doff);
+    return e == hipSuccess ? S2H_OK : S2H_ERR_HIP;
+  }
+};
+
+RingJob make_job(int64_t off, int nphi, int row, double phi0, double w, ChirpBuilder& cb) {
+  RingJob j{};
+  j.pix_offset = off;
+  j.nphi = nphi;
+  j.row = row;
+  j.phi0_over_pi = phi0;
+  j.weight = w;
+  j.conv_len = cb.add(nphi, &j.chirp_off);
+  return j;
+}
+
+int lds_of(const std::vector<RingJob>& jobs) {
+  int m = 0;
+  for (auto& j : jobs) m = std::max(m, j.conv_len ? j.conv_len : j.nphi);
+  return m;
+}
+
+// Latitudinal ring list over rows [r0, r1] of a theta grid.
+std::vector<LatRing> lat_rings(const std::vector<double>& th, int r0, int r1, bool paired) {
+  std::vector<LatRing> out;
+  auto mk = [&](int ta, int tb) {
+    LatRing r{};
+    r.ta = ta;
+    r.tb = tb;
+    r.x = std::cos(th[ta]);
+    r.c2 = std::cos(th[ta] / 2.0);
+    r.s2 = std::sin(th[ta] / 2.0);
+    return r;
+  };
+  if (!paired) {
+    for (int t = r0; t <= r1; ++t) out.push_back(mk(t, -1));
+  } else {
+    for (int t = r0, u = r1; t <= u; ++t, --u) out.push_back(mk(t, u));
+  }
+  return out;
+}
+
+}  // namespace
+
+// Reference-layout HEALPix ring tables, cached per (device, nside, L).
+int healpix_tables(int nside, int L, HpTables* out) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int>, HpTables> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return S2H_ERR_HIP;
+  std::lock_guard<std::mutex> lock(mu);
+  auto key = std::make_tuple(dev, nside, L);
+  auto it = cache.find(key);
+  if (it != cache.end()) {
+    *out = it->second;
+    return S2H_OK;
+  }
+  if (allow_large_lds() != hipSuccess) return S2H_ERR_HIP;
+  ChirpBuilder cb;
+  std::vector<RingJob> jobs;
+  int64_t off = 0;
+  for (int t = 0; t < 4 * nside - 1; ++t) {
+    const int n = healpix_nphi(t, nside);
+    jobs.push_back(make_job(off, n, t, healpix_phi0_over_pi(t, nside), 1.0, cb));
+    off += n;
+  }
+  HpTables tb;
+  int st = upload<RingJob>(nullptr, jobs, &tb.jobs);
+  if (st) return st;
+  tb.njobs = (int)jobs.size();
+  tb.lds = lds_of(jobs);
+  st = cb.build(&tb.chirp, nullptr);
+  if (st) return st;
+  cache[key] = tb;
+  *out = tb;
+  return S2H_OK;
+}
+
+}  // namespace s2h
+
+using namespace s2h;
+
+static int plan_build(s2h_plan* p) {
+  const s2h_desc& d = p->d;
+  const int L = d.L, nside = d.nside, spin = d.spin;
+  const bool real = d.reality != 0;
+  const bool hp = d.sampling == S2H_HEALPIX;
+  const int n = -spin;
+  if (allow_large_lds() != hipSuccess) return S2H_ERR_HIP;
+
+  p->m_lo = real ? 0 : -(L - 1);
+  p->nm = L - p->m_lo;
+  p->Mc = p->nm;
+  p->sign = (double)d.extra_sign * ((d.norm != S2H_NORM_NONE && (spin & 1)) ? -1.0 : 1.0);
+  const bool pair_ok = spin == 0;
+
+  ChirpBuilder cb;
+  std::vector<double> thi = thetas(d.sampling, L, nside);
+
+  // ---------------- inverse geometry
+  std::vector<LatRing> inv_r;
+  std::vector<RingJob> inv_j;
+  switch (d.sampling) {
+    case S2H_HEALPIX: {
+      p->inv_rows = 4 * nside - 1;
+      inv_r = lat_rings(thi, 0, p->inv_rows - 1, pair_ok);
+      int64_t off = 0;
+      for (int t = 0; t < p->inv_rows; ++t) {
+        const int np = healpix_nphi(t, nside);
+        inv_j.push_back(make_job(off, np, t, healpix_phi0_over_pi(t, nside), 1.0, cb));
+        off += np;
+      }
+      p->npix = off;
+      break;
+    }
+    case S2H_DH:
+    case S2H_GL: {
+      p->inv_rows = (int)thi.size();
+      inv_r = lat_rings(thi, 0, p->inv_rows - 1, pair_ok);
+      const int np = 2 * L - 1;
+      for (int t = 0; t < p->inv_rows; ++t) inv_j.push_back(make_job((int64_t)t * np, np, t, 0.0, 1.0, cb));
+      p->npix = (int64_t)p->inv_rows * np;
+      break;
+    }
+    case S2H_MW: {
+      p->inv_rows = L;
+      inv_r = lat_rings(thi, 0, L - 2, false);  // theta_{L-1} = pi is the pole row
+      p->inv_row_south = L - 1;
+      const int np = 2 * L - 1;
+      for (int t = 0; t < L; ++t) inv_j.push_back(make_job((int64_t)t * np, np, t, 0.0, 1.0, cb));
+      p->npix = (int64_t)L * np;
+      break;
+    }
+    case S2H_MWSS: {
+      p->inv_rows = L + 1;
+      inv_r = lat_rings(thi, 1, L - 1, pair_ok);
+      p->inv_row_north = 0;
+      p->inv_row_south = L;
+      const int np = 2 * L;
+      for (int t = 0; t <= L; ++t) inv_j.push_back(make_job((int64_t)t * np, np, t, 0.0, 1.0, cb));
+      p->npix = (int64_t)(L + 1) * np;
+      break;
+    }
+    default:
+      return S2H_ERR_INVALID;
+  }
+  p->inv_paired = pair_ok && d.sampling != S2H_MW;
+  p->inv_out_complex = hp || !real;
+
+  // ---------------- forward geometry
+  std::vector<LatRing> fwd_r;
+  std::vector<RingJob> fwd_j;
+  std::vector<double> wf;
+  if (d.sampling == S2H_HEALPIX || d.sampling == S2H_DH || d.sampling == S2H_GL) {
+    p->fwd_kind = 0;
+    p->fwd_rows = p->inv_rows;
+    fwd_r = inv_r;
+    p->fwd_paired = p->inv_paired;
+    wf = forward_weights(d.sampling, L, nside);
+    fwd_j = inv_j;
+    for (size_t t = 0; t < fwd_j.size(); ++t) fwd_j[t].weight = wf[t];
+  } else {
+    // resampled to MWSS at 2L: rows 0..2L, poles at 0 and 2L
+    p->fwd_kind = d.sampling == S2H_MW ? 1 : 2;
+    p->fwd_rows = 2 * L + 1;
+    std::vector<double> thf;
+    for (int t = 0; t <= 2 * L; ++t) thf.push_back(2.0 * t * M_PI / (4.0 * L));
+    fwd_r = lat_rings(thf, 1, 2 * L - 1, pair_ok);
+    p->fwd_paired = pair_ok;
+    p->fwd_row_north = 0;
+    p->fwd_row_south = 2 * L;
+    wf = forward_weights(d.sampling, L, nside);
+    p->fwd_in_rows = p->inv_rows;
+    fwd_j = inv_j;  // phi-DFT of the input rows, weight 1
+    return S2H_ERR_UNSUPPORTED;  // resampling kernels not yet wired
+  }
+
+  // ---------------- recursion tables
+  std::vector<MStart> msf, msi;
+  for (int m = p->m_lo; m < L; ++m) {
+    msf.push_back(make_mstart(m, n, d.norm, false, p->sign));
+    msi.push_back(make_mstart(m, n, d.norm, true, p->sign));
+  }
+  std::vector<Coef> cf, ci;
+  make_coefs(L, n, d.norm, false, cf);
+  const bool same = d.norm != S2H_NORM_WIGNER;
+  if (!same) make_coefs(L, n, d.norm, true, ci);
+  std::vector<double> nf(L), ni(L);
+  for (int l = 0; l < L; ++l) {
+    nf[l] = (double)(ell_norm(d.norm, l, false) * p->sign);
+    ni[l] = (double)(ell_norm(d.norm, l, true) * p->sign);
+  }
+
+  int st;
+  if ((st = upload(p, inv_r, &p->inv_rings))) return st;
+  p->n_inv_rings = (int)inv_r.size();
+  if ((st = upload(p, fwd_r, &p->fwd_rings))) return st;
+  p->n_fwd_rings = (int)fwd_r.size();
+  if ((st = upload(p, inv_j, &p->inv_jobs))) return st;
+  p->n_inv_jobs = (int)inv_j.size();
+  p->inv_lds = lds_of(inv_j);
+  if ((st = upload(p, fwd_j, &p->fwd_jobs))) return st;
+  p->n_fwd_jobs = (int)fwd_j.size();
+  p->fwd_lds = lds_of(fwd_j);
+  if ((st = upload(p, msf, &p->fwd_mstart))) return st;
+  if ((st = upload(p, msi, &p->inv_mstart))) return st;
+  if ((st = upload(p, cf, &p->fwd_coef))) return st;
+  if (same) {
+    p->inv_coef = p->fwd_coef;
+  } else if ((st = upload(p, ci, &p->inv_coef))) {
+    return st;
+  }
+  if ((st = upload(p, nf, &p->fwd_norm))) return st;
+  if ((st = upload(p, ni, &p->inv_norm))) return st;
+  if (!wf.empty() && p->fwd_kind != 0) {
+    if ((st = upload(p, wf, &p->fwd_weights))) return st;
+  }
+  if ((st = cb.build(&p->chirp, &p->allocations))) return st;
+  return S2H_OK;
+}
+
+extern "C" int s2h_plan_create(s2h_plan** out, const s2h_desc* desc) {
+  if (!out || !desc) return S2H_ERR_INVALID;
+  *out = nullptr;
+  const s2h_desc& d = *desc;
+  if (d.sampling < S2H_MW || d.sampling > S2H_HEALPIX) return S2H_ERR_INVALID;
+  if (d.L < 2) return S2H_ERR_INVALID;
+  if (d.spin >= 8 || d.spin <= -8) return S2H_ERR_SPIN;
+  if (std::abs(d.spin) >= d.L) return S2H_ERR_INVALID;
+  if (d.L_lower < 0 || d.L_lower >= d.L) return S2H_ERR_INVALID;
+  if (d.reality && d.spin != 0) return S2H_ERR_INVALID;
+  if (d.norm < S2H_NORM_SPIN || d.norm > S2H_NORM_NONE) return S2H_ERR_INVALID;
+  if (d.extra_sign != 1 && d.extra_sign != -1) return S2H_ERR_INVALID;
+  if (d.sampling == S2H_HEALPIX) {
+    if (d.nside < 1) return S2H_ERR_INVALID;
+    if (d.L < 2 * d.nside) return S2H_ERR_BANDLIMIT;
+    if (4 * d.nside > kMaxRingLen) return S2H_ERR_UNSUPPORTED;
+  } else {
+    if (2 * d.L > kMaxRingLen) return S2H_ERR_UNSUPPORTED;
+  }
+  s2h_plan* p = new (std::nothrow) s2h_plan();
+  if (!p) return S2H_ERR_NOMEM;
+  p->d = d;
+  if (hipGetDevice(&p->device) != hipSuccess) {
+    delete p;
+    return S2H_ERR_HIP;
+  }
+  const int st = plan_build(p);
+  if (st != S2H_OK) {
+    s2h_plan_destroy(p);
+    return st;
+  }
+  *out = p;
+  return S2H_OK;
+}
+
+extern "C" int s2h_plan_destroy(s2h_plan* p) {
+  if (!p) return S2H_OK;
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  (void)hipSetDevice(p->device);
+  for (void* ptr : p->allocations) (void)hipFree(ptr);
+  (void)hipSetDevice(cur);
+  delete p;
+  return S2H_OK;
+}

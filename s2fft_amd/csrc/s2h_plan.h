@@ -1,0 +1,62 @@
+// Host-side plan: constant tables for one (sampling, L, nside, spin, ...) set.
+#pragma once
+
+#include <vector>
+
+#include "s2h_kernels.h"
+
+struct s2h_plan {
+  s2h_desc d{};
+  int device = 0;
+  int m_lo = 0;      // first m computed (0 with reality, else -(L-1))
+  int nm = 0;        // number of m computed
+  int Mc = 0;        // columns of the internal ftm (m in [m_lo, L))
+  double sign = 1;   // global sign folded into the recursion
+
+  // forward
+  int fwd_kind = 0;  // 0 direct ring FFT, 1 MW resample, 2 MWSS resample
+  int fwd_rows = 0;  // rows of the internal forward ftm
+  int fwd_in_rows = 0;  // rows of the phi-spectrum before resampling (kinds 1, 2)
+  s2h::RingJob* fwd_jobs = nullptr;
+  int n_fwd_jobs = 0;
+  int fwd_lds = 0;
+  s2h::LatRing* fwd_rings = nullptr;
+  int n_fwd_rings = 0;
+  bool fwd_paired = false;
+  int fwd_row_south = -1, fwd_row_north = -1;
+  s2h::MStart* fwd_mstart = nullptr;
+  s2h::Coef* fwd_coef = nullptr;
+  double* fwd_norm = nullptr;
+  double* fwd_weights = nullptr;  // MWSS-at-2L weights (kinds 1, 2)
+  int rs_info[8] = {0};           // resampling DFT lengths / conv lengths
+  int64_t rs_chirp[4] = {0};
+
+  // inverse
+  int inv_rows = 0;
+  s2h::RingJob* inv_jobs = nullptr;
+  int n_inv_jobs = 0;
+  int inv_lds = 0;
+  s2h::LatRing* inv_rings = nullptr;
+  int n_inv_rings = 0;
+  bool inv_paired = false;
+  int inv_row_south = -1, inv_row_north = -1;
+  s2h::MStart* inv_mstart = nullptr;
+  s2h::Coef* inv_coef = nullptr;
+  double* inv_norm = nullptr;
+  bool inv_out_complex = true;
+
+  s2h::cplx* chirp = nullptr;
+  int64_t npix = 0;   // pixels per map (forward input / inverse output)
+  std::vector<void*> allocations;
+};
+
+namespace s2h {
+// Ring-job tables for the reference-layout HEALPix FFTs (cached per device).
+struct HpTables {
+  RingJob* jobs = nullptr;
+  int njobs = 0;
+  int lds = 0;
+  cplx* chirp = nullptr;
+};
+int healpix_tables(int nside, int L, HpTables* out);
+}  // namespace s2h

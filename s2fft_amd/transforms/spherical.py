@@ -1,0 +1,172 @@
+"""Spin spherical-harmonic transforms with ``method="hip"``.
+
+Drop-in for s2fft.transforms.spherical.forward / inverse
+(s2fft/transforms/spherical.py:21-109 and 352-464): same positional and
+keyword arguments, same array layouts (flm is (L, 2L-1) with column L-1+m,
+f as s2fft.sampling.s2_samples.f_shape), same exceptions.  Every transform
+runs on the GPU through the C ABI of ``libs2fft_amd.so``; there is no CPU
+path.  Inputs may be NumPy arrays (the result is returned as NumPy) or torch
+tensors (the result stays on the tensor's device); a leading batch axis is
+accepted and transformed in one launch per kernel.
+"""
+
+import threading
+
+import numpy as np
+import torch
+
+from .. import _lib
+from ..sampling import s2_samples as samples
+
+METHODS = ("hip",)
+
+_plan_lock = threading.Lock()
+_plans = {}
+
+
+def get_plan(sampling, L, nside=None, spin=0, L_lower=0, reality=False,
+             norm=_lib.NORM_SPIN, extra_sign=1, device=None):
+    """Cached native plan for the current (or given) CUDA/HIP device."""
+    dev = torch.cuda.current_device() if device is None else torch.device(device).index
+    if dev is None:
+        dev = torch.cuda.current_device()
+    s = sampling.lower()
+    key = (dev, s, int(L), int(nside or 0) if s == "healpix" else 0, int(spin), int(L_lower),
+           bool(reality), int(norm), int(extra_sign))
+    with _plan_lock:
+        plan = _plans.get(key)
+        if plan is None:
+            with torch.cuda.device(dev):
+                plan = _lib.Plan(s, L, nside if s == "healpix" else 0, spin, L_lower, reality,
+                                 norm, extra_sign)
+            _plans[key] = plan
+    return plan
+
+
+def clear_plans():
+    with _plan_lock:
+        _plans.clear()
+
+
+# ------------------------------------------------------------------ helpers
+def _as_device_tensor(x, complex_only=False):
+    """(tensor on a GPU, restore) where restore maps a result back to x's kind."""
+    if isinstance(x, torch.Tensor):
+        if x.is_cuda:
+            t, restore = x, (lambda y: y)
+        else:
+            t = x.to("cuda")
+            restore = (lambda y: y.cpu())
+    else:
+        t = torch.from_numpy(np.ascontiguousarray(x)).to("cuda")
+        restore = (lambda y: y.cpu().numpy())
+    if t.is_complex():
+        t = t.to(torch.complex128)
+    elif complex_only:
+        t = t.to(torch.complex128)
+    else:
+        t = t.to(torch.float64)
+    return t.contiguous(), restore
+
+
+def _stream(device):
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def _split_batch(t, core_shape, what):
+    core = tuple(core_shape)
+    if tuple(t.shape) == core:
+        return t.reshape((1,) + core), False
+    if t.dim() == len(core) + 1 and tuple(t.shape[1:]) == core:
+        return t, True
+    raise ValueError(f"{what} has shape {tuple(t.shape)}, expected {core} or (batch,) + {core}")
+
+
+def _check_common(method, spin, sampling, nside):
+    if method not in METHODS:
+        raise ValueError(f"Method {method} not recognised.")
+    if spin >= 8:
+        raise Warning("Recursive transform may provide lower precision beyond spin ~ 8")
+    if sampling.lower() not in samples.SAMPLINGS:
+        raise ValueError(f"Sampling scheme sampling={sampling} not supported")
+    if sampling.lower() == "healpix" and nside is None:
+        raise ValueError(f"Sampling scheme sampling={sampling} with nside={nside} not supported")
+
+
+# ------------------------------------------------------------------ device level
+def inverse_hip(flm, L, spin=0, nside=None, sampling="mw", reality=False, L_lower=0,
+                plan=None):
+    """Batched inverse on device tensors: flm (B, L, 2L-1) complex128 -> f (B, ...)."""
+    s = sampling.lower()
+    plan = plan or get_plan(s, L, nside, spin, L_lower, reality, device=flm.device)
+    B = flm.shape[0]
+    fshape = samples.f_shape(L, s, nside)
+    dtype = torch.complex128 if plan.inverse_output_is_complex() else torch.float64
+    f = torch.empty((B,) + fshape, dtype=dtype, device=flm.device)
+    work = torch.empty(plan.workspace_bytes(B), dtype=torch.uint8, device=flm.device)
+    flm = flm.contiguous()
+    _lib.check(_lib.lib().s2h_inverse(plan.handle, flm.data_ptr(), f.data_ptr(),
+                                      work.data_ptr(), B, _stream(flm.device)), "inverse")
+    return f
+
+
+def forward_hip(f, L, spin=0, nside=None, sampling="mw", reality=False, L_lower=0, plan=None):
+    """Batched forward on device tensors: f (B, ...) -> flm (B, L, 2L-1) complex128."""
+    s = sampling.lower()
+    plan = plan or get_plan(s, L, nside, spin, L_lower, reality, device=f.device)
+    B = f.shape[0]
+    flm = torch.empty((B, L, 2 * L - 1), dtype=torch.complex128, device=f.device)
+    work = torch.empty(plan.workspace_bytes(B), dtype=torch.uint8, device=f.device)
+    f = f.contiguous()
+    _lib.check(_lib.lib().s2h_forward(plan.handle, f.data_ptr(), int(f.is_complex()),
+                                      flm.data_ptr(), work.data_ptr(), B, _stream(f.device)),
+               "forward")
+    return flm
+
+
+def forward_iterative_hip(f, L, spin=0, nside=None, sampling="mw", reality=False, L_lower=0,
+                          n_iter=0):
+    """flm = F(f); repeat n_iter: flm += F(f - I(flm)).
+
+    s2fft/utils/iterative_refinement.py:9-46 as used by spherical.py:453-462;
+    the residual and the update stay in FP64 on the device.
+    """
+    flm = forward_hip(f, L, spin, nside, sampling, reality, L_lower)
+    for _ in range(n_iter):
+        f_rec = inverse_hip(flm, L, spin, nside, sampling, reality, L_lower)
+        flm = flm + forward_hip(f - f_rec, L, spin, nside, sampling, reality, L_lower)
+    return flm
+
+
+# ------------------------------------------------------------------ reference API
+def inverse(flm, L, spin=0, nside=None, sampling="mw", method="hip", reality=False,
+            precomps=None, spmd=False, L_lower=0, _ssht_backend=1):
+    """Inverse spin-spherical harmonic transform (s2fft/transforms/spherical.py:21-109).
+
+    ``precomps``, ``spmd`` and ``_ssht_backend`` are accepted for signature
+    compatibility; the recursion tables live in a cached native plan and the
+    transform runs on the current GPU.
+    """
+    _check_common(method, spin, sampling, nside)
+    t, restore = _as_device_tensor(flm, complex_only=True)
+    tb, batched = _split_batch(t, samples.flm_shape(L), "flm")
+    with torch.cuda.device(tb.device):
+        f = inverse_hip(tb, L, spin, nside, sampling, reality, L_lower)
+    return restore(f if batched else f[0])
+
+
+def forward(f, L, spin=0, nside=None, sampling="mw", method="hip", reality=False,
+            precomps=None, spmd=False, L_lower=0, iter=None, _ssht_backend=1):
+    """Forward spin-spherical harmonic transform (s2fft/transforms/spherical.py:352-464).
+
+    ``iter`` > 0 applies that many steps of iterative refinement
+    (default 0, as for every method except jax_healpy in the reference).
+    """
+    _check_common(method, spin, sampling, nside)
+    if iter is None:
+        iter = 0
+    t, restore = _as_device_tensor(f)
+    tb, batched = _split_batch(t, samples.f_shape(L, sampling, nside), "f")
+    with torch.cuda.device(tb.device):
+        flm = forward_iterative_hip(tb, L, spin, nside, sampling, reality, L_lower, iter)
+    return restore(flm if batched else flm[0])

@@ -1,0 +1,174 @@
+"""HIP transforms vs the CPU oracle on the same seeded inputs (needs a GPU).
+
+Parameter grids follow the reference's tests/test_spherical_transform.py:14-21
+(L in {6, 7}, spins {-2, 0, 1}, L_lower {0, 2}, nside {4, 5}) plus larger
+sizes that exercise the Bluestein ring lengths (prime factors > 13) and the
+high-m underflow handling.  Tolerance: the plain FP64 kernels must agree with
+the oracle to 1e-12 norm-wise relative error.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import ffts as offts
+from oracle import pm as opm
+from oracle import signals, transforms as otr
+from oracle import geometry as ogeo
+from tests.conftest import max_rel
+
+import s2fft_amd
+from s2fft_amd import _lib
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def _flm(rng, L, spin=0, reality=False, L_lower=0):
+    return signals.generate_flm(rng, L, L_lower, spin, reality)
+
+
+@pytest.mark.parametrize("sampling", ["mw", "mwss", "dh", "gl"])
+@pytest.mark.parametrize("L", [6, 7])
+@pytest.mark.parametrize("spin", [-2, 0, 1])
+@pytest.mark.parametrize("reality", [False, True])
+@pytest.mark.parametrize("L_lower", [0, 2])
+def test_inverse_equiangular(rng, sampling, L, spin, reality, L_lower):
+    if reality and spin != 0:
+        pytest.skip("reality only for spin 0")
+    flm = _flm(rng, L, spin, reality, L_lower)
+    ref = otr.inverse(flm, L, spin, sampling=sampling, reality=reality, L_lower=L_lower)
+    got = s2fft_amd.inverse(flm, L, spin, sampling=sampling, method="hip", reality=reality,
+                            L_lower=L_lower)
+    assert got.shape == ref.shape
+    assert max_rel(got, ref) < TOL
+
+
+@pytest.mark.parametrize("sampling", ["dh", "gl"])
+@pytest.mark.parametrize("L", [6, 7, 9])
+@pytest.mark.parametrize("spin", [-2, 0, 1])
+@pytest.mark.parametrize("reality", [False, True])
+@pytest.mark.parametrize("L_lower", [0, 2])
+def test_forward_equiangular(rng, sampling, L, spin, reality, L_lower):
+    if reality and spin != 0:
+        pytest.skip("reality only for spin 0")
+    flm = _flm(rng, L, spin, reality, L_lower)
+    f = otr.inverse(flm, L, spin, sampling=sampling, reality=reality, L_lower=L_lower)
+    ref = otr.forward(f, L, spin, sampling=sampling, reality=reality, L_lower=L_lower)
+    got = s2fft_amd.forward(f, L, spin, sampling=sampling, method="hip", reality=reality,
+                            L_lower=L_lower)
+    assert max_rel(got, ref) < TOL
+    assert max_rel(got, flm) < 1e-11   # sampling theorem: exact round trip
+
+
+@pytest.mark.parametrize("nside", [4, 5, 16, 32])
+@pytest.mark.parametrize("reality", [False, True])
+def test_healpix_inverse_forward(rng, nside, reality):
+    L = 2 * nside
+    flm = _flm(rng, L, 0, reality)
+    ref_f = otr.inverse(flm, L, 0, nside, "healpix", reality)
+    got_f = s2fft_amd.inverse(flm, L, 0, nside, "healpix", "hip", reality)
+    assert got_f.dtype == np.complex128
+    assert max_rel(got_f, ref_f) < TOL
+    f = np.real(ref_f) if reality else ref_f
+    ref = otr.forward(f, L, 0, nside, "healpix", reality)
+    got = s2fft_amd.forward(f, L, 0, nside, "healpix", "hip", reality)
+    assert max_rel(got, ref) < TOL
+
+
+@pytest.mark.parametrize("nside,L,spin", [(4, 8, 2), (5, 12, -1), (8, 24, 0)])
+def test_healpix_spin_and_oversampled(rng, nside, L, spin):
+    flm = _flm(rng, L, spin)
+    ref_f = otr.inverse(flm, L, spin, nside, "healpix")
+    got_f = s2fft_amd.inverse(flm, L, spin, nside, "healpix", "hip")
+    assert max_rel(got_f, ref_f) < TOL
+    ref = otr.forward(ref_f, L, spin, nside, "healpix")
+    got = s2fft_amd.forward(ref_f, L, spin, nside, "healpix", "hip")
+    assert max_rel(got, ref) < TOL
+
+
+@pytest.mark.parametrize("nside", [4, 5])
+@pytest.mark.parametrize("n_iter", [0, 1, 3])
+def test_healpix_iterative(rng, nside, n_iter):
+    L = 2 * nside
+    flm = _flm(rng, L, 0, True)
+    f = np.real(otr.inverse(flm, L, 0, nside, "healpix", True))
+    ref = otr.forward_iterative(f, L, 0, nside, "healpix", True, n_iter=n_iter)
+    got = s2fft_amd.forward(f, L, 0, nside, "healpix", "hip", True, iter=n_iter)
+    assert max_rel(got, ref) < TOL
+
+
+@pytest.mark.parametrize("nside", [4, 5, 32])
+@pytest.mark.parametrize("L_mult", [2, 3])
+@pytest.mark.parametrize("reality", [False, True])
+def test_healpix_ring_fft_entry(rng, nside, L_mult, reality):
+    L = L_mult * nside
+    npix = 12 * nside * nside
+    f = rng.standard_normal(npix) + (0 if reality else 1j * rng.standard_normal(npix))
+    f = f.astype(np.complex128)
+    ref = offts.healpix_fft(f, L, nside, reality)
+    ft = torch.from_numpy(f).cuda()
+    ftm = torch.empty((4 * nside - 1, 2 * L), dtype=torch.complex128, device="cuda")
+    s = torch.cuda.current_stream().cuda_stream
+    _lib.check(_lib.lib().s2h_healpix_fft(nside, L, int(reality), ft.data_ptr(), ftm.data_ptr(),
+                                          1, s))
+    assert max_rel(ftm.cpu().numpy(), ref) < TOL
+    back_ref = offts.healpix_ifft(ref.copy(), L, nside, reality)
+    back = torch.empty(npix, dtype=torch.complex128, device="cuda")
+    _lib.check(_lib.lib().s2h_healpix_ifft(nside, L, int(reality), ftm.data_ptr(),
+                                           back.data_ptr(), 1, s))
+    assert max_rel(back.cpu().numpy(), back_ref) < TOL
+
+
+@pytest.mark.parametrize("sampling,L,nside", [("healpix", 8, 4), ("dh", 7, None),
+                                              ("gl", 6, None), ("mw", 7, None)])
+@pytest.mark.parametrize("spin", [0, 2])
+def test_latitudinal_entry(rng, sampling, L, nside, spin):
+    """s2h_lat_inverse / s2h_lat_forward vs the oracle's latitudinal steps."""
+    plan = _lib.Plan(sampling, L, nside or 0, spin, 0, False, _lib.NORM_NONE)
+    beta = ogeo.thetas(L, sampling, nside)
+    interior = slice(0, L - 1) if sampling == "mw" else slice(None)
+    flm = _flm(rng, L, spin)
+    ref = opm.inverse_latitudinal_step(flm, beta, L, spin, nside, sampling)
+    nrow, ncol = ref.shape
+    d_flm = torch.from_numpy(flm).cuda()
+    d_ftm = torch.empty((nrow, ncol), dtype=torch.complex128, device="cuda")
+    s = torch.cuda.current_stream().cuda_stream
+    _lib.check(_lib.lib().s2h_lat_inverse(plan.handle, d_flm.data_ptr(), d_ftm.data_ptr(), 1, s))
+    got = d_ftm.cpu().numpy()
+    assert max_rel(got[interior], ref[interior]) < TOL
+    ftm = rng.standard_normal(ref.shape) + 1j * rng.standard_normal(ref.shape)
+    if sampling == "mw":
+        ftm[L - 1] = 0
+    ref2 = opm.forward_latitudinal_step(ftm, beta, L, spin, nside, sampling)
+    d_in = torch.from_numpy(ftm).cuda()
+    d_out = torch.empty((L, 2 * L - 1), dtype=torch.complex128, device="cuda")
+    _lib.check(_lib.lib().s2h_lat_forward(plan.handle, d_in.data_ptr(), d_out.data_ptr(), 1, s))
+    assert max_rel(d_out.cpu().numpy(), ref2) < TOL
+
+
+def test_batched_equals_single(rng):
+    nside, L = 8, 16
+    flms = np.stack([_flm(rng, L, 0, True) for _ in range(3)])
+    f_b = s2fft_amd.inverse(flms, L, 0, nside, "healpix", "hip", True)
+    for i in range(3):
+        f_i = s2fft_amd.inverse(flms[i], L, 0, nside, "healpix", "hip", True)
+        assert np.array_equal(f_b[i], f_i)
+    g_b = s2fft_amd.forward(np.real(f_b), L, 0, nside, "healpix", "hip", True)
+    for i in range(3):
+        g_i = s2fft_amd.forward(np.real(f_b[i]), L, 0, nside, "healpix", "hip", True)
+        assert np.array_equal(g_b[i], g_i)
+
+
+@pytest.mark.parametrize("nside", [64, 128])
+def test_healpix_larger_vs_oracle(rng, nside):
+    """Larger HEALPix case: Bluestein rings and deep polar underflow."""
+    L = 2 * nside
+    flm = _flm(rng, L, 0, True)
+    ref_f = otr.inverse(flm, L, 0, nside, "healpix", True)
+    got_f = s2fft_amd.inverse(flm, L, 0, nside, "healpix", "hip", True)
+    assert max_rel(got_f, ref_f) < TOL
+    f = np.real(ref_f)
+    ref = otr.forward(f, L, 0, nside, "healpix", True)
+    got = s2fft_amd.forward(f, L, 0, nside, "healpix", "hip", True)
+    assert max_rel(got, ref) < TOL
