@@ -77,7 +77,11 @@ extern "C" int s2h_forward(const s2h_plan* p, const void* f, int f_is_complex, v
   const int64_t fs = flm_stride(p);
 
   if ((st = hip_status(launch_zero(flm, 2 * fs * batch, s)))) return st;
-  if (p->fwd_kind != 0) return S2H_ERR_UNSUPPORTED;
+  const bool resample = p->fwd_kind != 0;
+  // resampled forward: phi-spectrum of the input rows first, then the
+  // resampled MWSS-at-2L ftm after it in the workspace
+  cplx* spec = ftm;
+  if (resample) ftm = spec + (int64_t)batch * p->fwd_in_rows * p->Mc;
 
   RingFwdArgs r{};
   r.jobs = p->fwd_jobs;
@@ -94,25 +98,66 @@ extern "C" int s2h_forward(const s2h_plan* p, const void* f, int f_is_complex, v
       r.real_only_all = 1;
   }
   r.map_stride = p->npix;
-  r.out = ftm;
-  r.out_map_stride = (int64_t)p->fwd_rows * p->Mc;
+  r.out = resample ? spec : ftm;
+  r.out_map_stride = (int64_t)(resample ? p->fwd_in_rows : p->fwd_rows) * p->Mc;
   r.ncol = p->Mc;
   r.m_first = p->m_lo;
-  r.apply_weight = 1;
+  r.apply_weight = resample ? 0 : 1;
   r.apply_phase = p->d.sampling == S2H_HEALPIX;
   r.zero_neg_2L = 0;
   r.L = L;
   r.chirp_pool = p->chirp;
   if ((st = hip_status(launch_ring_fft_forward(r, p->n_fwd_jobs, batch, p->fwd_lds, s)))) return st;
+  const int64_t ftm_stride = (int64_t)p->fwd_rows * p->Mc;
+  if (resample) {
+    ResampleArgs ra{};
+    ra.in = spec;
+    ra.in_map_stride = r.out_map_stride;
+    ra.out = ftm;
+    ra.out_map_stride = ftm_stride;
+    ra.ncol = p->Mc;
+    ra.m_lo = p->m_lo;
+    ra.nm = p->nm;
+    ra.L = L;
+    ra.spin = p->d.spin;
+    ra.mw = p->fwd_kind == 1;
+    ra.weights = p->fwd_weights;
+    ra.phi_factor = ra.mw ? (2.0 * L) / (2.0 * L - 1.0) : 1.0;
+    ra.lds = p->rs_info[3];
+    for (int i = 0; i < 3; ++i) {
+      ra.conv[i] = p->rs_info[i];
+      ra.chirp_off[i] = p->rs_chirp[i];
+    }
+    ra.pool = p->chirp;
+    if ((st = hip_status(launch_resample(ra, batch, s)))) return st;
+  }
 
   LatArgs a = lat_args(p, true, batch);
   a.flm = static_cast<cplx*>(flm);
   a.ftm = ftm;
-  a.ftm_map_stride = r.out_map_stride;
+  a.ftm_map_stride = ftm_stride;
   a.ftm_ncol = p->Mc;
   a.ftm_m_first = p->m_lo;
   a.lmin_out = std::max(std::abs(p->d.spin), p->d.L_lower);
   if ((st = hip_status(launch_lat_forward(a, p->fwd_paired, s)))) return st;
+
+  if (p->fwd_row_south >= 0 || p->fwd_row_north >= 0) {
+    PoleArgs pa{};
+    pa.flm = static_cast<cplx*>(flm);
+    pa.flm_map_stride = fs;
+    pa.ftm = ftm;
+    pa.ftm_map_stride = ftm_stride;
+    pa.ftm_ncol = p->Mc;
+    pa.ftm_m_first = p->m_lo;
+    pa.L = L;
+    pa.spin = p->d.spin;
+    pa.lmin = a.lmin_out;
+    pa.row_south = p->fwd_row_south;
+    pa.row_north = p->fwd_row_north;
+    pa.norm = p->fwd_norm;
+    pa.batch = batch;
+    if ((st = hip_status(launch_pole_forward(pa, s)))) return st;
+  }
 
   if (real) {
     if ((st = hip_status(launch_reality_fill(static_cast<cplx*>(flm), fs, L, batch, s)))) return st;

@@ -95,7 +95,24 @@ hipError_t launch_pole_inverse(const PoleArgs& a, hipStream_t s);
 hipError_t launch_pole_forward(const PoleArgs& a, hipStream_t s);
 hipError_t launch_zero(void* p, int64_t ndoubles, hipStream_t s);
 hipError_t launch_reality_fill(cplx* flm, int64_t map_stride, int L, int batch, hipStream_t s);
-hipError_t launch_mw_resample(const cplx* g, cplx* out, const double* weights, int L, int spin,
-                              int mw, int m_lo, int nm, int batch, hipStream_t s);
+
+struct ResampleArgs {
+  const cplx* in;           // phi-spectrum [b][row][col], col <-> m = m_lo + col
+  int64_t in_map_stride;
+  cplx* out;                // [b][2L+1][col]
+  int64_t out_map_stride;
+  int ncol;                 // columns of both arrays
+  int m_lo, nm;
+  int L, spin;
+  int mw;                   // 1: MW input (L rows), 0: MWSS input (L+1 rows)
+  const double* weights;    // 2L+1 quadrature weights of MWSS at 2L
+  double phi_factor;        // 2L/(2L-1) for MW, 1 for MWSS
+  int lds;                  // LDS elements (complex) per workgroup
+  int conv[3];              // Bluestein lengths of the DFTs 2L-1, 2L, 4L (0 = mixed)
+  int64_t chirp_off[3];
+  const cplx* pool;
+};
+hipError_t launch_resample(const ResampleArgs& a, int batch, hipStream_t s);
+hipError_t allow_large_lds_resample();
 
 }  // namespace s2h

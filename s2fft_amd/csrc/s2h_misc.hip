@@ -116,3 +116,164 @@ hipError_t launch_zero(void* p, int64_t ndoubles, hipStream_t s) {
 }
 
 }  // namespace s2h
+
+namespace s2h {
+
+// ------------------------------------------------------------------ resampling
+// MW / MWSS forward resampling in (theta, m) space, one workgroup per (m, map).
+// s2fft/utils/resampling.py: mw_to_mwss (248-355) followed by
+// upsample_by_two_mwss (64-160).  Every step there is linear along theta per
+// phi-Fourier column m (the phi-rolls of the mirrored rows are (-1)^m), so
+// the whole chain runs on one column in LDS:
+//   MW:   extend L rows to 2L-1 with (-1)^(m+s) mirror, DFT_{2L-1}/(2L-1),
+//         phase e^{-i p pi/(2L-1)}, IDFT_{2L} -> rows 0..L of MWSS
+//   MWSS: extend L+1 rows to 2L with (-1)^(m+s) mirror, DFT_{2L}/(2L),
+//         zero-pad, IDFT_{4L} -> rows 0..2L of MWSS at 2L
+// then the quadrature weight of each output row and, for MW, the 2L/(2L-1)
+// of the phi zero-padding.
+__device__ __forceinline__ void col_dft(double* re, double* im, int n, int conv, int64_t off,
+                                        const cplx* pool, int sign) {
+  if (conv == 0)
+    fft_mixed<kMaxConvLen>(re, im, n, sign);
+  else
+    fft_bluestein<kMaxConvLen>(re, im, n, conv, pool + off, sign);
+}
+
+__global__ void __launch_bounds__(kThreads) resample_kernel(ResampleArgs a) {
+  extern __shared__ double smem[];
+  const int L = a.L;
+  const int col = blockIdx.x;
+  const int m = a.m_lo + col;
+  const int64_t b = blockIdx.y;
+  double* re = smem;
+  double* im = smem + a.lds;
+  const double msgn = (((m + a.spin) % 2) + 2) % 2 ? -1.0 : 1.0;
+  const cplx* g = a.in + b * a.in_map_stride + col;
+
+  int nrow;  // rows of the MWSS column at L now held in LDS
+  if (a.mw) {
+    const int n1 = 2 * L - 1;
+    for (int t = threadIdx.x; t < n1; t += kThreads) {
+      cplx v;
+      if (t < L) {
+        v = g[(int64_t)t * a.ncol];
+      } else {
+        const cplx u = g[(int64_t)(2 * L - 2 - t) * a.ncol];
+        v = {msgn * u.x, msgn * u.y};
+      }
+      re[t] = v.x;
+      im[t] = v.y;
+    }
+    __syncthreads();
+    col_dft(re, im, n1, a.conv[0], a.chirp_off[0], a.pool, -1);
+    // scale, phase and move bins p<0 from index n1+p to 2L+p (register staged)
+    constexpr int kPer = kMaxConvLen / kThreads;
+    cplx v[kPer];
+    int idx[kPer];
+#pragma unroll
+    for (int c = 0; c < kPer; ++c) {
+      const int k = threadIdx.x + c * kThreads;
+      idx[c] = -1;
+      if (k < n1) {
+        const int p = k < L ? k : k - n1;
+        cplx x = {re[k] / n1, im[k] / n1};
+        x = cmul(x, expi_pi(-(int64_t)p, n1));
+        v[c] = x;
+        idx[c] = p >= 0 ? p : 2 * L + p;
+      }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < 2 * L; k += kThreads) {
+      re[k] = 0.0;
+      im[k] = 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < kPer; ++c) {
+      if (idx[c] >= 0) {
+        re[idx[c]] = v[c].x;
+        im[idx[c]] = v[c].y;
+      }
+    }
+    __syncthreads();
+    col_dft(re, im, 2 * L, a.conv[1], a.chirp_off[1], a.pool, +1);
+    nrow = L + 1;
+  } else {
+    for (int t = threadIdx.x; t <= L; t += kThreads) {
+      const cplx v = g[(int64_t)t * a.ncol];
+      re[t] = v.x;
+      im[t] = v.y;
+    }
+    __syncthreads();
+    nrow = L + 1;
+  }
+  (void)nrow;
+  // mirror rows L+1..2L-1 from rows L-1..1
+  {
+    constexpr int kPer = kMaxConvLen / kThreads;
+    cplx v[kPer];
+#pragma unroll
+    for (int c = 0; c < kPer; ++c) {
+      const int t = L + 1 + threadIdx.x + c * kThreads;
+      if (t < 2 * L) v[c] = {msgn * re[2 * L - t], msgn * im[2 * L - t]};
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < kPer; ++c) {
+      const int t = L + 1 + threadIdx.x + c * kThreads;
+      if (t < 2 * L) {
+        re[t] = v[c].x;
+        im[t] = v[c].y;
+      }
+    }
+    __syncthreads();
+  }
+  col_dft(re, im, 2 * L, a.conv[1], a.chirp_off[1], a.pool, -1);
+  // scale by 1/(2L), move q<0 from 2L+q to 4L+q, zero the middle
+  {
+    constexpr int kPer = kMaxConvLen / kThreads;
+    cplx v[kPer];
+#pragma unroll
+    for (int c = 0; c < kPer; ++c) {
+      const int k = threadIdx.x + c * kThreads;
+      if (k < 2 * L) v[c] = {re[k] / (2 * L), im[k] / (2 * L)};
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < 4 * L; k += kThreads) {
+      re[k] = 0.0;
+      im[k] = 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < kPer; ++c) {
+      const int k = threadIdx.x + c * kThreads;
+      if (k < 2 * L) {
+        const int dst = k < L ? k : k + 2 * L;
+        re[dst] = v[c].x;
+        im[dst] = v[c].y;
+      }
+    }
+    __syncthreads();
+  }
+  col_dft(re, im, 4 * L, a.conv[2], a.chirp_off[2], a.pool, +1);
+  cplx* out = a.out + b * a.out_map_stride + col;
+  for (int t = threadIdx.x; t <= 2 * L; t += kThreads) {
+    const double w = a.weights[t] * a.phi_factor;
+    out[(int64_t)t * a.ncol] = {re[t] * w, im[t] * w};
+  }
+}
+
+hipError_t launch_resample(const ResampleArgs& a, int batch, hipStream_t s) {
+  if (batch == 0 || a.nm == 0) return hipSuccess;
+  const size_t lds = 2 * sizeof(double) * (size_t)a.lds;
+  hipLaunchKernelGGL(resample_kernel, dim3(a.nm, batch), dim3(kThreads), lds, s, a);
+  return hipGetLastError();
+}
+
+hipError_t allow_large_lds_resample() {
+  return hipFuncSetAttribute((const void*)resample_kernel,
+                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)(2 * sizeof(double) * kMaxConvLen));
+}
+
+}  // namespace s2h

@@ -480,7 +480,15 @@ static int plan_build(s2h_plan* p) {
     wf = forward_weights(d.sampling, L, nside);
     p->fwd_in_rows = p->inv_rows;
     fwd_j = inv_j;  // phi-DFT of the input rows, weight 1
-    return S2H_ERR_UNSUPPORTED;  // resampling kernels not yet wired
+    const int lens[3] = {2 * L - 1, 2 * L, 4 * L};
+    int need = 4 * L;
+    for (int i = 0; i < 3; ++i) {
+      p->rs_info[i] = cb.add(lens[i], &p->rs_chirp[i]);
+      need = std::max(need, p->rs_info[i] ? p->rs_info[i] : lens[i]);
+    }
+    if (need > kMaxConvLen) return S2H_ERR_UNSUPPORTED;
+    p->rs_info[3] = need;
+    if (allow_large_lds_resample() != hipSuccess) return S2H_ERR_HIP;
   }
 
   // ---------------- recursion tables

@@ -1,0 +1,75 @@
+"""C ABI and host-side checks that need no GPU."""
+
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import s2fft_amd
+from s2fft_amd import _lib
+
+HEADER = Path(__file__).resolve().parents[1] / "include" / "s2fft_amd.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text)
+    keywords = {"if", "sizeof", "defined", "extern"}
+    return sorted({n for n in names if n not in keywords and not n.isupper()})
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.lib()
+    names = declared_functions()
+    assert "s2h_forward" in names and "healpix_fft_hip" in names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(_lib.EXPORTED) == set(names)
+
+
+def test_status_strings():
+    lib = _lib.lib()
+    for code in range(-6, 1):
+        assert lib.s2h_status_string(code).decode() != "unknown status"
+    assert lib.s2h_status_string(-99).decode() == "unknown status"
+    assert lib.s2h_version() >= 1
+
+
+def test_null_arguments_rejected_without_gpu():
+    lib = _lib.lib()
+    assert lib.s2h_plan_create(None, None) == _lib.S2H_ERR_INVALID
+    assert lib.s2h_forward(None, None, 0, None, None, 1, None) == _lib.S2H_ERR_INVALID
+    assert lib.s2h_inverse(None, None, None, None, 1, None) == _lib.S2H_ERR_INVALID
+    assert lib.s2h_healpix_fft(0, 8, 0, None, None, 1, None) == _lib.S2H_ERR_INVALID
+    assert lib.s2h_gamma_dft(9, 4, 1, 1.0, None, None, None) == _lib.S2H_ERR_INVALID
+    assert lib.s2h_workspace_bytes(None, 1) == 0
+
+
+def test_status_to_exception_mapping():
+    with pytest.raises(Warning):
+        _lib.check(_lib.S2H_ERR_SPIN)
+    with pytest.raises(AssertionError):
+        _lib.check(_lib.S2H_ERR_BANDLIMIT)
+    with pytest.raises(ValueError):
+        _lib.check(_lib.S2H_ERR_INVALID)
+    with pytest.raises(NotImplementedError):
+        _lib.check(_lib.S2H_ERR_UNSUPPORTED)
+    with pytest.raises(RuntimeError):
+        _lib.check(_lib.S2H_ERR_HIP)
+
+
+def test_api_argument_errors_before_any_device_work():
+    """Reference error conventions: spherical.py:85-89, 425-429."""
+    flm = np.zeros((4, 7), dtype=np.complex128)
+    with pytest.raises(ValueError, match="not recognised"):
+        s2fft_amd.inverse(flm, 4, method="numpy")
+    with pytest.raises(ValueError, match="not recognised"):
+        s2fft_amd.forward(np.zeros((4, 7)), 4, method="jax")
+    with pytest.raises(Warning):
+        s2fft_amd.inverse(np.zeros((10, 19), dtype=np.complex128), 10, spin=8, method="hip")
+    with pytest.raises(Warning):
+        s2fft_amd.forward(np.zeros((10, 19)), 10, spin=9, method="hip")
+    with pytest.raises(ValueError):
+        s2fft_amd.forward(np.zeros(48), 4, sampling="healpix", method="hip")

@@ -44,7 +44,7 @@ def test_inverse_equiangular(rng, sampling, L, spin, reality, L_lower):
     assert max_rel(got, ref) < TOL
 
 
-@pytest.mark.parametrize("sampling", ["dh", "gl"])
+@pytest.mark.parametrize("sampling", ["mw", "mwss", "dh", "gl"])
 @pytest.mark.parametrize("L", [6, 7, 9])
 @pytest.mark.parametrize("spin", [-2, 0, 1])
 @pytest.mark.parametrize("reality", [False, True])
