@@ -24,6 +24,7 @@
  *                                  descriptor of lib/include/s2fft.h:29-59
  *   s2h_gamma_dft                  the FFT over gamma of
  *                                  s2fft/transforms/wigner.py:183-195,495-502
+ *   s2h_wigner_reality_fill        s2fft/transforms/wigner.py:523-526
  */
 #ifndef S2FFT_AMD_H
 #define S2FFT_AMD_H
@@ -111,13 +112,34 @@ void healpix_fft_hip(void *stream, void **buffers, const char *opaque, size_t op
  * no return value). */
 int healpix_fft_hip_last_status(void);
 
-/* DFT over the first (gamma) axis of an SO(3) signal with 2N-1 gamma samples:
- * fban[n] = scale * sum_g f[g] exp(-2 pi i n g / (2N-1)) for n in
- * [-(N-1), N-1] (forward != 0), or the unnormalised inverse
- * f[g] = sum_n fban[n] exp(+2 pi i n g / (2N-1)) (forward == 0).
- * npix = number of (beta, alpha) samples; complex128 in and out. */
-int s2h_gamma_dft(int N, int64_t npix, int forward, double scale, const void *in,
-                  void *out, s2h_stream stream);
+/* DFT over the first (gamma) axis of an SO(3) signal with G = 2N-1 gamma
+ * samples, for each of the npix (beta, alpha) samples (N <= 8):
+ *   mode 0: complex128 f[g]        -> fban[k] = scale sum_g f[g] e^{-2 pi i n g/G}
+ *   mode 2: float64 f[g]           -> same, for a real signal
+ *   mode 1: complex128 fban[k]     -> f[g] = scale sum_n fban[k] e^{+2 pi i n g/G}
+ *   mode 3: complex128 fban[n >= 0] (N rows) -> float64 f[g], Hermitian in n
+ * with n = k - (N-1).  in and out must not alias. */
+int s2h_gamma_dft(int N, int64_t npix, int mode, double scale, const void *in, void *out,
+                  s2h_stream stream);
+/* Negative-n planes of a real signal's flmn (batch of one):
+ * flmn[N-1-n, l, L-1+m] = (-1)^(m+n) conj(flmn[N-1+n, l, L-1-m]), n = 1..N-1
+ * (s2fft/transforms/wigner.py:523-526). */
+int s2h_wigner_reality_fill(void *flmn, int L, int N, s2h_stream stream);
+
+/* Kernel timing for benchmarks (no counterpart in the reference).  While
+ * enabled, s2h_forward / s2h_inverse record a HIP event pair on the launch
+ * stream around each kernel; s2h_profile_query synchronises on them and
+ * returns the summed milliseconds and the launch count of one kernel kind.
+ * s2h_profile_enable(0/1) clears the records.  Off by default. */
+enum s2h_kernel_kind {
+  S2H_KERNEL_LAT_FORWARD = 0,
+  S2H_KERNEL_LAT_INVERSE = 1,
+  S2H_KERNEL_RING_FFT_FORWARD = 2,
+  S2H_KERNEL_RING_FFT_INVERSE = 3,
+  S2H_KERNEL_RESAMPLE = 4
+};
+int s2h_profile_enable(int on);
+int s2h_profile_query(int kernel, double *total_ms, int *count);
 
 #ifdef __cplusplus
 }

@@ -5,7 +5,7 @@ Drop-in ``method="hip"`` implementations of s2fft.forward / s2fft.inverse
 """
 
 from . import sampling  # noqa: F401
-from .transforms import spherical  # noqa: F401
+from .transforms import spherical, wigner  # noqa: F401
 from .transforms.spherical import forward, inverse  # noqa: F401
 
-__all__ = ["forward", "inverse", "spherical", "sampling"]
+__all__ = ["forward", "inverse", "spherical", "wigner", "sampling"]

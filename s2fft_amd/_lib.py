@@ -41,7 +41,11 @@ EXPORTED = (
     "s2h_workspace_bytes", "s2h_inverse_output_is_complex", "s2h_forward", "s2h_inverse",
     "s2h_lat_forward", "s2h_lat_inverse", "s2h_healpix_fft", "s2h_healpix_ifft",
     "healpix_fft_hip", "healpix_fft_hip_last_status", "s2h_gamma_dft",
+    "s2h_profile_enable", "s2h_profile_query", "s2h_wigner_reality_fill",
 )
+
+KERNEL_LAT_FORWARD, KERNEL_LAT_INVERSE, KERNEL_RING_FFT_FORWARD = 0, 1, 2
+KERNEL_RING_FFT_INVERSE, KERNEL_RESAMPLE = 3, 4
 
 
 def _declare(lib):
@@ -62,6 +66,10 @@ def _declare(lib):
         "healpix_fft_hip": (None, [vp, ctypes.POINTER(vp), ctypes.c_char_p, sz]),
         "healpix_fft_hip_last_status": (i32, []),
         "s2h_gamma_dft": (i32, [i32, i64, i32, ctypes.c_double, vp, vp, vp]),
+        "s2h_wigner_reality_fill": (i32, [vp, i32, i32, vp]),
+        "s2h_profile_enable": (i32, [i32]),
+        "s2h_profile_query": (i32, [i32, ctypes.POINTER(ctypes.c_double),
+                                    ctypes.POINTER(ctypes.c_int)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -128,3 +136,15 @@ class Plan:
         if h is not None and h.value and _lib is not None:
             _lib.s2h_plan_destroy(h)
             self.handle = None
+
+
+def profile_enable(on=True):
+    check(lib().s2h_profile_enable(int(bool(on))), "profile")
+
+
+def profile_query(kernel):
+    """(total milliseconds, launch count) of one kernel kind since enabling."""
+    ms = ctypes.c_double()
+    n = ctypes.c_int()
+    check(lib().s2h_profile_query(kernel, ctypes.byref(ms), ctypes.byref(n)), "profile")
+    return ms.value, n.value

@@ -1,5 +1,7 @@
 // extern "C" entry points: argument checks, kernel sequencing, status codes.
 #include <cstring>
+#include <mutex>
+#include <vector>
 
 #include "s2h_plan.h"
 
@@ -35,7 +37,75 @@ LatArgs lat_args(const s2h_plan* p, bool forward, int batch) {
   return a;
 }
 
+// Optional kernel timing: HIP events recorded on the launch stream around
+// each kernel while enabled (s2h_profile_enable).  Off by default.
+struct ProfRecord {
+  int kernel;
+  hipEvent_t start, stop;
+};
+std::mutex g_prof_mu;
+bool g_prof_on = false;
+std::vector<ProfRecord> g_prof_records;
+std::vector<std::pair<hipEvent_t, hipEvent_t>> g_prof_pool;
+
+struct ProfScope {
+  int kernel;
+  hipStream_t s;
+  ProfRecord rec{};
+  bool active = false;
+  ProfScope(int k, hipStream_t st) : kernel(k), s(st) {
+    std::lock_guard<std::mutex> lock(g_prof_mu);
+    if (!g_prof_on) return;
+    if (g_prof_pool.empty()) {
+      hipEvent_t a, b;
+      if (hipEventCreate(&a) != hipSuccess) return;
+      if (hipEventCreate(&b) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return;
+      }
+      g_prof_pool.push_back({a, b});
+    }
+    rec.kernel = kernel;
+    rec.start = g_prof_pool.back().first;
+    rec.stop = g_prof_pool.back().second;
+    g_prof_pool.pop_back();
+    active = hipEventRecord(rec.start, s) == hipSuccess;
+  }
+  ~ProfScope() {
+    if (!active) return;
+    (void)hipEventRecord(rec.stop, s);
+    std::lock_guard<std::mutex> lock(g_prof_mu);
+    g_prof_records.push_back(rec);
+  }
+};
+
 }  // namespace
+
+extern "C" int s2h_profile_enable(int on) {
+  std::lock_guard<std::mutex> lock(g_prof_mu);
+  for (auto& r : g_prof_records) g_prof_pool.push_back({r.start, r.stop});
+  g_prof_records.clear();
+  g_prof_on = on != 0;
+  return S2H_OK;
+}
+
+extern "C" int s2h_profile_query(int kernel, double* total_ms, int* count) {
+  if (!total_ms || !count) return S2H_ERR_INVALID;
+  std::lock_guard<std::mutex> lock(g_prof_mu);
+  double tot = 0.0;
+  int n = 0;
+  for (auto& r : g_prof_records) {
+    if (r.kernel != kernel) continue;
+    if (hipEventSynchronize(r.stop) != hipSuccess) return S2H_ERR_HIP;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, r.start, r.stop) != hipSuccess) return S2H_ERR_HIP;
+    tot += ms;
+    ++n;
+  }
+  *total_ms = tot;
+  *count = n;
+  return S2H_OK;
+}
 
 extern "C" const char* s2h_status_string(int s) {
   switch (s) {
@@ -107,7 +177,10 @@ extern "C" int s2h_forward(const s2h_plan* p, const void* f, int f_is_complex, v
   r.zero_neg_2L = 0;
   r.L = L;
   r.chirp_pool = p->chirp;
-  if ((st = hip_status(launch_ring_fft_forward(r, p->n_fwd_jobs, batch, p->fwd_lds, s)))) return st;
+  {
+    ProfScope ps(S2H_KERNEL_RING_FFT_FORWARD, s);
+    if ((st = hip_status(launch_ring_fft_forward(r, p->n_fwd_jobs, batch, p->fwd_lds, s)))) return st;
+  }
   const int64_t ftm_stride = (int64_t)p->fwd_rows * p->Mc;
   if (resample) {
     ResampleArgs ra{};
@@ -129,6 +202,7 @@ extern "C" int s2h_forward(const s2h_plan* p, const void* f, int f_is_complex, v
       ra.chirp_off[i] = p->rs_chirp[i];
     }
     ra.pool = p->chirp;
+    ProfScope ps(S2H_KERNEL_RESAMPLE, s);
     if ((st = hip_status(launch_resample(ra, batch, s)))) return st;
   }
 
@@ -139,7 +213,10 @@ extern "C" int s2h_forward(const s2h_plan* p, const void* f, int f_is_complex, v
   a.ftm_ncol = p->Mc;
   a.ftm_m_first = p->m_lo;
   a.lmin_out = std::max(std::abs(p->d.spin), p->d.L_lower);
-  if ((st = hip_status(launch_lat_forward(a, p->fwd_paired, s)))) return st;
+  {
+    ProfScope ps(S2H_KERNEL_LAT_FORWARD, s);
+    if ((st = hip_status(launch_lat_forward(a, p->fwd_paired, s)))) return st;
+  }
 
   if (p->fwd_row_south >= 0 || p->fwd_row_north >= 0) {
     PoleArgs pa{};
@@ -183,7 +260,10 @@ extern "C" int s2h_inverse(const s2h_plan* p, const void* flm, void* f, void* wo
   a.ftm_map_stride = ftm_stride;
   a.ftm_ncol = p->Mc;
   a.ftm_m_first = p->m_lo;
-  if ((st = hip_status(launch_lat_inverse(a, p->inv_paired, s)))) return st;
+  {
+    ProfScope ps(S2H_KERNEL_LAT_INVERSE, s);
+    if ((st = hip_status(launch_lat_inverse(a, p->inv_paired, s)))) return st;
+  }
 
   if (p->inv_row_south >= 0 || p->inv_row_north >= 0) {
     PoleArgs pa{};
@@ -219,6 +299,7 @@ extern "C" int s2h_inverse(const s2h_plan* p, const void* flm, void* f, void* wo
   r.map_stride = p->npix;
   r.chirp_pool = p->chirp;
   r.L = L;
+  ProfScope ps(S2H_KERNEL_RING_FFT_INVERSE, s);
   return hip_status(launch_ring_fft_inverse(r, p->n_inv_jobs, batch, p->inv_lds, s));
 }
 
@@ -336,9 +417,16 @@ extern "C" void healpix_fft_hip(void* stream, void** buffers, const char* opaque
 
 extern "C" int healpix_fft_hip_last_status(void) { return g_last_xla_status; }
 
-extern "C" int s2h_gamma_dft(int N, int64_t npix, int forward, double scale, const void* in,
+extern "C" int s2h_gamma_dft(int N, int64_t npix, int mode, double scale, const void* in,
                              void* out, s2h_stream stream) {
-  if (N < 1 || N > 8 || npix < 0 || !in || !out || in == out) return S2H_ERR_INVALID;
-  return hip_status(launch_gamma_dft(N, npix, forward, scale, static_cast<const cplx*>(in),
-                                     static_cast<cplx*>(out), static_cast<hipStream_t>(stream)));
+  if (N < 1 || N > 8 || npix < 0 || mode < 0 || mode > 3 || !in || !out || in == out)
+    return S2H_ERR_INVALID;
+  return hip_status(
+      launch_gamma_dft(N, npix, mode, scale, in, out, static_cast<hipStream_t>(stream)));
+}
+
+extern "C" int s2h_wigner_reality_fill(void* flmn, int L, int N, s2h_stream stream) {
+  if (!flmn || L < 1 || N < 1) return S2H_ERR_INVALID;
+  return hip_status(launch_wigner_fill(static_cast<cplx*>(flmn), L, N,
+                                       static_cast<hipStream_t>(stream)));
 }

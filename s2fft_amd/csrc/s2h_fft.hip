@@ -139,28 +139,37 @@ __global__ void __launch_bounds__(kThreads) chirp_setup_kernel(const int* length
   for (int j = threadIdx.x; j < M; j += kThreads) dst[j] = {re[j] * s, im[j] * s};
 }
 
-// Direct DFT over gamma (2N-1 <= 15 points) for every (beta, alpha) sample.
-__global__ void __launch_bounds__(kThreads) gamma_dft_kernel(int N, int64_t npix, int forward,
-                                                             double scale, const cplx* in,
-                                                             cplx* out) {
+// Direct DFT over gamma (G = 2N-1 <= 15 points) for every (beta, alpha)
+// sample.  Modes: 0 forward from complex f, 2 forward from real f (both give
+// fban[k], k = n + N-1, n in [-(N-1), N-1]), 1 inverse to complex f from all
+// 2N-1 rows, 3 inverse to real f from the rows n >= 0 only (Hermitian).
+__global__ void __launch_bounds__(kThreads) gamma_dft_kernel(int N, int64_t npix, int mode,
+                                                             double scale, const void* in_,
+                                                             void* out_) {
   const int64_t p = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   if (p >= npix) return;
   const int G = 2 * N - 1;
   cplx x[15];
-  for (int g = 0; g < G; ++g) x[g] = in[(int64_t)g * npix + p];
-  for (int k = 0; k < G; ++k) {
-    const int n = k - (N - 1);  // fftshifted frequency
-    cplx acc = {0.0, 0.0};
-    for (int g = 0; g < G; ++g) {
-      const cplx w = expi_pi((forward ? -2 : 2) * (int64_t)n * g, G);
-      acc = cadd(acc, cmul(x[g], w));
-    }
-    if (forward) {
+  if (mode == 0 || mode == 1) {
+    const cplx* in = static_cast<const cplx*>(in_);
+    for (int g = 0; g < G; ++g) x[g] = in[(int64_t)g * npix + p];
+  } else if (mode == 2) {
+    const double* in = static_cast<const double*>(in_);
+    for (int g = 0; g < G; ++g) x[g] = {in[(int64_t)g * npix + p], 0.0};
+  } else {
+    const cplx* in = static_cast<const cplx*>(in_);
+    for (int k = 0; k < N; ++k) x[k] = in[(int64_t)k * npix + p];
+  }
+  if (mode == 0 || mode == 2) {
+    cplx* out = static_cast<cplx*>(out_);
+    for (int k = 0; k < G; ++k) {
+      const int n = k - (N - 1);
+      cplx acc = {0.0, 0.0};
+      for (int g = 0; g < G; ++g) acc = cadd(acc, cmul(x[g], expi_pi(-2 * (int64_t)n * g, G)));
       out[(int64_t)k * npix + p] = {acc.x * scale, acc.y * scale};
     }
-  }
-  if (!forward) {
-    // inverse: in is indexed by shifted n, out by gamma sample
+  } else if (mode == 1) {
+    cplx* out = static_cast<cplx*>(out_);
     for (int g = 0; g < G; ++g) {
       cplx acc = {0.0, 0.0};
       for (int k = 0; k < G; ++k) {
@@ -169,7 +178,28 @@ __global__ void __launch_bounds__(kThreads) gamma_dft_kernel(int N, int64_t npix
       }
       out[(int64_t)g * npix + p] = {acc.x * scale, acc.y * scale};
     }
+  } else {
+    double* out = static_cast<double*>(out_);
+    for (int g = 0; g < G; ++g) {
+      double acc = x[0].x;
+      for (int n = 1; n < N; ++n) acc += 2.0 * cmul(x[n], expi_pi(2 * (int64_t)n * g, G)).x;
+      out[(int64_t)g * npix + p] = acc * scale;
+    }
   }
+}
+
+// flmn[N-1-n][l][L-1+m] = (-1)^(m+n) conj(flmn[N-1+n][l][L-1-m]) for n >= 1
+// (s2fft/transforms/wigner.py:523-526).
+__global__ void __launch_bounds__(kThreads) wigner_fill_kernel(cplx* flmn, int L, int N) {
+  const int n = blockIdx.z + 1;
+  const int l = blockIdx.y;
+  const int c = blockIdx.x * kThreads + threadIdx.x;
+  if (c >= 2 * L - 1) return;
+  const int m = c - (L - 1);
+  const int64_t plane = (int64_t)L * (2 * L - 1);
+  const cplx v = flmn[(N - 1 + n) * plane + (int64_t)l * (2 * L - 1) + (L - 1 - m)];
+  const double s = ((m + n) & 1) ? -1.0 : 1.0;
+  flmn[(N - 1 - n) * plane + (int64_t)l * (2 * L - 1) + c] = {s * v.x, -s * v.y};
 }
 
 // ------------------------------------------------------------ launchers
@@ -198,12 +228,19 @@ hipError_t launch_chirp_setup(const int* lengths, const int* conv_lens, const in
   return hipGetLastError();
 }
 
-hipError_t launch_gamma_dft(int N, int64_t npix, int forward, double scale, const cplx* in,
-                            cplx* out, hipStream_t s) {
+hipError_t launch_gamma_dft(int N, int64_t npix, int mode, double scale, const void* in,
+                            void* out, hipStream_t s) {
   if (npix == 0) return hipSuccess;
   const int64_t blocks = (npix + kThreads - 1) / kThreads;
   hipLaunchKernelGGL(gamma_dft_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, s, N, npix,
-                     forward, scale, in, out);
+                     mode, scale, in, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_wigner_fill(cplx* flmn, int L, int N, hipStream_t s) {
+  if (N < 2) return hipSuccess;
+  hipLaunchKernelGGL(wigner_fill_kernel, dim3((2 * L - 1 + kThreads - 1) / kThreads, L, N - 1),
+                     dim3(kThreads), 0, s, flmn, L, N);
   return hipGetLastError();
 }
 

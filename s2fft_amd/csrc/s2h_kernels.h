@@ -67,8 +67,9 @@ hipError_t launch_ring_fft_inverse(const RingInvArgs& a, int njobs, int batch, i
                                    hipStream_t s);
 hipError_t launch_chirp_setup(const int* lengths, const int* conv_lens, const int64_t* offs,
                               cplx* pool, int count, int max_conv, hipStream_t s);
-hipError_t launch_gamma_dft(int N, int64_t npix, int forward, double scale, const cplx* in,
-                            cplx* out, hipStream_t s);
+hipError_t launch_gamma_dft(int N, int64_t npix, int mode, double scale, const void* in,
+                            void* out, hipStream_t s);
+hipError_t launch_wigner_fill(cplx* flmn, int L, int N, hipStream_t s);
 hipError_t allow_large_lds();
 
 hipError_t launch_lat_inverse(const LatArgs& a, bool paired, hipStream_t s);

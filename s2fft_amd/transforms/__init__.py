@@ -1,1 +1,1 @@
-from . import spherical  # noqa: F401
+from . import spherical, wigner  # noqa: F401

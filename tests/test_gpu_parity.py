@@ -172,3 +172,22 @@ def test_healpix_larger_vs_oracle(rng, nside):
     ref = otr.forward(f, L, 0, nside, "healpix", True)
     got = s2fft_amd.forward(f, L, 0, nside, "healpix", "hip", True)
     assert max_rel(got, ref) < TOL
+
+
+@pytest.mark.parametrize("sampling,nside", [("mw", None), ("mwss", None), ("dh", None),
+                                            ("gl", None), ("healpix", 4)])
+@pytest.mark.parametrize("reality", [False, True])
+@pytest.mark.parametrize("N", [1, 2, 3])
+def test_wigner_vs_oracle(rng, sampling, nside, reality, N):
+    """SO(3) transforms (reference tests/test_wigner_transform.py:37-79 grid)."""
+    L = 8 if sampling == "healpix" else 6
+    flmn = signals.generate_flmn(rng, L, N, reality=reality)
+    ref_f = otr.wigner_inverse(flmn, L, N, nside, sampling, reality)
+    got_f = s2fft_amd.wigner.inverse(flmn, L, N, nside, sampling, "hip", reality)
+    assert got_f.shape == ref_f.shape
+    assert max_rel(got_f, ref_f) < TOL
+    ref = otr.wigner_forward(ref_f, L, N, nside, sampling, reality)
+    got = s2fft_amd.wigner.forward(ref_f, L, N, nside, sampling, "hip", reality)
+    assert max_rel(got, ref) < TOL
+    if sampling != "healpix":
+        assert max_rel(got, flmn) < 1e-11
