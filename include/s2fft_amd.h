@@ -87,6 +87,21 @@ int s2h_forward(const s2h_plan *plan, const void *f, int f_is_complex, void *flm
 int s2h_inverse(const s2h_plan *plan, const void *flm, void *f, void *work, int batch,
                 s2h_stream stream);
 
+/* Adjoints (vector-Jacobian products) of s2h_forward and s2h_inverse for
+ * the torch complex-gradient convention: grad_f = F^H g, grad_flm = I^H g.
+ * Replace the custom VJPs of s2fft/transforms/spherical.py:291-325 and
+ * 695-730 (whose bodies are the opposite latitudinal step) together with the
+ * JAX autodiff of the FFT, weight and phase stages around them.
+ * s2h_forward_adjoint: g is batch x (L, 2L-1) complex128; grad_f is float64
+ * with reality, else complex128 (f layout); for MW/MWSS it includes the
+ * adjoint of the theta resampling.
+ * s2h_inverse_adjoint: g in the f layout (complex128 if g_is_complex, else
+ * float64); grad_flm is batch x (L, 2L-1) complex128. */
+int s2h_forward_adjoint(const s2h_plan *plan, const void *g, void *grad_f, void *work,
+                        int batch, s2h_stream stream);
+int s2h_inverse_adjoint(const s2h_plan *plan, const void *g, int g_is_complex, void *grad_flm,
+                        void *work, int batch, s2h_stream stream);
+
 /* Latitudinal steps on the plan's theta grid (plan created with
  * S2H_NORM_NONE).  ftm: batch x (ntheta, 2L or 2L-1) complex128 in the
  * reference ftm layout; flm: batch x (L, 2L-1) complex128.  */
@@ -140,6 +155,12 @@ enum s2h_kernel_kind {
 };
 int s2h_profile_enable(int on);
 int s2h_profile_query(int kernel, double *total_ms, int *count);
+
+/* Diagnostics (no counterpart in the reference): one V_MFMA_F64_16X16X4F64
+ * with a[lane], b[lane] (64 doubles each) as the A and B operands and a zero
+ * accumulator; out[4*lane + r] = result register r of lane (256 doubles).
+ * Lets a test confirm the operand / result lane layouts the kernels assume. */
+int s2h_probe_mfma_f64(const double *a, const double *b, double *out, s2h_stream stream);
 
 #ifdef __cplusplus
 }

@@ -13,6 +13,8 @@ thread_local int g_last_xla_status = S2H_OK;
 
 int hip_status(hipError_t e) { return e == hipSuccess ? S2H_OK : S2H_ERR_HIP; }
 
+constexpr int kMfmaMinBatch = 4;
+
 int check_device(const s2h_plan* p) {
   int dev = -1;
   if (hipGetDevice(&dev) != hipSuccess) return S2H_ERR_HIP;
@@ -126,7 +128,9 @@ extern "C" size_t s2h_workspace_bytes(const s2h_plan* p, int batch) {
   if (!p || batch < 0) return 0;
   size_t rows = (size_t)std::max(p->fwd_rows, p->inv_rows);
   if (p->fwd_kind != 0) rows += (size_t)p->fwd_in_rows;
-  return rows * (size_t)p->Mc * sizeof(cplx) * (size_t)batch;
+  // + one flm-sized buffer for the adjoint of the conjugate fill
+  const size_t flm = (size_t)p->d.L * (2 * p->d.L - 1);
+  return (rows * (size_t)p->Mc + flm) * sizeof(cplx) * (size_t)batch;
 }
 
 extern "C" int s2h_inverse_output_is_complex(const s2h_plan* p) {
@@ -215,7 +219,12 @@ extern "C" int s2h_forward(const s2h_plan* p, const void* f, int f_is_complex, v
   a.lmin_out = std::max(std::abs(p->d.spin), p->d.L_lower);
   {
     ProfScope ps(S2H_KERNEL_LAT_FORWARD, s);
-    if ((st = hip_status(launch_lat_forward(a, p->fwd_paired, s)))) return st;
+    // batches of 4 or more maps share each d-value across an 8-map MFMA tile
+    if (batch >= kMfmaMinBatch)
+      st = hip_status(launch_lat_forward_mfma(a, p->fwd_paired, s));
+    else
+      st = hip_status(launch_lat_forward(a, p->fwd_paired, s));
+    if (st) return st;
   }
 
   if (p->fwd_row_south >= 0 || p->fwd_row_north >= 0) {
@@ -301,6 +310,171 @@ extern "C" int s2h_inverse(const s2h_plan* p, const void* flm, void* f, void* wo
   r.L = L;
   ProfScope ps(S2H_KERNEL_RING_FFT_INVERSE, s);
   return hip_status(launch_ring_fft_inverse(r, p->n_inv_jobs, batch, p->inv_lds, s));
+}
+
+// Adjoint of s2h_forward (the VJP of a complex-linear map, torch convention):
+// grad_f = F^H g.  F = sign N_l Lat_fwd . (e^{-im phi0} w_t) . DFT_ext (+ poles),
+// so F^H = DFT_ext^H . (w_t e^{+im phi0}) . Lat_inv with the forward plan's
+// tables: the ring iFFT with folding and weights after the latitudinal
+// synthesis on the forward grid.  With reality the fill's adjoint folds the
+// m < 0 cotangents onto m > 0 and the result is real.
+extern "C" int s2h_forward_adjoint(const s2h_plan* p, const void* g, void* grad_f, void* work,
+                                   int batch, s2h_stream stream) {
+  if (!p || !g || !grad_f || batch < 0 || (batch > 0 && !work)) return S2H_ERR_INVALID;
+  if (batch == 0) return S2H_OK;
+  int st = check_device(p);
+  if (st) return st;
+  if (p->d.norm == S2H_NORM_NONE) return S2H_ERR_INVALID;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int L = p->d.L;
+  const bool real = p->d.reality != 0;
+  const int64_t fs = flm_stride(p);
+  const bool resample = p->fwd_kind != 0;
+  cplx* spec = static_cast<cplx*>(work);
+  cplx* ftm = resample ? spec + (int64_t)batch * p->fwd_in_rows * p->Mc : spec;
+  const int64_t ftm_stride = (int64_t)p->fwd_rows * p->Mc;
+  const cplx* gin = static_cast<const cplx*>(g);
+  if (real) {
+    size_t rows = (size_t)std::max(p->fwd_rows, p->inv_rows) + (resample ? p->fwd_in_rows : 0);
+    cplx* folded = spec + (int64_t)batch * rows * p->Mc;
+    if ((st = hip_status(launch_reality_fold(gin, folded, fs, L, batch, s)))) return st;
+    gin = folded;
+  }
+  LatArgs a = lat_args(p, true, batch);
+  a.flm = const_cast<cplx*>(gin);
+  a.ftm = ftm;
+  a.ftm_map_stride = ftm_stride;
+  a.ftm_ncol = p->Mc;
+  a.ftm_m_first = p->m_lo;
+  a.lmin_out = std::max(std::abs(p->d.spin), p->d.L_lower);
+  if ((st = hip_status(launch_lat_inverse(a, p->fwd_paired, s)))) return st;
+  if (p->fwd_row_south >= 0 || p->fwd_row_north >= 0) {
+    // adjoint of the forward pole terms: the pole rows of the synthesis
+    PoleArgs pa{};
+    pa.flm = const_cast<cplx*>(gin);
+    pa.flm_map_stride = fs;
+    pa.ftm = ftm;
+    pa.ftm_map_stride = ftm_stride;
+    pa.ftm_ncol = p->Mc;
+    pa.ftm_m_first = p->m_lo;
+    pa.L = L;
+    pa.spin = p->d.spin;
+    pa.lmin = a.lmin_out;
+    pa.row_south = p->fwd_row_south;
+    pa.row_north = p->fwd_row_north;
+    pa.norm = p->fwd_norm;
+    pa.batch = batch;
+    if ((st = hip_status(launch_pole_inverse(pa, s)))) return st;
+  }
+  int64_t in_stride = ftm_stride;
+  if (resample) {
+    ResampleArgs ra{};
+    ra.in = spec;
+    ra.in_map_stride = (int64_t)p->fwd_in_rows * p->Mc;
+    ra.out = ftm;
+    ra.out_map_stride = ftm_stride;
+    ra.ncol = p->Mc;
+    ra.m_lo = p->m_lo;
+    ra.nm = p->nm;
+    ra.L = L;
+    ra.spin = p->d.spin;
+    ra.mw = p->fwd_kind == 1;
+    ra.weights = p->fwd_weights;
+    ra.phi_factor = ra.mw ? (2.0 * L) / (2.0 * L - 1.0) : 1.0;
+    ra.lds = p->rs_info[3];
+    for (int i = 0; i < 3; ++i) {
+      ra.conv[i] = p->rs_info[i];
+      ra.chirp_off[i] = p->rs_chirp[i];
+    }
+    ra.pool = p->chirp;
+    if ((st = hip_status(launch_resample_adjoint(ra, batch, s)))) return st;
+    in_stride = ra.in_map_stride;
+  }
+
+  RingInvArgs r{};
+  r.jobs = p->fwd_jobs;
+  r.in = resample ? spec : ftm;
+  r.in_map_stride = in_stride;
+  r.ncol = p->Mc;
+  r.m_first = p->m_lo;
+  r.m_lo = -(L - 1);
+  r.m_hi = L - 1;
+  r.herm_mode = real ? 1 : 0;
+  r.apply_phase = p->d.sampling == S2H_HEALPIX;
+  r.apply_weight = resample ? 0 : 1;
+  r.f = grad_f;
+  r.f_complex = real ? 0 : 1;
+  r.map_stride = p->npix;
+  r.chirp_pool = p->chirp;
+  r.L = L;
+  return hip_status(launch_ring_fft_inverse(r, p->n_fwd_jobs, batch, p->fwd_lds, s));
+}
+
+// Adjoint of s2h_inverse: grad_flm = I^H g = sign N_l Lat_fwd . (e^{-im phi0})
+// . DFT_ext on the inverse grid, plus the pole rows' adjoint.  With reality
+// the inverse reads only m >= 0 and completes Hermitian-symmetrically, so the
+// m > 0 cotangents double and m < 0 stay zero.
+extern "C" int s2h_inverse_adjoint(const s2h_plan* p, const void* g, int g_is_complex,
+                                   void* grad_flm, void* work, int batch, s2h_stream stream) {
+  if (!p || !g || !grad_flm || batch < 0 || (batch > 0 && !work)) return S2H_ERR_INVALID;
+  if (batch == 0) return S2H_OK;
+  int st = check_device(p);
+  if (st) return st;
+  if (p->d.norm == S2H_NORM_NONE) return S2H_ERR_INVALID;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int L = p->d.L;
+  const bool real = p->d.reality != 0;
+  const int64_t fs = flm_stride(p);
+  cplx* ftm = static_cast<cplx*>(work);
+  const int64_t ftm_stride = (int64_t)p->inv_rows * p->Mc;
+  if ((st = hip_status(launch_zero(grad_flm, 2 * fs * batch, s)))) return st;
+
+  RingFwdArgs r{};
+  r.jobs = p->inv_jobs;
+  r.f = g;
+  r.f_complex = g_is_complex;
+  r.real_only_all = (real && g_is_complex) ? 1 : 0;
+  r.map_stride = p->npix;
+  r.out = ftm;
+  r.out_map_stride = ftm_stride;
+  r.ncol = p->Mc;
+  r.m_first = p->m_lo;
+  r.apply_weight = 0;
+  r.apply_phase = p->d.sampling == S2H_HEALPIX;
+  r.L = L;
+  r.chirp_pool = p->chirp;
+  if ((st = hip_status(launch_ring_fft_forward(r, p->n_inv_jobs, batch, p->inv_lds, s)))) return st;
+
+  LatArgs a = lat_args(p, false, batch);
+  a.flm = static_cast<cplx*>(grad_flm);
+  a.ftm = ftm;
+  a.ftm_map_stride = ftm_stride;
+  a.ftm_ncol = p->Mc;
+  a.ftm_m_first = p->m_lo;
+  a.lmin_out = p->d.L_lower;
+  if ((st = hip_status(launch_lat_forward(a, p->inv_paired, s)))) return st;
+  if (p->inv_row_south >= 0 || p->inv_row_north >= 0) {
+    PoleArgs pa{};
+    pa.flm = static_cast<cplx*>(grad_flm);
+    pa.flm_map_stride = fs;
+    pa.ftm = ftm;
+    pa.ftm_map_stride = ftm_stride;
+    pa.ftm_ncol = p->Mc;
+    pa.ftm_m_first = p->m_lo;
+    pa.L = L;
+    pa.spin = p->d.spin;
+    pa.lmin = p->d.L_lower;
+    pa.row_south = p->inv_row_south;
+    pa.row_north = p->inv_row_north;
+    pa.norm = p->inv_norm;
+    pa.batch = batch;
+    if ((st = hip_status(launch_pole_forward(pa, s)))) return st;
+  }
+  if (real) {
+    if ((st = hip_status(launch_scale_pos_m(static_cast<cplx*>(grad_flm), fs, L, batch, 2.0, s))))
+      return st;
+  }
+  return S2H_OK;
 }
 
 extern "C" int s2h_lat_forward(const s2h_plan* p, const void* ftm, void* flm, int batch,
@@ -429,4 +603,10 @@ extern "C" int s2h_wigner_reality_fill(void* flmn, int L, int N, s2h_stream stre
   if (!flmn || L < 1 || N < 1) return S2H_ERR_INVALID;
   return hip_status(launch_wigner_fill(static_cast<cplx*>(flmn), L, N,
                                        static_cast<hipStream_t>(stream)));
+}
+
+extern "C" int s2h_probe_mfma_f64(const double* a, const double* b, double* out,
+                                  s2h_stream stream) {
+  if (!a || !b || !out) return S2H_ERR_INVALID;
+  return hip_status(launch_mfma_f64_probe(a, b, out, static_cast<hipStream_t>(stream)));
 }

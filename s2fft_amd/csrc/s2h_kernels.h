@@ -74,6 +74,8 @@ hipError_t allow_large_lds();
 
 hipError_t launch_lat_inverse(const LatArgs& a, bool paired, hipStream_t s);
 hipError_t launch_lat_forward(const LatArgs& a, bool paired, hipStream_t s);
+hipError_t launch_lat_forward_mfma(const LatArgs& a, bool paired, hipStream_t s);
+hipError_t launch_mfma_f64_probe(const double* a, const double* b, double* out, hipStream_t s);
 
 // Small fix-up kernels (s2h_misc.hip).
 struct PoleArgs {
@@ -96,6 +98,10 @@ hipError_t launch_pole_inverse(const PoleArgs& a, hipStream_t s);
 hipError_t launch_pole_forward(const PoleArgs& a, hipStream_t s);
 hipError_t launch_zero(void* p, int64_t ndoubles, hipStream_t s);
 hipError_t launch_reality_fill(cplx* flm, int64_t map_stride, int L, int batch, hipStream_t s);
+hipError_t launch_reality_fold(const cplx* g, cplx* out, int64_t map_stride, int L, int batch,
+                               hipStream_t s);
+hipError_t launch_scale_pos_m(cplx* flm, int64_t map_stride, int L, int batch, double factor,
+                              hipStream_t s);
 
 struct ResampleArgs {
   const cplx* in;           // phi-spectrum [b][row][col], col <-> m = m_lo + col
@@ -115,5 +121,7 @@ struct ResampleArgs {
 };
 hipError_t launch_resample(const ResampleArgs& a, int batch, hipStream_t s);
 hipError_t allow_large_lds_resample();
+hipError_t launch_resample_adjoint(const ResampleArgs& a, int batch, hipStream_t s);
+hipError_t allow_large_lds_resample_adjoint();
 
 }  // namespace s2h

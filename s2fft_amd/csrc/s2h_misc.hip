@@ -277,3 +277,196 @@ hipError_t allow_large_lds_resample() {
 }
 
 }  // namespace s2h
+
+namespace s2h {
+
+// Adjoint of the conjugate fill (reality): for m >= 1
+// out[l, m] = (g[l, m] + (-1)^m conj(g[l, -m])) / 2, out[l, 0] = g[l, 0].
+// The 1/2 pairs with the Hermitian completion of the inverse ring kernel.
+__global__ void __launch_bounds__(kThreads) reality_fold_kernel(const cplx* g, cplx* out,
+                                                                int64_t map_stride, int L) {
+  const int64_t b = blockIdx.z;
+  const int l = blockIdx.y;
+  const int m = blockIdx.x * kThreads + threadIdx.x;
+  if (m >= L) return;
+  const cplx* row = g + b * map_stride + (int64_t)l * (2 * L - 1);
+  cplx* o = out + b * map_stride + (int64_t)l * (2 * L - 1);
+  if (m == 0) {
+    o[L - 1] = row[L - 1];
+    return;
+  }
+  const cplx a = row[L - 1 + m], c = row[L - 1 - m];
+  const double s = (m & 1) ? -1.0 : 1.0;
+  o[L - 1 + m] = {0.5 * (a.x + s * c.x), 0.5 * (a.y - s * c.y)};
+}
+
+// flm[l, m] *= factor for m >= 1 (adjoint of the Hermitian completion).
+__global__ void __launch_bounds__(kThreads) scale_pos_m_kernel(cplx* flm, int64_t map_stride,
+                                                               int L, double factor) {
+  const int64_t b = blockIdx.z;
+  const int l = blockIdx.y;
+  const int m = blockIdx.x * kThreads + threadIdx.x + 1;
+  if (m >= L) return;
+  cplx* o = flm + b * map_stride + (int64_t)l * (2 * L - 1) + (L - 1 + m);
+  o->x *= factor;
+  o->y *= factor;
+}
+
+hipError_t launch_reality_fold(const cplx* g, cplx* out, int64_t map_stride, int L, int batch,
+                               hipStream_t s) {
+  if (batch == 0) return hipSuccess;
+  hipLaunchKernelGGL(reality_fold_kernel, dim3((L + kThreads - 1) / kThreads, L, batch),
+                     dim3(kThreads), 0, s, g, out, map_stride, L);
+  return hipGetLastError();
+}
+
+hipError_t launch_scale_pos_m(cplx* flm, int64_t map_stride, int L, int batch, double factor,
+                              hipStream_t s) {
+  if (batch == 0 || L < 2) return hipSuccess;
+  hipLaunchKernelGGL(scale_pos_m_kernel, dim3((L - 1 + kThreads - 1) / kThreads, L, batch),
+                     dim3(kThreads), 0, s, flm, map_stride, L, factor);
+  return hipGetLastError();
+}
+
+}  // namespace s2h
+
+namespace s2h {
+
+// Adjoint of resample_kernel (same column-per-workgroup layout, steps in
+// reverse with each step's adjoint): weights, zero-extension of the kept
+// rows, DFT for every IDFT, bin extraction for every zero-pad, IDFT/n for
+// every DFT/n, and folding of the mirrored rows.
+__global__ void __launch_bounds__(kThreads) resample_adjoint_kernel(ResampleArgs a) {
+  extern __shared__ double smem[];
+  const int L = a.L;
+  const int col = blockIdx.x;
+  const int m = a.m_lo + col;
+  const int64_t b = blockIdx.y;
+  double* re = smem;
+  double* im = smem + a.lds;
+  const double msgn = (((m + a.spin) % 2) + 2) % 2 ? -1.0 : 1.0;
+  constexpr int kPer = kMaxConvLen / kThreads;
+
+  // rows 0..2L of the upsampled grid, weighted, zero-extended to 4L
+  const cplx* g = a.out + b * a.out_map_stride + col;
+  for (int t = threadIdx.x; t < 4 * L; t += kThreads) {
+    cplx v = {0.0, 0.0};
+    if (t <= 2 * L) {
+      const cplx u = g[(int64_t)t * a.ncol];
+      const double w = a.weights[t] * a.phi_factor;
+      v = {u.x * w, u.y * w};
+    }
+    re[t] = v.x;
+    im[t] = v.y;
+  }
+  __syncthreads();
+  col_dft(re, im, 4 * L, a.conv[2], a.chirp_off[2], a.pool, -1);
+  // keep bins q in [-L, L): index q (q >= 0) and 4L+q (q < 0) -> 2L layout
+  {
+    cplx v[kPer];
+#pragma unroll
+    for (int c = 0; c < kPer; ++c) {
+      const int k = threadIdx.x + c * kThreads;
+      if (k < 2 * L) {
+        const int src = k < L ? k : k + 2 * L;
+        v[c] = {re[src], im[src]};
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < kPer; ++c) {
+      const int k = threadIdx.x + c * kThreads;
+      if (k < 2 * L) {
+        re[k] = v[c].x;
+        im[k] = v[c].y;
+      }
+    }
+    __syncthreads();
+  }
+  col_dft(re, im, 2 * L, a.conv[1], a.chirp_off[1], a.pool, +1);
+  // 1/(2L) and fold rows L+1..2L-1 onto L-1..1
+  {
+    cplx v[kPer];
+#pragma unroll
+    for (int c = 0; c < kPer; ++c) {
+      const int t = threadIdx.x + c * kThreads;
+      if (t >= 1 && t <= L - 1) v[c] = {re[t] + msgn * re[2 * L - t], im[t] + msgn * im[2 * L - t]};
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < kPer; ++c) {
+      const int t = threadIdx.x + c * kThreads;
+      if (t >= 1 && t <= L - 1) {
+        re[t] = v[c].x;
+        im[t] = v[c].y;
+      }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t <= L; t += kThreads) {
+      re[t] /= (2 * L);
+      im[t] /= (2 * L);
+    }
+    __syncthreads();
+  }
+  cplx* dst = const_cast<cplx*>(a.in) + b * a.in_map_stride + col;
+  if (!a.mw) {
+    for (int t = threadIdx.x; t <= L; t += kThreads) dst[(int64_t)t * a.ncol] = {re[t], im[t]};
+    return;
+  }
+  // MW part: rows 0..L zero-extended to 2L, DFT_{2L}, bins p in [-(L-1), L-1]
+  for (int t = L + 1 + threadIdx.x; t < 2 * L; t += kThreads) {
+    re[t] = 0.0;
+    im[t] = 0.0;
+  }
+  __syncthreads();
+  col_dft(re, im, 2 * L, a.conv[1], a.chirp_off[1], a.pool, -1);
+  const int n1 = 2 * L - 1;
+  {
+    cplx v[kPer];
+#pragma unroll
+    for (int c = 0; c < kPer; ++c) {
+      const int k = threadIdx.x + c * kThreads;
+      if (k < n1) {
+        const int p = k < L ? k : k - n1;
+        const int src = p >= 0 ? p : 2 * L + p;
+        cplx x = {re[src], im[src]};
+        x = cmul(x, expi_pi((int64_t)p, n1));
+        v[c] = {x.x / n1, x.y / n1};
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < kPer; ++c) {
+      const int k = threadIdx.x + c * kThreads;
+      if (k < n1) {
+        re[k] = v[c].x;
+        im[k] = v[c].y;
+      }
+    }
+    __syncthreads();
+  }
+  col_dft(re, im, n1, a.conv[0], a.chirp_off[0], a.pool, +1);
+  for (int t = threadIdx.x; t < L; t += kThreads) {
+    cplx v = {re[t], im[t]};
+    if (t <= L - 2) {
+      v.x += msgn * re[2 * L - 2 - t];
+      v.y += msgn * im[2 * L - 2 - t];
+    }
+    dst[(int64_t)t * a.ncol] = v;
+  }
+}
+
+hipError_t launch_resample_adjoint(const ResampleArgs& a, int batch, hipStream_t s) {
+  if (batch == 0 || a.nm == 0) return hipSuccess;
+  const size_t lds = 2 * sizeof(double) * (size_t)a.lds;
+  hipLaunchKernelGGL(resample_adjoint_kernel, dim3(a.nm, batch), dim3(kThreads), lds, s, a);
+  return hipGetLastError();
+}
+
+hipError_t allow_large_lds_resample_adjoint() {
+  return hipFuncSetAttribute((const void*)resample_adjoint_kernel,
+                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)(2 * sizeof(double) * kMaxConvLen));
+}
+
+}  // namespace s2h

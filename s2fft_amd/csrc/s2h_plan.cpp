@@ -489,6 +489,7 @@ static int plan_build(s2h_plan* p) {
     if (need > kMaxConvLen) return S2H_ERR_UNSUPPORTED;
     p->rs_info[3] = need;
     if (allow_large_lds_resample() != hipSuccess) return S2H_ERR_HIP;
+    if (allow_large_lds_resample_adjoint() != hipSuccess) return S2H_ERR_HIP;
   }
 
   // ---------------- recursion tables

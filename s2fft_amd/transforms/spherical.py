@@ -124,6 +124,84 @@ def forward_hip(f, L, spin=0, nside=None, sampling="mw", reality=False, L_lower=
     return flm
 
 
+def forward_adjoint_hip(g, L, spin=0, nside=None, sampling="mw", reality=False, L_lower=0,
+                        plan=None):
+    """F^H g on device tensors (g: (B, L, 2L-1) complex128)."""
+    s = sampling.lower()
+    plan = plan or get_plan(s, L, nside, spin, L_lower, reality, device=g.device)
+    B = g.shape[0]
+    dtype = torch.float64 if reality else torch.complex128
+    out = torch.empty((B,) + samples.f_shape(L, s, nside), dtype=dtype, device=g.device)
+    work = torch.empty(plan.workspace_bytes(B), dtype=torch.uint8, device=g.device)
+    g = g.contiguous().to(torch.complex128)
+    _lib.check(_lib.lib().s2h_forward_adjoint(plan.handle, g.data_ptr(), out.data_ptr(),
+                                              work.data_ptr(), B, _stream(g.device)),
+               "forward adjoint")
+    return out
+
+
+def inverse_adjoint_hip(g, L, spin=0, nside=None, sampling="mw", reality=False, L_lower=0,
+                        plan=None):
+    """I^H g on device tensors (g in the f layout) -> (B, L, 2L-1) complex128."""
+    s = sampling.lower()
+    plan = plan or get_plan(s, L, nside, spin, L_lower, reality, device=g.device)
+    B = g.shape[0]
+    out = torch.empty((B, L, 2 * L - 1), dtype=torch.complex128, device=g.device)
+    work = torch.empty(plan.workspace_bytes(B), dtype=torch.uint8, device=g.device)
+    g = g.contiguous()
+    if not g.is_complex():
+        g = g.to(torch.float64)
+    _lib.check(_lib.lib().s2h_inverse_adjoint(plan.handle, g.data_ptr(), int(g.is_complex()),
+                                              out.data_ptr(), work.data_ptr(), B,
+                                              _stream(g.device)), "inverse adjoint")
+    return out
+
+
+class _Forward(torch.autograd.Function):
+    """flm = F(f) with the hand-written adjoint as its backward (no stored d)."""
+
+    @staticmethod
+    def forward(ctx, f, L, spin, nside, sampling, reality, L_lower):
+        ctx.args = (L, spin, nside, sampling, reality, L_lower)
+        ctx.f_complex = f.is_complex()
+        return forward_hip(f, L, spin, nside, sampling, reality, L_lower)
+
+    @staticmethod
+    def backward(ctx, g):
+        grad = forward_adjoint_hip(g, *ctx.args)
+        if ctx.f_complex and not grad.is_complex():
+            grad = grad.to(torch.complex128)
+        elif not ctx.f_complex and grad.is_complex():
+            grad = grad.real.contiguous()
+        return grad, None, None, None, None, None, None
+
+
+class _Inverse(torch.autograd.Function):
+    """f = I(flm) with the hand-written adjoint as its backward."""
+
+    @staticmethod
+    def forward(ctx, flm, L, spin, nside, sampling, reality, L_lower):
+        ctx.args = (L, spin, nside, sampling, reality, L_lower)
+        return inverse_hip(flm, L, spin, nside, sampling, reality, L_lower)
+
+    @staticmethod
+    def backward(ctx, g):
+        return (inverse_adjoint_hip(g, *ctx.args), None, None, None, None, None, None)
+
+
+def forward_diff(f, L, spin=0, nside=None, sampling="mw", reality=False, L_lower=0):
+    """forward_hip that records the adjoint for autograd when needed."""
+    if torch.is_grad_enabled() and f.requires_grad:
+        return _Forward.apply(f, L, spin, nside, sampling, reality, L_lower)
+    return forward_hip(f, L, spin, nside, sampling, reality, L_lower)
+
+
+def inverse_diff(flm, L, spin=0, nside=None, sampling="mw", reality=False, L_lower=0):
+    if torch.is_grad_enabled() and flm.requires_grad:
+        return _Inverse.apply(flm, L, spin, nside, sampling, reality, L_lower)
+    return inverse_hip(flm, L, spin, nside, sampling, reality, L_lower)
+
+
 def forward_iterative_hip(f, L, spin=0, nside=None, sampling="mw", reality=False, L_lower=0,
                           n_iter=0):
     """flm = F(f); repeat n_iter: flm += F(f - I(flm)).
@@ -131,10 +209,10 @@ def forward_iterative_hip(f, L, spin=0, nside=None, sampling="mw", reality=False
     s2fft/utils/iterative_refinement.py:9-46 as used by spherical.py:453-462;
     the residual and the update stay in FP64 on the device.
     """
-    flm = forward_hip(f, L, spin, nside, sampling, reality, L_lower)
+    flm = forward_diff(f, L, spin, nside, sampling, reality, L_lower)
     for _ in range(n_iter):
-        f_rec = inverse_hip(flm, L, spin, nside, sampling, reality, L_lower)
-        flm = flm + forward_hip(f - f_rec, L, spin, nside, sampling, reality, L_lower)
+        f_rec = inverse_diff(flm, L, spin, nside, sampling, reality, L_lower)
+        flm = flm + forward_diff(f - f_rec, L, spin, nside, sampling, reality, L_lower)
     return flm
 
 
@@ -151,7 +229,7 @@ def inverse(flm, L, spin=0, nside=None, sampling="mw", method="hip", reality=Fal
     t, restore = _as_device_tensor(flm, complex_only=True)
     tb, batched = _split_batch(t, samples.flm_shape(L), "flm")
     with torch.cuda.device(tb.device):
-        f = inverse_hip(tb, L, spin, nside, sampling, reality, L_lower)
+        f = inverse_diff(tb, L, spin, nside, sampling, reality, L_lower)
     return restore(f if batched else f[0])
 
 

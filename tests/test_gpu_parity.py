@@ -191,3 +191,35 @@ def test_wigner_vs_oracle(rng, sampling, nside, reality, N):
     assert max_rel(got, ref) < TOL
     if sampling != "healpix":
         assert max_rel(got, flmn) < 1e-11
+
+
+def test_mfma_f64_layout():
+    """The A/B/result lane layouts the MFMA forward kernel assumes."""
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((16, 4))
+    B = rng.standard_normal((4, 16))
+    lanes = np.arange(64)
+    a = A[lanes % 16, lanes // 16]          # A[i][k] at lane i + 16 k
+    b = B[lanes // 16, lanes % 16]          # B[k][j] at lane j + 16 k
+    da, db = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+    out = torch.zeros(256, dtype=torch.float64, device="cuda")
+    _lib.check(_lib.lib().s2h_probe_mfma_f64(da.data_ptr(), db.data_ptr(), out.data_ptr(),
+                                             torch.cuda.current_stream().cuda_stream))
+    got = out.cpu().numpy().reshape(64, 4)
+    C = A @ B
+    expect = np.array([[C[4 * (l // 16) + r, l % 16] for r in range(4)] for l in range(64)])
+    assert np.allclose(got, expect, atol=1e-13)
+
+
+@pytest.mark.parametrize("sampling,nside,L,spin,reality,B", [
+    ("healpix", 16, 32, 0, True, 9), ("healpix", 8, 16, 0, False, 4),
+    ("gl", None, 12, 2, False, 5), ("dh", None, 10, 0, True, 8), ("mw", None, 9, 0, True, 4)])
+def test_batched_mfma_forward_matches_oracle(rng, sampling, nside, L, spin, reality, B):
+    flms = np.stack([_flm(rng, L, spin, reality) for _ in range(B)])
+    fs = np.stack([otr.inverse(flms[i], L, spin, nside, sampling, reality) for i in range(B)])
+    if reality:
+        fs = np.real(fs)
+    got = s2fft_amd.forward(fs, L, spin, nside, sampling, "hip", reality)
+    for i in range(B):
+        ref = otr.forward(fs[i], L, spin, nside, sampling, reality)
+        assert max_rel(got[i], ref) < TOL
