@@ -139,6 +139,45 @@ def dslices(beta, L, spin, reality=False, L_lower=0, precomps=None,
                 lrenorm[i] = np.where(active, lrenorm[i] + np.log(mag), lrenorm[i])
 
 
+def dslices_exact(beta, L, spin, reality=False, L_lower=0):
+    """Yield ``(m, D)`` like ``dslices`` from Wigner's explicit sum (mpmath).
+
+    For spin != 0 the reference's m-recursion meets exact zeros of
+    intermediate values at some HEALPix colatitudes (e.g. cos(theta) = 1/6):
+    its renormalisation 1/|d| (otf_recursions.py:164-169) turns them into NaN
+    and nansum drops those terms.  This generator gives the mathematically
+    exact d-values for those cases; it is slow and meant for L <= 16.
+    """
+    from . import pins
+
+    beta = np.asarray(beta, dtype=np.float64)
+    for m in range(0 if reality else -(L - 1), L):
+        D = np.zeros((len(beta), L - L_lower))
+        for l in range(max(abs(m), abs(spin), L_lower), L):
+            D[:, l - L_lower] = [pins.wigner_d(l, m, -spin, b) for b in beta]
+        yield m, D
+
+
+_EXACT = {"on": False}
+
+
+class exact_d:
+    """Context manager: latitudinal steps use ``dslices_exact``."""
+
+    def __enter__(self):
+        _EXACT["on"] = True
+        return self
+
+    def __exit__(self, *exc):
+        _EXACT["on"] = False
+
+
+def _slices(beta, L, spin, reality, L_lower, precomps, sampling, nside, forward):
+    if _EXACT["on"]:
+        return dslices_exact(beta, L, spin, reality, L_lower)
+    return dslices(beta, L, spin, reality, L_lower, precomps, sampling, nside, forward)
+
+
 def _ftm_col_offset(sampling):
     return 1 if sampling.lower() in ("mwss", "healpix") else 0
 
@@ -153,8 +192,7 @@ def inverse_latitudinal_step(flm, beta, L, spin, nside=None, sampling="mw",
     """
     off = _ftm_col_offset(sampling)
     ftm = np.zeros((len(beta), 2 * L - 1 + off), dtype=np.complex128)
-    for m, D in dslices(beta, L, spin, reality, L_lower, precomps, sampling, nside,
-                        forward=False):
+    for m, D in _slices(beta, L, spin, reality, L_lower, precomps, sampling, nside, False):
         ftm[:, L - 1 + m + off] = np.nansum(D * flm[L_lower:, L - 1 + m][None, :], axis=1)
     return ftm
 
@@ -167,7 +205,6 @@ def forward_latitudinal_step(ftm, beta, L, spin, nside=None, sampling="mw",
     """
     off = _ftm_col_offset(sampling)
     flm = np.zeros((L, 2 * L - 1), dtype=np.complex128)
-    for m, D in dslices(beta, L, spin, reality, L_lower, precomps, sampling, nside,
-                        forward=True):
+    for m, D in _slices(beta, L, spin, reality, L_lower, precomps, sampling, nside, True):
         flm[L_lower:, L - 1 + m] = np.nansum(D * ftm[:, L - 1 + m + off][:, None], axis=0)
     return flm
