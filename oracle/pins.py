@@ -14,6 +14,9 @@ Test infrastructure only -- see ``oracle/__init__.py``.
 * ``healpix_pix2ang``-- the HEALPix ring-scheme closed form (Gorski et al. 2005,
                         eqs. 4-9), independent of the ring-index formulas.
 * ``dft``            -- brute-force discrete Fourier sums.
+* ``lat_inverse_longdouble`` -- the spin-0 latitudinal synthesis in 80-bit
+                        long double (direct l-recursion from d^m_{m0}), a
+                        high-precision pin for L up to a few hundred.
 """
 
 import math
@@ -117,3 +120,35 @@ def dft(x):
     n = len(x)
     p = np.arange(n)
     return np.exp(-2j * np.pi * np.outer(p, p) / n) @ x
+
+
+def lat_inverse_longdouble(flm, beta, L):
+    """ftm[t, m] = sum_l flm[l, L-1+m] d^l_{m0}(beta_t), m = 0..L-1, in long double.
+
+    The three-term recurrence in l of d^l_{m0} (Bonnet-type, normalised
+    Wigner d) from d^m_{m0} = (-1)^m sqrt((2m)!/(m!^2)) (cos(b/2) sin(b/2))^m;
+    the 80-bit exponent range holds the start values unscaled for L <= ~600.
+    Returns complex128 of shape (len(beta), L).
+    """
+    ld = np.longdouble
+    th = np.asarray(beta, dtype=np.float64).astype(ld)
+    x, c2, s2 = np.cos(th), np.cos(th / 2), np.sin(th / 2)
+    out = np.zeros((len(th), L), dtype=np.clongdouble)
+    for m in range(L):
+        K = ld(1)
+        for i in range(1, m + 1):
+            K *= ld(m + i) / ld(i)
+        d = (-1) ** m * np.sqrt(K) * (c2 * s2) ** m
+        dm1 = np.zeros_like(d)
+        acc = np.zeros(len(th), dtype=np.clongdouble)
+        for l in range(m, L):
+            acc += d * flm[l, L - 1 + m]
+            if l == 0:
+                dn = x * d
+            else:
+                D = ld(l) * np.sqrt(ld((l + 1) ** 2 - m * m) * ld((l + 1) ** 2))
+                dn = (ld(2 * l + 1) * l * (l + 1) / D) * x * d \
+                    - (ld(l + 1) * np.sqrt(ld(l * l - m * m) * ld(l * l)) / D) * dm1
+            dm1, d = d, dn
+        out[:, m] = acc
+    return out.astype(np.complex128)

@@ -198,6 +198,7 @@ extern "C" int s2h_forward(const s2h_plan* p, const void* f, int f_is_complex, v
     ra.L = L;
     ra.spin = p->d.spin;
     ra.mw = p->fwd_kind == 1;
+    ra.nyq_split = real ? 1 : 0;
     ra.weights = p->fwd_weights;
     ra.phi_factor = ra.mw ? (2.0 * L) / (2.0 * L - 1.0) : 1.0;
     ra.lds = p->rs_info[3];
@@ -379,6 +380,7 @@ extern "C" int s2h_forward_adjoint(const s2h_plan* p, const void* g, void* grad_
     ra.L = L;
     ra.spin = p->d.spin;
     ra.mw = p->fwd_kind == 1;
+    ra.nyq_split = real ? 1 : 0;
     ra.weights = p->fwd_weights;
     ra.phi_factor = ra.mw ? (2.0 * L) / (2.0 * L - 1.0) : 1.0;
     ra.lds = p->rs_info[3];

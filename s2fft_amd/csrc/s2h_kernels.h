@@ -112,6 +112,7 @@ struct ResampleArgs {
   int m_lo, nm;
   int L, spin;
   int mw;                   // 1: MW input (L rows), 0: MWSS input (L+1 rows)
+  int nyq_split;            // real signals: split the theta-Nyquist bin (see kernel)
   const double* weights;    // 2L+1 quadrature weights of MWSS at 2L
   double phi_factor;        // 2L/(2L-1) for MW, 1 for MWSS
   int lds;                  // LDS elements (complex) per workgroup

@@ -249,8 +249,18 @@ __global__ void __launch_bounds__(kThreads) resample_kernel(ResampleArgs a) {
       const int k = threadIdx.x + c * kThreads;
       if (k < 2 * L) {
         const int dst = k < L ? k : k + 2 * L;
-        re[dst] = v[c].x;
-        im[dst] = v[c].y;
+        if (a.nyq_split && k == L) {
+          // real signals: the reference upsamples the complex map and then
+          // takes its real part (spherical.py:528-534); on one column that is
+          // the Nyquist bin shared half/half between q = -L and q = +L
+          re[3 * L] = 0.5 * v[c].x;
+          im[3 * L] = 0.5 * v[c].y;
+          re[L] = 0.5 * v[c].x;
+          im[L] = 0.5 * v[c].y;
+        } else {
+          re[dst] = v[c].x;
+          im[dst] = v[c].y;
+        }
       }
     }
     __syncthreads();
@@ -370,6 +380,7 @@ __global__ void __launch_bounds__(kThreads) resample_adjoint_kernel(ResampleArgs
       if (k < 2 * L) {
         const int src = k < L ? k : k + 2 * L;
         v[c] = {re[src], im[src]};
+        if (a.nyq_split && k == L) v[c] = {0.5 * (re[3 * L] + re[L]), 0.5 * (im[3 * L] + im[L])};
       }
     }
     __syncthreads();
