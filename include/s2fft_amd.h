@@ -161,6 +161,8 @@ int s2h_profile_query(int kernel, double *total_ms, int *count);
  * accumulator; out[4*lane + r] = result register r of lane (256 doubles).
  * Lets a test confirm the operand / result lane layouts the kernels assume. */
 int s2h_probe_mfma_f64(const double *a, const double *b, double *out, s2h_stream stream);
+/* The same for V_MFMA_F32_16X16X4F32 (64 floats in each of a, b; 256 out). */
+int s2h_probe_mfma_f32(const float *a, const float *b, float *out, s2h_stream stream);
 
 #ifdef __cplusplus
 }

@@ -43,6 +43,7 @@ EXPORTED = (
     "healpix_fft_hip", "healpix_fft_hip_last_status", "s2h_gamma_dft",
     "s2h_profile_enable", "s2h_profile_query", "s2h_wigner_reality_fill",
     "s2h_forward_adjoint", "s2h_inverse_adjoint", "s2h_probe_mfma_f64",
+    "s2h_probe_mfma_f32",
 )
 
 KERNEL_LAT_FORWARD, KERNEL_LAT_INVERSE, KERNEL_RING_FFT_FORWARD = 0, 1, 2
@@ -62,6 +63,7 @@ def _declare(lib):
         "s2h_inverse": (i32, [vp, vp, vp, vp, i32, vp]),
         "s2h_forward_adjoint": (i32, [vp, vp, vp, vp, i32, vp]),
         "s2h_probe_mfma_f64": (i32, [vp, vp, vp, vp]),
+        "s2h_probe_mfma_f32": (i32, [vp, vp, vp, vp]),
         "s2h_inverse_adjoint": (i32, [vp, vp, i32, vp, vp, i32, vp]),
         "s2h_lat_forward": (i32, [vp, vp, vp, i32, vp]),
         "s2h_lat_inverse": (i32, [vp, vp, vp, i32, vp]),

@@ -272,7 +272,12 @@ extern "C" int s2h_inverse(const s2h_plan* p, const void* flm, void* f, void* wo
   a.ftm_m_first = p->m_lo;
   {
     ProfScope ps(S2H_KERNEL_LAT_INVERSE, s);
-    if ((st = hip_status(launch_lat_inverse(a, p->inv_paired, s)))) return st;
+    // batches of 4 or more maps: each d-value feeds 16 maps through the MFMA
+    if (batch >= kMfmaMinBatch)
+      st = hip_status(launch_lat_inverse_mma(a, p->inv_paired, false, s));
+    else
+      st = hip_status(launch_lat_inverse(a, p->inv_paired, s));
+    if (st) return st;
   }
 
   if (p->inv_row_south >= 0 || p->inv_row_north >= 0) {
@@ -611,4 +616,9 @@ extern "C" int s2h_probe_mfma_f64(const double* a, const double* b, double* out,
                                   s2h_stream stream) {
   if (!a || !b || !out) return S2H_ERR_INVALID;
   return hip_status(launch_mfma_f64_probe(a, b, out, static_cast<hipStream_t>(stream)));
+}
+
+extern "C" int s2h_probe_mfma_f32(const float* a, const float* b, float* out, s2h_stream stream) {
+  if (!a || !b || !out) return S2H_ERR_INVALID;
+  return hip_status(launch_mfma_f32_probe(a, b, out, static_cast<hipStream_t>(stream)));
 }

@@ -284,6 +284,7 @@ static hipError_t lat_inv_nb(const LatArgs& a, bool paired, hipStream_t s) {
 
 hipError_t launch_lat_inverse(const LatArgs& a, bool paired, hipStream_t s) {
   if (a.nm == 0 || a.nrings == 0 || a.batch == 0) return hipSuccess;
+  if (a.batch >= 8) return lat_inv_nb<8>(a, paired, s);
   if (a.batch >= 4) return lat_inv_nb<4>(a, paired, s);
   if (a.batch >= 2) return lat_inv_nb<2>(a, paired, s);
   return lat_inv_nb<1>(a, paired, s);

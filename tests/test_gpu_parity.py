@@ -252,8 +252,39 @@ def test_mfma_f64_layout():
                                              torch.cuda.current_stream().cuda_stream))
     got = out.cpu().numpy().reshape(64, 4)
     C = A @ B
-    expect = np.array([[C[4 * (l // 16) + r, l % 16] for r in range(4)] for l in range(64)])
+    # FP64: register r of lane l holds row 4r + l//16
+    expect = np.array([[C[4 * r + l // 16, l % 16] for r in range(4)] for l in range(64)])
     assert np.allclose(got, expect, atol=1e-13)
+
+
+def test_mfma_f32_layout():
+    """FP32 MFMA layouts the mixed-precision kernels assume."""
+    rng = np.random.default_rng(6)
+    A = rng.standard_normal((16, 4)).astype(np.float32)
+    B = rng.standard_normal((4, 16)).astype(np.float32)
+    lanes = np.arange(64)
+    a = np.ascontiguousarray(A[lanes % 16, lanes // 16])
+    b = np.ascontiguousarray(B[lanes // 16, lanes % 16])
+    da, db = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+    out = torch.zeros(256, dtype=torch.float32, device="cuda")
+    _lib.check(_lib.lib().s2h_probe_mfma_f32(da.data_ptr(), db.data_ptr(), out.data_ptr(),
+                                             torch.cuda.current_stream().cuda_stream))
+    got = out.cpu().numpy().reshape(64, 4).astype(np.float64)
+    C = A.astype(np.float64) @ B.astype(np.float64)
+    # FP32: register r of lane l holds row 4 (l//16) + r
+    expect = np.array([[C[4 * (l // 16) + r, l % 16] for r in range(4)] for l in range(64)])
+    assert np.allclose(got, expect, atol=1e-5)
+
+
+@pytest.mark.parametrize("sampling,nside,L,spin,reality,B", [
+    ("healpix", 16, 32, 0, True, 17), ("healpix", 8, 16, 0, False, 4),
+    ("gl", None, 12, 2, False, 5), ("mwss", None, 10, 0, True, 8), ("mw", None, 9, 1, False, 6)])
+def test_batched_mma_inverse_matches_oracle(rng, sampling, nside, L, spin, reality, B):
+    flms = np.stack([_flm(rng, L, spin, reality) for _ in range(B)])
+    got = s2fft_amd.inverse(flms, L, spin, nside, sampling, "hip", reality)
+    for i in range(B):
+        ref = otr.inverse(flms[i], L, spin, nside, sampling, reality)
+        assert max_rel(got[i], ref) < TOL
 
 
 @pytest.mark.parametrize("sampling,nside,L,spin,reality,B", [
