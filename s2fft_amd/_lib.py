@@ -43,11 +43,12 @@ EXPORTED = (
     "healpix_fft_hip", "healpix_fft_hip_last_status", "s2h_gamma_dft",
     "s2h_profile_enable", "s2h_profile_query", "s2h_wigner_reality_fill",
     "s2h_forward_adjoint", "s2h_inverse_adjoint", "s2h_probe_mfma_f64",
-    "s2h_probe_mfma_f32",
+    "s2h_probe_mfma_f32", "s2h_forward_ex", "s2h_inverse_ex",
 )
 
 KERNEL_LAT_FORWARD, KERNEL_LAT_INVERSE, KERNEL_RING_FFT_FORWARD = 0, 1, 2
 KERNEL_RING_FFT_INVERSE, KERNEL_RESAMPLE = 3, 4
+PREC_FP64, PREC_MIXED = 0, 1
 
 
 def _declare(lib):
@@ -61,6 +62,8 @@ def _declare(lib):
         "s2h_inverse_output_is_complex": (i32, [vp]),
         "s2h_forward": (i32, [vp, vp, i32, vp, vp, i32, vp]),
         "s2h_inverse": (i32, [vp, vp, vp, vp, i32, vp]),
+        "s2h_forward_ex": (i32, [vp, vp, i32, vp, vp, i32, i32, vp]),
+        "s2h_inverse_ex": (i32, [vp, vp, vp, vp, i32, i32, vp]),
         "s2h_forward_adjoint": (i32, [vp, vp, vp, vp, i32, vp]),
         "s2h_probe_mfma_f64": (i32, [vp, vp, vp, vp]),
         "s2h_probe_mfma_f32": (i32, [vp, vp, vp, vp]),

@@ -139,7 +139,14 @@ extern "C" int s2h_inverse_output_is_complex(const s2h_plan* p) {
 
 extern "C" int s2h_forward(const s2h_plan* p, const void* f, int f_is_complex, void* flm,
                            void* work, int batch, s2h_stream stream) {
+  return s2h_forward_ex(p, f, f_is_complex, flm, work, batch, S2H_PREC_FP64, stream);
+}
+
+extern "C" int s2h_forward_ex(const s2h_plan* p, const void* f, int f_is_complex, void* flm,
+                              void* work, int batch, int precision, s2h_stream stream) {
   if (!p || !f || !flm || batch < 0 || (batch > 0 && !work)) return S2H_ERR_INVALID;
+  if (precision != S2H_PREC_FP64 && precision != S2H_PREC_MIXED) return S2H_ERR_INVALID;
+  const bool mixed = precision == S2H_PREC_MIXED;
   if (batch == 0) return S2H_OK;
   int st = check_device(p);
   if (st) return st;
@@ -220,9 +227,10 @@ extern "C" int s2h_forward(const s2h_plan* p, const void* f, int f_is_complex, v
   a.lmin_out = std::max(std::abs(p->d.spin), p->d.L_lower);
   {
     ProfScope ps(S2H_KERNEL_LAT_FORWARD, s);
-    // batches of 4 or more maps share each d-value across an 8-map MFMA tile
-    if (batch >= kMfmaMinBatch)
-      st = hip_status(launch_lat_forward_mfma(a, p->fwd_paired, s));
+    // batches of 4 or more maps share each d-value across an 8-map MFMA tile;
+    // the mixed-precision projection always runs on the FP32 matrix cores
+    if (mixed || batch >= kMfmaMinBatch)
+      st = hip_status(launch_lat_forward_mfma(a, p->fwd_paired, mixed, s));
     else
       st = hip_status(launch_lat_forward(a, p->fwd_paired, s));
     if (st) return st;
@@ -254,7 +262,14 @@ extern "C" int s2h_forward(const s2h_plan* p, const void* f, int f_is_complex, v
 
 extern "C" int s2h_inverse(const s2h_plan* p, const void* flm, void* f, void* work, int batch,
                            s2h_stream stream) {
+  return s2h_inverse_ex(p, flm, f, work, batch, S2H_PREC_FP64, stream);
+}
+
+extern "C" int s2h_inverse_ex(const s2h_plan* p, const void* flm, void* f, void* work, int batch,
+                              int precision, s2h_stream stream) {
   if (!p || !flm || !f || batch < 0 || (batch > 0 && !work)) return S2H_ERR_INVALID;
+  if (precision != S2H_PREC_FP64 && precision != S2H_PREC_MIXED) return S2H_ERR_INVALID;
+  const bool mixed = precision == S2H_PREC_MIXED;
   if (batch == 0) return S2H_OK;
   int st = check_device(p);
   if (st) return st;
@@ -273,8 +288,8 @@ extern "C" int s2h_inverse(const s2h_plan* p, const void* flm, void* f, void* wo
   {
     ProfScope ps(S2H_KERNEL_LAT_INVERSE, s);
     // batches of 4 or more maps: each d-value feeds 16 maps through the MFMA
-    if (batch >= kMfmaMinBatch)
-      st = hip_status(launch_lat_inverse_mma(a, p->inv_paired, false, s));
+    if (mixed || batch >= kMfmaMinBatch)
+      st = hip_status(launch_lat_inverse_mma(a, p->inv_paired, mixed, s));
     else
       st = hip_status(launch_lat_inverse(a, p->inv_paired, s));
     if (st) return st;

@@ -74,7 +74,7 @@ hipError_t allow_large_lds();
 
 hipError_t launch_lat_inverse(const LatArgs& a, bool paired, hipStream_t s);
 hipError_t launch_lat_forward(const LatArgs& a, bool paired, hipStream_t s);
-hipError_t launch_lat_forward_mfma(const LatArgs& a, bool paired, hipStream_t s);
+hipError_t launch_lat_forward_mfma(const LatArgs& a, bool paired, bool fp32, hipStream_t s);
 hipError_t launch_mfma_f64_probe(const double* a, const double* b, double* out, hipStream_t s);
 hipError_t launch_lat_inverse_mma(const LatArgs& a, bool paired, bool fp32, hipStream_t s);
 hipError_t launch_mfma_f32_probe(const float* a, const float* b, float* out, hipStream_t s);

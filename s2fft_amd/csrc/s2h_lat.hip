@@ -14,6 +14,7 @@
 // O(1) ones).
 #include "s2h_fft_lds.h"
 #include "s2h_kernels.h"
+#include "s2h_lat_common.h"
 
 namespace s2h {
 
@@ -21,69 +22,9 @@ namespace {
 
 constexpr int kCheck = 8;  // steps between rescale checks (growth < 2^100 per check)
 
-struct Ext {
-  double m;
-  int e;
-};
-
-__device__ __forceinline__ Ext ext_of(double v) {
-  int e;
-  const double m = frexp(v, &e);
-  return {m, e};
-}
-__device__ __forceinline__ Ext ext_mul(Ext a, Ext b) {
-  int e;
-  const double m = frexp(a.m * b.m, &e);
-  return {m, a.e + b.e + e};
-}
-__device__ __forceinline__ Ext ext_pow(double base, int k) {
-  Ext r = {0.5, 1};
-  Ext x = ext_of(base);
-  while (k) {
-    if (k & 1) r = ext_mul(r, x);
-    k >>= 1;
-    if (k) x = ext_mul(x, x);
-  }
-  return r;
-}
-
-// Start value d^{l0}_{m,n}(theta) split into (stored, lscale).
-__device__ __forceinline__ void start_value(const MStart& st, double c2, double s2, double& d,
-                                            int& lscale) {
-  Ext v = {st.kmant, st.kexp};
-  if (st.ecos) v = ext_mul(v, ext_pow(c2, st.ecos));
-  if (st.esin) v = ext_mul(v, ext_pow(s2, st.esin));
-  if (v.m == 0.0) {
-    d = 0.0;
-    lscale = 0;
-    return;
-  }
-  int ls = -((256 - v.e) >> 9);
-  if (ls > 0) ls = 0;
-  lscale = ls;
-  d = ldexp(v.m, v.e - 512 * ls);
-}
-
-__device__ __forceinline__ void rescale(double& d, double& dm1, int& lscale) {
-  const bool big = fabs(d) > 0x1p256;
-  const double f = big ? 0x1p-512 : 1.0;
-  d *= f;
-  dm1 *= f;
-  lscale += big ? 1 : 0;
-}
-
-// blockIdx.x -> m: 4 adjacent m per XCD in a row (workgroups are dealt to the
-// 8 XCDs round-robin), heaviest (smallest |m|) first.
-__device__ __forceinline__ int m_of_block(int i, int nm, int m_lo) {
-  const int nfull = nm & ~31;  // the remap is a bijection on [0, nfull)
-  int idx = i;
-  if (i < nfull) {
-    const int xcd = i & 7, k = i >> 3;
-    idx = ((k >> 2) << 5) + (xcd << 2) + (k & 3);
-  }
-  if (m_lo == 0) return idx;
-  return (idx & 1) ? -((idx + 1) >> 1) : (idx >> 1);
-}
+using lat::m_of_block;
+using lat::rescale;
+using lat::start_value;
 
 }  // namespace
 

@@ -1,0 +1,116 @@
+// Device helpers shared by the latitudinal (Wigner-d) kernels.
+//
+// Start values: d^{l0}_{m,n}(theta) = K cos(theta/2)^ecos sin(theta/2)^esin
+// evaluated as (mantissa, exponent) so that sin^a cos^b cannot underflow at
+// high m; the result is split into a stored double and an integer scale index
+// lscale <= 0 with true value = stored * 2^(512 lscale).  A lane contributes
+// only while lscale == 0 (earlier terms are below 2^-256 of the O(1) ones).
+#pragma once
+
+#include "s2h_internal.h"
+
+namespace s2h {
+namespace lat {
+
+struct Ext {
+  double m;
+  int e;
+};
+
+__device__ __forceinline__ Ext ext_of(double v) {
+  int e;
+  const double m = frexp(v, &e);
+  return {m, e};
+}
+__device__ __forceinline__ Ext ext_mul(Ext a, Ext b) {
+  int e;
+  const double m = frexp(a.m * b.m, &e);
+  return {m, a.e + b.e + e};
+}
+__device__ __forceinline__ Ext ext_pow(double base, int k) {
+  Ext r = {0.5, 1};
+  Ext x = ext_of(base);
+  while (k) {
+    if (k & 1) r = ext_mul(r, x);
+    k >>= 1;
+    if (k) x = ext_mul(x, x);
+  }
+  return r;
+}
+
+// Start value d^{l0}_{m,n}(theta) split into (stored, lscale).
+__device__ __forceinline__ void start_value(const MStart& st, double c2, double s2, double& d,
+                                            int& lscale) {
+  Ext v = {st.kmant, st.kexp};
+  if (st.ecos) v = ext_mul(v, ext_pow(c2, st.ecos));
+  if (st.esin) v = ext_mul(v, ext_pow(s2, st.esin));
+  if (v.m == 0.0) {
+    d = 0.0;
+    lscale = 0;
+    return;
+  }
+  int ls = -((256 - v.e) >> 9);
+  if (ls > 0) ls = 0;
+  lscale = ls;
+  d = ldexp(v.m, v.e - 512 * ls);
+}
+
+// Moves (d, dm1) down by 2^512 once |d| passes 2^256.  The recursion grows
+// by less than 2.5 per step, so checks every 32 steps cannot overflow.
+__device__ __forceinline__ void rescale(double& d, double& dm1, int& lscale) {
+  const bool big = fabs(d) > 0x1p256;
+  const double f = big ? 0x1p-512 : 1.0;
+  d *= f;
+  dm1 *= f;
+  lscale += big ? 1 : 0;
+}
+
+// blockIdx.x -> m: 4 adjacent m per XCD in a row (workgroups are dealt to the
+// 8 XCDs round-robin), heaviest (smallest |m|) first.  For m_lo < 0 the order
+// is 0, -1, 1, -2, 2, ...
+__device__ __forceinline__ int m_of_block(int i, int nm, int m_lo) {
+  const int nfull = nm & ~31;  // the remap is a bijection on [0, nfull)
+  int idx = i;
+  if (i < nfull) {
+    const int xcd = i & 7, k = i >> 3;
+    idx = ((k >> 2) << 5) + (xcd << 2) + (k & 3);
+  }
+  if (m_lo == 0) return idx;
+  return (idx & 1) ? -((idx + 1) >> 1) : (idx >> 1);
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// 16x16x4 matrix-core products.  Operand layouts (both types): A[i][k] in
+// lane i + 16k, B[k][j] in lane j + 16k.  The result layouts differ:
+// FP64 holds C[i][j] in lane j + 16 (i%4), register i/4; FP32 holds it in
+// lane j + 16 (i/4), register i%4.  row_of(r, kq) is the row held in
+// register r by lanes 16 kq .. 16 kq + 15.
+template <typename T>
+struct Mma;
+template <>
+struct Mma<double> {
+  typedef v4d acc;
+  __device__ static acc run(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __device__ static int row_of(int r, int kq) { return 4 * r + kq; }
+};
+template <>
+struct Mma<float> {
+  typedef v4f acc;
+  __device__ static acc run(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  __device__ static int row_of(int r, int kq) { return 4 * kq + r; }
+};
+
+}  // namespace lat
+}  // namespace s2h

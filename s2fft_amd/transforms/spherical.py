@@ -93,10 +93,21 @@ def _check_common(method, spin, sampling, nside):
         raise ValueError(f"Sampling scheme sampling={sampling} with nside={nside} not supported")
 
 
+def _precision(name):
+    try:
+        return {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED}[name]
+    except KeyError:
+        raise ValueError(f"precision={name!r}: expected 'fp64' or 'mixed'") from None
+
+
 # ------------------------------------------------------------------ device level
 def inverse_hip(flm, L, spin=0, nside=None, sampling="mw", reality=False, L_lower=0,
-                plan=None):
-    """Batched inverse on device tensors: flm (B, L, 2L-1) complex128 -> f (B, ...)."""
+                plan=None, precision="fp64"):
+    """Batched inverse on device tensors: flm (B, L, 2L-1) complex128 -> f (B, ...).
+
+    ``precision``: "fp64" (default) or "mixed" (FP64 recursion, FP32 matrix-core
+    synthesis with FP32 accumulation within 32-l chunks).
+    """
     s = sampling.lower()
     plan = plan or get_plan(s, L, nside, spin, L_lower, reality, device=flm.device)
     B = flm.shape[0]
@@ -105,22 +116,28 @@ def inverse_hip(flm, L, spin=0, nside=None, sampling="mw", reality=False, L_lowe
     f = torch.empty((B,) + fshape, dtype=dtype, device=flm.device)
     work = torch.empty(plan.workspace_bytes(B), dtype=torch.uint8, device=flm.device)
     flm = flm.contiguous()
-    _lib.check(_lib.lib().s2h_inverse(plan.handle, flm.data_ptr(), f.data_ptr(),
-                                      work.data_ptr(), B, _stream(flm.device)), "inverse")
+    _lib.check(_lib.lib().s2h_inverse_ex(plan.handle, flm.data_ptr(), f.data_ptr(),
+                                         work.data_ptr(), B, _precision(precision),
+                                         _stream(flm.device)), "inverse")
     return f
 
 
-def forward_hip(f, L, spin=0, nside=None, sampling="mw", reality=False, L_lower=0, plan=None):
-    """Batched forward on device tensors: f (B, ...) -> flm (B, L, 2L-1) complex128."""
+def forward_hip(f, L, spin=0, nside=None, sampling="mw", reality=False, L_lower=0, plan=None,
+                precision="fp64"):
+    """Batched forward on device tensors: f (B, ...) -> flm (B, L, 2L-1) complex128.
+
+    ``precision``: "fp64" (default) or "mixed" (FP64 recursion and ring FFTs,
+    FP32 matrix-core projection with FP32 accumulation over 128 rings).
+    """
     s = sampling.lower()
     plan = plan or get_plan(s, L, nside, spin, L_lower, reality, device=f.device)
     B = f.shape[0]
     flm = torch.empty((B, L, 2 * L - 1), dtype=torch.complex128, device=f.device)
     work = torch.empty(plan.workspace_bytes(B), dtype=torch.uint8, device=f.device)
     f = f.contiguous()
-    _lib.check(_lib.lib().s2h_forward(plan.handle, f.data_ptr(), int(f.is_complex()),
-                                      flm.data_ptr(), work.data_ptr(), B, _stream(f.device)),
-               "forward")
+    _lib.check(_lib.lib().s2h_forward_ex(plan.handle, f.data_ptr(), int(f.is_complex()),
+                                         flm.data_ptr(), work.data_ptr(), B,
+                                         _precision(precision), _stream(f.device)), "forward")
     return flm
 
 

@@ -1,0 +1,91 @@
+"""Mixed-precision transforms (FP64 recursion, FP32 matrix-core projection).
+
+Tolerances are stated per test.  A single mixed-precision transform carries
+the FP32 rounding of the d-values and of the data plus FP32 accumulation over
+128 rings (forward) or 32 l (inverse): about 1e-7 of max |flm| at these sizes;
+the tests allow 2e-6 and also require the error to be above 1e-12, which
+shows that the FP32 path (not the FP64 one) ran.  Refinement with an FP64
+residual must reach the FP64 target of 1e-7 against the seeded coefficients.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import signals, transforms as otr
+from tests.conftest import max_rel
+
+from s2fft_amd.transforms import refine, spherical as sph
+
+pytestmark = pytest.mark.gpu
+MIXED_TOL = 2e-6
+
+
+def _flms(rng, B, L, spin, reality):
+    return np.stack([signals.generate_flm(rng, L, 0, spin, reality) for _ in range(B)])
+
+
+CASES = [("healpix", 16, 32, 0, True, 3), ("gl", None, 24, 2, False, 2),
+         ("mw", None, 16, 0, True, 5), ("mwss", None, 12, -1, False, 4),
+         ("dh", None, 10, 0, False, 1)]
+
+
+@pytest.mark.parametrize("sampling,nside,L,spin,reality,B", CASES)
+def test_mixed_inverse_vs_oracle(rng, sampling, nside, L, spin, reality, B):
+    flms = _flms(rng, B, L, spin, reality)
+    got = sph.inverse_hip(torch.from_numpy(flms).cuda(), L, spin, nside, sampling, reality,
+                          precision="mixed").cpu().numpy()
+    for i in range(B):
+        ref = otr.inverse(flms[i], L, spin, nside, sampling, reality)
+        if reality:
+            ref = np.real(ref)
+        err = max_rel(got[i], ref)
+        assert 1e-12 < err < MIXED_TOL, err
+
+
+@pytest.mark.parametrize("sampling,nside,L,spin,reality,B", CASES)
+def test_mixed_forward_vs_oracle(rng, sampling, nside, L, spin, reality, B):
+    flms = _flms(rng, B, L, spin, reality)
+    fs = np.stack([otr.inverse(flms[i], L, spin, nside, sampling, reality) for i in range(B)])
+    if reality:
+        fs = np.real(fs)
+    got = sph.forward_hip(torch.from_numpy(fs).cuda(), L, spin, nside, sampling, reality,
+                          precision="mixed").cpu().numpy()
+    for i in range(B):
+        ref = otr.forward(fs[i], L, spin, nside, sampling, reality)
+        err = max_rel(got[i], ref)
+        assert 1e-12 < err < MIXED_TOL, err
+
+
+def test_mixed_vs_fp64_larger(rng):
+    """nside 128, L 256, 4 real maps: mixed against the FP64 kernels."""
+    nside, L, B = 128, 256, 4
+    flm = torch.from_numpy(_flms(rng, B, L, 0, True)).cuda()
+    f64 = sph.inverse_hip(flm, L, 0, nside, "healpix", True)
+    fmx = sph.inverse_hip(flm, L, 0, nside, "healpix", True, precision="mixed")
+    e_inv = float((fmx - f64).abs().max() / f64.abs().max())
+    g64 = sph.forward_hip(f64, L, 0, nside, "healpix", True)
+    gmx = sph.forward_hip(f64, L, 0, nside, "healpix", True, precision="mixed")
+    e_fwd = float((gmx - g64).abs().max() / g64.abs().max())
+    print(f"mixed vs fp64 at nside {nside}: inverse {e_inv:.2e}, forward {e_fwd:.2e}")
+    assert 1e-12 < e_inv < MIXED_TOL and 1e-12 < e_fwd < MIXED_TOL
+
+
+def test_mixed_refinement_reaches_fp64_target(rng):
+    """HEALPix nside 64, L 128, 4 real maps: refine_mixed reaches 1e-7."""
+    nside, L, B = 64, 128, 4
+    flm_true = torch.from_numpy(_flms(rng, B, L, 0, True)).cuda()
+    f = sph.inverse_hip(flm_true, L, 0, nside, "healpix", True).real.contiguous()
+    hist_m, hist_d = [], []
+    refine.refine_mixed(f, L, 0, nside, "healpix", True, 10, history=hist_m, flm_true=flm_true)
+    flm = sph.forward_hip(f, L, 0, nside, "healpix", True)
+    for _ in range(10):
+        flm = flm + sph.forward_hip(f - sph.inverse_hip(flm, L, 0, nside, "healpix", True).real,
+                                    L, 0, nside, "healpix", True)
+        hist_d.append(float((flm - flm_true).abs().max() / flm_true.abs().max()))
+    print("mixed", ["%.1e" % e for e in hist_m])
+    print("fp64 ", ["%.1e" % e for e in hist_d])
+    assert min(hist_m) <= 1e-7
+    # the mixed iteration contracts like the FP64 one (at most one step more)
+    k = next(i for i, e in enumerate(hist_d) if e <= 1e-7)
+    assert min(hist_m[k + 1:k + 3]) <= 1e-7
