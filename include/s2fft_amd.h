@@ -170,6 +170,39 @@ enum s2h_kernel_kind {
 int s2h_profile_enable(int on);
 int s2h_profile_query(int kernel, double *total_ms, int *count);
 
+/* Learned preconditioner for the iterative forward transform (no
+ * counterpart in the reference, whose refinement is plain Jacobi,
+ * s2fft/utils/iterative_refinement.py:9-46).  A per-(l, m) MLP
+ *   (re, im, l/L, m/L) -> correction (re, im)
+ * with two hidden layers of 128 (GELU, tanh form), BF16-rounded weights and
+ * FP32 accumulation; (re, im) is the harmonic residual divided by its RMS
+ * over the map's coefficients, and the correction is scaled back by it.
+ * params: s2h_mlp_param_count() FP32 values, layout
+ *   W1[128][4] b1[128] W2[128][128] b2[128] W3[2][128] b3[2]. */
+int s2h_mlp_param_count(void);
+/* delta = correction for every packed coefficient (m in [0, l] with reality,
+ * whose m < 0 entries are filled by conjugate symmetry; else m in [-l, l]),
+ * zero for l < lmin.  h, delta: batch x (L, 2L-1) complex128, distinct;
+ * work: >= 8 * batch bytes. */
+int s2h_mlp_precondition(const float *params, const void *h, void *delta, int L, int reality,
+                         int lmin, int batch, void *work, s2h_stream stream);
+/* Training rows from a residual h and the error e it should predict:
+ * X[b*P + p] = (h/s, l/L, m/L) (4 floats), Y = e/s (2 floats), with P the
+ * packed rows per map; rows with l < lmin are zero.  work: >= 8 * batch. */
+int s2h_mlp_features(const void *h, const void *e, int L, int reality, int lmin, int batch,
+                     float *X, float *Y, void *work, s2h_stream stream);
+/* Gradient of sum_rows w_r |y(X_r) - Y_r|^2 * loss_scale over n rows
+ * (row_weight may be NULL = 1) into grad (param layout); *loss (device
+ * double, may be NULL) = sum_rows w_r |y - Y|^2.  nwg workgroups; work:
+ * s2h_mlp_grad_workspace_bytes(nwg) bytes.  Deterministic for fixed nwg. */
+size_t s2h_mlp_grad_workspace_bytes(int nwg);
+int s2h_mlp_grad(const float *params, const float *X, const float *Y, const float *row_weight,
+                 int64_t n, float loss_scale, float *grad, double *loss, void *work, int nwg,
+                 s2h_stream stream);
+/* One Adam step (step >= 1) on the FP32 master parameters. */
+int s2h_mlp_adam(float *params, const float *grad, float *m1, float *m2, int step, float lr,
+                 float beta1, float beta2, float eps, s2h_stream stream);
+
 /* Diagnostics (no counterpart in the reference): one V_MFMA_F64_16X16X4F64
  * with a[lane], b[lane] (64 doubles each) as the A and B operands and a zero
  * accumulator; out[4*lane + r] = result register r of lane (256 doubles).
@@ -177,6 +210,9 @@ int s2h_profile_query(int kernel, double *total_ms, int *count);
 int s2h_probe_mfma_f64(const double *a, const double *b, double *out, s2h_stream stream);
 /* The same for V_MFMA_F32_16X16X4F32 (64 floats in each of a, b; 256 out). */
 int s2h_probe_mfma_f32(const float *a, const float *b, float *out, s2h_stream stream);
+/* The same for V_MFMA_F32_16X16X32_BF16: a, b hold 8 bf16 bit patterns per
+ * lane (lane-major, 512 each); out 256 floats. */
+int s2h_probe_mfma_bf16(const uint16_t *a, const uint16_t *b, float *out, s2h_stream stream);
 
 #ifdef __cplusplus
 }

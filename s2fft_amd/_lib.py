@@ -44,6 +44,8 @@ EXPORTED = (
     "s2h_profile_enable", "s2h_profile_query", "s2h_wigner_reality_fill",
     "s2h_forward_adjoint", "s2h_inverse_adjoint", "s2h_probe_mfma_f64",
     "s2h_probe_mfma_f32", "s2h_forward_ex", "s2h_inverse_ex",
+    "s2h_mlp_param_count", "s2h_mlp_precondition", "s2h_mlp_features",
+    "s2h_mlp_grad_workspace_bytes", "s2h_mlp_grad", "s2h_mlp_adam", "s2h_probe_mfma_bf16",
 )
 
 KERNEL_LAT_FORWARD, KERNEL_LAT_INVERSE, KERNEL_RING_FFT_FORWARD = 0, 1, 2
@@ -67,6 +69,14 @@ def _declare(lib):
         "s2h_forward_adjoint": (i32, [vp, vp, vp, vp, i32, vp]),
         "s2h_probe_mfma_f64": (i32, [vp, vp, vp, vp]),
         "s2h_probe_mfma_f32": (i32, [vp, vp, vp, vp]),
+        "s2h_probe_mfma_bf16": (i32, [vp, vp, vp, vp]),
+        "s2h_mlp_param_count": (i32, []),
+        "s2h_mlp_precondition": (i32, [vp, vp, vp, i32, i32, i32, i32, vp, vp]),
+        "s2h_mlp_features": (i32, [vp, vp, i32, i32, i32, i32, vp, vp, vp, vp]),
+        "s2h_mlp_grad_workspace_bytes": (sz, [i32]),
+        "s2h_mlp_grad": (i32, [vp, vp, vp, vp, i64, ctypes.c_float, vp, vp, vp, i32, vp]),
+        "s2h_mlp_adam": (i32, [vp, vp, vp, vp, i32, ctypes.c_float, ctypes.c_float,
+                               ctypes.c_float, ctypes.c_float, vp]),
         "s2h_inverse_adjoint": (i32, [vp, vp, i32, vp, vp, i32, vp]),
         "s2h_lat_forward": (i32, [vp, vp, vp, i32, vp]),
         "s2h_lat_inverse": (i32, [vp, vp, vp, i32, vp]),

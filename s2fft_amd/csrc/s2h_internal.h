@@ -11,6 +11,9 @@ namespace s2h {
 constexpr int kThreads = 256;        // workgroup size of every kernel here
 constexpr int kMaxRingLen = 4096;    // longest ring FFT held in LDS (Stockham)
 constexpr int kMaxConvLen = 8192;    // longest Bluestein convolution held in LDS
+constexpr int kMlpHidden = 128;       // learned preconditioner: hidden width
+constexpr int kMlpParams = kMlpHidden * 4 + kMlpHidden + kMlpHidden * kMlpHidden + kMlpHidden +
+                           2 * kMlpHidden + 2;  // W1 b1 W2 b2 W3 b3
 
 // One ring FFT job: a contiguous run of nphi pixels.
 struct RingJob {

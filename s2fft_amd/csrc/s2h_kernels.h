@@ -79,6 +79,28 @@ hipError_t launch_mfma_f64_probe(const double* a, const double* b, double* out, 
 hipError_t launch_lat_inverse_mma(const LatArgs& a, bool paired, bool fp32, hipStream_t s);
 hipError_t launch_mfma_f32_probe(const float* a, const float* b, float* out, hipStream_t s);
 
+// learned preconditioner (per-(l, m) rows of one or more flm arrays)
+struct MlpRowArgs {
+  const cplx* h;       // batch x (L, 2L-1)
+  cplx* out;           // batch x (L, 2L-1) (apply)
+  int L, reality, lmin, batch;
+  int64_t P;           // packed rows per map: L(L+1)/2 (reality) or L^2
+  int64_t count;       // rows with l >= lmin (for the RMS)
+  int64_t map_stride;  // L (2L-1)
+};
+hipError_t launch_mlp_sumsq(const MlpRowArgs& a, double* sumsq, hipStream_t s);
+hipError_t launch_mlp_apply(const MlpRowArgs& a, const float* w, const double* sumsq,
+                            hipStream_t s);
+hipError_t launch_mlp_features(const MlpRowArgs& a, const cplx* e, const double* sumsq, float* X,
+                               float* Y, hipStream_t s);
+hipError_t launch_mlp_grad(const float* w, const float* X, const float* Y, const float* rw,
+                           int64_t n, float loss_scale, int nwg, float* partial,
+                           double* loss_partial, float* grad, double* loss, hipStream_t s);
+hipError_t launch_mlp_adam(float* w, const float* grad, float* m1, float* m2, int step, float lr,
+                           float beta1, float beta2, float eps, hipStream_t s);
+hipError_t launch_mfma_bf16_probe(const uint16_t* a, const uint16_t* b, float* out,
+                                  hipStream_t s);
+
 // Small fix-up kernels (s2h_misc.hip).
 struct PoleArgs {
   // inverse: set ftm rows of the poles; forward: add pole terms into flm

@@ -1,0 +1,519 @@
+// Learned preconditioner for the iterative forward transform: a per-(l, m)
+// MLP  (re, im, l/L, m/L) -> correction (re, im),  two hidden layers of 128
+// with GELU (tanh form), BF16 weights and FP32 accumulation.
+//
+//   z1 = W1q x + b1,  a1 = gelu(z1),  z2 = W2q bf16(a1) + b2,
+//   a2 = gelu(z2),    y  = W3q a2 + b3,           Wq = bf16(W)
+//
+// x = (h_re / s, h_im / s, l / L, m / L) for the harmonic residual h of a map
+// whose RMS over its coefficients is s; the correction is s * y.  Inference
+// runs layer 2 on the BF16 matrix cores (V_MFMA_F32_16X16X32_BF16); training
+// (forward + backward, FP32 on the vector ALUs) accumulates per-workgroup
+// gradient partials that a second kernel reduces in a fixed order, and Adam
+// updates the FP32 master weights.  The backward treats the BF16 roundings
+// as the identity (straight-through).
+#include "s2h_internal.h"
+#include "s2h_kernels.h"
+
+namespace s2h {
+
+namespace {
+
+constexpr int kH = kMlpHidden;  // 128
+
+// master parameter offsets (floats)
+constexpr int oW1 = 0;               // [128][4]
+constexpr int oB1 = oW1 + kH * 4;    // [128]
+constexpr int oW2 = oB1 + kH;        // [128 out][128 in]
+constexpr int oB2 = oW2 + kH * kH;   // [128]
+constexpr int oW3 = oB2 + kH;        // [2][128]
+constexpr int oB3 = oW3 + 2 * kH;    // [2]
+static_assert(oB3 + 2 == kMlpParams, "parameter layout");
+
+__device__ __forceinline__ float bf16_round(float v) {
+  return (float)(__bf16)v;
+}
+
+// GELU, tanh form: 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3)))
+//                = x / (1 + exp(-2 sqrt(2/pi) (x + 0.044715 x^3)))
+__device__ __forceinline__ float gelu(float x) {
+  const float u = 1.5957691216057308f * fmaf(0.044715f * x, x * x, x);
+  return x / (1.0f + __expf(-u));
+}
+// d gelu / dx for the same form: sig + x sig (1 - sig) u'
+__device__ __forceinline__ float gelu_grad(float x) {
+  const float u = 1.5957691216057308f * fmaf(0.044715f * x, x * x, x);
+  const float sg = 1.0f / (1.0f + __expf(-u));
+  const float du = 1.5957691216057308f * fmaf(3.0f * 0.044715f, x * x, 1.0f);
+  return fmaf(x * sg * (1.0f - sg), du, sg);
+}
+
+// packed (l, m) enumeration: reality -> m in [0, l], P = L(L+1)/2;
+// else m in [-l, l], P = L^2
+__device__ __forceinline__ void lm_of(int64_t p, int reality, int& l, int& m) {
+  if (reality) {
+    int ll = (int)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
+    while ((int64_t)(ll + 1) * (ll + 2) / 2 <= p) ++ll;
+    while ((int64_t)ll * (ll + 1) / 2 > p) --ll;
+    l = ll;
+    m = (int)(p - (int64_t)ll * (ll + 1) / 2);
+  } else {
+    int ll = (int)sqrt((double)p);
+    while ((int64_t)(ll + 1) * (ll + 1) <= p) ++ll;
+    while ((int64_t)ll * ll > p) --ll;
+    l = ll;
+    m = (int)(p - (int64_t)ll * ll - ll);
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- statistics
+// sumsq[b] += sum |h[b, l, m]|^2 over the packed coefficients with l >= lmin
+__global__ void __launch_bounds__(kThreads) mlp_sumsq_kernel(MlpRowArgs a, double* sumsq) {
+  const int b = blockIdx.y;
+  __shared__ double red[kThreads / 64];
+  double acc = 0.0;
+  for (int64_t p = (int64_t)blockIdx.x * kThreads + threadIdx.x; p < a.P;
+       p += (int64_t)gridDim.x * kThreads) {
+    int l, m;
+    lm_of(p, a.reality, l, m);
+    if (l < a.lmin) continue;
+    const cplx v = a.h[(int64_t)b * a.map_stride + (int64_t)l * (2 * a.L - 1) + a.L - 1 + m];
+    acc += v.x * v.x + v.y * v.y;
+  }
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kThreads / 64; ++w) t += red[w];
+    atomicAdd(sumsq + b, t);
+  }
+}
+
+// ---------------------------------------------------------------- inference
+// One wave handles 16 rows per pass: layer 1 on the vector ALUs straight into
+// the A-operand layout of the BF16 MFMA (A[i][k]: lane i + 16 (k/8 % 4),
+// element k % 8 of the 8-vector, for each 32-k block), layer 2 as 8 column
+// tiles x 4 k-blocks of V_MFMA_F32_16X16X32_BF16 with W2q staged in LDS in
+// the B-operand layout, layer 3 as a 16-lane shuffle reduction.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kInfWaves = 4;
+
+__global__ void __launch_bounds__(kInfWaves * 64) mlp_apply_kernel(MlpRowArgs a,
+                                                                   const float* __restrict__ w,
+                                                                   const double* sumsq) {
+  // B operand of column tile ct, k-block kb for lane l: W2q[16 ct + l%16][32 kb + 8 (l/16) + e]
+  __shared__ bf16x8 wb[8][4][64];
+  __shared__ float w1[kH * 4], b1[kH], b2[kH], w3[2 * kH];
+  __shared__ float b3[2];
+  for (int e = threadIdx.x; e < 8 * 4 * 64; e += kInfWaves * 64) {
+    const int ct = e >> 8, kb = (e >> 6) & 3, l = e & 63;
+    const int j = 16 * ct + (l & 15), k0 = 32 * kb + 8 * (l >> 4);
+    bf16x8 v;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = (__bf16)w[oW2 + j * kH + k0 + q];
+    wb[ct][kb][l] = v;
+  }
+  for (int e = threadIdx.x; e < kH * 4; e += kInfWaves * 64) w1[e] = bf16_round(w[oW1 + e]);
+  for (int e = threadIdx.x; e < kH; e += kInfWaves * 64) {
+    b1[e] = w[oB1 + e];
+    b2[e] = w[oB2 + e];
+  }
+  for (int e = threadIdx.x; e < 2 * kH; e += kInfWaves * 64) w3[e] = bf16_round(w[oW3 + e]);
+  if (threadIdx.x < 2) b3[threadIdx.x] = w[oB3 + threadIdx.x];
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int b = blockIdx.y;
+  const double cnt = (double)a.count;
+  const double sig = sumsq[b] > 0.0 ? sqrt(sumsq[b] / cnt) : 1.0;
+  const float inv_sig = (float)(1.0 / sig);
+  const float inv_L = 1.0f / (float)a.L;
+  const cplx* __restrict__ h = a.h + (int64_t)b * a.map_stride;
+  cplx* __restrict__ out = a.out + (int64_t)b * a.map_stride;
+  const int row = lane & 15, kq = lane >> 4;
+
+  const int64_t ntile = (a.P + 15) / 16;
+  for (int64_t tile = (int64_t)blockIdx.x * kInfWaves + wave; tile < ntile;
+       tile += (int64_t)gridDim.x * kInfWaves) {
+    // features of row `row` (every lane of the 16-lane group computes them)
+    const int64_t p = tile * 16 + row;
+    int l = 0, m = 0;
+    float x0 = 0.f, x1 = 0.f, x2 = 0.f, x3 = 0.f;
+    bool valid = false;
+    if (p < a.P) {
+      lm_of(p, a.reality, l, m);
+      valid = l >= a.lmin;
+      if (valid) {
+        const cplx v = h[(int64_t)l * (2 * a.L - 1) + a.L - 1 + m];
+        x0 = (float)v.x * inv_sig;
+        x1 = (float)v.y * inv_sig;
+        x2 = (float)l * inv_L;
+        x3 = (float)m * inv_L;
+      }
+    }
+    f32x4 acc[8];
+#pragma unroll
+    for (int ct = 0; ct < 8; ++ct) acc[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) {
+      bf16x8 av;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int k = 32 * kb + 8 * kq + q;
+        const float z = fmaf(w1[4 * k + 3], x3,
+                             fmaf(w1[4 * k + 2], x2,
+                                  fmaf(w1[4 * k + 1], x1, fmaf(w1[4 * k], x0, b1[k]))));
+        av[q] = (__bf16)gelu(z);
+      }
+#pragma unroll
+      for (int ct = 0; ct < 8; ++ct)
+        acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, wb[ct][kb][lane], acc[ct], 0, 0, 0);
+    }
+    // result: register r of lane holds row 4 kq + r, column 16 ct + row
+    float y0[4] = {0.f, 0.f, 0.f, 0.f}, y1[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ct = 0; ct < 8; ++ct) {
+      const int j = 16 * ct + row;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float a2 = gelu(acc[ct][r] + b2[j]);
+        y0[r] = fmaf(w3[j], a2, y0[r]);
+        y1[r] = fmaf(w3[kH + j], a2, y1[r]);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+#pragma unroll
+      for (int o = 8; o > 0; o >>= 1) {
+        y0[r] += __shfl_xor(y0[r], o);
+        y1[r] += __shfl_xor(y1[r], o);
+      }
+    }
+    // lane (kq, row = r) writes row 4 kq + r: pick register r == row
+    if (row < 4) {
+      float o0 = y0[0], o1 = y1[0];
+#pragma unroll
+      for (int r = 1; r < 4; ++r)
+        if (row == r) {
+          o0 = y0[r];
+          o1 = y1[r];
+        }
+      const int64_t q = tile * 16 + 4 * kq + row;
+      if (q < a.P) {
+        int lq, mq;
+        lm_of(q, a.reality, lq, mq);
+        const int64_t idx = (int64_t)lq * (2 * a.L - 1) + a.L - 1 + mq;
+        if (lq < a.lmin) {
+          out[idx] = cplx{0.0, 0.0};
+        } else {
+          double re = sig * (double)(o0 + b3[0]);
+          double im = sig * (double)(o1 + b3[1]);
+          if (a.reality && mq == 0) im = 0.0;
+          out[idx] = cplx{re, im};
+          if (a.reality && mq > 0) {
+            const double sg = (mq & 1) ? -1.0 : 1.0;
+            out[(int64_t)lq * (2 * a.L - 1) + a.L - 1 - mq] = cplx{sg * re, -sg * im};
+          }
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- features
+// Training rows from (h, e): X = (h / s, l / L, m / L), Y = e / s, one row
+// per packed coefficient with l >= lmin (others get a zero row and weight 0).
+__global__ void __launch_bounds__(kThreads) mlp_features_kernel(MlpRowArgs a, const cplx* e,
+                                                                const double* sumsq, float* X,
+                                                                float* Y) {
+  const int b = blockIdx.y;
+  const double sig = sumsq[b] > 0.0 ? sqrt(sumsq[b] / (double)a.count) : 1.0;
+  for (int64_t p = (int64_t)blockIdx.x * kThreads + threadIdx.x; p < a.P;
+       p += (int64_t)gridDim.x * kThreads) {
+    int l, m;
+    lm_of(p, a.reality, l, m);
+    const int64_t row = (int64_t)b * a.P + p;
+    float4 xv = {0.f, 0.f, 0.f, 0.f};
+    float2 yv = {0.f, 0.f};
+    if (l >= a.lmin) {
+      const int64_t idx = (int64_t)b * a.map_stride + (int64_t)l * (2 * a.L - 1) + a.L - 1 + m;
+      const cplx hv = a.h[idx], ev = e[idx];
+      xv = float4{(float)(hv.x / sig), (float)(hv.y / sig), (float)l / a.L, (float)m / a.L};
+      yv = float2{(float)(ev.x / sig), (float)(ev.y / sig)};
+    }
+    reinterpret_cast<float4*>(X)[row] = xv;
+    reinterpret_cast<float2*>(Y)[row] = yv;
+  }
+}
+
+// ---------------------------------------------------------------- training
+// 256 threads, rows in blocks of 16.  Thread t owns hidden unit k = t % 128
+// for rows r = 2i + t/128 (i < 8) in layers 1 and 2, and the 64 entries
+// (j, k) = (2i + t/128, t % 128), i < 64, of dW2.
+constexpr int kTR = 16;            // rows per block
+constexpr int kTT = 256;           // threads
+constexpr int kW2S = kH + 1;       // padded row stride of W2 in LDS
+
+__global__ void __launch_bounds__(kTT) mlp_grad_kernel(const float* __restrict__ w,
+                                                       const float* __restrict__ X,
+                                                       const float* __restrict__ Y,
+                                                       const float* __restrict__ rw,
+                                                       int64_t n, float loss_scale,
+                                                       float* __restrict__ partial,
+                                                       double* __restrict__ loss_partial) {
+  __shared__ float W2s[kH * kW2S];  // W2q[j][k] at j * kW2S + k
+  __shared__ float W1s[kH * 4], B1s[kH], B2s[kH], W3s[2 * kH], B3s[2];
+  __shared__ float xs[kTR][4], a1s[kTR][kW2S], a2s[kTR][kW2S], dz2s[kTR][kW2S];
+  __shared__ float douts[kTR][2];
+  __shared__ double lred[kTT / 64];
+  const int t = threadIdx.x;
+  for (int e = t; e < kH * kH; e += kTT) W2s[(e / kH) * kW2S + (e % kH)] = bf16_round(w[oW2 + e]);
+  for (int e = t; e < kH * 4; e += kTT) W1s[e] = bf16_round(w[oW1 + e]);
+  for (int e = t; e < kH; e += kTT) {
+    B1s[e] = w[oB1 + e];
+    B2s[e] = w[oB2 + e];
+  }
+  for (int e = t; e < 2 * kH; e += kTT) W3s[e] = bf16_round(w[oW3 + e]);
+  if (t < 2) B3s[t] = w[oB3 + t];
+  __syncthreads();
+
+  const int k = t % kH, half = t / kH;
+  float gW2[64];
+#pragma unroll
+  for (int i = 0; i < 64; ++i) gW2[i] = 0.f;
+  float gW1[4] = {0.f, 0.f, 0.f, 0.f}, gB1 = 0.f, gB2 = 0.f, gW3[2] = {0.f, 0.f};
+  float gB3[2] = {0.f, 0.f};
+  double lsum = 0.0;
+
+  const int64_t nblk = (n + kTR - 1) / kTR;
+  for (int64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    const int64_t r0 = blk * kTR;
+    if (t < kTR * 4) {
+      const int r = t / 4, f = t % 4;
+      xs[r][f] = (r0 + r < n) ? X[(r0 + r) * 4 + f] : 0.f;
+    }
+    __syncthreads();
+    // layer 1 (unit k, rows 2i + half)
+    float z1[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int r = 2 * i + half;
+      z1[i] = fmaf(W1s[4 * k + 3], xs[r][3],
+                   fmaf(W1s[4 * k + 2], xs[r][2],
+                        fmaf(W1s[4 * k + 1], xs[r][1], fmaf(W1s[4 * k], xs[r][0], B1s[k]))));
+      a1s[r][k] = bf16_round(gelu(z1[i]));
+    }
+    __syncthreads();
+    // layer 2 (unit j = k)
+    float z2[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int r = 2 * i + half;
+      float s = B2s[k];
+#pragma unroll 8
+      for (int q = 0; q < kH; ++q) s = fmaf(W2s[k * kW2S + q], a1s[r][q], s);
+      z2[i] = s;
+      a2s[r][k] = gelu(s);
+    }
+    __syncthreads();
+    // layer 3 and the loss: threads 0..31 -> (row t/2, output t%2)
+    if (t < 2 * kTR) {
+      const int r = t >> 1, o = t & 1;
+      float s = B3s[o];
+      for (int q = 0; q < kH; ++q) s = fmaf(W3s[o * kH + q], a2s[r][q], s);
+      float d = 0.f;
+      if (r0 + r < n) {
+        const float wt = rw ? rw[r0 + r] : 1.0f;
+        const float diff = s - Y[(r0 + r) * 2 + o];
+        d = 2.0f * diff * wt * loss_scale;
+        lsum += (double)(diff * diff * wt);
+      }
+      douts[r][o] = d;
+    }
+    __syncthreads();
+    // backward through layer 3 and gelu(z2)
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int r = 2 * i + half;
+      const float d0 = douts[r][0], d1 = douts[r][1];
+      const float a2 = a2s[r][k];
+      gW3[0] = fmaf(d0, a2, gW3[0]);
+      gW3[1] = fmaf(d1, a2, gW3[1]);
+      const float dz = (d0 * W3s[k] + d1 * W3s[kH + k]) * gelu_grad(z2[i]);
+      dz2s[r][k] = dz;
+      gB2 += dz;
+    }
+    if (t < 2) {
+#pragma unroll
+      for (int r = 0; r < kTR; ++r) gB3[t] += douts[r][t];
+    }
+    __syncthreads();
+    // dW2[j][k] += dz2[r][j] a1q[r][k] for the owned (j, k) = (2i + half, k)
+#pragma unroll 1
+    for (int r = 0; r < kTR; ++r) {
+      const float a1 = a1s[r][k];
+#pragma unroll
+      for (int i = 0; i < 64; ++i) gW2[i] = fmaf(dz2s[r][2 * i + half], a1, gW2[i]);
+    }
+    // da1[r][k] = sum_j dz2[r][j] W2q[j][k]; back through gelu(z1)
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int r = 2 * i + half;
+      float s = 0.f;
+#pragma unroll 8
+      for (int j = 0; j < kH; ++j) s = fmaf(dz2s[r][j], W2s[j * kW2S + k], s);
+      const float dz1 = s * gelu_grad(z1[i]);
+      gB1 += dz1;
+#pragma unroll
+      for (int f = 0; f < 4; ++f) gW1[f] = fmaf(dz1, xs[r][f], gW1[f]);
+    }
+    __syncthreads();
+  }
+
+  // per-workgroup partials: the two halves of a unit combine through LDS
+  float* pw = partial + (int64_t)blockIdx.x * kMlpParams;
+#pragma unroll
+  for (int i = 0; i < 64; ++i) pw[oW2 + (2 * i + half) * kH + k] = gW2[i];
+  __shared__ float comb[kH][8];
+  if (half == 1) {
+    comb[k][0] = gW1[0];
+    comb[k][1] = gW1[1];
+    comb[k][2] = gW1[2];
+    comb[k][3] = gW1[3];
+    comb[k][4] = gB1;
+    comb[k][5] = gB2;
+    comb[k][6] = gW3[0];
+    comb[k][7] = gW3[1];
+  }
+  __syncthreads();
+  if (half == 0) {
+#pragma unroll
+    for (int f = 0; f < 4; ++f) pw[oW1 + 4 * k + f] = gW1[f] + comb[k][f];
+    pw[oB1 + k] = gB1 + comb[k][4];
+    pw[oB2 + k] = gB2 + comb[k][5];
+    pw[oW3 + k] = gW3[0] + comb[k][6];
+    pw[oW3 + kH + k] = gW3[1] + comb[k][7];
+  }
+  if (t < 2) pw[oB3 + t] = gB3[t];
+  for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o);
+  if ((t & 63) == 0) lred[t >> 6] = lsum;
+  __syncthreads();
+  if (t == 0) {
+    double s = 0.0;
+    for (int q = 0; q < kTT / 64; ++q) s += lred[q];
+    loss_partial[blockIdx.x] = s;
+  }
+}
+
+// grad[p] = sum over workgroups of partial[wg][p] (fixed order, FP64 sum)
+__global__ void __launch_bounds__(kThreads) mlp_grad_reduce_kernel(const float* partial, int nwg,
+                                                                   float* grad,
+                                                                   const double* loss_partial,
+                                                                   double* loss) {
+  const int p = blockIdx.x * kThreads + threadIdx.x;
+  if (p < kMlpParams) {
+    double s = 0.0;
+    for (int g = 0; g < nwg; ++g) s += (double)partial[(int64_t)g * kMlpParams + p];
+    grad[p] = (float)s;
+  }
+  if (p == 0 && loss) {
+    double s = 0.0;
+    for (int g = 0; g < nwg; ++g) s += loss_partial[g];
+    *loss = s;
+  }
+}
+
+// Adam (Kingma & Ba 2015) on the FP32 master weights.
+__global__ void __launch_bounds__(kThreads) mlp_adam_kernel(float* w, const float* grad, float* m1,
+                                                            float* m2, int step, float lr,
+                                                            float beta1, float beta2, float eps) {
+  const int p = blockIdx.x * kThreads + threadIdx.x;
+  if (p >= kMlpParams) return;
+  const float g = grad[p];
+  const float a = beta1 * m1[p] + (1.0f - beta1) * g;
+  const float v = beta2 * m2[p] + (1.0f - beta2) * g * g;
+  m1[p] = a;
+  m2[p] = v;
+  const float c1 = 1.0f - powf(beta1, (float)step);
+  const float c2 = 1.0f - powf(beta2, (float)step);
+  w[p] -= lr * (a / c1) / (sqrtf(v / c2) + eps);
+}
+
+// ---------------------------------------------------------------- launchers
+hipError_t launch_mlp_sumsq(const MlpRowArgs& a, double* sumsq, hipStream_t s) {
+  if (a.batch == 0 || a.P == 0) return hipSuccess;
+  hipError_t e = launch_zero(sumsq, a.batch, s);
+  if (e != hipSuccess) return e;
+  const int64_t want = (a.P + kThreads - 1) / kThreads;
+  dim3 grid((unsigned)(want < 1024 ? want : 1024), a.batch);
+  hipLaunchKernelGGL(mlp_sumsq_kernel, grid, dim3(kThreads), 0, s, a, sumsq);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_apply(const MlpRowArgs& a, const float* w, const double* sumsq,
+                            hipStream_t s) {
+  if (a.batch == 0 || a.P == 0) return hipSuccess;
+  const int64_t ntile = (a.P + 15) / 16;
+  const int64_t want = (ntile + kInfWaves - 1) / kInfWaves;
+  dim3 grid((unsigned)(want < 2048 ? want : 2048), a.batch);
+  hipLaunchKernelGGL(mlp_apply_kernel, grid, dim3(kInfWaves * 64), 0, s, a, w, sumsq);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_features(const MlpRowArgs& a, const cplx* e, const double* sumsq, float* X,
+                               float* Y, hipStream_t s) {
+  if (a.batch == 0 || a.P == 0) return hipSuccess;
+  const int64_t want = (a.P + kThreads - 1) / kThreads;
+  dim3 grid((unsigned)(want < 4096 ? want : 4096), a.batch);
+  hipLaunchKernelGGL(mlp_features_kernel, grid, dim3(kThreads), 0, s, a, e, sumsq, X, Y);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_grad(const float* w, const float* X, const float* Y, const float* rw,
+                           int64_t n, float loss_scale, int nwg, float* partial,
+                           double* loss_partial, float* grad, double* loss, hipStream_t s) {
+  if (nwg < 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(mlp_grad_kernel, dim3(nwg), dim3(kTT), 0, s, w, X, Y, rw, n, loss_scale,
+                     partial, loss_partial);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(mlp_grad_reduce_kernel, dim3((kMlpParams + kThreads - 1) / kThreads),
+                     dim3(kThreads), 0, s, partial, nwg, grad, loss_partial, loss);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_adam(float* w, const float* grad, float* m1, float* m2, int step, float lr,
+                           float beta1, float beta2, float eps, hipStream_t s) {
+  hipLaunchKernelGGL(mlp_adam_kernel, dim3((kMlpParams + kThreads - 1) / kThreads),
+                     dim3(kThreads), 0, s, w, grad, m1, m2, step, lr, beta1, beta2, eps);
+  return hipGetLastError();
+}
+
+// One V_MFMA_F32_16X16X32_BF16 on caller operands (64 x 8 bf16 bit patterns
+// each), for checking the lane layouts: out[4 lane + r] = register r.
+__global__ void mfma_bf16_probe_kernel(const uint16_t* a, const uint16_t* b, float* out) {
+  const int lane = threadIdx.x;
+  bf16x8 av, bv;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    uint16_t ua = a[lane * 8 + q], ub = b[lane * 8 + q];
+    av[q] = __builtin_bit_cast(__bf16, ua);
+    bv[q] = __builtin_bit_cast(__bf16, ub);
+  }
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
+  for (int r = 0; r < 4; ++r) out[lane * 4 + r] = acc[r];
+}
+
+hipError_t launch_mfma_bf16_probe(const uint16_t* a, const uint16_t* b, float* out,
+                                  hipStream_t s) {
+  hipLaunchKernelGGL(mfma_bf16_probe_kernel, dim3(1), dim3(64), 0, s, a, b, out);
+  return hipGetLastError();
+}
+
+}  // namespace s2h
