@@ -30,14 +30,16 @@ def refine_fp64(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=No
 
 
 def refine_mixed(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=None,
-                 precond=None, refresh_every=0, history=None, flm_true=None):
+                 precond=None, refresh_every=0, history=None, flm_true=None, stats=None):
     """Mixed-precision refinement with an FP64 residual (see module docstring).
 
-    precond: None (Jacobi) or a callable (h, r, flm) -> delta.
+    precond: None (Jacobi) or an object with ``apply(h, L, reality, lmin)``
+    returning a correction (the learned preconditioner); its steps are judged
+    per map against the residual (``_judged_step``).
     refresh_every: recompute the residual with the FP64 synthesis every this
     many steps (0 = only at the start).
     history: optional list that receives the max-relative error per step when
-    ``flm_true`` is given.
+    ``flm_true`` is given.  stats: optional dict counting accepted learned steps.
     """
     s = sampling.lower()
     plan = plan or sph.get_plan(s, L, nside, spin, L_lower, reality, device=f.device)
@@ -48,15 +50,51 @@ def refine_mixed(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=N
     _track(history, flm, flm_true)
     for k in range(n_iter):
         h = sph.forward_hip(r, precision="mixed", **kw)
-        delta = h if precond is None else precond(h, r, flm)
-        flm = flm + delta
-        if k + 1 < n_iter:
-            if refresh_every and (k + 1) % refresh_every == 0:
-                r = f - _real_if(sph.inverse_hip(flm, **kw), f)
-            else:
+        last = k + 1 == n_iter
+        if precond is None:
+            delta = h
+            if not last:
                 r = r - _real_if(sph.inverse_hip(delta, precision="mixed", **kw), f)
+        else:
+            delta, r = _judged_step(precond, h, r, f, kw, stats)
+        flm = flm + delta
+        if not last and refresh_every and (k + 1) % refresh_every == 0:
+            r = f - _real_if(sph.inverse_hip(flm, **kw), f)
         _track(history, flm, flm_true)
     return flm
+
+
+def _judged_step(precond, h, r, f, kw, stats):
+    """Learned step per map where it lowers the FP64-held residual, else Jacobi.
+
+    The candidate delta = P(h) moves the residual to r - I~(delta); a map keeps
+    it only if that residual's 2-norm is below the current one.  Rejected maps
+    take the Jacobi step delta = h instead (one more mixed synthesis, for
+    those maps only).
+    """
+    L = kw["L"]
+    lmin = max(abs(kw["spin"]), kw["L_lower"])
+    delta = precond.apply(h, L, kw["reality"], lmin)
+    r_new = r - _real_if(sph.inverse_hip(delta, precision="mixed", **kw), f)
+    dims = tuple(range(1, r.dim()))
+    ok = _norm(r_new, dims) < _norm(r, dims)
+    if stats is not None:
+        stats["accepted"] = stats.get("accepted", 0) + int(ok.sum())
+        stats["steps"] = stats.get("steps", 0) + int(ok.numel())
+    if bool(ok.all()):
+        return delta, r_new
+    bad = (~ok).nonzero().flatten()
+    hj = h[bad].contiguous()
+    rj = r[bad] - _real_if(sph.inverse_hip(hj, precision="mixed", **kw), f)
+    delta = delta.clone()
+    r_new = r_new.clone()
+    delta[bad] = hj
+    r_new[bad] = rj
+    return delta, r_new
+
+
+def _norm(x, dims):
+    return (x.abs() ** 2).sum(dim=dims)
 
 
 def _real_if(x, like):
