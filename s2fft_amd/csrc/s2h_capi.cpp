@@ -165,7 +165,6 @@ extern "C" int s2h_forward_ex(const s2h_plan* p, const void* f, int f_is_complex
   if (resample) ftm = spec + (int64_t)batch * p->fwd_in_rows * p->Mc;
 
   RingFwdArgs r{};
-  r.jobs = p->fwd_jobs;
   r.f = f;
   r.f_complex = f_is_complex;
   r.real_only_2L = 0;
@@ -190,7 +189,7 @@ extern "C" int s2h_forward_ex(const s2h_plan* p, const void* f, int f_is_complex
   r.chirp_pool = p->chirp;
   {
     ProfScope ps(S2H_KERNEL_RING_FFT_FORWARD, s);
-    if ((st = hip_status(launch_ring_fft_forward(r, p->n_fwd_jobs, batch, p->fwd_lds, s)))) return st;
+    if ((st = hip_status(launch_ring_fft_forward(r, p->fwd_set, batch, s)))) return st;
   }
   const int64_t ftm_stride = (int64_t)p->fwd_rows * p->Mc;
   if (resample) {
@@ -314,7 +313,6 @@ extern "C" int s2h_inverse_ex(const s2h_plan* p, const void* flm, void* f, void*
   }
 
   RingInvArgs r{};
-  r.jobs = p->inv_jobs;
   r.in = ftm;
   r.in_map_stride = ftm_stride;
   r.ncol = p->Mc;
@@ -330,7 +328,7 @@ extern "C" int s2h_inverse_ex(const s2h_plan* p, const void* flm, void* f, void*
   r.chirp_pool = p->chirp;
   r.L = L;
   ProfScope ps(S2H_KERNEL_RING_FFT_INVERSE, s);
-  return hip_status(launch_ring_fft_inverse(r, p->n_inv_jobs, batch, p->inv_lds, s));
+  return hip_status(launch_ring_fft_inverse(r, p->inv_set, batch, s));
 }
 
 // Adjoint of s2h_forward (the VJP of a complex-linear map, torch convention):
@@ -414,7 +412,6 @@ extern "C" int s2h_forward_adjoint(const s2h_plan* p, const void* g, void* grad_
   }
 
   RingInvArgs r{};
-  r.jobs = p->fwd_jobs;
   r.in = resample ? spec : ftm;
   r.in_map_stride = in_stride;
   r.ncol = p->Mc;
@@ -429,7 +426,7 @@ extern "C" int s2h_forward_adjoint(const s2h_plan* p, const void* g, void* grad_
   r.map_stride = p->npix;
   r.chirp_pool = p->chirp;
   r.L = L;
-  return hip_status(launch_ring_fft_inverse(r, p->n_fwd_jobs, batch, p->fwd_lds, s));
+  return hip_status(launch_ring_fft_inverse(r, p->fwd_set, batch, s));
 }
 
 // Adjoint of s2h_inverse: grad_flm = I^H g = sign N_l Lat_fwd . (e^{-im phi0})
@@ -452,7 +449,6 @@ extern "C" int s2h_inverse_adjoint(const s2h_plan* p, const void* g, int g_is_co
   if ((st = hip_status(launch_zero(grad_flm, 2 * fs * batch, s)))) return st;
 
   RingFwdArgs r{};
-  r.jobs = p->inv_jobs;
   r.f = g;
   r.f_complex = g_is_complex;
   r.real_only_all = (real && g_is_complex) ? 1 : 0;
@@ -465,7 +461,7 @@ extern "C" int s2h_inverse_adjoint(const s2h_plan* p, const void* g, int g_is_co
   r.apply_phase = p->d.sampling == S2H_HEALPIX;
   r.L = L;
   r.chirp_pool = p->chirp;
-  if ((st = hip_status(launch_ring_fft_forward(r, p->n_inv_jobs, batch, p->inv_lds, s)))) return st;
+  if ((st = hip_status(launch_ring_fft_forward(r, p->inv_set, batch, s)))) return st;
 
   LatArgs a = lat_args(p, false, batch);
   a.flm = static_cast<cplx*>(grad_flm);
@@ -548,13 +544,12 @@ extern "C" int s2h_healpix_fft(int nside, int L, int reality, const void* f, voi
                                s2h_stream stream) {
   if (nside < 1 || !f || !ftm || batch < 0) return S2H_ERR_INVALID;
   if (L < 2 * nside) return S2H_ERR_BANDLIMIT;
-  if (4 * nside > kMaxRingLen) return S2H_ERR_UNSUPPORTED;
+  if (4 * nside > kMaxRingLen || L > kMaxL) return S2H_ERR_UNSUPPORTED;
   if (batch == 0) return S2H_OK;
   HpTables tb;
   int st = healpix_tables(nside, L, &tb);
   if (st) return st;
   RingFwdArgs r{};
-  r.jobs = tb.jobs;
   r.f = f;
   r.f_complex = 1;
   r.real_only_2L = reality ? 1 : 0;
@@ -567,20 +562,19 @@ extern "C" int s2h_healpix_fft(int nside, int L, int reality, const void* f, voi
   r.L = L;
   r.chirp_pool = tb.chirp;
   return hip_status(
-      launch_ring_fft_forward(r, tb.njobs, batch, tb.lds, static_cast<hipStream_t>(stream)));
+      launch_ring_fft_forward(r, tb.set, batch, static_cast<hipStream_t>(stream)));
 }
 
 extern "C" int s2h_healpix_ifft(int nside, int L, int reality, const void* ftm, void* f,
                                 int batch, s2h_stream stream) {
   if (nside < 1 || !f || !ftm || batch < 0) return S2H_ERR_INVALID;
   if (L < 2 * nside) return S2H_ERR_BANDLIMIT;
-  if (4 * nside > kMaxRingLen) return S2H_ERR_UNSUPPORTED;
+  if (4 * nside > kMaxRingLen || L > kMaxL) return S2H_ERR_UNSUPPORTED;
   if (batch == 0) return S2H_OK;
   HpTables tb;
   int st = healpix_tables(nside, L, &tb);
   if (st) return st;
   RingInvArgs r{};
-  r.jobs = tb.jobs;
   r.in = static_cast<const cplx*>(ftm);
   r.in_map_stride = (int64_t)(4 * nside - 1) * 2 * L;
   r.ncol = 2 * L;
@@ -594,7 +588,7 @@ extern "C" int s2h_healpix_ifft(int nside, int L, int reality, const void* ftm, 
   r.chirp_pool = tb.chirp;
   r.L = L;
   return hip_status(
-      launch_ring_fft_inverse(r, tb.njobs, batch, tb.lds, static_cast<hipStream_t>(stream)));
+      launch_ring_fft_inverse(r, tb.set, batch, static_cast<hipStream_t>(stream)));
 }
 
 extern "C" void healpix_fft_hip(void* stream, void** buffers, const char* opaque,

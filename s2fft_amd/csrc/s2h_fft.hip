@@ -1,117 +1,12 @@
-// Ragged ring FFT kernels: one workgroup per (ring, map), the whole ring in LDS.
-//
-// Forward  (s2fft/utils/healpix_ffts.py:187-226 + spherical.py:533-546):
-//   out[row, col] = w_t * exp(-i m phi0_t) * F_t[m mod nphi_t],  m = m_first + col
-//   with F_t the unnormalised DFT of the ring; the periodic extension, the
-//   quadrature weight and the ring phase are fused into the store.
-// Inverse  (healpix_ffts.py:335-371 + spherical.py:183-200):
-//   F_t[q] = sum_{m = q mod nphi} exp(+i m phi0_t) c(m), f_t[p] = sum_q F_t[q] e^{2 pi i q p/nphi}
-//   the folding and the phase are fused into the LDS load.
+// Bluestein chirp tables, the DFT over gamma of the Wigner transforms and the
+// reality fill of Wigner coefficients.  (The ring FFTs live in
+// s2h_ring_fft.hip.)
 #include "s2h_fft_lds.h"
 #include "s2h_kernels.h"
 
 namespace s2h {
 
 constexpr int kFftMax = kMaxConvLen;
-
-__device__ __forceinline__ void ring_dft(double* re, double* im, const RingJob& job,
-                                         const cplx* chirp_pool, int sign) {
-  if (job.conv_len == 0)
-    fft_mixed<kFftMax>(re, im, job.nphi, sign);
-  else
-    fft_bluestein<kFftMax>(re, im, job.nphi, job.conv_len, chirp_pool + job.chirp_off, sign);
-}
-
-__global__ void __launch_bounds__(kThreads) ring_fft_forward_kernel(RingFwdArgs a) {
-  extern __shared__ double smem[];
-  const RingJob job = a.jobs[blockIdx.x];
-  const int64_t b = blockIdx.y;
-  const int n = job.nphi;
-  const int cap = job.conv_len ? job.conv_len : n;
-  double* re = smem;
-  double* im = smem + cap;
-
-  const bool real_only = !a.f_complex || a.real_only_all || (a.real_only_2L && n == 2 * a.L);
-  if (a.f_complex) {
-    const cplx* src = reinterpret_cast<const cplx*>(a.f) + b * a.map_stride + job.pix_offset;
-    for (int p = threadIdx.x; p < n; p += kThreads) {
-      const cplx v = src[p];
-      re[p] = v.x;
-      im[p] = real_only ? 0.0 : v.y;
-    }
-  } else {
-    const double* src = reinterpret_cast<const double*>(a.f) + b * a.map_stride + job.pix_offset;
-    for (int p = threadIdx.x; p < n; p += kThreads) {
-      re[p] = src[p];
-      im[p] = 0.0;
-    }
-  }
-  __syncthreads();
-  ring_dft(re, im, job, a.chirp_pool, -1);
-
-  cplx* dst = a.out + b * a.out_map_stride + (int64_t)job.row * a.ncol;
-  const double w = a.apply_weight ? job.weight : 1.0;
-  for (int col = threadIdx.x; col < a.ncol; col += kThreads) {
-    const int m = a.m_first + col;
-    int k = m % n;
-    if (k < 0) k += n;
-    cplx v = {re[k] * w, im[k] * w};
-    if (a.apply_phase) {
-      double s, c;
-      sincospi(-(double)m * job.phi0_over_pi, &s, &c);
-      v = cmul(v, cplx{c, s});
-    }
-    if (a.zero_neg_2L && n == 2 * a.L && m < 0) v = {0.0, 0.0};
-    dst[col] = v;
-  }
-}
-
-__global__ void __launch_bounds__(kThreads) ring_fft_inverse_kernel(RingInvArgs a) {
-  extern __shared__ double smem[];
-  const RingJob job = a.jobs[blockIdx.x];
-  const int64_t b = blockIdx.y;
-  const int n = job.nphi;
-  const int cap = job.conv_len ? job.conv_len : n;
-  double* re = smem;
-  double* im = smem + cap;
-  const cplx* src = a.in + b * a.in_map_stride + (int64_t)job.row * a.ncol;
-  const bool herm = a.herm_mode == 1 || (a.herm_mode == 2 && n == 2 * a.L);
-  // folded range of m
-  const int lo = herm ? -(a.L - 1) : a.m_lo;
-  const int hi = herm ? (a.L - 1) : a.m_hi;
-  const double w = a.apply_weight ? job.weight : 1.0;
-  for (int q = threadIdx.x; q < n; q += kThreads) {
-    int m = lo + (((q - lo) % n) + n) % n;
-    cplx acc = {0.0, 0.0};
-    for (; m <= hi; m += n) {
-      cplx v;
-      if (herm && m < 0) {
-        const cplx t = src[-m - a.m_first];
-        v = {t.x, -t.y};
-      } else {
-        v = src[m - a.m_first];
-      }
-      if (a.apply_phase) {
-        double s, c;
-        sincospi((double)m * job.phi0_over_pi, &s, &c);
-        v = cmul(v, cplx{c, s});
-      }
-      acc = cadd(acc, v);
-    }
-    re[q] = acc.x * w;
-    im[q] = acc.y * w;
-  }
-  __syncthreads();
-  ring_dft(re, im, job, a.chirp_pool, +1);
-
-  if (a.f_complex) {
-    cplx* dst = reinterpret_cast<cplx*>(a.f) + b * a.map_stride + job.pix_offset;
-    for (int p = threadIdx.x; p < n; p += kThreads) dst[p] = {re[p], herm ? 0.0 : im[p]};
-  } else {
-    double* dst = reinterpret_cast<double*>(a.f) + b * a.map_stride + job.pix_offset;
-    for (int p = threadIdx.x; p < n; p += kThreads) dst[p] = re[p];
-  }
-}
 
 // FFT_M(b)/M for Bluestein, b_j = exp(+i pi j^2 / n) for |j| < n, wrapped.
 __global__ void __launch_bounds__(kThreads) chirp_setup_kernel(const int* lengths,
@@ -203,22 +98,6 @@ __global__ void __launch_bounds__(kThreads) wigner_fill_kernel(cplx* flmn, int L
 }
 
 // ------------------------------------------------------------ launchers
-hipError_t launch_ring_fft_forward(const RingFwdArgs& a, int njobs, int batch, int lds_elems,
-                                   hipStream_t s) {
-  if (njobs == 0 || batch == 0) return hipSuccess;
-  const size_t lds = 2 * sizeof(double) * (size_t)lds_elems;
-  hipLaunchKernelGGL(ring_fft_forward_kernel, dim3(njobs, batch), dim3(kThreads), lds, s, a);
-  return hipGetLastError();
-}
-
-hipError_t launch_ring_fft_inverse(const RingInvArgs& a, int njobs, int batch, int lds_elems,
-                                   hipStream_t s) {
-  if (njobs == 0 || batch == 0) return hipSuccess;
-  const size_t lds = 2 * sizeof(double) * (size_t)lds_elems;
-  hipLaunchKernelGGL(ring_fft_inverse_kernel, dim3(njobs, batch), dim3(kThreads), lds, s, a);
-  return hipGetLastError();
-}
-
 hipError_t launch_chirp_setup(const int* lengths, const int* conv_lens, const int64_t* offs,
                               cplx* pool, int count, int max_conv, hipStream_t s) {
   if (count == 0) return hipSuccess;
@@ -245,13 +124,9 @@ hipError_t launch_wigner_fill(cplx* flmn, int L, int N, hipStream_t s) {
 }
 
 hipError_t allow_large_lds() {
+  hipError_t e = allow_large_lds_rings();
+  if (e != hipSuccess) return e;
   const int bytes = 2 * sizeof(double) * kFftMax;
-  hipError_t e = hipFuncSetAttribute((const void*)ring_fft_forward_kernel,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute((const void*)ring_fft_inverse_kernel,
-                          hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  if (e != hipSuccess) return e;
   return hipFuncSetAttribute((const void*)chirp_setup_kernel,
                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }

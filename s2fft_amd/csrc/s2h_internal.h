@@ -11,6 +11,7 @@ namespace s2h {
 constexpr int kThreads = 256;        // workgroup size of every kernel here
 constexpr int kMaxRingLen = 4096;    // longest ring FFT held in LDS (Stockham)
 constexpr int kMaxConvLen = 8192;    // longest Bluestein convolution held in LDS
+constexpr int kMaxL = 4096;           // ring phase tables cover |m| < kMaxL
 constexpr int kMlpHidden = 128;       // learned preconditioner: hidden width
 constexpr int kMlpParams = kMlpHidden * 4 + kMlpHidden + kMlpHidden * kMlpHidden + kMlpHidden +
                            2 * kMlpHidden + 2;  // W1 b1 W2 b2 W3 b3
@@ -25,6 +26,17 @@ struct RingJob {
   int32_t conv_len;     // 0: mixed-radix; else Bluestein length M (power of 2)
   int32_t pad;
   int64_t chirp_off;    // offset (complex elements) of FFT(chirp)/M in the chirp pool
+};
+
+// Ring jobs of one transform on the device, in three consecutive classes:
+// n_pow2 power-of-two lengths, n_smooth other lengths with prime factors
+// <= 13, n_blue Bluestein rings; each class sorted by decreasing length.
+struct JobSet {
+  RingJob* jobs = nullptr;
+  int n_pow2 = 0, n_smooth = 0, n_blue = 0;
+  int max_pow2 = 0;     // longest power-of-two ring
+  int max_smooth = 0;   // longest other mixed-radix ring
+  int max_conv = 0;     // longest Bluestein convolution
 };
 
 // Start of the ell-recursion for one signed m.

@@ -21,6 +21,7 @@ struct RingFwdArgs {
   int zero_neg_2L;          // reference reality: m < 0 zero on rings with nphi == 2L
   int L;
   const cplx* chirp_pool;
+  int batch;                // set by the launcher
 };
 
 struct RingInvArgs {
@@ -38,6 +39,7 @@ struct RingInvArgs {
   int64_t map_stride;
   const cplx* chirp_pool;
   int L;
+  int batch;                // set by the launcher
 };
 
 struct LatArgs {
@@ -61,10 +63,11 @@ struct LatArgs {
   int groups;               // ring groups per m (forward); >1 accumulates atomically
 };
 
-hipError_t launch_ring_fft_forward(const RingFwdArgs& a, int njobs, int batch, int lds_elems,
+hipError_t launch_ring_fft_forward(const RingFwdArgs& a, const JobSet& js, int batch,
                                    hipStream_t s);
-hipError_t launch_ring_fft_inverse(const RingInvArgs& a, int njobs, int batch, int lds_elems,
+hipError_t launch_ring_fft_inverse(const RingInvArgs& a, const JobSet& js, int batch,
                                    hipStream_t s);
+hipError_t allow_large_lds_rings();
 hipError_t launch_chirp_setup(const int* lengths, const int* conv_lens, const int64_t* offs,
                               cplx* pool, int count, int max_conv, hipStream_t s);
 hipError_t launch_gamma_dft(int N, int64_t npix, int mode, double scale, const void* in,

@@ -322,10 +322,41 @@ RingJob make_job(int64_t off, int nphi, int row, double phi0, double w, ChirpBui
   return j;
 }
 
-int lds_of(const std::vector<RingJob>& jobs) {
-  int m = 0;
-  for (auto& j : jobs) m = std::max(m, j.conv_len ? j.conv_len : j.nphi);
-  return m;
+// Orders the jobs (powers of two, other mixed-radix lengths, Bluestein;
+// longest first within a class so that the heavy workgroups start early) and
+// uploads them.
+int ring_class(const RingJob& j) {
+  if (j.conv_len) return 2;
+  return (j.nphi & (j.nphi - 1)) == 0 ? 0 : 1;
+}
+
+int make_set(s2h_plan* p, std::vector<RingJob> jobs, JobSet* out) {
+  std::stable_sort(jobs.begin(), jobs.end(), [](const RingJob& x, const RingJob& y) {
+    const int cx = ring_class(x), cy = ring_class(y);
+    if (cx != cy) return cx < cy;
+    const int kx = cx == 2 ? x.conv_len : x.nphi, ky = cy == 2 ? y.conv_len : y.nphi;
+    return kx > ky;
+  });
+  JobSet js;
+  for (auto& j : jobs) {
+    switch (ring_class(j)) {
+      case 0:
+        ++js.n_pow2;
+        js.max_pow2 = std::max(js.max_pow2, j.nphi);
+        break;
+      case 1:
+        ++js.n_smooth;
+        js.max_smooth = std::max(js.max_smooth, j.nphi);
+        break;
+      default:
+        ++js.n_blue;
+        js.max_conv = std::max(js.max_conv, j.conv_len);
+    }
+  }
+  const int st = upload<RingJob>(p, jobs, &js.jobs);
+  if (st) return st;
+  *out = js;
+  return S2H_OK;
 }
 
 // Latitudinal ring list over rows [r0, r1] of a theta grid.
@@ -373,10 +404,8 @@ int healpix_tables(int nside, int L, HpTables* out) {
     off += n;
   }
   HpTables tb;
-  int st = upload<RingJob>(nullptr, jobs, &tb.jobs);
+  int st = make_set(nullptr, jobs, &tb.set);
   if (st) return st;
-  tb.njobs = (int)jobs.size();
-  tb.lds = lds_of(jobs);
   st = cb.build(&tb.chirp, nullptr);
   if (st) return st;
   cache[key] = tb;
@@ -513,12 +542,8 @@ static int plan_build(s2h_plan* p) {
   p->n_inv_rings = (int)inv_r.size();
   if ((st = upload(p, fwd_r, &p->fwd_rings))) return st;
   p->n_fwd_rings = (int)fwd_r.size();
-  if ((st = upload(p, inv_j, &p->inv_jobs))) return st;
-  p->n_inv_jobs = (int)inv_j.size();
-  p->inv_lds = lds_of(inv_j);
-  if ((st = upload(p, fwd_j, &p->fwd_jobs))) return st;
-  p->n_fwd_jobs = (int)fwd_j.size();
-  p->fwd_lds = lds_of(fwd_j);
+  if ((st = make_set(p, inv_j, &p->inv_set))) return st;
+  if ((st = make_set(p, fwd_j, &p->fwd_set))) return st;
   if ((st = upload(p, msf, &p->fwd_mstart))) return st;
   if ((st = upload(p, msi, &p->inv_mstart))) return st;
   if ((st = upload(p, cf, &p->fwd_coef))) return st;
@@ -551,7 +576,7 @@ extern "C" int s2h_plan_create(s2h_plan** out, const s2h_desc* desc) {
   if (d.sampling == S2H_HEALPIX) {
     if (d.nside < 1) return S2H_ERR_INVALID;
     if (d.L < 2 * d.nside) return S2H_ERR_BANDLIMIT;
-    if (4 * d.nside > kMaxRingLen) return S2H_ERR_UNSUPPORTED;
+    if (4 * d.nside > kMaxRingLen || d.L > kMaxL) return S2H_ERR_UNSUPPORTED;
   } else {
     if (2 * d.L > kMaxRingLen) return S2H_ERR_UNSUPPORTED;
   }

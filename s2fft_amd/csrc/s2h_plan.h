@@ -17,9 +17,7 @@ struct s2h_plan {
   int fwd_kind = 0;  // 0 direct ring FFT, 1 MW resample, 2 MWSS resample
   int fwd_rows = 0;  // rows of the internal forward ftm
   int fwd_in_rows = 0;  // rows of the phi-spectrum before resampling (kinds 1, 2)
-  s2h::RingJob* fwd_jobs = nullptr;
-  int n_fwd_jobs = 0;
-  int fwd_lds = 0;
+  s2h::JobSet fwd_set;
   s2h::LatRing* fwd_rings = nullptr;
   int n_fwd_rings = 0;
   bool fwd_paired = false;
@@ -33,9 +31,7 @@ struct s2h_plan {
 
   // inverse
   int inv_rows = 0;
-  s2h::RingJob* inv_jobs = nullptr;
-  int n_inv_jobs = 0;
-  int inv_lds = 0;
+  s2h::JobSet inv_set;
   s2h::LatRing* inv_rings = nullptr;
   int n_inv_rings = 0;
   bool inv_paired = false;
@@ -53,9 +49,7 @@ struct s2h_plan {
 namespace s2h {
 // Ring-job tables for the reference-layout HEALPix FFTs (cached per device).
 struct HpTables {
-  RingJob* jobs = nullptr;
-  int njobs = 0;
-  int lds = 0;
+  JobSet set;
   cplx* chirp = nullptr;
 };
 int healpix_tables(int nside, int L, HpTables* out);

@@ -1,0 +1,543 @@
+// Ragged ring FFT kernels: one workgroup per (ring, map or map pair), the
+// whole ring in LDS.
+//
+// Forward  (s2fft/utils/healpix_ffts.py:187-226 + spherical.py:533-546):
+//   out[row, col] = w_t * exp(-i m phi0_t) * F_t[m mod nphi_t],  m = m_first + col
+//   with F_t the unnormalised DFT of the ring; the periodic extension, the
+//   quadrature weight and the ring phase are fused into the store.
+// Inverse  (healpix_ffts.py:335-371 + spherical.py:183-200):
+//   F_t[q] = sum_{m = q mod nphi} exp(+i m phi0_t) c(m), f_t[p] = sum_q F_t[q] e^{2 pi i q p/nphi}
+//   the folding and the phase are fused into the LDS load.
+//
+// Rings fall in three classes launched separately so that each gets the LDS
+// and registers it needs: powers of two (radix-8 Stockham passes and one
+// radix 4 or 2), other lengths whose prime factors are all <= 13 (radices
+// 3, 5, 7, 11, 13 as well) and the rest (Bluestein's chirp-z over a
+// power-of-two convolution, 1024 threads).  Twiddles, chirps and ring
+// phases come from two-level tables in LDS (e^{i a k} = hi[k / 64] lo[k % 64])
+// built once per workgroup, so no transcendental is evaluated per point.
+// Two real maps share one complex transform (z = x + i y, split by the
+// conjugate symmetry of the spectra), halving the transforms for real data.
+#include "s2h_fft_lds.h"
+#include "s2h_kernels.h"
+
+namespace s2h {
+
+namespace {
+
+constexpr int kTabLo = 64;
+
+// e^{i pi num/den}-based two-level table: value(k) = hi[k >> 6] * lo[k & 63],
+// value(k) = e^{i pi s k / d} for 0 <= k < count.
+struct Tab2 {
+  const cplx* hi;
+  const cplx* lo;
+  __device__ cplx at(int k) const { return cmul(hi[k >> 6], lo[k & 63]); }
+};
+
+// Builds hi[0..ceil(count/64)) and lo[0..64) for e^{i pi s k / d}.
+template <int TPB>
+__device__ void build_tab(cplx* hi, cplx* lo, int count, int s, int d) {
+  const int nhi = (count + kTabLo - 1) / kTabLo;
+  for (int k = threadIdx.x; k < nhi + kTabLo; k += TPB) {
+    if (k < nhi)
+      hi[k] = expi_pi((int64_t)s * kTabLo * k, d);
+    else
+      lo[k - nhi] = expi_pi((int64_t)s * (k - nhi), d);
+  }
+}
+
+// Phase table e^{i sgn (m0 + c) phi0} for c in [0, count): double-precision
+// angles (the reference forms m * phi0 the same way).
+template <int TPB>
+__device__ void build_phase(cplx* hi, cplx* lo, int count, int m0, double phi0_over_pi,
+                            double sgn) {
+  const int nhi = (count + kTabLo - 1) / kTabLo;
+  for (int k = threadIdx.x; k < nhi + kTabLo; k += TPB) {
+    double s, c;
+    if (k < nhi) {
+      sincospi(sgn * (double)(m0 + kTabLo * k) * phi0_over_pi, &s, &c);
+      hi[k] = {c, s};
+    } else {
+      sincospi(sgn * (double)(k - nhi) * phi0_over_pi, &s, &c);
+      lo[k - nhi] = {c, s};
+    }
+  }
+}
+
+__device__ __forceinline__ cplx conj_if(cplx v, bool c) { return c ? cplx{v.x, -v.y} : v; }
+
+// Radix-R DFT in place.  Odd R uses the symmetric form: with s_k = v_k + v_{R-k},
+// d_k = v_k - v_{R-k} (k = 1..H, H = (R-1)/2) and c_m + i t_m = e^{sign 2 pi i m/R},
+//   X_j = v_0 + sum_k c_{jk} s_k + i t_{jk} d_k,  X_{R-j} = the same with -i,
+// so only R + 2H complex values are live.  cs[m] = (c_m, t_m), m = 1..H.
+template <int R>
+__device__ __forceinline__ void radix_dft(cplx* v, int sign, const cplx* cs) {
+  if constexpr (R == 2 || R == 4 || R == 8) {
+    SmallDft<R>::run(v, sign, nullptr);
+  } else {
+    constexpr int H = (R - 1) / 2;
+    cplx sm[H], df[H];
+    const cplx x0 = v[0];
+    cplx tot = x0;
+#pragma unroll
+    for (int k = 1; k <= H; ++k) {
+      sm[k - 1] = cadd(v[k], v[R - k]);
+      df[k - 1] = csub(v[k], v[R - k]);
+      tot = cadd(tot, sm[k - 1]);
+    }
+    v[0] = tot;
+#pragma unroll
+    for (int j = 1; j <= H; ++j) {
+      cplx a = x0, b = {0.0, 0.0};
+#pragma unroll
+      for (int k = 1; k <= H; ++k) {
+        const int m = (j * k) % R;
+        const double c = m <= H ? cs[m].x : cs[R - m].x;
+        const double t = m <= H ? cs[m].y : -cs[R - m].y;
+        a.x = fma(c, sm[k - 1].x, a.x);
+        a.y = fma(c, sm[k - 1].y, a.y);
+        b.x = fma(t, df[k - 1].x, b.x);
+        b.y = fma(t, df[k - 1].y, b.y);
+      }
+      v[j] = {a.x - b.y, a.y + b.x};
+      v[R - j] = {a.x + b.y, a.y - b.x};
+    }
+  }
+}
+
+// One radix-R Stockham pass over n points (Ns = product of the radices done).
+// tw.at(k) = e^{-2 pi i k / n}; the inverse (sign > 0) uses its conjugate.
+// The last pass (Ns R = n) writes exactly the points each butterfly read, so
+// it streams one butterfly at a time; earlier passes hold their butterflies
+// in registers across the barrier between the reads and the writes.
+template <int R>
+__device__ __forceinline__ void butterfly(cplx* v, const double* re, const double* im, int j,
+                                          int nb, int Ns, int step, bool cj, int sign,
+                                          const Tab2& tw, const cplx* cs) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = {re[j + r * nb], im[j + r * nb]};
+  const int jl = j % Ns;
+  if (jl != 0) {
+    const cplx w1 = conj_if(tw.at(jl * step), cj);
+    cplx wr = w1;
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      v[r] = cmul(v[r], wr);
+      if (r + 1 < R) wr = cmul(wr, w1);
+    }
+  }
+  radix_dft<R>(v, sign, cs);
+}
+
+template <int R, int TPB, int MAXN>
+__device__ __forceinline__ void pass_t(double* re, double* im, int n, int Ns, int sign, const Tab2& tw) {
+  constexpr int MAXB = (MAXN / R + TPB - 1) / TPB;
+  constexpr int H = R / 2 + 1;
+  const int nb = n / R;
+  const bool cj = sign > 0;
+  const int step = n / (Ns * R);
+  cplx cs[H];
+  if constexpr (R != 2 && R != 4 && R != 8) {
+#pragma unroll
+    for (int k = 1; k < H; ++k) cs[k] = conj_if(tw.at(k * (n / R)), cj);
+  }
+  // radices 2 and 4 only ever run as the last pass (see fft_t)
+  if (R <= 4 || Ns * R == n) {
+    for (int j = threadIdx.x; j < nb; j += TPB) {
+      cplx v[R];
+      butterfly<R>(v, re, im, j, nb, Ns, step, cj, sign, tw, cs);
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        re[j + k * Ns] = v[k].x;
+        im[j + k * Ns] = v[k].y;
+      }
+    }
+    __syncthreads();
+    return;
+  }
+  if constexpr (R > 4) {
+    cplx v[MAXB][R];
+#pragma unroll
+    for (int c = 0; c < MAXB; ++c) {
+      const int j = threadIdx.x + c * TPB;
+      if (j < nb) butterfly<R>(v[c], re, im, j, nb, Ns, step, cj, sign, tw, cs);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < MAXB; ++c) {
+      const int j = threadIdx.x + c * TPB;
+      if (j < nb) {
+        const int jl = j % Ns;
+        const int base = (j - jl) * R + jl;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+          re[base + k * Ns] = v[c][k].x;
+          im[base + k * Ns] = v[c][k].y;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Out-of-line pass: the smooth and Bluestein kernels call their passes so
+// that each pass gets its own register allocation.
+template <int R, int TPB, int MAXN>
+__device__ __noinline__ void pass_call(double* re, double* im, int n, int Ns, int sign,
+                                       const Tab2 tw) {
+  pass_t<R, TPB, MAXN>(re, im, n, Ns, sign, tw);
+}
+
+template <int R, int TPB, int MAXN, bool INL>
+__device__ __forceinline__ void pass_any(double* re, double* im, int n, int Ns, int sign,
+                                         const Tab2& tw) {
+  if constexpr (INL)
+    pass_t<R, TPB, MAXN>(re, im, n, Ns, sign, tw);
+  else
+    pass_call<R, TPB, MAXN>(re, im, n, Ns, sign, tw);
+}
+
+template <int TPB, int MAXN, bool POW2, bool INL>
+__device__ __forceinline__ void run_pass(int R, double* re, double* im, int n, int Ns, int sign,
+                                         const Tab2& tw) {
+  switch (R) {
+    case 8: pass_any<8, TPB, MAXN, INL>(re, im, n, Ns, sign, tw); break;
+    case 4: pass_any<4, TPB, MAXN, INL>(re, im, n, Ns, sign, tw); break;
+    case 2: pass_any<2, TPB, MAXN, INL>(re, im, n, Ns, sign, tw); break;
+    default:
+      if constexpr (!POW2) {
+        switch (R) {
+          case 3: pass_any<3, TPB, MAXN, INL>(re, im, n, Ns, sign, tw); break;
+          case 5: pass_any<5, TPB, MAXN, INL>(re, im, n, Ns, sign, tw); break;
+          case 7: pass_any<7, TPB, MAXN, INL>(re, im, n, Ns, sign, tw); break;
+          case 11: pass_any<11, TPB, MAXN, INL>(re, im, n, Ns, sign, tw); break;
+          case 13: pass_any<13, TPB, MAXN, INL>(re, im, n, Ns, sign, tw); break;
+          default: break;
+        }
+      }
+  }
+}
+
+// Mixed-radix FFT in place (prime factors <= 13, or powers of two only when
+// POW2); sign -1 forward, +1 inverse.  Radices in the order: odd primes
+// ascending, 8s, then the one 4 or 2 -- so a 4 or 2 is always the last
+// (streamed) pass.
+template <int TPB, int MAXN, bool POW2, bool INL>
+__device__ void fft_t(double* re, double* im, int n, int sign, const Tab2& tw) {
+  int fac[16];
+  int nf = 0, rem = n;
+  if (!POW2) {
+    const int odd[5] = {3, 5, 7, 11, 13};
+    for (int q = 0; q < 5; ++q)
+      while (rem % odd[q] == 0 && nf < 16) { fac[nf++] = odd[q]; rem /= odd[q]; }
+  }
+  while (rem % 8 == 0 && nf < 16) { fac[nf++] = 8; rem /= 8; }
+  if (rem % 4 == 0) { fac[nf++] = 4; rem /= 4; }
+  if (rem % 2 == 0) { fac[nf++] = 2; rem /= 2; }
+  if (rem != 1) return;  // unreachable: the host routes other lengths to Bluestein
+  int Ns = 1;
+  for (int i = 0; i < nf; ++i) {
+    run_pass<TPB, MAXN, POW2, INL>(fac[i], re, im, n, Ns, sign, tw);
+    Ns *= fac[i];
+  }
+}
+
+// LDS layout of a ring workgroup: re[N] im[N] | tables (576 complex).
+// Tables: FFT twiddles (hi <= 128, lo 64), chirp e^{-i pi k/n} for k < 2n
+// (hi <= 128, lo 64; Bluestein only), phase over <= 8192 values of m
+// (hi <= 128, lo 64).
+struct RingLds {
+  double* re;
+  double* im;
+  cplx* tw_hi;
+  cplx* tw_lo;
+  cplx* ch_hi;
+  cplx* ch_lo;
+  cplx* ph_hi;
+  cplx* ph_lo;
+};
+
+__device__ __forceinline__ RingLds ring_lds(double* smem, int N) {
+  RingLds s;
+  s.re = smem;
+  s.im = smem + N;
+  cplx* t = reinterpret_cast<cplx*>(smem + 2 * N);
+  s.tw_hi = t;
+  s.tw_lo = t + 128;
+  s.ch_hi = t + 192;
+  s.ch_lo = t + 320;
+  s.ph_hi = t + 384;
+  s.ph_lo = t + 512;
+  return s;
+}
+
+// The DFT of the n points in re/im (sign -1 forward, +1 inverse, unnormalised),
+// direct or by Bluestein; tables must be built.
+// KIND: kPow2 (power-of-two ring), kSmooth (prime factors <= 13), kBlue.
+constexpr int kPow2 = 0, kSmooth = 1, kBlue = 2;
+
+template <int TPB, int MAXN, int KIND>
+__device__ void ring_transform(const RingLds& s, int n, int M, const cplx* chirp_hat, int sign) {
+  const Tab2 tw = {s.tw_hi, s.tw_lo};
+  if constexpr (KIND != kBlue) {
+    fft_t<TPB, MAXN, KIND == kPow2, KIND == kPow2>(s.re, s.im, n, sign, tw);
+  } else {
+  const Tab2 ch = {s.ch_hi, s.ch_lo};  // e^{-i pi k / n}, k < 2n
+  const bool cj = sign > 0;
+  for (int p = threadIdx.x; p < M; p += TPB) {
+    if (p < n) {
+      cplx x = {s.re[p], cj ? -s.im[p] : s.im[p]};
+      x = cmul(x, ch.at((int)(((int64_t)p * p) % (2 * n))));
+      s.re[p] = x.x;
+      s.im[p] = x.y;
+    } else {
+      s.re[p] = 0.0;
+      s.im[p] = 0.0;
+    }
+  }
+  __syncthreads();
+  fft_t<TPB, MAXN, true, false>(s.re, s.im, M, -1, tw);
+  for (int k = threadIdx.x; k < M; k += TPB) {
+    const cplx a = cmul(cplx{s.re[k], s.im[k]}, chirp_hat[k]);
+    s.re[k] = a.x;
+    s.im[k] = a.y;
+  }
+  __syncthreads();
+  fft_t<TPB, MAXN, true, false>(s.re, s.im, M, +1, tw);
+  for (int k = threadIdx.x; k < n; k += TPB) {
+    const cplx a = cmul(cplx{s.re[k], s.im[k]}, ch.at((int)(((int64_t)k * k) % (2 * n))));
+    s.re[k] = a.x;
+    s.im[k] = cj ? -a.y : a.y;
+  }
+  __syncthreads();
+  }
+}
+
+template <int TPB, bool BLUE>
+__device__ void build_fft_tables(const RingLds& s, int n, int M) {
+  const int N = BLUE ? M : n;
+  build_tab<TPB>(s.tw_hi, s.tw_lo, N, -2, N);        // e^{-2 pi i k / N}
+  if (BLUE) build_tab<TPB>(s.ch_hi, s.ch_lo, 2 * n, -1, n);  // e^{-i pi k / n}
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ forward
+// gridDim.y: maps (complex input) or map pairs (real input, `paired`).
+template <int TPB, int MAXN, int KIND>
+__global__ void __launch_bounds__(TPB) ring_fwd_kernel(RingFwdArgs a, int job0, int paired) {
+  constexpr bool BLUE = KIND == kBlue;
+  extern __shared__ double smem[];
+  const RingJob job = a.jobs[job0 + blockIdx.x];
+  const int n = job.nphi;
+  const int M = BLUE ? job.conv_len : n;
+  const RingLds s = ring_lds(smem, M);
+  const int64_t b0 = paired ? 2 * (int64_t)blockIdx.y : (int64_t)blockIdx.y;
+  const bool two = paired && b0 + 1 < a.batch;
+
+  build_fft_tables<TPB, BLUE>(s, n, M);
+  if (a.apply_phase) build_phase<TPB>(s.ph_hi, s.ph_lo, a.ncol, a.m_first, job.phi0_over_pi, -1.0);
+
+  if (paired) {
+    // real data: z = x_b0 + i x_b0+1
+    if (a.f_complex) {
+      const cplx* s0 = reinterpret_cast<const cplx*>(a.f) + b0 * a.map_stride + job.pix_offset;
+      for (int p = threadIdx.x; p < n; p += TPB) {
+        s.re[p] = s0[p].x;
+        s.im[p] = two ? s0[a.map_stride + p].x : 0.0;
+      }
+    } else {
+      const double* s0 = reinterpret_cast<const double*>(a.f) + b0 * a.map_stride + job.pix_offset;
+      for (int p = threadIdx.x; p < n; p += TPB) {
+        s.re[p] = s0[p];
+        s.im[p] = two ? s0[a.map_stride + p] : 0.0;
+      }
+    }
+  } else {
+    const bool real_only = a.real_only_all || (a.real_only_2L && n == 2 * a.L);
+    const cplx* src = reinterpret_cast<const cplx*>(a.f) + b0 * a.map_stride + job.pix_offset;
+    for (int p = threadIdx.x; p < n; p += TPB) {
+      const cplx v = src[p];
+      s.re[p] = v.x;
+      s.im[p] = real_only ? 0.0 : v.y;
+    }
+  }
+  __syncthreads();
+  ring_transform<TPB, MAXN, KIND>(s, n, M, a.chirp_pool + job.chirp_off, -1);
+
+  const double w = a.apply_weight ? job.weight : 1.0;
+  const Tab2 ph = {s.ph_hi, s.ph_lo};
+  cplx* d0 = a.out + b0 * a.out_map_stride + (int64_t)job.row * a.ncol;
+  for (int col = threadIdx.x; col < a.ncol; col += TPB) {
+    const int m = a.m_first + col;
+    int k = m % n;
+    if (k < 0) k += n;
+    const bool zero = a.zero_neg_2L && n == 2 * a.L && m < 0;
+    const cplx f = a.apply_phase ? ph.at(col) : cplx{1.0, 0.0};
+    const cplx z = {s.re[k], s.im[k]};
+    if (paired) {
+      const int k2 = k == 0 ? 0 : n - k;
+      const cplx zc = {s.re[k2], -s.im[k2]};
+      // X = (z + zc)/2, Y = (z - zc)/(2i)
+      const cplx X = {0.5 * (z.x + zc.x), 0.5 * (z.y + zc.y)};
+      const cplx Y = {0.5 * (z.y - zc.y), -0.5 * (z.x - zc.x)};
+      const cplx vx = cmul(cplx{X.x * w, X.y * w}, f);
+      d0[col] = zero ? cplx{0.0, 0.0} : vx;
+      if (two) {
+        const cplx vy = cmul(cplx{Y.x * w, Y.y * w}, f);
+        d0[a.out_map_stride + col] = zero ? cplx{0.0, 0.0} : vy;
+      }
+    } else {
+      const cplx v = cmul(cplx{z.x * w, z.y * w}, f);
+      d0[col] = zero ? cplx{0.0, 0.0} : v;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ inverse
+// Fold the (Hermitian-completed) spectrum of one map into F[q].
+__device__ __forceinline__ cplx fold_one(const cplx* src, const RingInvArgs& a, int q, int n,
+                                         int lo, int hi, bool herm, const Tab2& ph) {
+  int m = lo + (((q - lo) % n) + n) % n;
+  cplx acc = {0.0, 0.0};
+  for (; m <= hi; m += n) {
+    cplx v;
+    if (herm && m < 0) {
+      const cplx t = src[-m - a.m_first];
+      v = {t.x, -t.y};
+    } else {
+      v = src[m - a.m_first];
+    }
+    if (a.apply_phase) v = cmul(v, ph.at(m - lo));
+    acc = cadd(acc, v);
+  }
+  return acc;
+}
+
+template <int TPB, int MAXN, int KIND>
+__global__ void __launch_bounds__(TPB) ring_inv_kernel(RingInvArgs a, int job0, int paired) {
+  constexpr bool BLUE = KIND == kBlue;
+  extern __shared__ double smem[];
+  const RingJob job = a.jobs[job0 + blockIdx.x];
+  const int n = job.nphi;
+  const int M = BLUE ? job.conv_len : n;
+  const RingLds s = ring_lds(smem, M);
+  const int64_t b0 = paired ? 2 * (int64_t)blockIdx.y : (int64_t)blockIdx.y;
+  const bool two = paired && b0 + 1 < a.batch;
+  const bool herm = a.herm_mode == 1 || (a.herm_mode == 2 && n == 2 * a.L);
+  const int lo = herm ? -(a.L - 1) : a.m_lo;
+  const int hi = herm ? (a.L - 1) : a.m_hi;
+
+  build_fft_tables<TPB, BLUE>(s, n, M);
+  if (a.apply_phase) build_phase<TPB>(s.ph_hi, s.ph_lo, hi - lo + 1, lo, job.phi0_over_pi, 1.0);
+  __syncthreads();
+
+  const Tab2 ph = {s.ph_hi, s.ph_lo};
+  const double w = a.apply_weight ? job.weight : 1.0;
+  const cplx* src0 = a.in + b0 * a.in_map_stride + (int64_t)job.row * a.ncol;
+  for (int q = threadIdx.x; q < n; q += TPB) {
+    const cplx X = fold_one(src0, a, q, n, lo, hi, herm, ph);
+    if (paired) {
+      // Z = X + i Y: the inverse of two Hermitian spectra gives x + i y
+      const cplx Y = two ? fold_one(src0 + a.in_map_stride, a, q, n, lo, hi, herm, ph)
+                         : cplx{0.0, 0.0};
+      s.re[q] = (X.x - Y.y) * w;
+      s.im[q] = (X.y + Y.x) * w;
+    } else {
+      s.re[q] = X.x * w;
+      s.im[q] = X.y * w;
+    }
+  }
+  __syncthreads();
+  ring_transform<TPB, MAXN, KIND>(s, n, M, a.chirp_pool + job.chirp_off, +1);
+
+  if (paired) {
+    double* d0 = reinterpret_cast<double*>(a.f) + b0 * a.map_stride + job.pix_offset;
+    for (int p = threadIdx.x; p < n; p += TPB) {
+      d0[p] = s.re[p];
+      if (two) d0[a.map_stride + p] = s.im[p];
+    }
+  } else if (a.f_complex) {
+    cplx* dst = reinterpret_cast<cplx*>(a.f) + b0 * a.map_stride + job.pix_offset;
+    for (int p = threadIdx.x; p < n; p += TPB) dst[p] = {s.re[p], herm ? 0.0 : s.im[p]};
+  } else {
+    double* dst = reinterpret_cast<double*>(a.f) + b0 * a.map_stride + job.pix_offset;
+    for (int p = threadIdx.x; p < n; p += TPB) dst[p] = s.re[p];
+  }
+}
+
+// ------------------------------------------------------------------ launchers
+namespace {
+
+constexpr int kTpbDirect = 512;
+constexpr int kTpbBlue = 1024;
+constexpr int kTabBytes = 576 * sizeof(cplx);
+
+size_t ring_lds_bytes(int N) { return 2 * sizeof(double) * (size_t)N + kTabBytes; }
+
+bool fwd_paired(const RingFwdArgs& a) { return !a.f_complex || a.real_only_all; }
+bool inv_paired(const RingInvArgs& a) { return a.herm_mode == 1 && !a.f_complex; }
+
+}  // namespace
+
+hipError_t launch_ring_fft_forward(const RingFwdArgs& a0, const JobSet& js, int batch,
+                                   hipStream_t s) {
+  if (batch == 0 || js.n_pow2 + js.n_smooth + js.n_blue == 0) return hipSuccess;
+  RingFwdArgs a = a0;
+  a.jobs = js.jobs;
+  a.batch = batch;
+  const int paired = fwd_paired(a) ? 1 : 0;
+  const unsigned gy = paired ? (unsigned)((batch + 1) / 2) : (unsigned)batch;
+  if (js.n_pow2)
+    hipLaunchKernelGGL((ring_fwd_kernel<kTpbDirect, kMaxRingLen, kPow2>), dim3(js.n_pow2, gy),
+                       dim3(kTpbDirect), ring_lds_bytes(js.max_pow2), s, a, 0, paired);
+  if (js.n_smooth)
+    hipLaunchKernelGGL((ring_fwd_kernel<kTpbDirect, kMaxRingLen, kSmooth>),
+                       dim3(js.n_smooth, gy), dim3(kTpbDirect), ring_lds_bytes(js.max_smooth), s,
+                       a, js.n_pow2, paired);
+  if (js.n_blue)
+    hipLaunchKernelGGL((ring_fwd_kernel<kTpbBlue, kMaxConvLen, kBlue>), dim3(js.n_blue, gy),
+                       dim3(kTpbBlue), ring_lds_bytes(js.max_conv), s, a,
+                       js.n_pow2 + js.n_smooth, paired);
+  return hipGetLastError();
+}
+
+hipError_t launch_ring_fft_inverse(const RingInvArgs& a0, const JobSet& js, int batch,
+                                   hipStream_t s) {
+  if (batch == 0 || js.n_pow2 + js.n_smooth + js.n_blue == 0) return hipSuccess;
+  RingInvArgs a = a0;
+  a.jobs = js.jobs;
+  a.batch = batch;
+  const int paired = inv_paired(a) ? 1 : 0;
+  const unsigned gy = paired ? (unsigned)((batch + 1) / 2) : (unsigned)batch;
+  if (js.n_pow2)
+    hipLaunchKernelGGL((ring_inv_kernel<kTpbDirect, kMaxRingLen, kPow2>), dim3(js.n_pow2, gy),
+                       dim3(kTpbDirect), ring_lds_bytes(js.max_pow2), s, a, 0, paired);
+  if (js.n_smooth)
+    hipLaunchKernelGGL((ring_inv_kernel<kTpbDirect, kMaxRingLen, kSmooth>),
+                       dim3(js.n_smooth, gy), dim3(kTpbDirect), ring_lds_bytes(js.max_smooth), s,
+                       a, js.n_pow2, paired);
+  if (js.n_blue)
+    hipLaunchKernelGGL((ring_inv_kernel<kTpbBlue, kMaxConvLen, kBlue>), dim3(js.n_blue, gy),
+                       dim3(kTpbBlue), ring_lds_bytes(js.max_conv), s, a,
+                       js.n_pow2 + js.n_smooth, paired);
+  return hipGetLastError();
+}
+
+hipError_t allow_large_lds_rings() {
+  const int bytes = (int)ring_lds_bytes(kMaxConvLen);
+  const void* fns[] = {(const void*)ring_fwd_kernel<kTpbDirect, kMaxRingLen, kPow2>,
+                       (const void*)ring_fwd_kernel<kTpbDirect, kMaxRingLen, kSmooth>,
+                       (const void*)ring_fwd_kernel<kTpbBlue, kMaxConvLen, kBlue>,
+                       (const void*)ring_inv_kernel<kTpbDirect, kMaxRingLen, kPow2>,
+                       (const void*)ring_inv_kernel<kTpbDirect, kMaxRingLen, kSmooth>,
+                       (const void*)ring_inv_kernel<kTpbBlue, kMaxConvLen, kBlue>};
+  for (const void* f : fns) {
+    const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+}  // namespace s2h
