@@ -89,13 +89,18 @@ int s2h_inverse(const s2h_plan *plan, const void *flm, void *f, void *work, int 
 
 /* Arithmetic of the latitudinal projection / synthesis (no counterpart in
  * the reference, which is FP64 throughout).  S2H_PREC_FP64: FP64 everywhere.
- * S2H_PREC_MIXED: the Wigner-d recursion, its renormalisation, the ring FFTs
- * and every sum across waves stay FP64; the d-values and the ring (or flm)
- * data are rounded to FP32 and multiplied on the FP32 matrix cores with FP32
- * accumulation over one wave's 128 rings (forward) or one 32-l chunk
- * (inverse).  Relative error ~1e-7 of max |flm| for band-limited maps at
- * L = 2048; meant for iterative refinement whose residual is FP64. */
-enum s2h_precision { S2H_PREC_FP64 = 0, S2H_PREC_MIXED = 1 };
+ * In the other modes the Wigner-d recursion, its renormalisation, the ring
+ * FFTs and every sum across waves stay FP64:
+ * S2H_PREC_MIXED: the d-values and the ring (or flm) data are rounded to FP32
+ *   and multiplied on the FP32 matrix cores with FP32 accumulation over one
+ *   wave's 128 rings (forward) or one 32-l chunk (inverse); relative error
+ *   ~1e-7 of max |flm| for band-limited maps at L = 2048.
+ * S2H_PREC_BF16: each operand split into two BF16 (hi + lo, 16 significant
+ *   bits), products hi*hi + hi*lo + lo*hi on the BF16 matrix cores with FP32
+ *   accumulation (forward: over one wave's 64 ring pairs; inverse: over all
+ *   l); relative error ~1e-5.
+ * Both are meant for iterative refinement whose residual is FP64. */
+enum s2h_precision { S2H_PREC_FP64 = 0, S2H_PREC_MIXED = 1, S2H_PREC_BF16 = 2 };
 int s2h_forward_ex(const s2h_plan *plan, const void *f, int f_is_complex, void *flm,
                    void *work, int batch, int precision, s2h_stream stream);
 int s2h_inverse_ex(const s2h_plan *plan, const void *flm, void *f, void *work, int batch,

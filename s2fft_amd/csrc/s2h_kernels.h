@@ -80,6 +80,8 @@ hipError_t launch_lat_forward(const LatArgs& a, bool paired, hipStream_t s);
 hipError_t launch_lat_forward_mfma(const LatArgs& a, bool paired, bool fp32, hipStream_t s);
 hipError_t launch_mfma_f64_probe(const double* a, const double* b, double* out, hipStream_t s);
 hipError_t launch_lat_inverse_mma(const LatArgs& a, bool paired, bool fp32, hipStream_t s);
+hipError_t launch_lat_forward_bf16(const LatArgs& a, bool paired, hipStream_t s);
+hipError_t launch_lat_inverse_bf16(const LatArgs& a, bool paired, hipStream_t s);
 hipError_t launch_mfma_f32_probe(const float* a, const float* b, float* out, hipStream_t s);
 
 // learned preconditioner (per-(l, m) rows of one or more flm arrays)

@@ -1,0 +1,358 @@
+// Latitudinal steps with the projection / synthesis on the BF16 matrix cores
+// (split-BF16 operands, FP32 accumulation); the ell-recursion stays FP64.
+//
+// Every FP64 operand v (a d-value or a ring / flm value) is split into
+// hi = bf16(v) and lo = bf16(v - hi), so v = hi + lo to 2^-17 relative, and
+// each product is formed as hi*hi + hi*lo + lo*hi on
+// V_MFMA_F32_16X16X32_BF16 (A[i][k]: lane i + 16 (k/8), element k%8; B[k][j]:
+// lane j + 16 (k/8), element k%8; C[i][j]: lane j + 16 (i/4), register i%4).
+// Products carry ~1e-5 relative error: this is the fast inner transform of
+// a refinement whose residual is FP64.
+//
+// Forward:  flm[b, l, m] = sum_p S_par(l)[b, p] d_l(p)   (K = ring pairs)
+//   A = S (16 rows = 8 maps x re/im, per 8-map tile), B = d (K = 32 pairs x
+//   16 l of one parity).  Each lane runs the recursion of one ring pair and
+//   writes its split d-values to LDS as [l][pair], the B layout.
+// Inverse:  ftm[b, p, m] = sum_l d_l(p) flm[b, l, m]      (K = l)
+//   A = d (16 pairs x K = 32 l), B = flm (32 l x 16 columns = 8 maps x re/im).
+//   Each lane writes its d-values as [pair][l], the A layout; the flm chunk
+//   is staged once per workgroup as [column][l], the B layout.  Even and odd
+//   l (relative to m) accumulate separately to serve theta and pi - theta.
+#include "s2h_fft_lds.h"
+#include "s2h_kernels.h"
+#include "s2h_lat_common.h"
+
+namespace s2h {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+using lat::m_of_block;
+using lat::start_value;
+
+__device__ __forceinline__ void split2(double v, __bf16& hi, __bf16& lo) {
+  const float f = (float)v;
+  hi = (__bf16)f;
+  lo = (__bf16)(f - (float)hi);
+}
+
+__device__ __forceinline__ f32x4 mma(const bf16x8& a, const bf16x8& b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// hi*hi + hi*lo + lo*hi
+__device__ __forceinline__ f32x4 mma3(const bf16x8& ah, const bf16x8& al, const bf16x8& bh,
+                                      const bf16x8& bl, f32x4 c) {
+  c = mma(al, bh, c);
+  c = mma(ah, bl, c);
+  return mma(ah, bh, c);
+}
+
+constexpr int kFW = 8;          // forward: waves per workgroup
+constexpr int kFT = kFW * 64;
+constexpr int kFTiles = 2;      // 8-map tiles per workgroup (16 maps)
+
+}  // namespace
+
+// ------------------------------------------------------------------ forward
+template <bool PAIRED>
+__global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
+  // d tile per wave: [l_local 32][pair 64] hi and lo
+  __shared__ __bf16 dth[kFW][32 * 64];
+  __shared__ __bf16 dtl[kFW][32 * 64];
+  // per-wave results: [tile][parity][row 16][col 16]
+  __shared__ float ctile[kFW][kFTiles * 2 * 256];
+  const int m = m_of_block(blockIdx.x, a.nm, a.m_lo);
+  const int am = m < 0 ? -m : m;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int row = lane & 15, kg = lane >> 4;
+  const int L = a.L;
+  const int b0 = blockIdx.z * (8 * kFTiles);
+  const MStart st = a.mstart[m - a.m_lo];
+  const double bsign = m < 0 ? -1.0 : 1.0;
+  const Coef* __restrict__ cf = a.coef + (int64_t)am * L;
+  __bf16* dh = dth[wave];
+  __bf16* dl = dtl[wave];
+
+  // recursion state of this lane's pair
+  const int base = (blockIdx.y * kFW + wave) * 64;
+  const int p = base + lane;
+  double x = 0.0, d = 0.0, dm1 = 0.0;
+  int ls = 0;
+  if (p < a.nrings) {
+    const LatRing rg = a.rings[p];
+    x = rg.x;
+    start_value(st, rg.c2, rg.s2, d, ls);
+  }
+
+  // A operands: row = (map row/2 of the tile, re/im row%2), K = pairs
+  // 32 kb + 8 kg + e of this wave
+  bf16x8 seh[kFTiles][2], sel[kFTiles][2], soh[kFTiles][2], sol[kFTiles][2];
+#pragma unroll
+  for (int t = 0; t < kFTiles; ++t) {
+    const int amap = b0 + 8 * t + (row >> 1);
+    const bool have = amap < a.batch;
+    const cplx* ftm = a.ftm + (int64_t)amap * a.ftm_map_stride + (m - a.ftm_m_first);
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int q = base + 32 * kb + 8 * kg + e;
+        double ve = 0.0, vo = 0.0;
+        if (have && q < a.nrings) {
+          const LatRing rq = a.rings[q];
+          const cplx fa = ftm[(int64_t)rq.ta * a.ftm_ncol];
+          cplx fb = {0.0, 0.0};
+          if (PAIRED && rq.tb >= 0 && rq.tb != rq.ta) fb = ftm[(int64_t)rq.tb * a.ftm_ncol];
+          const double ca = (row & 1) ? fa.y : fa.x;
+          const double cb = (row & 1) ? fb.y : fb.x;
+          ve = ca + cb;
+          vo = PAIRED ? ca - cb : ca;
+        }
+        __bf16 h, l;
+        split2(ve, h, l);
+        seh[t][kb][e] = h;
+        sel[t][kb][e] = l;
+        split2(vo, h, l);
+        soh[t][kb][e] = h;
+        sol[t][kb][e] = l;
+      }
+    }
+  }
+
+  cplx* out = a.flm + (int64_t)b0 * a.flm_map_stride + (L - 1 + m);
+  for (int blk = st.ell0; blk < L; blk += 32) {
+    const bool live = ls == 0;
+#pragma unroll
+    for (int i = 0; i < 32; ++i) {
+      const int l = blk + i;
+      const double v = (live && l < L) ? d : 0.0;
+      __bf16 h, lo;
+      split2(v, h, lo);
+      // even / odd l (relative to l0) in separate 16-column halves
+      const int col = ((i & 1) << 4) | (i >> 1);
+      dh[col * 64 + lane] = h;
+      dl[col * 64 + lane] = lo;
+      if (l + 1 < L) {
+        const Coef c = cf[l];
+        const double dn = fma(fma(c.a, x, -c.b * bsign), d, -c.c * dm1);
+        dm1 = d;
+        d = dn;
+      }
+    }
+    lat::rescale(d, dm1, ls);
+    lat::wave_sync();
+    f32x4 acc[kFTiles][2];
+#pragma unroll
+    for (int t = 0; t < kFTiles; ++t) {
+      acc[t][0] = f32x4{0.f, 0.f, 0.f, 0.f};
+      acc[t][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int par = 0; par < 2; ++par) {
+#pragma unroll
+      for (int kb = 0; kb < 2; ++kb) {
+        // B[k][j]: d of pair 32 kb + 8 kg + e at column j = row (parity par)
+        const int off = (par * 16 + row) * 64 + 32 * kb + 8 * kg;
+        const bf16x8 bh = *reinterpret_cast<const bf16x8*>(dh + off);
+        const bf16x8 bl = *reinterpret_cast<const bf16x8*>(dl + off);
+#pragma unroll
+        for (int t = 0; t < kFTiles; ++t) {
+          if (par == 0)
+            acc[t][0] = mma3(seh[t][kb], sel[t][kb], bh, bl, acc[t][0]);
+          else
+            acc[t][1] = mma3(soh[t][kb], sol[t][kb], bh, bl, acc[t][1]);
+        }
+      }
+    }
+    lat::wave_sync();
+    // C[i][j]: lane j + 16 (i/4), register i%4
+#pragma unroll
+    for (int t = 0; t < kFTiles; ++t)
+#pragma unroll
+      for (int par = 0; par < 2; ++par)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          ctile[wave][(t * 2 + par) * 256 + (4 * kg + r) * 16 + row] = acc[t][par][r];
+    __syncthreads();
+    // element (t, parity, row i, col j) -> l = blk + 2 j + parity, map 8t + i/2
+    for (int e = threadIdx.x; e < kFTiles * 2 * 256; e += kFT) {
+      const int t = e >> 9, par = (e >> 8) & 1, ri = (e >> 4) & 15, cj = e & 15;
+      const int l = blk + 2 * cj + par;
+      const int bm = 8 * t + (ri >> 1);
+      double v = 0.0;
+#pragma unroll
+      for (int w = 0; w < kFW; ++w) v += (double)ctile[w][e];
+      if (l < L && l >= a.lmin_out && b0 + bm < a.batch) {
+        double* o = reinterpret_cast<double*>(out + (int64_t)bm * a.flm_map_stride +
+                                              (int64_t)l * (2 * L - 1)) + (ri & 1);
+        if (a.groups > 1)
+          atomicAdd(o, v);
+        else
+          *o = v;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ inverse
+namespace {
+constexpr int kIW = 4;          // inverse: waves per workgroup
+constexpr int kIT = kIW * 64;
+constexpr int kIMaps = 16;      // maps per workgroup (two column tiles)
+}  // namespace
+
+template <bool PAIRED>
+__global__ void __launch_bounds__(kIT) lat_inverse_bf16_kernel(LatArgs a) {
+  // d tile per wave: [pair 64][l_local 32] hi / lo (A layout)
+  __shared__ __bf16 dth[kIW][64 * 32];
+  __shared__ __bf16 dtl[kIW][64 * 32];
+  // flm chunk: [parity][column 32 = 16 maps x re/im][l 32] hi / lo (B layout),
+  // entries of the other parity zero
+  __shared__ __bf16 bth[2][32 * 32];
+  __shared__ __bf16 btl[2][32 * 32];
+  const int m = m_of_block(blockIdx.x, a.nm, a.m_lo);
+  const int am = m < 0 ? -m : m;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int row = lane & 15, kg = lane >> 4;
+  const int L = a.L;
+  const int b0 = blockIdx.z * kIMaps;
+  const MStart st = a.mstart[m - a.m_lo];
+  const double bsign = m < 0 ? -1.0 : 1.0;
+  const Coef* __restrict__ cf = a.coef + (int64_t)am * L;
+  __bf16* dh = dth[wave];
+  __bf16* dl = dtl[wave];
+
+  const int pbase = (blockIdx.y * kIW + wave) * 64;
+  const int p = pbase + lane;
+  double x = 0.0, d = 0.0, dm1 = 0.0;
+  int ls = 0;
+  if (p < a.nrings) {
+    const LatRing rg = a.rings[p];
+    x = rg.x;
+    start_value(st, rg.c2, rg.s2, d, ls);
+  }
+  // accumulators [q: 16-pair tile][column tile][parity]
+  f32x4 acc[4][2][2];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      acc[q][t][0] = f32x4{0.f, 0.f, 0.f, 0.f};
+      acc[q][t][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+
+  const cplx* flm = a.flm + (int64_t)b0 * a.flm_map_stride + (L - 1 + m);
+  for (int blk = st.ell0; blk < L; blk += 32) {
+    // stage flm[b0 + col/2][blk + i][m] (component col%2) by parity of i
+    for (int e = threadIdx.x; e < 32 * 32; e += kIT) {
+      const int col = e >> 5, i = e & 31;
+      const int l = blk + i;
+      const int bm = col >> 1;
+      double v = 0.0;
+      if (l < L && l >= a.lmin_out && b0 + bm < a.batch) {
+        const cplx f = flm[(int64_t)bm * a.flm_map_stride + (int64_t)l * (2 * L - 1)];
+        v = (col & 1) ? f.y : f.x;
+      }
+      __bf16 h, lo;
+      split2(v, h, lo);
+      const int par = i & 1;
+      bth[par][col * 32 + i] = h;
+      btl[par][col * 32 + i] = lo;
+      bth[par ^ 1][col * 32 + i] = (__bf16)0.0f;
+      btl[par ^ 1][col * 32 + i] = (__bf16)0.0f;
+    }
+    // 32 recursion steps into the A tile, 8 l per 16-byte store
+    const bool live = ls == 0;
+#pragma unroll
+    for (int g8 = 0; g8 < 4; ++g8) {
+      bf16x8 vh, vl;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int l = blk + 8 * g8 + e;
+        const double v = (live && l < L) ? d : 0.0;
+        __bf16 h, lo;
+        split2(v, h, lo);
+        vh[e] = h;
+        vl[e] = lo;
+        if (l + 1 < L) {
+          const Coef c = cf[l];
+          const double dn = fma(fma(c.a, x, -c.b * bsign), d, -c.c * dm1);
+          dm1 = d;
+          d = dn;
+        }
+      }
+      *reinterpret_cast<bf16x8*>(dh + lane * 32 + 8 * g8) = vh;
+      *reinterpret_cast<bf16x8*>(dl + lane * 32 + 8 * g8) = vl;
+    }
+    lat::rescale(d, dm1, ls);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      // A[i][k]: pair 16 q + row, l = 8 kg + e
+      const int aoff = (16 * q + row) * 32 + 8 * kg;
+      const bf16x8 ah = *reinterpret_cast<const bf16x8*>(dh + aoff);
+      const bf16x8 al = *reinterpret_cast<const bf16x8*>(dl + aoff);
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int par = 0; par < 2; ++par) {
+          // B[k][j]: column 16 t + row, l = 8 kg + e
+          const int boff = (16 * t + row) * 32 + 8 * kg;
+          const bf16x8 bh = *reinterpret_cast<const bf16x8*>(&bth[par][boff]);
+          const bf16x8 bl = *reinterpret_cast<const bf16x8*>(&btl[par][boff]);
+          acc[q][t][par] = mma3(ah, al, bh, bl, acc[q][t][par]);
+        }
+    }
+    __syncthreads();
+  }
+  // C[i][j]: pair 16 q + 4 kg + r, column 16 t + row -> map t*8 + row/2
+  // Parity is relative to l = blk offsets, i.e. to l0 = ell0, whose parity is
+  // that of |m| for spin 0 (the paired case).
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int bm = b0 + t * 8 + (row >> 1);
+    if (bm >= a.batch) continue;
+    double* out = reinterpret_cast<double*>(a.ftm + (int64_t)bm * a.ftm_map_stride +
+                                            (m - a.ftm_m_first)) + (row & 1);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int pp = pbase + 16 * q + 4 * kg + r;
+        if (pp >= a.nrings) continue;
+        const LatRing rg = a.rings[pp];
+        const double e = (double)acc[q][t][0][r], o = (double)acc[q][t][1][r];
+        out[2 * (int64_t)rg.ta * a.ftm_ncol] = e + o;
+        if (PAIRED && rg.tb >= 0 && rg.tb != rg.ta) out[2 * (int64_t)rg.tb * a.ftm_ncol] = e - o;
+      }
+  }
+}
+
+// ------------------------------------------------------------------ launchers
+hipError_t launch_lat_forward_bf16(const LatArgs& a0, bool paired, hipStream_t s) {
+  if (a0.nm == 0 || a0.nrings == 0 || a0.batch == 0) return hipSuccess;
+  LatArgs a = a0;
+  a.groups = (a.nrings + kFT - 1) / kFT;
+  dim3 grid(a.nm, a.groups, (a.batch + 8 * kFTiles - 1) / (8 * kFTiles));
+  if (paired)
+    hipLaunchKernelGGL(lat_forward_bf16_kernel<true>, grid, dim3(kFT), 0, s, a);
+  else
+    hipLaunchKernelGGL(lat_forward_bf16_kernel<false>, grid, dim3(kFT), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_lat_inverse_bf16(const LatArgs& a, bool paired, hipStream_t s) {
+  if (a.nm == 0 || a.nrings == 0 || a.batch == 0) return hipSuccess;
+  dim3 grid(a.nm, (a.nrings + kIT - 1) / kIT, (a.batch + kIMaps - 1) / kIMaps);
+  if (paired)
+    hipLaunchKernelGGL(lat_inverse_bf16_kernel<true>, grid, dim3(kIT), 0, s, a);
+  else
+    hipLaunchKernelGGL(lat_inverse_bf16_kernel<false>, grid, dim3(kIT), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace s2h

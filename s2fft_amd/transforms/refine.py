@@ -30,7 +30,8 @@ def refine_fp64(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=No
 
 
 def refine_mixed(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=None,
-                 precond=None, refresh_every=0, history=None, flm_true=None, stats=None):
+                 precond=None, refresh_every=0, history=None, flm_true=None, stats=None,
+                 precision="mixed", inv_precision=None):
     """Mixed-precision refinement with an FP64 residual (see module docstring).
 
     precond: None (Jacobi) or an object with ``apply(h, L, reality, lmin)``
@@ -40,31 +41,35 @@ def refine_mixed(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=N
     many steps (0 = only at the start).
     history: optional list that receives the max-relative error per step when
     ``flm_true`` is given.  stats: optional dict counting accepted learned steps.
+    precision / inv_precision: arithmetic of F~ / I~ ("mixed" FP32 or "bf16"
+    split-BF16 matrix cores; inv_precision defaults to precision).
     """
     s = sampling.lower()
     plan = plan or sph.get_plan(s, L, nside, spin, L_lower, reality, device=f.device)
     kw = dict(L=L, spin=spin, nside=nside, sampling=s, reality=reality, L_lower=L_lower,
               plan=plan)
-    flm = sph.forward_hip(f, precision="mixed", **kw)
+    fp, ip = precision, inv_precision or precision
+    flm = sph.forward_hip(f, precision=fp, **kw)
     r = f - _real_if(sph.inverse_hip(flm, **kw), f)
     _track(history, flm, flm_true)
     for k in range(n_iter):
-        h = sph.forward_hip(r, precision="mixed", **kw)
+        h = sph.forward_hip(r, precision=fp, **kw)
         last = k + 1 == n_iter
+        refresh = not last and refresh_every and (k + 1) % refresh_every == 0
         if precond is None:
             delta = h
-            if not last:
-                r = r - _real_if(sph.inverse_hip(delta, precision="mixed", **kw), f)
+            if not last and not refresh:
+                r = r - _real_if(sph.inverse_hip(delta, precision=ip, **kw), f)
         else:
-            delta, r = _judged_step(precond, h, r, f, kw, stats)
+            delta, r = _judged_step(precond, h, r, f, kw, stats, ip)
         flm = flm + delta
-        if not last and refresh_every and (k + 1) % refresh_every == 0:
+        if refresh:
             r = f - _real_if(sph.inverse_hip(flm, **kw), f)
         _track(history, flm, flm_true)
     return flm
 
 
-def _judged_step(precond, h, r, f, kw, stats):
+def _judged_step(precond, h, r, f, kw, stats, ip):
     """Learned step per map where it lowers the FP64-held residual, else Jacobi.
 
     The candidate delta = P(h) moves the residual to r - I~(delta); a map keeps
@@ -75,7 +80,7 @@ def _judged_step(precond, h, r, f, kw, stats):
     L = kw["L"]
     lmin = max(abs(kw["spin"]), kw["L_lower"])
     delta = precond.apply(h, L, kw["reality"], lmin)
-    r_new = r - _real_if(sph.inverse_hip(delta, precision="mixed", **kw), f)
+    r_new = r - _real_if(sph.inverse_hip(delta, precision=ip, **kw), f)
     dims = tuple(range(1, r.dim()))
     ok = _norm(r_new, dims) < _norm(r, dims)
     if stats is not None:
@@ -85,7 +90,7 @@ def _judged_step(precond, h, r, f, kw, stats):
         return delta, r_new
     bad = (~ok).nonzero().flatten()
     hj = h[bad].contiguous()
-    rj = r[bad] - _real_if(sph.inverse_hip(hj, precision="mixed", **kw), f)
+    rj = r[bad] - _real_if(sph.inverse_hip(hj, precision=ip, **kw), f)
     delta = delta.clone()
     r_new = r_new.clone()
     delta[bad] = hj

@@ -93,11 +93,14 @@ def _check_common(method, spin, sampling, nside):
         raise ValueError(f"Sampling scheme sampling={sampling} with nside={nside} not supported")
 
 
+PRECISIONS = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED, "bf16": _lib.PREC_BF16}
+
+
 def _precision(name):
     try:
-        return {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED}[name]
+        return PRECISIONS[name]
     except KeyError:
-        raise ValueError(f"precision={name!r}: expected 'fp64' or 'mixed'") from None
+        raise ValueError(f"precision={name!r}: expected one of {sorted(PRECISIONS)}") from None
 
 
 # ------------------------------------------------------------------ device level
@@ -105,8 +108,9 @@ def inverse_hip(flm, L, spin=0, nside=None, sampling="mw", reality=False, L_lowe
                 plan=None, precision="fp64"):
     """Batched inverse on device tensors: flm (B, L, 2L-1) complex128 -> f (B, ...).
 
-    ``precision``: "fp64" (default) or "mixed" (FP64 recursion, FP32 matrix-core
-    synthesis with FP32 accumulation within 32-l chunks).
+    ``precision``: "fp64" (default), "mixed" (FP64 recursion, FP32 matrix-core
+    synthesis with FP32 accumulation within 32-l chunks) or "bf16" (FP64
+    recursion, split-BF16 matrix-core synthesis, FP32 accumulation).
     """
     s = sampling.lower()
     plan = plan or get_plan(s, L, nside, spin, L_lower, reality, device=flm.device)
@@ -126,8 +130,9 @@ def forward_hip(f, L, spin=0, nside=None, sampling="mw", reality=False, L_lower=
                 precision="fp64"):
     """Batched forward on device tensors: f (B, ...) -> flm (B, L, 2L-1) complex128.
 
-    ``precision``: "fp64" (default) or "mixed" (FP64 recursion and ring FFTs,
-    FP32 matrix-core projection with FP32 accumulation over 128 rings).
+    ``precision``: "fp64" (default), "mixed" (FP64 recursion and ring FFTs,
+    FP32 matrix-core projection with FP32 accumulation over 128 rings) or
+    "bf16" (split-BF16 matrix-core projection, FP32 accumulation).
     """
     s = sampling.lower()
     plan = plan or get_plan(s, L, nside, spin, L_lower, reality, device=f.device)

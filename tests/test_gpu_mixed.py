@@ -1,11 +1,13 @@
-"""Mixed-precision transforms (FP64 recursion, FP32 matrix-core projection).
+"""Mixed-precision transforms (FP64 recursion, FP32 or split-BF16 matrix cores).
 
-Tolerances are stated per test.  A single mixed-precision transform carries
-the FP32 rounding of the d-values and of the data plus FP32 accumulation over
-128 rings (forward) or 32 l (inverse): about 1e-7 of max |flm| at these sizes;
-the tests allow 2e-6 and also require the error to be above 1e-12, which
-shows that the FP32 path (not the FP64 one) ran.  Refinement with an FP64
-residual must reach the FP64 target of 1e-7 against the seeded coefficients.
+Tolerances are stated per test and precision.  "mixed": FP32 rounding of the
+d-values and of the data plus FP32 accumulation over 128 rings (forward) or
+32 l (inverse), about 1e-7 of max |flm| at these sizes; the tests allow 2e-6.
+"bf16": hi + lo BF16 operands (16 significant bits), products without the
+lo*lo term, FP32 accumulation, about 1e-5; the tests allow 1e-4.  Every test
+also requires the error to be above 1e-12, which shows that the reduced
+precision path (not the FP64 one) ran.  Refinement with an FP64 residual must
+reach the FP64 target of 1e-7 against the seeded coefficients.
 """
 
 import numpy as np
@@ -19,6 +21,7 @@ from s2fft_amd.transforms import refine, spherical as sph
 
 pytestmark = pytest.mark.gpu
 MIXED_TOL = 2e-6
+TOLS = {"mixed": 2e-6, "bf16": 1e-4}
 
 
 def _flms(rng, B, L, spin, reality):
@@ -30,45 +33,48 @@ CASES = [("healpix", 16, 32, 0, True, 3), ("gl", None, 24, 2, False, 2),
          ("dh", None, 10, 0, False, 1)]
 
 
+@pytest.mark.parametrize("precision", ["mixed", "bf16"])
 @pytest.mark.parametrize("sampling,nside,L,spin,reality,B", CASES)
-def test_mixed_inverse_vs_oracle(rng, sampling, nside, L, spin, reality, B):
+def test_mixed_inverse_vs_oracle(rng, sampling, nside, L, spin, reality, B, precision):
     flms = _flms(rng, B, L, spin, reality)
     got = sph.inverse_hip(torch.from_numpy(flms).cuda(), L, spin, nside, sampling, reality,
-                          precision="mixed").cpu().numpy()
+                          precision=precision).cpu().numpy()
     for i in range(B):
         ref = otr.inverse(flms[i], L, spin, nside, sampling, reality)
         if reality:
             ref = np.real(ref)
         err = max_rel(got[i], ref)
-        assert 1e-12 < err < MIXED_TOL, err
+        assert 1e-12 < err < TOLS[precision], err
 
 
+@pytest.mark.parametrize("precision", ["mixed", "bf16"])
 @pytest.mark.parametrize("sampling,nside,L,spin,reality,B", CASES)
-def test_mixed_forward_vs_oracle(rng, sampling, nside, L, spin, reality, B):
+def test_mixed_forward_vs_oracle(rng, sampling, nside, L, spin, reality, B, precision):
     flms = _flms(rng, B, L, spin, reality)
     fs = np.stack([otr.inverse(flms[i], L, spin, nside, sampling, reality) for i in range(B)])
     if reality:
         fs = np.real(fs)
     got = sph.forward_hip(torch.from_numpy(fs).cuda(), L, spin, nside, sampling, reality,
-                          precision="mixed").cpu().numpy()
+                          precision=precision).cpu().numpy()
     for i in range(B):
         ref = otr.forward(fs[i], L, spin, nside, sampling, reality)
         err = max_rel(got[i], ref)
-        assert 1e-12 < err < MIXED_TOL, err
+        assert 1e-12 < err < TOLS[precision], err
 
 
-def test_mixed_vs_fp64_larger(rng):
-    """nside 128, L 256, 4 real maps: mixed against the FP64 kernels."""
-    nside, L, B = 128, 256, 4
+@pytest.mark.parametrize("precision", ["mixed", "bf16"])
+def test_mixed_vs_fp64_larger(rng, precision):
+    """nside 128, L 256, 20 real maps (a partial map tile): against the FP64 kernels."""
+    nside, L, B = 128, 256, 20
     flm = torch.from_numpy(_flms(rng, B, L, 0, True)).cuda()
     f64 = sph.inverse_hip(flm, L, 0, nside, "healpix", True)
-    fmx = sph.inverse_hip(flm, L, 0, nside, "healpix", True, precision="mixed")
+    fmx = sph.inverse_hip(flm, L, 0, nside, "healpix", True, precision=precision)
     e_inv = float((fmx - f64).abs().max() / f64.abs().max())
     g64 = sph.forward_hip(f64, L, 0, nside, "healpix", True)
-    gmx = sph.forward_hip(f64, L, 0, nside, "healpix", True, precision="mixed")
+    gmx = sph.forward_hip(f64, L, 0, nside, "healpix", True, precision=precision)
     e_fwd = float((gmx - g64).abs().max() / g64.abs().max())
-    print(f"mixed vs fp64 at nside {nside}: inverse {e_inv:.2e}, forward {e_fwd:.2e}")
-    assert 1e-12 < e_inv < MIXED_TOL and 1e-12 < e_fwd < MIXED_TOL
+    print(f"{precision} vs fp64 at nside {nside}: inverse {e_inv:.2e}, forward {e_fwd:.2e}")
+    assert 1e-12 < e_inv < TOLS[precision] and 1e-12 < e_fwd < TOLS[precision]
 
 
 def test_mixed_refinement_reaches_fp64_target(rng):
@@ -89,3 +95,15 @@ def test_mixed_refinement_reaches_fp64_target(rng):
     # the mixed iteration contracts like the FP64 one (at most one step more)
     k = next(i for i, e in enumerate(hist_d) if e <= 1e-7)
     assert min(hist_m[k + 1:k + 3]) <= 1e-7
+
+
+def test_bf16_refinement_with_refresh_reaches_fp64_target(rng):
+    """Split-BF16 inner transforms with an FP64 residual every 3 steps reach 1e-7."""
+    nside, L, B = 64, 128, 4
+    flm_true = torch.from_numpy(_flms(rng, B, L, 0, True)).cuda()
+    f = sph.inverse_hip(flm_true, L, 0, nside, "healpix", True).real.contiguous()
+    hist = []
+    refine.refine_mixed(f, L, 0, nside, "healpix", True, 12, history=hist, flm_true=flm_true,
+                        precision="bf16", refresh_every=3)
+    print("bf16+refresh", ["%.1e" % e for e in hist])
+    assert min(hist) <= 1e-7
