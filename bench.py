@@ -1,15 +1,25 @@
 """Headline benchmark: HEALPix forward transforms at L=2048, maps/s on one GPU.
 
 Workload (one step): the forward spin-0 transform of a batch of 16 real
-HEALPix maps at nside=1024 (L=2048), refined with Jacobi iterations
-(flm += F(f - I(flm))) until the max-relative error against the seeded
-coefficients is <= 1e-7.  Maps are band-limited: f = I(flm) for flm drawn
-with seeds 300..315 (the reference's signal-generator distribution).  The
-iteration count is fixed before timing by running the refinement once.
+HEALPix maps at nside=1024 (L=2048), refined until the max-relative error
+against the seeded coefficients is <= 1e-7.  Maps are band-limited:
+f = I(flm) for flm drawn with seeds 300..315 (the reference's
+signal-generator distribution).  Two pipelines are timed on the same maps:
 
-Prints one JSON line (rank 0) with the measured maps/s, the roofline of the
-dominant kernel (timed live with HIP events on its launch stream) and a CPU
-baseline from the NumPy oracle on a bounded sample of the same workload.
+* plain   -- the reference's Jacobi refinement flm += F(f - I(flm)) on the
+             FP64 kernels (the "plain FP64 GPU" baseline of the metric);
+* mixed   -- the mixed-precision refinement (FP64 recursion and residual,
+             split-BF16 matrix-core projection / synthesis, FP64 residual
+             refresh) with the learned preconditioner when its trained
+             weights are present (else its Jacobi fallback; the JSON says
+             which).
+
+Iteration counts are fixed before timing by running each pipeline once.
+`value` is the mixed pipeline's maps/s; the plain one and the ratio sit in
+`config`.  The roofline is reported for the plain pipeline's dominant kernel
+(timed live with HIP events on its launch stream) and, under
+`config.mixed_roofline`, for the mixed pipeline's.  The CPU baseline is the
+NumPy oracle on a bounded sample of the same workload.
 """
 
 import argparse
@@ -21,6 +31,8 @@ import time
 
 METRIC = "maps/s at L=2048 HEALPix: mixed+learned vs plain FP64 GPU at ≤1e-7 rel err"
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix
+HBM_PEAK_GBS = 8000.0
 
 
 def parse():
@@ -33,6 +45,10 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-7)
     ap.add_argument("--max-iter", type=int, default=40)
     ap.add_argument("--seed0", type=int, default=300)
+    ap.add_argument("--mixed-precision", default="bf16", choices=["bf16", "mixed"])
+    ap.add_argument("--refresh-every", type=int, default=3)
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     return ap.parse_args()
 
@@ -41,14 +57,14 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(nside, L, n_transforms, rings=256, msteps=256):
+def cpu_baseline(nside, L, n_transforms, rings=512, msteps=256):
     """Oracle (NumPy, one core) on a bounded sample of the same workload.
 
     Sample: the oracle's Price-McEwen latitudinal recursion for `rings` of
     the 4*nside-1 rings and the first `msteps` of its L m-steps (every step
     costs the same: it updates full (rings x L) arrays), plus one full
     ring FFT of a map.  Scaled to one transform, times the transforms of a
-    refined forward.
+    plain refined forward.
     """
     import numpy as np
 
@@ -83,6 +99,41 @@ def cpu_baseline(nside, L, n_transforms, rings=256, msteps=256):
     }
 
 
+def kernel_times(_lib):
+    names = {"lat_forward": _lib.KERNEL_LAT_FORWARD, "lat_inverse": _lib.KERNEL_LAT_INVERSE,
+             "ring_fft_forward": _lib.KERNEL_RING_FFT_FORWARD,
+             "ring_fft_inverse": _lib.KERNEL_RING_FFT_INVERSE}
+    return {k: _lib.profile_query(v) for k, v in names.items()}
+
+
+def roofline(kernels, nside, L, B, lat_precision):
+    """Roofline entry of the dominant kernel kind (by total time).
+
+    Latitudinal kernels: algorithmic flops per launch = ceil(ntheta/2) ring
+    pairs x P = L(L+1)/2 coefficients x (5 + 4 B) (5 for the recursion step,
+    4 per map for one complex x real multiply-add).  FP64 launches are priced
+    against the FP64 peak, split-BF16 ones against the dense BF16 peak.
+    Ring FFTs: algorithmic bytes per launch = B maps x (8 B per pixel + 16 B
+    per ftm element, ntheta x L).
+    """
+    dom, (ms, n) = max(kernels.items(), key=lambda kv: kv[1][0])
+    avg_ms = ms / max(n, 1)
+    ntheta = 4 * nside - 1
+    if dom.startswith("lat"):
+        flop = math.ceil(ntheta / 2) * (L * (L + 1) // 2) * (5 + 4 * B)
+        achieved = flop / (avg_ms * 1e-3) / 1e12
+        peak = BF16_PEAK_TFLOPS if lat_precision == "bf16" else FP64_PEAK_TFLOPS
+        return {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                "frac": achieved / peak, "traffic": None, "kernel": dom,
+                "arith": "split-bf16" if lat_precision == "bf16" else "fp64",
+                "flop_per_launch": flop, "avg_launch_ms": avg_ms}
+    nbytes = B * (8 * 12 * nside * nside + 16 * ntheta * L)
+    achieved = nbytes / (avg_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+            "bytes_per_launch": nbytes, "avg_launch_ms": avg_ms}
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -94,6 +145,7 @@ def main():
     import torch
 
     from s2fft_amd import _lib
+    from s2fft_amd.transforms import refine
     from s2fft_amd.transforms import spherical as sph
     from s2fft_amd.utils.signal_generator import generate_flm
 
@@ -114,69 +166,77 @@ def main():
     def max_rel(x):
         return float((x - flm_true).abs().max() / flm_true.abs().max())
 
-    # fix the iteration count: refine until <= tol
-    log("[bench] calibrating the refinement count")
-    flm = sph.forward_hip(f, L, 0, nside, "healpix", True, plan=plan)
-    errs = [max_rel(flm)]
-    n_iter = 0
-    while errs[-1] > args.tol and n_iter < args.max_iter:
-        rec = sph.inverse_hip(flm, L, 0, nside, "healpix", True, plan=plan)
-        flm = flm + sph.forward_hip(f - rec, L, 0, nside, "healpix", True, plan=plan)
-        n_iter += 1
-        errs.append(max_rel(flm))
-        log(f"[bench]   iter {n_iter}: max-rel err {errs[-1]:.3e}")
-    del flm, rec
-    torch.cuda.empty_cache()
-
-    def step():
-        return sph.forward_iterative_hip(f, L, 0, nside, "healpix", True, 0, n_iter)
-
-    for i in range(args.warmup):
-        out = step()
+    def timed(step, label):
+        out = None
+        for i in range(args.warmup):
+            out = step()
+            torch.cuda.synchronize()
+            log(f"[bench] {label} warmup {i + 1}/{args.warmup} done")
+        err = max_rel(out) if out is not None else float("nan")
+        del out
+        _lib.profile_enable(True)
         torch.cuda.synchronize()
-        log(f"[bench] warmup {i + 1}/{args.warmup} done")
-    final_err = max_rel(out) if args.warmup else float("nan")
-    del out
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            out = step()
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        kt = kernel_times(_lib)
+        _lib.profile_enable(False)
+        if not args.warmup:
+            err = max_rel(out)
+        del out
+        log(f"[bench] {label}: {args.steps} steps in {elapsed:.3f} s, err {err:.2e}")
+        return elapsed, err, kt
 
-    _lib.profile_enable(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        out = step()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    log(f"[bench] timed {args.steps} steps in {elapsed:.3f} s")
-    lat_f = _lib.profile_query(_lib.KERNEL_LAT_FORWARD)
-    lat_i = _lib.profile_query(_lib.KERNEL_LAT_INVERSE)
-    fft_f = _lib.profile_query(_lib.KERNEL_RING_FFT_FORWARD)
-    fft_i = _lib.profile_query(_lib.KERNEL_RING_FFT_INVERSE)
-    _lib.profile_enable(False)
-    if not args.warmup:
-        final_err = max_rel(out)
+    # ---------------------------------------------------------------- plain
+    plain = None
+    if not args.no_plain:
+        log("[bench] plain FP64: calibrating the Jacobi count")
+        flm = sph.forward_hip(f, L, 0, nside, "healpix", True, plan=plan)
+        errs = [max_rel(flm)]
+        n_plain = 0
+        while errs[-1] > args.tol and n_plain < args.max_iter:
+            rec = sph.inverse_hip(flm, L, 0, nside, "healpix", True, plan=plan)
+            flm = flm + sph.forward_hip(f - rec, L, 0, nside, "healpix", True, plan=plan)
+            n_plain += 1
+            errs.append(max_rel(flm))
+            log(f"[bench]   iter {n_plain}: max-rel err {errs[-1]:.3e}")
+        del flm
+        torch.cuda.empty_cache()
+        el, err, kt = timed(lambda: refine.refine_fp64(f, L, 0, nside, "healpix", True, n_plain),
+                            "plain")
+        plain = {"elapsed": el, "err": err, "kernels": kt, "iters": n_plain, "calib": errs,
+                 "roofline": roofline(kt, nside, L, B, "fp64")}
 
-    ntheta = 4 * nside - 1
-    P = L * (L + 1) // 2
-    flop_launch = math.ceil(ntheta / 2) * P * (5 + 4 * B)   # one launch = B maps
-    kernels = {"lat_forward": lat_f, "lat_inverse": lat_i, "ring_fft_forward": fft_f,
-               "ring_fft_inverse": fft_i}
-    dom_name, (dom_ms, dom_n) = max(kernels.items(), key=lambda kv: kv[1][0])
-    if dom_name.startswith("lat"):
-        avg_ms = dom_ms / max(dom_n, 1)
-        achieved = flop_launch / (avg_ms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                "kernel": dom_name, "pipe": "fp64 VALU (peak equals the FP64 MFMA peak)",
-                "flop_per_launch": flop_launch, "avg_launch_ms": avg_ms}
-    else:
-        avg_ms = dom_ms / max(dom_n, 1)
-        nbytes = B * (8 * 12 * nside * nside + 16 * ntheta * L)
-        achieved = nbytes / (avg_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                "frac": achieved / 8000.0, "traffic": None, "kernel": dom_name,
-                "bytes_per_launch": nbytes, "avg_launch_ms": avg_ms}
+    # ---------------------------------------------------------------- mixed
+    precond, learned = None, False
+    if not args.no_learned:
+        from s2fft_amd.learned import precond as lp
+        if lp.WEIGHTS.exists():
+            precond = lp.MLPPreconditioner.load(device=dev)
+            learned = True
+        else:
+            log("[bench] no trained preconditioner weights: the mixed leg runs Jacobi")
+    prec = args.mixed_precision
+    log(f"[bench] mixed ({prec}, learned={learned}): calibrating the step count")
+    hist, stats = [], {}
+    refine.refine_mixed(f, L, 0, nside, "healpix", True, args.max_iter, plan=plan,
+                        precond=precond, refresh_every=args.refresh_every, history=hist,
+                        flm_true=flm_true, stats=stats, precision=prec)
+    ok = [i for i, e in enumerate(hist) if e <= args.tol]
+    n_mixed = ok[0] if ok else args.max_iter
+    log(f"[bench]   mixed errors {['%.1e' % e for e in hist[:n_mixed + 2]]} -> {n_mixed} steps")
+    torch.cuda.empty_cache()
+    el, err, kt = timed(lambda: refine.refine_mixed(
+        f, L, 0, nside, "healpix", True, n_mixed, plan=plan, precond=precond,
+        refresh_every=args.refresh_every, precision=prec), "mixed")
+    mixed = {"elapsed": el, "err": err, "kernels": kt, "iters": n_mixed,
+             "roofline": roofline(kt, nside, L, B, prec)}
 
-    n_transforms = 1 + 2 * n_iter
-    value = B * args.steps / elapsed
+    value = B * args.steps / mixed["elapsed"]
+    plain_value = B * args.steps / plain["elapsed"] if plain else None
+    head = plain["roofline"] if plain else mixed["roofline"]
     result = {
         "metric": METRIC,
         "value": value,
@@ -184,26 +244,43 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps,
+        "ms_per_step": 1e3 * mixed["elapsed"] / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": (f"HEALPix nside={nside} L={L} real spin-0 forward of a {B}-map batch, "
-                         f"FP64 Jacobi refinement to <= {args.tol:g} max-rel error"),
-            "method": "plain FP64 HIP kernels",
-            "nside": nside, "L": L, "batch": B, "refine_iters": n_iter,
-            "transforms_per_step": n_transforms, "max_rel_err": final_err,
-            "calibration_errs": errs,
-            "kernel_ms": {k: {"total_ms": v[0], "launches": v[1]} for k, v in kernels.items()},
+            "workload": (f"HEALPix nside={nside} L={L} real spin-0 forward of a {B}-map batch "
+                         f"refined to <= {args.tol:g} max-rel error vs the seeded flm"),
+            "method": (f"mixed: FP64 recursion + {prec} matrix-core projection/synthesis, "
+                       f"FP64 residual refreshed every {args.refresh_every} steps, "
+                       f"{'learned MLP preconditioner (Jacobi-judged)' if learned else 'Jacobi'}"),
+            "nside": nside, "L": L, "batch": B, "learned": learned,
+            "mixed_steps": n_mixed, "mixed_max_rel_err": mixed["err"],
+            "mixed_calibration_errs": hist[:n_mixed + 1],
+            "learned_steps_accepted": stats.get("accepted"),
+            "learned_steps_judged": stats.get("steps"),
+            "mixed_kernel_ms": {k: {"total_ms": v[0], "launches": v[1]}
+                                for k, v in mixed["kernels"].items()},
+            "mixed_roofline": mixed["roofline"],
         },
-        "roofline": roof,
+        "roofline": head,
     }
+    if plain:
+        result["config"].update({
+            "plain_fp64_maps_per_s": plain_value,
+            "speedup_vs_plain_fp64": value / plain_value,
+            "plain_ms_per_step": 1e3 * plain["elapsed"] / args.steps,
+            "plain_refine_iters": plain["iters"], "plain_max_rel_err": plain["err"],
+            "plain_calibration_errs": plain["calib"],
+            "plain_kernel_ms": {k: {"total_ms": v[0], "launches": v[1]}
+                                for k, v in plain["kernels"].items()},
+        })
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle sample)")
-        result["cpu_baseline"] = cpu_baseline(nside, L, n_transforms)
+        n_tr = 1 + 2 * (plain["iters"] if plain else n_mixed)
+        result["cpu_baseline"] = cpu_baseline(nside, L, n_tr)
     print(json.dumps(result), flush=True)
 
 

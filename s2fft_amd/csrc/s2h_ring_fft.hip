@@ -143,7 +143,7 @@ __device__ __forceinline__ void pass_t(double* re, double* im, int n, int Ns, in
     for (int k = 1; k < H; ++k) cs[k] = conj_if(tw.at(k * (n / R)), cj);
   }
   // radices 2 and 4 only ever run as the last pass (see fft_t)
-  if (R <= 4 || Ns * R == n) {
+  if (R == 2 || R == 4 || Ns * R == n) {
     for (int j = threadIdx.x; j < nb; j += TPB) {
       cplx v[R];
       butterfly<R>(v, re, im, j, nb, Ns, step, cj, sign, tw, cs);
@@ -156,7 +156,7 @@ __device__ __forceinline__ void pass_t(double* re, double* im, int n, int Ns, in
     __syncthreads();
     return;
   }
-  if constexpr (R > 4) {
+  if constexpr (R != 2 && R != 4) {
     cplx v[MAXB][R];
 #pragma unroll
     for (int c = 0; c < MAXB; ++c) {

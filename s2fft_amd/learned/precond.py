@@ -13,12 +13,12 @@ the network learns e_k from h_k.  Training runs on the GPU with the
 library's forward/backward kernels and Adam.
 
 The reference has no counterpart (its refinement is plain Jacobi,
-s2fft/utils/iterative_refinement.py:9-46); Jacobi stays the fallback: a
-learned step is kept for a map only where it lowers that map's residual norm
-more than the Jacobi step would be expected to (see ``refine.refine_mixed``).
+s2fft/utils/iterative_refinement.py:9-46); Jacobi stays the fallback and
+judge: a learned step is kept for a map only where it lowers that map's
+FP64-held residual norm, else the map takes the Jacobi step
+(``refine._judged_step``).
 """
 
-import ctypes
 import math
 from pathlib import Path
 
@@ -136,11 +136,13 @@ class Adam:
             float(self.betas[1]), float(self.eps), _stream(self.params)), "mlp adam")
 
 
-def make_training_rows(nside, n_maps, seed=1000, batch=16, iters=(0, 1, 2), log=None):
+def make_training_rows(nside, n_maps, seed=1000, batch=16, iters=(0, 1, 2), keep=1.0,
+                       log=None):
     """Rows (X, Y) from Jacobi iterates of the plain FP64 HEALPix transform.
 
     Maps: seeded band-limited real spin-0 signals at L = 2 nside (flm entries
-    complex N(0,1) for m > 0, real N(0,1) for m = 0).
+    complex N(0,1) for m > 0, real N(0,1) for m = 0).  ``keep`` < 1 keeps a
+    seeded random fraction of the rows of every (map batch, iterate).
     """
     L = 2 * nside
     dev = torch.device("cuda")
@@ -168,6 +170,9 @@ def make_training_rows(nside, n_maps, seed=1000, batch=16, iters=(0, 1, 2), log=
             h = sph.forward_hip(r.contiguous(), L, 0, nside, "healpix", True, plan=plan)
             if k in iters:
                 X, Y = features(h, flm - x, L, True)
+                if keep < 1.0:
+                    sel = torch.rand(X.shape[0], generator=gen, device=dev) < keep
+                    X, Y = X[sel], Y[sel]
                 Xs.append(X)
                 Ys.append(Y)
             x = x + h

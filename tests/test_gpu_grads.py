@@ -76,7 +76,7 @@ def test_inverse_adjoint_dot_product(rng, sampling, L, nside, spin, reality):
 
 @pytest.mark.parametrize("sampling,L,nside,spin,reality", CASES[:4] + CASES[7:9])
 def test_forward_loss_gradient_finite_differences(rng, sampling, L, nside, spin, reality):
-    """Central FD, h = 1e-6, of sum |forward(f) - t|^2 (SURVEY config 4a form)."""
+    """Central FD, h = 1e-6, of sum |forward(f) - t|^2 (the Gauss-Legendre gradient case)."""
     f0 = _rand_f(rng, L, sampling, nside, reality)
     t = signals.generate_flm(rng, L, spin=spin, reality=reality)
     tt = torch.from_numpy(t).cuda()
