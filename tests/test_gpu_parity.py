@@ -170,16 +170,19 @@ def test_latitudinal_entry(rng, sampling, L, nside, spin):
 
 
 def test_batched_equals_single(rng):
+    """A batch gives each map's single-map result.  Real maps share their ring
+    FFTs in pairs (z = x + i y), so a map's rounding depends on its partner:
+    agreement to 1e-14 rather than bit for bit."""
     nside, L = 8, 16
     flms = np.stack([_flm(rng, L, 0, True) for _ in range(3)])
     f_b = s2fft_amd.inverse(flms, L, 0, nside, "healpix", "hip", True)
     for i in range(3):
         f_i = s2fft_amd.inverse(flms[i], L, 0, nside, "healpix", "hip", True)
-        assert np.array_equal(f_b[i], f_i)
+        assert max_rel(f_b[i], f_i) < 1e-14
     g_b = s2fft_amd.forward(np.real(f_b), L, 0, nside, "healpix", "hip", True)
     for i in range(3):
         g_i = s2fft_amd.forward(np.real(f_b[i]), L, 0, nside, "healpix", "hip", True)
-        assert np.array_equal(g_b[i], g_i)
+        assert max_rel(g_b[i], g_i) < 1e-14
 
 
 @pytest.mark.parametrize("nside", [64, 128])
