@@ -55,9 +55,15 @@ __global__ void __launch_bounds__(kThreads) lat_inverse_kernel(LatArgs a) {
   const int nb = min(NB, a.batch - b0);
   for (int blk = st.ell0; blk < L; blk += kCheck) {
     const bool live = lscale == 0;
-    const int end = min(blk + kCheck, L);
-    for (int l = blk; l < end; ++l) {
-      if (l >= a.lmin_out) {
+    // the block's coefficients are loaded together (one memory latency per
+    // kCheck steps instead of one per step)
+    lat::ChunkCoef cs[kCheck];
+#pragma unroll
+    for (int i = 0; i < kCheck; ++i) cs[i] = lat::load_coef(cf, blk, i, L, bsign);
+#pragma unroll
+    for (int i = 0; i < kCheck; ++i) {
+      const int l = blk + i;
+      if (l < L && l >= a.lmin_out) {
         const bool odd = PAIRED && ((l + m) & 1);
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
@@ -73,12 +79,9 @@ __global__ void __launch_bounds__(kThreads) lat_inverse_kernel(LatArgs a) {
           }
         }
       }
-      if (l + 1 < L) {
-        const Coef c = cf[l];
-        const double dn = fma(fma(c.a, x, -c.b * bsign), d, -c.c * dm1);
-        dm1 = d;
-        d = dn;
-      }
+      const double dn = lat::rec_step(cs[i], x, d, dm1);
+      dm1 = d;
+      d = dn;
     }
     if (!live) {
 #pragma unroll
@@ -152,6 +155,9 @@ __global__ void __launch_bounds__(kThreads) lat_forward_kernel(LatArgs a) {
       uo[r] = live ? so[r] : cplx{0.0, 0.0};
     }
     cplx part[kChunk];
+    lat::ChunkCoef cs[kChunk];
+#pragma unroll
+    for (int i = 0; i < kChunk; ++i) cs[i] = lat::load_coef(cf, blk, i, L, bsign);
 #pragma unroll
     for (int i = 0; i < kChunk; ++i) {
       const int l = blk + i;
@@ -164,15 +170,11 @@ __global__ void __launch_bounds__(kThreads) lat_forward_kernel(LatArgs a) {
         acc.y = fma(d[r], u.y, acc.y);
       }
       part[i] = acc;
-      if (l + 1 < L) {
-        const Coef c = cf[l];
-        const double ab = -c.b * bsign;
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          const double dn = fma(fma(c.a, x[r], ab), d[r], -c.c * dm1[r]);
-          dm1[r] = d[r];
-          d[r] = dn;
-        }
+      for (int r = 0; r < R; ++r) {
+        const double dn = lat::rec_step(cs[i], x[r], d[r], dm1[r]);
+        dm1[r] = d[r];
+        d[r] = dn;
       }
     }
 #pragma unroll

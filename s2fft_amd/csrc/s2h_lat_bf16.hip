@@ -64,6 +64,7 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
   __shared__ __bf16 dtl[kFW][32 * 64];
   // per-wave results: [tile][parity][row 16][col 16]
   __shared__ float ctile[kFW][kFTiles * 2 * 256];
+  __shared__ lat::ChunkCoef ccs[2][lat::kChunk];  // recursion coefficients, double-buffered
   const int m = m_of_block(blockIdx.x, a.nm, a.m_lo);
   const int am = m < 0 ? -m : m;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -123,7 +124,14 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
   }
 
   cplx* out = a.flm + (int64_t)b0 * a.flm_map_stride + (L - 1 + m);
-  for (int blk = st.ell0; blk < L; blk += 32) {
+  lat::ChunkCoef cn = {0.0, 0.0, 0.0, 0.0};
+  if (threadIdx.x < lat::kChunk) ccs[0][threadIdx.x] = lat::load_coef(cf, st.ell0, threadIdx.x, L, bsign);
+  __syncthreads();
+  int buf = 0;
+  for (int blk = st.ell0; blk < L; blk += 32, buf ^= 1) {
+    const bool more = blk + 32 < L;
+    if (more && threadIdx.x < lat::kChunk) cn = lat::load_coef(cf, blk + 32, threadIdx.x, L, bsign);
+    const lat::ChunkCoef* cs = ccs[buf];
     const bool live = ls == 0;
 #pragma unroll
     for (int i = 0; i < 32; ++i) {
@@ -135,12 +143,9 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
       const int col = ((i & 1) << 4) | (i >> 1);
       dh[col * 64 + lane] = h;
       dl[col * 64 + lane] = lo;
-      if (l + 1 < L) {
-        const Coef c = cf[l];
-        const double dn = fma(fma(c.a, x, -c.b * bsign), d, -c.c * dm1);
-        dm1 = d;
-        d = dn;
-      }
+      const double dn = lat::rec_step(cs[i], x, d, dm1);
+      dm1 = d;
+      d = dn;
     }
     lat::rescale(d, dm1, ls);
     lat::wave_sync();
@@ -176,6 +181,7 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
 #pragma unroll
         for (int r = 0; r < 4; ++r)
           ctile[wave][(t * 2 + par) * 256 + (4 * kg + r) * 16 + row] = acc[t][par][r];
+    if (more && threadIdx.x < lat::kChunk) ccs[buf ^ 1][threadIdx.x] = cn;
     __syncthreads();
     // element (t, parity, row i, col j) -> l = blk + 2 j + parity, map 8t + i/2
     for (int e = threadIdx.x; e < kFTiles * 2 * 256; e += kFT) {
@@ -214,6 +220,7 @@ __global__ void __launch_bounds__(kIT) lat_inverse_bf16_kernel(LatArgs a) {
   // entries of the other parity zero
   __shared__ __bf16 bth[2][32 * 32];
   __shared__ __bf16 btl[2][32 * 32];
+  __shared__ lat::ChunkCoef ccs[2][lat::kChunk];  // recursion coefficients, double-buffered
   const int m = m_of_block(blockIdx.x, a.nm, a.m_lo);
   const int am = m < 0 ? -m : m;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -246,27 +253,56 @@ __global__ void __launch_bounds__(kIT) lat_inverse_bf16_kernel(LatArgs a) {
     }
 
   const cplx* flm = a.flm + (int64_t)b0 * a.flm_map_stride + (L - 1 + m);
-  for (int blk = st.ell0; blk < L; blk += 32) {
-    // stage flm[b0 + col/2][blk + i][m] (component col%2) by parity of i
-    for (int e = threadIdx.x; e < 32 * 32; e += kIT) {
+  // flm chunk staging: thread t holds elements e = t + kIT * j (j < 4) of
+  // [column 32][l_local 32]; the next chunk is loaded while this one runs
+  constexpr int kStage = 32 * 32 / kIT;
+  auto load_flm = [&](int blk, double* v) {
+#pragma unroll
+    for (int j = 0; j < kStage; ++j) {
+      const int e = threadIdx.x + kIT * j;
       const int col = e >> 5, i = e & 31;
-      const int l = blk + i;
-      const int bm = col >> 1;
-      double v = 0.0;
+      const int l = blk + i, bm = col >> 1;
+      double x0 = 0.0;
       if (l < L && l >= a.lmin_out && b0 + bm < a.batch) {
-        const cplx f = flm[(int64_t)bm * a.flm_map_stride + (int64_t)l * (2 * L - 1)];
-        v = (col & 1) ? f.y : f.x;
+        const double* f = reinterpret_cast<const double*>(
+            flm + (int64_t)bm * a.flm_map_stride + (int64_t)l * (2 * L - 1));
+        x0 = f[col & 1];
       }
+      v[j] = x0;
+    }
+  };
+  // by parity of i, the other parity's entry zero
+  auto store_flm = [&](const double* v) {
+#pragma unroll
+    for (int j = 0; j < kStage; ++j) {
+      const int e = threadIdx.x + kIT * j;
+      const int col = e >> 5, i = e & 31;
       __bf16 h, lo;
-      split2(v, h, lo);
+      split2(v[j], h, lo);
       const int par = i & 1;
       bth[par][col * 32 + i] = h;
       btl[par][col * 32 + i] = lo;
       bth[par ^ 1][col * 32 + i] = (__bf16)0.0f;
       btl[par ^ 1][col * 32 + i] = (__bf16)0.0f;
     }
+  };
+  double fv[kStage];
+  lat::ChunkCoef cn = {0.0, 0.0, 0.0, 0.0};
+  load_flm(st.ell0, fv);
+  store_flm(fv);
+  if (threadIdx.x < lat::kChunk) ccs[0][threadIdx.x] = lat::load_coef(cf, st.ell0, threadIdx.x, L, bsign);
+  __syncthreads();
+  int buf = 0;
+  for (int blk = st.ell0; blk < L; blk += 32, buf ^= 1) {
+    const int nxt = blk + 32;
+    const bool more = nxt < L;
+    if (more) {
+      load_flm(nxt, fv);
+      if (threadIdx.x < lat::kChunk) cn = lat::load_coef(cf, nxt, threadIdx.x, L, bsign);
+    }
     // 32 recursion steps into the A tile, 8 l per 16-byte store
     const bool live = ls == 0;
+    const lat::ChunkCoef* cs = ccs[buf];
 #pragma unroll
     for (int g8 = 0; g8 < 4; ++g8) {
       bf16x8 vh, vl;
@@ -278,17 +314,15 @@ __global__ void __launch_bounds__(kIT) lat_inverse_bf16_kernel(LatArgs a) {
         split2(v, h, lo);
         vh[e] = h;
         vl[e] = lo;
-        if (l + 1 < L) {
-          const Coef c = cf[l];
-          const double dn = fma(fma(c.a, x, -c.b * bsign), d, -c.c * dm1);
-          dm1 = d;
-          d = dn;
-        }
+        const double dn = lat::rec_step(cs[8 * g8 + e], x, d, dm1);
+        dm1 = d;
+        d = dn;
       }
       *reinterpret_cast<bf16x8*>(dh + lane * 32 + 8 * g8) = vh;
       *reinterpret_cast<bf16x8*>(dl + lane * 32 + 8 * g8) = vl;
     }
     lat::rescale(d, dm1, ls);
+    if (more && threadIdx.x < lat::kChunk) ccs[buf ^ 1][threadIdx.x] = cn;
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -308,6 +342,7 @@ __global__ void __launch_bounds__(kIT) lat_inverse_bf16_kernel(LatArgs a) {
         }
     }
     __syncthreads();
+    if (more) store_flm(fv);
   }
   // C[i][j]: pair 16 q + 4 kg + r, column 16 t + row -> map t*8 + row/2
   // Parity is relative to l = blk offsets, i.e. to l0 = ell0, whose parity is

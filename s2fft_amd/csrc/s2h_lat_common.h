@@ -79,6 +79,32 @@ __device__ __forceinline__ int m_of_block(int i, int nm, int m_lo) {
   return (idx & 1) ? -((idx + 1) >> 1) : (idx >> 1);
 }
 
+// Recursion coefficients of one 32-step chunk, staged in LDS so that every
+// step reads them with a broadcast LDS load instead of a dependent global or
+// scalar load.  (a, -b * bsign, c) for l = blk + i; zero from l = L - 1 on,
+// which makes the steps past the band-limit produce zeros.
+struct ChunkCoef {
+  double a, nb, c, pad;
+};
+constexpr int kChunk = 32;
+
+// Loads chunk `blk`'s coefficient for lane i (< 32) into registers.
+__device__ __forceinline__ ChunkCoef load_coef(const Coef* __restrict__ cf, int blk, int i, int L,
+                                               double bsign) {
+  const int l = blk + i;
+  ChunkCoef v = {0.0, 0.0, 0.0, 0.0};
+  if (l + 1 < L) {
+    const Coef c = cf[l];
+    v = {c.a, -c.b * bsign, c.c, 0.0};
+  }
+  return v;
+}
+
+// One recursion step with staged coefficients: returns d_{l+1}.
+__device__ __forceinline__ double rec_step(const ChunkCoef& c, double x, double d, double dm1) {
+  return fma(fma(c.a, x, c.nb), d, -c.c * dm1);
+}
+
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();

@@ -13,7 +13,7 @@
 // and registers it needs: powers of two (radix-8 Stockham passes and one
 // radix 4 or 2), other lengths whose prime factors are all <= 13 (radices
 // 3, 5, 7, 11, 13 as well) and the rest (Bluestein's chirp-z over a
-// power-of-two convolution, 1024 threads).  Twiddles, chirps and ring
+// power-of-two convolution).  Twiddles, chirps and ring
 // phases come from two-level tables in LDS (e^{i a k} = hi[k / 64] lo[k % 64])
 // built once per workgroup, so no transcendental is evaluated per point.
 // Two real maps share one complex transform (z = x + i y, split by the
@@ -283,12 +283,12 @@ __device__ void ring_transform(const RingLds& s, int n, int M, const cplx* chirp
   if constexpr (KIND != kBlue) {
     fft_t<TPB, MAXN, KIND == kPow2, KIND == kPow2>(s.re, s.im, n, sign, tw);
   } else {
-  const Tab2 ch = {s.ch_hi, s.ch_lo};  // e^{-i pi k / n}, k < 2n
+  const Tab2 ch = {s.ch_hi, s.ch_lo};  // e^{-i pi k / n}, k < 2n (n <= 4096: k^2 < 2^31)
   const bool cj = sign > 0;
   for (int p = threadIdx.x; p < M; p += TPB) {
     if (p < n) {
       cplx x = {s.re[p], cj ? -s.im[p] : s.im[p]};
-      x = cmul(x, ch.at((int)(((int64_t)p * p) % (2 * n))));
+      x = cmul(x, ch.at((int)(((unsigned)p * (unsigned)p) % (unsigned)(2 * n))));
       s.re[p] = x.x;
       s.im[p] = x.y;
     } else {
@@ -306,7 +306,7 @@ __device__ void ring_transform(const RingLds& s, int n, int M, const cplx* chirp
   __syncthreads();
   fft_t<TPB, MAXN, true, false>(s.re, s.im, M, +1, tw);
   for (int k = threadIdx.x; k < n; k += TPB) {
-    const cplx a = cmul(cplx{s.re[k], s.im[k]}, ch.at((int)(((int64_t)k * k) % (2 * n))));
+    const cplx a = cmul(cplx{s.re[k], s.im[k]}, ch.at((int)(((unsigned)k * (unsigned)k) % (unsigned)(2 * n))));
     s.re[k] = a.x;
     s.im[k] = cj ? -a.y : a.y;
   }
@@ -470,8 +470,9 @@ __global__ void __launch_bounds__(TPB) ring_inv_kernel(RingInvArgs a, int job0, 
 // ------------------------------------------------------------------ launchers
 namespace {
 
-constexpr int kTpbDirect = 512;
-constexpr int kTpbBlue = 1024;
+constexpr int kTpbPow2 = 512;
+constexpr int kTpbSmooth = 256;
+constexpr int kTpbBlue = 512;
 constexpr int kTabBytes = 576 * sizeof(cplx);
 
 size_t ring_lds_bytes(int N) { return 2 * sizeof(double) * (size_t)N + kTabBytes; }
@@ -490,11 +491,11 @@ hipError_t launch_ring_fft_forward(const RingFwdArgs& a0, const JobSet& js, int 
   const int paired = fwd_paired(a) ? 1 : 0;
   const unsigned gy = paired ? (unsigned)((batch + 1) / 2) : (unsigned)batch;
   if (js.n_pow2)
-    hipLaunchKernelGGL((ring_fwd_kernel<kTpbDirect, kMaxRingLen, kPow2>), dim3(js.n_pow2, gy),
-                       dim3(kTpbDirect), ring_lds_bytes(js.max_pow2), s, a, 0, paired);
+    hipLaunchKernelGGL((ring_fwd_kernel<kTpbPow2, kMaxRingLen, kPow2>), dim3(js.n_pow2, gy),
+                       dim3(kTpbPow2), ring_lds_bytes(js.max_pow2), s, a, 0, paired);
   if (js.n_smooth)
-    hipLaunchKernelGGL((ring_fwd_kernel<kTpbDirect, kMaxRingLen, kSmooth>),
-                       dim3(js.n_smooth, gy), dim3(kTpbDirect), ring_lds_bytes(js.max_smooth), s,
+    hipLaunchKernelGGL((ring_fwd_kernel<kTpbSmooth, kMaxRingLen, kSmooth>),
+                       dim3(js.n_smooth, gy), dim3(kTpbSmooth), ring_lds_bytes(js.max_smooth), s,
                        a, js.n_pow2, paired);
   if (js.n_blue)
     hipLaunchKernelGGL((ring_fwd_kernel<kTpbBlue, kMaxConvLen, kBlue>), dim3(js.n_blue, gy),
@@ -512,11 +513,11 @@ hipError_t launch_ring_fft_inverse(const RingInvArgs& a0, const JobSet& js, int 
   const int paired = inv_paired(a) ? 1 : 0;
   const unsigned gy = paired ? (unsigned)((batch + 1) / 2) : (unsigned)batch;
   if (js.n_pow2)
-    hipLaunchKernelGGL((ring_inv_kernel<kTpbDirect, kMaxRingLen, kPow2>), dim3(js.n_pow2, gy),
-                       dim3(kTpbDirect), ring_lds_bytes(js.max_pow2), s, a, 0, paired);
+    hipLaunchKernelGGL((ring_inv_kernel<kTpbPow2, kMaxRingLen, kPow2>), dim3(js.n_pow2, gy),
+                       dim3(kTpbPow2), ring_lds_bytes(js.max_pow2), s, a, 0, paired);
   if (js.n_smooth)
-    hipLaunchKernelGGL((ring_inv_kernel<kTpbDirect, kMaxRingLen, kSmooth>),
-                       dim3(js.n_smooth, gy), dim3(kTpbDirect), ring_lds_bytes(js.max_smooth), s,
+    hipLaunchKernelGGL((ring_inv_kernel<kTpbSmooth, kMaxRingLen, kSmooth>),
+                       dim3(js.n_smooth, gy), dim3(kTpbSmooth), ring_lds_bytes(js.max_smooth), s,
                        a, js.n_pow2, paired);
   if (js.n_blue)
     hipLaunchKernelGGL((ring_inv_kernel<kTpbBlue, kMaxConvLen, kBlue>), dim3(js.n_blue, gy),
@@ -527,11 +528,11 @@ hipError_t launch_ring_fft_inverse(const RingInvArgs& a0, const JobSet& js, int 
 
 hipError_t allow_large_lds_rings() {
   const int bytes = (int)ring_lds_bytes(kMaxConvLen);
-  const void* fns[] = {(const void*)ring_fwd_kernel<kTpbDirect, kMaxRingLen, kPow2>,
-                       (const void*)ring_fwd_kernel<kTpbDirect, kMaxRingLen, kSmooth>,
+  const void* fns[] = {(const void*)ring_fwd_kernel<kTpbPow2, kMaxRingLen, kPow2>,
+                       (const void*)ring_fwd_kernel<kTpbSmooth, kMaxRingLen, kSmooth>,
                        (const void*)ring_fwd_kernel<kTpbBlue, kMaxConvLen, kBlue>,
-                       (const void*)ring_inv_kernel<kTpbDirect, kMaxRingLen, kPow2>,
-                       (const void*)ring_inv_kernel<kTpbDirect, kMaxRingLen, kSmooth>,
+                       (const void*)ring_inv_kernel<kTpbPow2, kMaxRingLen, kPow2>,
+                       (const void*)ring_inv_kernel<kTpbSmooth, kMaxRingLen, kSmooth>,
                        (const void*)ring_inv_kernel<kTpbBlue, kMaxConvLen, kBlue>};
   for (const void* f : fns) {
     const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
