@@ -47,6 +47,7 @@ def parse():
     ap.add_argument("--seed0", type=int, default=300)
     ap.add_argument("--mixed-precision", default="bf16", choices=["bf16", "mixed"])
     ap.add_argument("--refresh-every", type=int, default=3)
+    ap.add_argument("--learned-steps", type=int, default=2)
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -223,14 +224,16 @@ def main():
     hist, stats = [], {}
     refine.refine_mixed(f, L, 0, nside, "healpix", True, args.max_iter, plan=plan,
                         precond=precond, refresh_every=args.refresh_every, history=hist,
-                        flm_true=flm_true, stats=stats, precision=prec)
+                        flm_true=flm_true, stats=stats, precision=prec,
+                        learned_steps=args.learned_steps)
     ok = [i for i, e in enumerate(hist) if e <= args.tol]
     n_mixed = ok[0] if ok else args.max_iter
     log(f"[bench]   mixed errors {['%.1e' % e for e in hist[:n_mixed + 2]]} -> {n_mixed} steps")
     torch.cuda.empty_cache()
     el, err, kt = timed(lambda: refine.refine_mixed(
         f, L, 0, nside, "healpix", True, n_mixed, plan=plan, precond=precond,
-        refresh_every=args.refresh_every, precision=prec), "mixed")
+        refresh_every=args.refresh_every, precision=prec, learned_steps=args.learned_steps),
+        "mixed")
     mixed = {"elapsed": el, "err": err, "kernels": kt, "iters": n_mixed,
              "roofline": roofline(kt, nside, L, B, prec)}
 
@@ -255,7 +258,8 @@ def main():
                          f"refined to <= {args.tol:g} max-rel error vs the seeded flm"),
             "method": (f"mixed: FP64 recursion + {prec} matrix-core projection/synthesis, "
                        f"FP64 residual refreshed every {args.refresh_every} steps, "
-                       f"{'learned MLP preconditioner (Jacobi-judged)' if learned else 'Jacobi'}"),
+                       (f"learned MLP preconditioner (Jacobi-judged) on the first "
+                        f"{args.learned_steps} steps, Jacobi after" if learned else "Jacobi")),
             "nside": nside, "L": L, "batch": B, "learned": learned,
             "mixed_steps": n_mixed, "mixed_max_rel_err": mixed["err"],
             "mixed_calibration_errs": hist[:n_mixed + 1],

@@ -31,12 +31,14 @@ def refine_fp64(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=No
 
 def refine_mixed(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=None,
                  precond=None, refresh_every=0, history=None, flm_true=None, stats=None,
-                 precision="mixed", inv_precision=None):
+                 precision="mixed", inv_precision=None, learned_steps=None):
     """Mixed-precision refinement with an FP64 residual (see module docstring).
 
     precond: None (Jacobi) or an object with ``apply(h, L, reality, lmin)``
     returning a correction (the learned preconditioner); its steps are judged
-    per map against the residual (``_judged_step``).
+    per map against the residual (``_judged_step``).  learned_steps: how many
+    of the first steps use it (default all but the last, whose residual is
+    never formed); later steps are Jacobi.
     refresh_every: recompute the residual with the FP64 synthesis every this
     many steps (0 = only at the start).
     history: optional list that receives the max-relative error per step when
@@ -49,38 +51,48 @@ def refine_mixed(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=N
     kw = dict(L=L, spin=spin, nside=nside, sampling=s, reality=reality, L_lower=L_lower,
               plan=plan)
     fp, ip = precision, inv_precision or precision
+    if learned_steps is None:
+        learned_steps = max(n_iter - 1, 0)
     flm = sph.forward_hip(f, precision=fp, **kw)
     r = f - _real_if(sph.inverse_hip(flm, **kw), f)
     _track(history, flm, flm_true)
     for k in range(n_iter):
         h = sph.forward_hip(r, precision=fp, **kw)
         last = k + 1 == n_iter
-        refresh = not last and refresh_every and (k + 1) % refresh_every == 0
-        if precond is None:
-            delta = h
-            if not last and not refresh:
-                r = r - _real_if(sph.inverse_hip(delta, precision=ip, **kw), f)
+        refresh = bool(not last and refresh_every and (k + 1) % refresh_every == 0)
+        if precond is not None and k < learned_steps and not last:
+            delta, r = _judged_step(precond, h, r, f, flm, kw, stats, ip, refresh)
+            flm = flm + delta
         else:
-            delta, r = _judged_step(precond, h, r, f, kw, stats, ip)
-        flm = flm + delta
-        if refresh:
-            r = f - _real_if(sph.inverse_hip(flm, **kw), f)
+            flm = flm + h
+            if refresh:
+                r = f - _real_if(sph.inverse_hip(flm, **kw), f)
+            elif not last:
+                r = r - _real_if(sph.inverse_hip(h, precision=ip, **kw), f)
         _track(history, flm, flm_true)
     return flm
 
 
-def _judged_step(precond, h, r, f, kw, stats, ip):
+def _judged_step(precond, h, r, f, flm, kw, stats, ip, exact):
     """Learned step per map where it lowers the FP64-held residual, else Jacobi.
 
-    The candidate delta = P(h) moves the residual to r - I~(delta); a map keeps
-    it only if that residual's 2-norm is below the current one.  Rejected maps
-    take the Jacobi step delta = h instead (one more mixed synthesis, for
-    those maps only).
+    The candidate delta = P(h) moves the residual to r - I~(delta) (or, on a
+    refresh step, to the exact f - I(flm + delta)); a map keeps it only if
+    that residual's 2-norm is below the current one.  Rejected maps take the
+    Jacobi step delta = h instead (one more synthesis, for those maps only).
+    Returns (delta, new residual).
     """
     L = kw["L"]
     lmin = max(abs(kw["spin"]), kw["L_lower"])
     delta = precond.apply(h, L, kw["reality"], lmin)
-    r_new = r - _real_if(sph.inverse_hip(delta, precision=ip, **kw), f)
+
+    def residual(d, rows):
+        if exact:
+            return f[rows] - _real_if(sph.inverse_hip((flm[rows] + d).contiguous(), **kw), f)
+        return r[rows] - _real_if(sph.inverse_hip(d.contiguous(), precision=ip, **kw), f)
+
+    every = slice(None)
+    r_new = residual(delta, every)
     dims = tuple(range(1, r.dim()))
     ok = _norm(r_new, dims) < _norm(r, dims)
     if stats is not None:
@@ -90,7 +102,7 @@ def _judged_step(precond, h, r, f, kw, stats, ip):
         return delta, r_new
     bad = (~ok).nonzero().flatten()
     hj = h[bad].contiguous()
-    rj = r[bad] - _real_if(sph.inverse_hip(hj, precision=ip, **kw), f)
+    rj = residual(hj, bad)
     delta = delta.clone()
     r_new = r_new.clone()
     delta[bad] = hj

@@ -73,3 +73,13 @@ def test_api_argument_errors_before_any_device_work():
         s2fft_amd.forward(np.zeros((10, 19)), 10, spin=9, method="hip")
     with pytest.raises(ValueError):
         s2fft_amd.forward(np.zeros(48), 4, sampling="healpix", method="hip")
+
+
+def test_precompute_api_errors_and_kernel_handles():
+    from s2fft_amd.precompute_transforms import construct, spherical as pspin
+    with pytest.raises(ValueError):
+        pspin.forward(np.zeros((8, 15)), 8, method="numpy")
+    k = construct.spin_spherical_kernel(8, 2, reality=True, sampling="mw")
+    assert k.reality is False and k.spin == 2 and k.L == 8
+    with pytest.raises(ValueError):
+        pspin._check_kernel(k, 9, 2, "mw", None, False, True)
