@@ -45,7 +45,8 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-7)
     ap.add_argument("--max-iter", type=int, default=40)
     ap.add_argument("--seed0", type=int, default=300)
-    ap.add_argument("--mixed-precision", default="bf16", choices=["bf16", "mixed"])
+    ap.add_argument("--mixed-precision", default="bf16x1", choices=["bf16x1", "bf16", "mixed"])
+    ap.add_argument("--mixed-inv-precision", default="bf16", choices=["bf16", "mixed"])
     ap.add_argument("--refresh-every", type=int, default=3)
     ap.add_argument("--learned-steps", type=int, default=2)
     ap.add_argument("--no-learned", action="store_true")
@@ -123,10 +124,11 @@ def roofline(kernels, nside, L, B, lat_precision):
     if dom.startswith("lat"):
         flop = math.ceil(ntheta / 2) * (L * (L + 1) // 2) * (5 + 4 * B)
         achieved = flop / (avg_ms * 1e-3) / 1e12
-        peak = BF16_PEAK_TFLOPS if lat_precision == "bf16" else FP64_PEAK_TFLOPS
+        bf = lat_precision.startswith("bf16")
+        peak = BF16_PEAK_TFLOPS if bf else FP64_PEAK_TFLOPS
         return {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                 "frac": achieved / peak, "traffic": None, "kernel": dom,
-                "arith": "split-bf16" if lat_precision == "bf16" else "fp64",
+                "arith": lat_precision,
                 "flop_per_launch": flop, "avg_launch_ms": avg_ms}
     nbytes = B * (8 * 12 * nside * nside + 16 * ntheta * L)
     achieved = nbytes / (avg_ms * 1e-3) / 1e9
@@ -225,14 +227,15 @@ def main():
     refine.refine_mixed(f, L, 0, nside, "healpix", True, args.max_iter, plan=plan,
                         precond=precond, refresh_every=args.refresh_every, history=hist,
                         flm_true=flm_true, stats=stats, precision=prec,
-                        learned_steps=args.learned_steps)
+                        inv_precision=args.mixed_inv_precision, learned_steps=args.learned_steps)
     ok = [i for i, e in enumerate(hist) if e <= args.tol]
     n_mixed = ok[0] if ok else args.max_iter
     log(f"[bench]   mixed errors {['%.1e' % e for e in hist[:n_mixed + 2]]} -> {n_mixed} steps")
     torch.cuda.empty_cache()
     el, err, kt = timed(lambda: refine.refine_mixed(
         f, L, 0, nside, "healpix", True, n_mixed, plan=plan, precond=precond,
-        refresh_every=args.refresh_every, precision=prec, learned_steps=args.learned_steps),
+        refresh_every=args.refresh_every, precision=prec,
+        inv_precision=args.mixed_inv_precision, learned_steps=args.learned_steps),
         "mixed")
     mixed = {"elapsed": el, "err": err, "kernels": kt, "iters": n_mixed,
              "roofline": roofline(kt, nside, L, B, prec)}
@@ -256,7 +259,8 @@ def main():
         "config": {
             "workload": (f"HEALPix nside={nside} L={L} real spin-0 forward of a {B}-map batch "
                          f"refined to <= {args.tol:g} max-rel error vs the seeded flm"),
-            "method": (f"mixed: FP64 recursion + {prec} matrix-core projection/synthesis, "
+            "method": (f"mixed: FP64 recursion, {prec} matrix-core projection, "
+                       f"{args.mixed_inv_precision} matrix-core synthesis, "
                        f"FP64 residual refreshed every {args.refresh_every} steps, "
                        (f"learned MLP preconditioner (Jacobi-judged) on the first "
                         f"{args.learned_steps} steps, Jacobi after" if learned else "Jacobi")),

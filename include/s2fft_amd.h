@@ -97,10 +97,18 @@ int s2h_inverse(const s2h_plan *plan, const void *flm, void *f, void *work, int 
  *   ~1e-7 of max |flm| for band-limited maps at L = 2048.
  * S2H_PREC_BF16: each operand split into two BF16 (hi + lo, 16 significant
  *   bits), products hi*hi + hi*lo + lo*hi on the BF16 matrix cores with FP32
- *   accumulation (forward: over one wave's 64 ring pairs; inverse: over all
- *   l); relative error ~1e-5.
- * Both are meant for iterative refinement whose residual is FP64. */
-enum s2h_precision { S2H_PREC_FP64 = 0, S2H_PREC_MIXED = 1, S2H_PREC_BF16 = 2 };
+ *   accumulation (forward: over one wave's ring pairs; inverse: over all l);
+ *   relative error ~1e-5.
+ * S2H_PREC_BF16X1: the forward with BF16 operands and one product (~4e-3
+ *   relative), the inverse as S2H_PREC_BF16: the correction transform of a
+ *   refinement, whose contraction it changes by its own relative error.
+ * All are meant for iterative refinement whose residual is FP64. */
+enum s2h_precision {
+  S2H_PREC_FP64 = 0,
+  S2H_PREC_MIXED = 1,
+  S2H_PREC_BF16 = 2,
+  S2H_PREC_BF16X1 = 3
+};
 int s2h_forward_ex(const s2h_plan *plan, const void *f, int f_is_complex, void *flm,
                    void *work, int batch, int precision, s2h_stream stream);
 int s2h_inverse_ex(const s2h_plan *plan, const void *flm, void *f, void *work, int batch,

@@ -50,7 +50,7 @@ EXPORTED = (
 
 KERNEL_LAT_FORWARD, KERNEL_LAT_INVERSE, KERNEL_RING_FFT_FORWARD = 0, 1, 2
 KERNEL_RING_FFT_INVERSE, KERNEL_RESAMPLE = 3, 4
-PREC_FP64, PREC_MIXED, PREC_BF16 = 0, 1, 2
+PREC_FP64, PREC_MIXED, PREC_BF16, PREC_BF16X1 = 0, 1, 2, 3
 
 
 def _declare(lib):

@@ -145,9 +145,10 @@ extern "C" int s2h_forward(const s2h_plan* p, const void* f, int f_is_complex, v
 extern "C" int s2h_forward_ex(const s2h_plan* p, const void* f, int f_is_complex, void* flm,
                               void* work, int batch, int precision, s2h_stream stream) {
   if (!p || !f || !flm || batch < 0 || (batch > 0 && !work)) return S2H_ERR_INVALID;
-  if (precision < S2H_PREC_FP64 || precision > S2H_PREC_BF16) return S2H_ERR_INVALID;
+  if (precision < S2H_PREC_FP64 || precision > S2H_PREC_BF16X1) return S2H_ERR_INVALID;
   const bool mixed = precision == S2H_PREC_MIXED;
-  const bool bf16 = precision == S2H_PREC_BF16;
+  const bool bf16 = precision == S2H_PREC_BF16 || precision == S2H_PREC_BF16X1;
+  const int split = precision == S2H_PREC_BF16X1 ? 1 : 3;
   if (batch == 0) return S2H_OK;
   int st = check_device(p);
   if (st) return st;
@@ -230,7 +231,7 @@ extern "C" int s2h_forward_ex(const s2h_plan* p, const void* f, int f_is_complex
     // batches of 4 or more maps share each d-value across an 8-map MFMA tile;
     // the mixed-precision projection always runs on the FP32 matrix cores
     if (bf16)
-      st = hip_status(launch_lat_forward_bf16(a, p->fwd_paired, s));
+      st = hip_status(launch_lat_forward_bf16(a, p->fwd_paired, split, s));
     else if (mixed || batch >= kMfmaMinBatch)
       st = hip_status(launch_lat_forward_mfma(a, p->fwd_paired, mixed, s));
     else
@@ -270,9 +271,10 @@ extern "C" int s2h_inverse(const s2h_plan* p, const void* flm, void* f, void* wo
 extern "C" int s2h_inverse_ex(const s2h_plan* p, const void* flm, void* f, void* work, int batch,
                               int precision, s2h_stream stream) {
   if (!p || !flm || !f || batch < 0 || (batch > 0 && !work)) return S2H_ERR_INVALID;
-  if (precision < S2H_PREC_FP64 || precision > S2H_PREC_BF16) return S2H_ERR_INVALID;
+  if (precision < S2H_PREC_FP64 || precision > S2H_PREC_BF16X1) return S2H_ERR_INVALID;
   const bool mixed = precision == S2H_PREC_MIXED;
-  const bool bf16 = precision == S2H_PREC_BF16;
+  const bool bf16 = precision == S2H_PREC_BF16 || precision == S2H_PREC_BF16X1;
+  const int split = precision == S2H_PREC_BF16X1 ? 1 : 3;
   if (batch == 0) return S2H_OK;
   int st = check_device(p);
   if (st) return st;

@@ -80,7 +80,7 @@ hipError_t launch_lat_forward(const LatArgs& a, bool paired, hipStream_t s);
 hipError_t launch_lat_forward_mfma(const LatArgs& a, bool paired, bool fp32, hipStream_t s);
 hipError_t launch_mfma_f64_probe(const double* a, const double* b, double* out, hipStream_t s);
 hipError_t launch_lat_inverse_mma(const LatArgs& a, bool paired, bool fp32, hipStream_t s);
-hipError_t launch_lat_forward_bf16(const LatArgs& a, bool paired, hipStream_t s);
+hipError_t launch_lat_forward_bf16(const LatArgs& a, bool paired, int split, hipStream_t s);
 hipError_t launch_flm_to_mmajor(const cplx* flm, cplx* flmT, int L, int m_lo, int nm, int batch,
                                 hipStream_t s);
 hipError_t launch_mmajor_to_flm(const cplx* flmT, cplx* flm, int L, int m_lo, int nm, int batch,

@@ -57,11 +57,16 @@ constexpr int kFTiles = 2;      // 8-map tiles per workgroup (16 maps)
 }  // namespace
 
 // ------------------------------------------------------------------ forward
-template <bool PAIRED>
+// NS: 64-pair sets per wave (their products accumulate into the same MFMA
+// accumulators, so a workgroup covers kFW * NS * 64 ring pairs).  SPLIT: 3 =
+// hi*hi + hi*lo + lo*hi, 1 = hi*hi only (BF16 operands, ~4e-3 relative: the
+// correction transform of a refinement whose residual is FP64).
+template <bool PAIRED, int NS, int SPLIT>
 __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
-  // d tile per wave: [l_local 32][pair 64] hi and lo
+  constexpr bool LO = SPLIT == 3;
+  // d tile per wave: [l_local 32][pair 64] hi (and lo)
   __shared__ __bf16 dth[kFW][32 * 64];
-  __shared__ __bf16 dtl[kFW][32 * 64];
+  __shared__ __bf16 dtl[LO ? kFW : 1][LO ? 32 * 64 : 1];
   // per-wave results: [tile][parity][row 16][col 16]
   __shared__ float ctile[kFW][kFTiles * 2 * 256];
   __shared__ lat::ChunkCoef ccs[2][lat::kChunk];  // recursion coefficients, double-buffered
@@ -75,50 +80,57 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
   const double bsign = m < 0 ? -1.0 : 1.0;
   const Coef* __restrict__ cf = a.coef + (int64_t)am * L;
   __bf16* dh = dth[wave];
-  __bf16* dl = dtl[wave];
+  __bf16* dl = dtl[LO ? wave : 0];
 
-  // recursion state of this lane's pair
-  const int base = (blockIdx.y * kFW + wave) * 64;
-  const int p = base + lane;
-  double x = 0.0, d = 0.0, dm1 = 0.0;
-  int ls = 0;
-  if (p < a.nrings) {
-    const LatRing rg = a.rings[p];
-    x = rg.x;
-    start_value(st, rg.c2, rg.s2, d, ls);
-  }
-
+  // recursion state of this lane's pair in each set
+  double x[NS], d[NS], dm1[NS];
+  int ls[NS];
   // A operands: row = (map row/2 of the tile, re/im row%2), K = pairs
-  // 32 kb + 8 kg + e of this wave
-  bf16x8 seh[kFTiles][2], sel[kFTiles][2], soh[kFTiles][2], sol[kFTiles][2];
+  // 32 kb + 8 kg + e of the set
+  bf16x8 seh[NS][kFTiles][2], soh[NS][kFTiles][2];
+  bf16x8 sel[LO ? NS : 1][kFTiles][2], sol[LO ? NS : 1][kFTiles][2];
 #pragma unroll
-  for (int t = 0; t < kFTiles; ++t) {
-    const int amap = b0 + 8 * t + (row >> 1);
-    const bool have = amap < a.batch;
-    const cplx* ftm = a.ftm + (int64_t)amap * a.ftm_map_stride + (m - a.ftm_m_first);
+  for (int s = 0; s < NS; ++s) {
+    const int base = ((blockIdx.y * kFW + wave) * NS + s) * 64;
+    const int p = base + lane;
+    dm1[s] = 0.0;
+    x[s] = 0.0;
+    d[s] = 0.0;
+    ls[s] = 0;
+    if (p < a.nrings) {
+      const LatRing rg = a.rings[p];
+      x[s] = rg.x;
+      start_value(st, rg.c2, rg.s2, d[s], ls[s]);
+    }
 #pragma unroll
-    for (int kb = 0; kb < 2; ++kb) {
+    for (int t = 0; t < kFTiles; ++t) {
+      const int amap = b0 + 8 * t + (row >> 1);
+      const bool have = amap < a.batch;
+      const cplx* ftm = a.ftm + (int64_t)amap * a.ftm_map_stride + (m - a.ftm_m_first);
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int q = base + 32 * kb + 8 * kg + e;
-        double ve = 0.0, vo = 0.0;
-        if (have && q < a.nrings) {
-          const LatRing rq = a.rings[q];
-          const cplx fa = ftm[(int64_t)rq.ta * a.ftm_ncol];
-          cplx fb = {0.0, 0.0};
-          if (PAIRED && rq.tb >= 0 && rq.tb != rq.ta) fb = ftm[(int64_t)rq.tb * a.ftm_ncol];
-          const double ca = (row & 1) ? fa.y : fa.x;
-          const double cb = (row & 1) ? fb.y : fb.x;
-          ve = ca + cb;
-          vo = PAIRED ? ca - cb : ca;
+      for (int kb = 0; kb < 2; ++kb) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int q = base + 32 * kb + 8 * kg + e;
+          double ve = 0.0, vo = 0.0;
+          if (have && q < a.nrings) {
+            const LatRing rq = a.rings[q];
+            const cplx fa = ftm[(int64_t)rq.ta * a.ftm_ncol];
+            cplx fb = {0.0, 0.0};
+            if (PAIRED && rq.tb >= 0 && rq.tb != rq.ta) fb = ftm[(int64_t)rq.tb * a.ftm_ncol];
+            const double ca = (row & 1) ? fa.y : fa.x;
+            const double cb = (row & 1) ? fb.y : fb.x;
+            ve = ca + cb;
+            vo = PAIRED ? ca - cb : ca;
+          }
+          __bf16 h, l;
+          split2(ve, h, l);
+          seh[s][t][kb][e] = h;
+          if constexpr (LO) sel[s][t][kb][e] = l;
+          split2(vo, h, l);
+          soh[s][t][kb][e] = h;
+          if constexpr (LO) sol[s][t][kb][e] = l;
         }
-        __bf16 h, l;
-        split2(ve, h, l);
-        seh[t][kb][e] = h;
-        sel[t][kb][e] = l;
-        split2(vo, h, l);
-        soh[t][kb][e] = h;
-        sol[t][kb][e] = l;
       }
     }
   }
@@ -132,23 +144,6 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
     const bool more = blk + 32 < L;
     if (more && threadIdx.x < lat::kChunk) cn = lat::load_coef(cf, blk + 32, threadIdx.x, L, bsign);
     const lat::ChunkCoef* cs = ccs[buf];
-    const bool live = ls == 0;
-#pragma unroll
-    for (int i = 0; i < 32; ++i) {
-      const int l = blk + i;
-      const double v = (live && l < L) ? d : 0.0;
-      __bf16 h, lo;
-      split2(v, h, lo);
-      // even / odd l (relative to l0) in separate 16-column halves
-      const int col = ((i & 1) << 4) | (i >> 1);
-      dh[col * 64 + lane] = h;
-      dl[col * 64 + lane] = lo;
-      const double dn = lat::rec_step(cs[i], x, d, dm1);
-      dm1 = d;
-      d = dn;
-    }
-    lat::rescale(d, dm1, ls);
-    lat::wave_sync();
     f32x4 acc[kFTiles][2];
 #pragma unroll
     for (int t = 0; t < kFTiles; ++t) {
@@ -156,23 +151,53 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
       acc[t][1] = f32x4{0.f, 0.f, 0.f, 0.f};
     }
 #pragma unroll
-    for (int par = 0; par < 2; ++par) {
+    for (int s = 0; s < NS; ++s) {
+      const bool live = ls[s] == 0;
 #pragma unroll
-      for (int kb = 0; kb < 2; ++kb) {
-        // B[k][j]: d of pair 32 kb + 8 kg + e at column j = row (parity par)
-        const int off = (par * 16 + row) * 64 + 32 * kb + 8 * kg;
-        const bf16x8 bh = *reinterpret_cast<const bf16x8*>(dh + off);
-        const bf16x8 bl = *reinterpret_cast<const bf16x8*>(dl + off);
+      for (int i = 0; i < 32; ++i) {
+        const int l = blk + i;
+        const double v = (live && l < L) ? d[s] : 0.0;
+        __bf16 h, lo;
+        split2(v, h, lo);
+        // even / odd l (relative to l0) in separate 16-column halves
+        const int col = ((i & 1) << 4) | (i >> 1);
+        dh[col * 64 + lane] = h;
+        if constexpr (LO) dl[col * 64 + lane] = lo;
+        const double dn = lat::rec_step(cs[i], x[s], d[s], dm1[s]);
+        dm1[s] = d[s];
+        d[s] = dn;
+      }
+      lat::rescale(d[s], dm1[s], ls[s]);
+      lat::wave_sync();
 #pragma unroll
-        for (int t = 0; t < kFTiles; ++t) {
-          if (par == 0)
-            acc[t][0] = mma3(seh[t][kb], sel[t][kb], bh, bl, acc[t][0]);
-          else
-            acc[t][1] = mma3(soh[t][kb], sol[t][kb], bh, bl, acc[t][1]);
+      for (int par = 0; par < 2; ++par) {
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb) {
+          // B[k][j]: d of pair 32 kb + 8 kg + e at column j = row (parity par)
+          const int off = (par * 16 + row) * 64 + 32 * kb + 8 * kg;
+          const bf16x8 bh = *reinterpret_cast<const bf16x8*>(dh + off);
+          if constexpr (LO) {
+            const bf16x8 bl = *reinterpret_cast<const bf16x8*>(dl + off);
+#pragma unroll
+            for (int t = 0; t < kFTiles; ++t) {
+              if (par == 0)
+                acc[t][0] = mma3(seh[s][t][kb], sel[s][t][kb], bh, bl, acc[t][0]);
+              else
+                acc[t][1] = mma3(soh[s][t][kb], sol[s][t][kb], bh, bl, acc[t][1]);
+            }
+          } else {
+#pragma unroll
+            for (int t = 0; t < kFTiles; ++t) {
+              if (par == 0)
+                acc[t][0] = mma(seh[s][t][kb], bh, acc[t][0]);
+              else
+                acc[t][1] = mma(soh[s][t][kb], bh, acc[t][1]);
+            }
+          }
         }
       }
+      lat::wave_sync();
     }
-    lat::wave_sync();
     // C[i][j]: lane j + 16 (i/4), register i%4
 #pragma unroll
     for (int t = 0; t < kFTiles; ++t)
@@ -368,16 +393,22 @@ __global__ void __launch_bounds__(kIT) lat_inverse_bf16_kernel(LatArgs a) {
 }
 
 // ------------------------------------------------------------------ launchers
-hipError_t launch_lat_forward_bf16(const LatArgs& a0, bool paired, hipStream_t s) {
-  if (a0.nm == 0 || a0.nrings == 0 || a0.batch == 0) return hipSuccess;
+template <int NS, int SPLIT>
+static hipError_t lat_fwd_bf16(const LatArgs& a0, bool paired, hipStream_t s) {
   LatArgs a = a0;
-  a.groups = (a.nrings + kFT - 1) / kFT;
+  const int per_wg = kFW * NS * 64;
+  a.groups = (a.nrings + per_wg - 1) / per_wg;
   dim3 grid(a.nm, a.groups, (a.batch + 8 * kFTiles - 1) / (8 * kFTiles));
   if (paired)
-    hipLaunchKernelGGL(lat_forward_bf16_kernel<true>, grid, dim3(kFT), 0, s, a);
+    hipLaunchKernelGGL((lat_forward_bf16_kernel<true, NS, SPLIT>), grid, dim3(kFT), 0, s, a);
   else
-    hipLaunchKernelGGL(lat_forward_bf16_kernel<false>, grid, dim3(kFT), 0, s, a);
+    hipLaunchKernelGGL((lat_forward_bf16_kernel<false, NS, SPLIT>), grid, dim3(kFT), 0, s, a);
   return hipGetLastError();
+}
+
+hipError_t launch_lat_forward_bf16(const LatArgs& a, bool paired, int split, hipStream_t s) {
+  if (a.nm == 0 || a.nrings == 0 || a.batch == 0) return hipSuccess;
+  return split == 1 ? lat_fwd_bf16<4, 1>(a, paired, s) : lat_fwd_bf16<2, 3>(a, paired, s);
 }
 
 hipError_t launch_lat_inverse_bf16(const LatArgs& a, bool paired, hipStream_t s) {

@@ -93,7 +93,8 @@ def _check_common(method, spin, sampling, nside):
         raise ValueError(f"Sampling scheme sampling={sampling} with nside={nside} not supported")
 
 
-PRECISIONS = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED, "bf16": _lib.PREC_BF16}
+PRECISIONS = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED, "bf16": _lib.PREC_BF16,
+              "bf16x1": _lib.PREC_BF16X1}
 
 
 def _precision(name):
@@ -131,8 +132,10 @@ def forward_hip(f, L, spin=0, nside=None, sampling="mw", reality=False, L_lower=
     """Batched forward on device tensors: f (B, ...) -> flm (B, L, 2L-1) complex128.
 
     ``precision``: "fp64" (default), "mixed" (FP64 recursion and ring FFTs,
-    FP32 matrix-core projection with FP32 accumulation over 128 rings) or
-    "bf16" (split-BF16 matrix-core projection, FP32 accumulation).
+    FP32 matrix-core projection with FP32 accumulation over 128 rings),
+    "bf16" (split-BF16 matrix-core projection, FP32 accumulation) or "bf16x1"
+    (single-product BF16 projection, ~4e-3 relative: a refinement's
+    correction transform).
     """
     s = sampling.lower()
     plan = plan or get_plan(s, L, nside, spin, L_lower, reality, device=f.device)

@@ -4,7 +4,8 @@ Tolerances are stated per test and precision.  "mixed": FP32 rounding of the
 d-values and of the data plus FP32 accumulation over 128 rings (forward) or
 32 l (inverse), about 1e-7 of max |flm| at these sizes; the tests allow 2e-6.
 "bf16": hi + lo BF16 operands (16 significant bits), products without the
-lo*lo term, FP32 accumulation, about 1e-5; the tests allow 1e-4.  Every test
+lo*lo term, FP32 accumulation, about 1e-5; the tests allow 1e-4.  "bf16x1"
+(forward only): BF16 operands, one product, about 4e-3; the tests allow 2e-2.  Every test
 also requires the error to be above 1e-12, which shows that the reduced
 precision path (not the FP64 one) ran.  Refinement with an FP64 residual must
 reach the FP64 target of 1e-7 against the seeded coefficients.
@@ -21,7 +22,7 @@ from s2fft_amd.transforms import refine, spherical as sph
 
 pytestmark = pytest.mark.gpu
 MIXED_TOL = 2e-6
-TOLS = {"mixed": 2e-6, "bf16": 1e-4}
+TOLS = {"mixed": 2e-6, "bf16": 1e-4, "bf16x1": 2e-2}
 
 
 def _flms(rng, B, L, spin, reality):
@@ -47,7 +48,7 @@ def test_mixed_inverse_vs_oracle(rng, sampling, nside, L, spin, reality, B, prec
         assert 1e-12 < err < TOLS[precision], err
 
 
-@pytest.mark.parametrize("precision", ["mixed", "bf16"])
+@pytest.mark.parametrize("precision", ["mixed", "bf16", "bf16x1"])
 @pytest.mark.parametrize("sampling,nside,L,spin,reality,B", CASES)
 def test_mixed_forward_vs_oracle(rng, sampling, nside, L, spin, reality, B, precision):
     flms = _flms(rng, B, L, spin, reality)
@@ -75,6 +76,18 @@ def test_mixed_vs_fp64_larger(rng, precision):
     e_fwd = float((gmx - g64).abs().max() / g64.abs().max())
     print(f"{precision} vs fp64 at nside {nside}: inverse {e_inv:.2e}, forward {e_fwd:.2e}")
     assert 1e-12 < e_inv < TOLS[precision] and 1e-12 < e_fwd < TOLS[precision]
+
+
+def test_bf16x1_forward_larger(rng):
+    """nside 128, L 256, 20 real maps: single-product forward vs FP64."""
+    nside, L, B = 128, 256, 20
+    flm = torch.from_numpy(_flms(rng, B, L, 0, True)).cuda()
+    f64 = sph.inverse_hip(flm, L, 0, nside, "healpix", True)
+    g64 = sph.forward_hip(f64, L, 0, nside, "healpix", True)
+    gx1 = sph.forward_hip(f64, L, 0, nside, "healpix", True, precision="bf16x1")
+    e = float((gx1 - g64).abs().max() / g64.abs().max())
+    print(f"bf16x1 forward vs fp64 at nside {nside}: {e:.2e}")
+    assert 1e-12 < e < TOLS["bf16x1"]
 
 
 def test_mixed_refinement_reaches_fp64_target(rng):
