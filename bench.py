@@ -262,8 +262,9 @@ def main():
             "method": (f"mixed: FP64 recursion, {prec} matrix-core projection, "
                        f"{args.mixed_inv_precision} matrix-core synthesis, "
                        f"FP64 residual refreshed every {args.refresh_every} steps, "
-                       (f"learned MLP preconditioner (Jacobi-judged) on the first "
-                        f"{args.learned_steps} steps, Jacobi after" if learned else "Jacobi")),
+                       + (f"learned MLP preconditioner (Jacobi-judged) on the first "
+                          f"{args.learned_steps} steps, Jacobi after" if learned
+                          else "Jacobi")),
             "nside": nside, "L": L, "batch": B, "learned": learned,
             "mixed_steps": n_mixed, "mixed_max_rel_err": mixed["err"],
             "mixed_calibration_errs": hist[:n_mixed + 1],
