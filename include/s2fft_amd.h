@@ -109,10 +109,18 @@ enum s2h_precision {
   S2H_PREC_BF16 = 2,
   S2H_PREC_BF16X1 = 3
 };
+/* flags of s2h_forward_ex: S2H_FWD_ACCUMULATE adds F(f) into flm (which must
+ * hold coefficients with the plan's symmetry) instead of overwriting it.
+ * flags of s2h_inverse_ex: S2H_INV_REAL_OUTPUT writes float64 pixels for a
+ * reality plan even where s2h_inverse_output_is_complex says complex (its
+ * imaginary part is zero there); S2H_INV_SUBTRACT writes sub_from - I(flm)
+ * (sub_from in the output layout, may alias f): a residual in one pass. */
+enum { S2H_FWD_ACCUMULATE = 1 };
+enum { S2H_INV_REAL_OUTPUT = 1, S2H_INV_SUBTRACT = 2 };
 int s2h_forward_ex(const s2h_plan *plan, const void *f, int f_is_complex, void *flm,
-                   void *work, int batch, int precision, s2h_stream stream);
+                   void *work, int batch, int precision, int flags, s2h_stream stream);
 int s2h_inverse_ex(const s2h_plan *plan, const void *flm, void *f, void *work, int batch,
-                   int precision, s2h_stream stream);
+                   int precision, int flags, const void *sub_from, s2h_stream stream);
 
 /* Adjoints (vector-Jacobian products) of s2h_forward and s2h_inverse for
  * the torch complex-gradient convention: grad_f = F^H g, grad_flm = I^H g.

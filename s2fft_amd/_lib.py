@@ -51,6 +51,8 @@ EXPORTED = (
 KERNEL_LAT_FORWARD, KERNEL_LAT_INVERSE, KERNEL_RING_FFT_FORWARD = 0, 1, 2
 KERNEL_RING_FFT_INVERSE, KERNEL_RESAMPLE = 3, 4
 PREC_FP64, PREC_MIXED, PREC_BF16, PREC_BF16X1 = 0, 1, 2, 3
+FWD_ACCUMULATE = 1
+INV_REAL_OUTPUT, INV_SUBTRACT = 1, 2
 
 
 def _declare(lib):
@@ -64,8 +66,8 @@ def _declare(lib):
         "s2h_inverse_output_is_complex": (i32, [vp]),
         "s2h_forward": (i32, [vp, vp, i32, vp, vp, i32, vp]),
         "s2h_inverse": (i32, [vp, vp, vp, vp, i32, vp]),
-        "s2h_forward_ex": (i32, [vp, vp, i32, vp, vp, i32, i32, vp]),
-        "s2h_inverse_ex": (i32, [vp, vp, vp, vp, i32, i32, vp]),
+        "s2h_forward_ex": (i32, [vp, vp, i32, vp, vp, i32, i32, i32, vp]),
+        "s2h_inverse_ex": (i32, [vp, vp, vp, vp, i32, i32, i32, vp, vp]),
         "s2h_forward_adjoint": (i32, [vp, vp, vp, vp, i32, vp]),
         "s2h_probe_mfma_f64": (i32, [vp, vp, vp, vp]),
         "s2h_probe_mfma_f32": (i32, [vp, vp, vp, vp]),

@@ -139,13 +139,16 @@ extern "C" int s2h_inverse_output_is_complex(const s2h_plan* p) {
 
 extern "C" int s2h_forward(const s2h_plan* p, const void* f, int f_is_complex, void* flm,
                            void* work, int batch, s2h_stream stream) {
-  return s2h_forward_ex(p, f, f_is_complex, flm, work, batch, S2H_PREC_FP64, stream);
+  return s2h_forward_ex(p, f, f_is_complex, flm, work, batch, S2H_PREC_FP64, 0, stream);
 }
 
 extern "C" int s2h_forward_ex(const s2h_plan* p, const void* f, int f_is_complex, void* flm,
-                              void* work, int batch, int precision, s2h_stream stream) {
+                              void* work, int batch, int precision, int flags,
+                              s2h_stream stream) {
   if (!p || !f || !flm || batch < 0 || (batch > 0 && !work)) return S2H_ERR_INVALID;
   if (precision < S2H_PREC_FP64 || precision > S2H_PREC_BF16X1) return S2H_ERR_INVALID;
+  if (flags & ~S2H_FWD_ACCUMULATE) return S2H_ERR_INVALID;
+  const bool accumulate = (flags & S2H_FWD_ACCUMULATE) != 0;
   const bool mixed = precision == S2H_PREC_MIXED;
   const bool bf16 = precision == S2H_PREC_BF16 || precision == S2H_PREC_BF16X1;
   const int split = precision == S2H_PREC_BF16X1 ? 1 : 3;
@@ -159,7 +162,7 @@ extern "C" int s2h_forward_ex(const s2h_plan* p, const void* f, int f_is_complex
   cplx* ftm = static_cast<cplx*>(work);
   const int64_t fs = flm_stride(p);
 
-  if ((st = hip_status(launch_zero(flm, 2 * fs * batch, s)))) return st;
+  if (!accumulate && (st = hip_status(launch_zero(flm, 2 * fs * batch, s)))) return st;
   const bool resample = p->fwd_kind != 0;
   // resampled forward: phi-spectrum of the input rows first, then the
   // resampled MWSS-at-2L ftm after it in the workspace
@@ -226,6 +229,7 @@ extern "C" int s2h_forward_ex(const s2h_plan* p, const void* f, int f_is_complex
   a.ftm_ncol = p->Mc;
   a.ftm_m_first = p->m_lo;
   a.lmin_out = std::max(std::abs(p->d.spin), p->d.L_lower);
+  a.accumulate = accumulate ? 1 : 0;
   {
     ProfScope ps(S2H_KERNEL_LAT_FORWARD, s);
     // batches of 4 or more maps share each d-value across an 8-map MFMA tile;
@@ -265,12 +269,17 @@ extern "C" int s2h_forward_ex(const s2h_plan* p, const void* f, int f_is_complex
 
 extern "C" int s2h_inverse(const s2h_plan* p, const void* flm, void* f, void* work, int batch,
                            s2h_stream stream) {
-  return s2h_inverse_ex(p, flm, f, work, batch, S2H_PREC_FP64, stream);
+  return s2h_inverse_ex(p, flm, f, work, batch, S2H_PREC_FP64, 0, nullptr, stream);
 }
 
 extern "C" int s2h_inverse_ex(const s2h_plan* p, const void* flm, void* f, void* work, int batch,
-                              int precision, s2h_stream stream) {
+                              int precision, int flags, const void* sub_from,
+                              s2h_stream stream) {
   if (!p || !flm || !f || batch < 0 || (batch > 0 && !work)) return S2H_ERR_INVALID;
+  if (flags & ~(S2H_INV_REAL_OUTPUT | S2H_INV_SUBTRACT)) return S2H_ERR_INVALID;
+  const bool real_out = (flags & S2H_INV_REAL_OUTPUT) != 0;
+  if (real_out && !p->d.reality) return S2H_ERR_INVALID;
+  if ((flags & S2H_INV_SUBTRACT) && !sub_from) return S2H_ERR_INVALID;
   if (precision < S2H_PREC_FP64 || precision > S2H_PREC_BF16X1) return S2H_ERR_INVALID;
   const bool mixed = precision == S2H_PREC_MIXED;
   const bool bf16 = precision == S2H_PREC_BF16 || precision == S2H_PREC_BF16X1;
@@ -331,10 +340,11 @@ extern "C" int s2h_inverse_ex(const s2h_plan* p, const void* flm, void* f, void*
   r.apply_phase = p->d.sampling == S2H_HEALPIX;
   r.apply_weight = 0;
   r.f = f;
-  r.f_complex = p->inv_out_complex ? 1 : 0;
+  r.f_complex = p->inv_out_complex && !real_out ? 1 : 0;
   r.map_stride = p->npix;
   r.chirp_pool = p->chirp;
   r.L = L;
+  r.sub_from = (flags & S2H_INV_SUBTRACT) ? sub_from : nullptr;
   ProfScope ps(S2H_KERNEL_RING_FFT_INVERSE, s);
   return hip_status(launch_ring_fft_inverse(r, p->inv_set, batch, s));
 }

@@ -40,6 +40,7 @@ struct RingInvArgs {
   const cplx* chirp_pool;
   int L;
   int batch;                // set by the launcher
+  const void* sub_from;     // non-null: f = sub_from - (transform), same layout as f
 };
 
 struct LatArgs {
@@ -61,6 +62,7 @@ struct LatArgs {
   int ftm_ncol;
   int ftm_m_first;
   int groups;               // ring groups per m (forward); >1 accumulates atomically
+  int accumulate;           // forward: add into flm instead of overwriting it
 };
 
 hipError_t launch_ring_fft_forward(const RingFwdArgs& a, const JobSet& js, int batch,

@@ -201,7 +201,7 @@ __global__ void __launch_bounds__(kThreads) lat_forward_kernel(LatArgs a) {
 #pragma unroll
         for (int w = 1; w < kThreads / 64; ++w) t = cadd(t, wsum[w][threadIdx.x]);
         cplx* o = out + (int64_t)l * (2 * L - 1);
-        if (a.groups > 1) {
+        if (a.groups > 1 || a.accumulate) {
           atomicAdd(&o->x, t.x);
           atomicAdd(&o->y, t.y);
         } else {

@@ -219,7 +219,7 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
       if (l < L && l >= a.lmin_out && b0 + bm < a.batch) {
         double* o = reinterpret_cast<double*>(out + (int64_t)bm * a.flm_map_stride +
                                               (int64_t)l * (2 * L - 1)) + (ri & 1);
-        if (a.groups > 1)
+        if (a.groups > 1 || a.accumulate)
           atomicAdd(o, v);
         else
           *o = v;

@@ -452,18 +452,22 @@ __global__ void __launch_bounds__(TPB) ring_inv_kernel(RingInvArgs a, int job0, 
   __syncthreads();
   ring_transform<TPB, MAXN, KIND>(s, n, M, a.chirp_pool + job.chirp_off, +1);
 
-  if (paired) {
-    double* d0 = reinterpret_cast<double*>(a.f) + b0 * a.map_stride + job.pix_offset;
+  // store; with sub_from the result is subtracted from it (a residual)
+  const int64_t off = b0 * a.map_stride + job.pix_offset;
+  if (paired || !a.f_complex) {
+    double* d0 = reinterpret_cast<double*>(a.f) + off;
+    const double* u0 = a.sub_from ? reinterpret_cast<const double*>(a.sub_from) + off : nullptr;
     for (int p = threadIdx.x; p < n; p += TPB) {
-      d0[p] = s.re[p];
-      if (two) d0[a.map_stride + p] = s.im[p];
+      d0[p] = u0 ? u0[p] - s.re[p] : s.re[p];
+      if (two) d0[a.map_stride + p] = u0 ? u0[a.map_stride + p] - s.im[p] : s.im[p];
     }
-  } else if (a.f_complex) {
-    cplx* dst = reinterpret_cast<cplx*>(a.f) + b0 * a.map_stride + job.pix_offset;
-    for (int p = threadIdx.x; p < n; p += TPB) dst[p] = {s.re[p], herm ? 0.0 : s.im[p]};
   } else {
-    double* dst = reinterpret_cast<double*>(a.f) + b0 * a.map_stride + job.pix_offset;
-    for (int p = threadIdx.x; p < n; p += TPB) dst[p] = s.re[p];
+    cplx* dst = reinterpret_cast<cplx*>(a.f) + off;
+    const cplx* u0 = a.sub_from ? reinterpret_cast<const cplx*>(a.sub_from) + off : nullptr;
+    for (int p = threadIdx.x; p < n; p += TPB) {
+      const cplx v = {s.re[p], herm ? 0.0 : s.im[p]};
+      dst[p] = u0 ? csub(u0[p], v) : v;
+    }
   }
 }
 

@@ -53,27 +53,33 @@ def refine_mixed(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=N
     fp, ip = precision, inv_precision or precision
     if learned_steps is None:
         learned_steps = max(n_iter - 1, 0)
+    real_out = bool(reality) and not f.is_complex()
+
+    def synth(x, prec="fp64", sub=None):
+        return sph.inverse_hip(x, precision=prec, real_output=real_out, sub_from=sub, **kw)
+
     flm = sph.forward_hip(f, precision=fp, **kw)
-    r = f - _real_if(sph.inverse_hip(flm, **kw), f)
+    r = synth(flm, sub=f)
     _track(history, flm, flm_true)
     for k in range(n_iter):
-        h = sph.forward_hip(r, precision=fp, **kw)
         last = k + 1 == n_iter
-        refresh = bool(not last and refresh_every and (k + 1) % refresh_every == 0)
-        if precond is not None and k < learned_steps and not last:
-            delta, r = _judged_step(precond, h, r, f, flm, kw, stats, ip, refresh)
-            flm = flm + delta
+        if last:
+            sph.forward_hip(r, precision=fp, accumulate_into=flm, **kw)
+            _track(history, flm, flm_true)
+            break
+        h = sph.forward_hip(r, precision=fp, **kw)
+        refresh = bool(refresh_every and (k + 1) % refresh_every == 0)
+        if precond is not None and k < learned_steps:
+            delta, r = _judged_step(precond, h, r, f, flm, kw, stats, synth, ip, refresh)
+            flm += delta
         else:
-            flm = flm + h
-            if refresh:
-                r = f - _real_if(sph.inverse_hip(flm, **kw), f)
-            elif not last:
-                r = r - _real_if(sph.inverse_hip(h, precision=ip, **kw), f)
+            flm += h
+            r = synth(flm, sub=f) if refresh else synth(h, ip, sub=r)
         _track(history, flm, flm_true)
     return flm
 
 
-def _judged_step(precond, h, r, f, flm, kw, stats, ip, exact):
+def _judged_step(precond, h, r, f, flm, kw, stats, synth, ip, exact):
     """Learned step per map where it lowers the FP64-held residual, else Jacobi.
 
     The candidate delta = P(h) moves the residual to r - I~(delta) (or, on a
@@ -88,8 +94,8 @@ def _judged_step(precond, h, r, f, flm, kw, stats, ip, exact):
 
     def residual(d, rows):
         if exact:
-            return f[rows] - _real_if(sph.inverse_hip((flm[rows] + d).contiguous(), **kw), f)
-        return r[rows] - _real_if(sph.inverse_hip(d.contiguous(), precision=ip, **kw), f)
+            return synth((flm[rows] + d).contiguous(), sub=f[rows])
+        return synth(d.contiguous(), ip, sub=r[rows])
 
     every = slice(None)
     r_new = residual(delta, every)

@@ -106,46 +106,68 @@ def _precision(name):
 
 # ------------------------------------------------------------------ device level
 def inverse_hip(flm, L, spin=0, nside=None, sampling="mw", reality=False, L_lower=0,
-                plan=None, precision="fp64"):
+                plan=None, precision="fp64", real_output=False, sub_from=None):
     """Batched inverse on device tensors: flm (B, L, 2L-1) complex128 -> f (B, ...).
 
     ``precision``: "fp64" (default), "mixed" (FP64 recursion, FP32 matrix-core
     synthesis with FP32 accumulation within 32-l chunks) or "bf16" (FP64
     recursion, split-BF16 matrix-core synthesis, FP32 accumulation).
+    ``real_output`` (reality only): float64 pixels even where the reference
+    returns complex128 with a zero imaginary part (HEALPix).  ``sub_from``:
+    return sub_from - I(flm) (a residual) in the same pass; it may be
+    overwritten in place when it is the returned tensor's only owner.
     """
     s = sampling.lower()
     plan = plan or get_plan(s, L, nside, spin, L_lower, reality, device=flm.device)
     B = flm.shape[0]
     fshape = samples.f_shape(L, s, nside)
-    dtype = torch.complex128 if plan.inverse_output_is_complex() else torch.float64
+    cplx_out = plan.inverse_output_is_complex() and not real_output
+    dtype = torch.complex128 if cplx_out else torch.float64
+    flags = _lib.INV_REAL_OUTPUT if real_output else 0
+    if sub_from is not None:
+        sub_from = sub_from.to(dtype).contiguous()
+        if tuple(sub_from.shape) != (B,) + tuple(fshape):
+            raise ValueError(f"sub_from has shape {tuple(sub_from.shape)}")
+        flags |= _lib.INV_SUBTRACT
     f = torch.empty((B,) + fshape, dtype=dtype, device=flm.device)
     work = torch.empty(plan.workspace_bytes(B), dtype=torch.uint8, device=flm.device)
     flm = flm.contiguous()
-    _lib.check(_lib.lib().s2h_inverse_ex(plan.handle, flm.data_ptr(), f.data_ptr(),
-                                         work.data_ptr(), B, _precision(precision),
-                                         _stream(flm.device)), "inverse")
+    _lib.check(_lib.lib().s2h_inverse_ex(
+        plan.handle, flm.data_ptr(), f.data_ptr(), work.data_ptr(), B, _precision(precision),
+        flags, sub_from.data_ptr() if sub_from is not None else None, _stream(flm.device)),
+        "inverse")
     return f
 
 
 def forward_hip(f, L, spin=0, nside=None, sampling="mw", reality=False, L_lower=0, plan=None,
-                precision="fp64"):
+                precision="fp64", accumulate_into=None):
     """Batched forward on device tensors: f (B, ...) -> flm (B, L, 2L-1) complex128.
 
     ``precision``: "fp64" (default), "mixed" (FP64 recursion and ring FFTs,
     FP32 matrix-core projection with FP32 accumulation over 128 rings),
     "bf16" (split-BF16 matrix-core projection, FP32 accumulation) or "bf16x1"
     (single-product BF16 projection, ~4e-3 relative: a refinement's
-    correction transform).
+    correction transform).  ``accumulate_into``: an flm tensor (contiguous
+    complex128, same shape) that receives flm += F(f) in place and is returned.
     """
     s = sampling.lower()
     plan = plan or get_plan(s, L, nside, spin, L_lower, reality, device=f.device)
     B = f.shape[0]
-    flm = torch.empty((B, L, 2 * L - 1), dtype=torch.complex128, device=f.device)
+    flags = 0
+    if accumulate_into is not None:
+        flm = accumulate_into
+        if (flm.dtype != torch.complex128 or not flm.is_contiguous()
+                or tuple(flm.shape) != (B, L, 2 * L - 1)):
+            raise ValueError("accumulate_into must be a contiguous (B, L, 2L-1) complex128 tensor")
+        flags = _lib.FWD_ACCUMULATE
+    else:
+        flm = torch.empty((B, L, 2 * L - 1), dtype=torch.complex128, device=f.device)
     work = torch.empty(plan.workspace_bytes(B), dtype=torch.uint8, device=f.device)
     f = f.contiguous()
     _lib.check(_lib.lib().s2h_forward_ex(plan.handle, f.data_ptr(), int(f.is_complex()),
                                          flm.data_ptr(), work.data_ptr(), B,
-                                         _precision(precision), _stream(f.device)), "forward")
+                                         _precision(precision), flags, _stream(f.device)),
+               "forward")
     return flm
 
 
@@ -232,9 +254,21 @@ def forward_iterative_hip(f, L, spin=0, nside=None, sampling="mw", reality=False
     """flm = F(f); repeat n_iter: flm += F(f - I(flm)).
 
     s2fft/utils/iterative_refinement.py:9-46 as used by spherical.py:453-462;
-    the residual and the update stay in FP64 on the device.
+    the residual and the update stay in FP64 on the device.  Without autograd
+    the residual comes out of the synthesis pass (f - I(flm)) and the update
+    is accumulated by the projection pass (flm += F(r)).
     """
     flm = forward_diff(f, L, spin, nside, sampling, reality, L_lower)
+    if n_iter and not (torch.is_grad_enabled() and f.requires_grad):
+        s = sampling.lower()
+        plan = get_plan(s, L, nside, spin, L_lower, reality, device=f.device)
+        real_out = bool(reality) and not f.is_complex()
+        for _ in range(n_iter):
+            r = inverse_hip(flm, L, spin, nside, s, reality, L_lower, plan=plan,
+                            real_output=real_out, sub_from=f)
+            forward_hip(r, L, spin, nside, s, reality, L_lower, plan=plan,
+                        accumulate_into=flm)
+        return flm
     for _ in range(n_iter):
         f_rec = inverse_diff(flm, L, spin, nside, sampling, reality, L_lower)
         flm = flm + forward_diff(f - f_rec, L, spin, nside, sampling, reality, L_lower)

@@ -316,3 +316,29 @@ def test_forward_reality_non_bandlimited(rng, sampling, L):
     refc = otr.forward(f + 0j, L, 0, sampling=sampling, reality=False)
     gotc = s2fft_amd.forward(f + 0j, L, 0, sampling=sampling, method="hip", reality=False)
     assert max_rel(gotc, refc) < TOL
+
+
+@pytest.mark.parametrize("sampling,nside,L,reality", [("healpix", 8, 16, True), ("mw", None, 9, True),
+                                                      ("gl", None, 8, False)])
+def test_fused_residual_and_accumulate(rng, sampling, nside, L, reality):
+    """inverse(sub_from=...) == f - inverse, real_output == real part,
+    forward(accumulate_into=...) == flm + forward: the refinement's fused passes."""
+    from s2fft_amd.transforms import spherical as sph
+    B = 3
+    flm = torch.from_numpy(np.stack([_flm(rng, L, 0, reality) for _ in range(B)])).cuda()
+    f = sph.inverse_hip(flm, L, 0, nside, sampling, reality)
+    if reality:
+        fr = f.real.contiguous() if f.is_complex() else f
+        ro = sph.inverse_hip(flm, L, 0, nside, sampling, reality, real_output=True)
+        assert ro.dtype == torch.float64
+        assert max_rel(ro.cpu().numpy(), fr.cpu().numpy()) < 1e-15
+    else:
+        fr = f
+    g = fr + 0.25 * torch.randn_like(fr)
+    r = sph.inverse_hip(flm, L, 0, nside, sampling, reality, real_output=reality, sub_from=g)
+    assert max_rel(r.cpu().numpy(), (g - fr).cpu().numpy()) < 1e-14
+    base = sph.forward_hip(g, L, 0, nside, sampling, reality)
+    acc = base.clone()
+    sph.forward_hip(r, L, 0, nside, sampling, reality, accumulate_into=acc)
+    ref = base + sph.forward_hip(r, L, 0, nside, sampling, reality)
+    assert max_rel(acc.cpu().numpy(), ref.cpu().numpy()) < 1e-14
