@@ -36,9 +36,14 @@ __device__ __forceinline__ float bf16_round(float v) {
 
 // GELU, tanh form: 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3)))
 //                = x / (1 + exp(-2 sqrt(2/pi) (x + 0.044715 x^3)))
+// evaluated as x * rcp(1 + exp2(t)), t = -2 sqrt(2/pi) log2(e) (x + 0.044715 x^3):
+// seven vector instructions, two of them transcendental (the hardware exp2 and
+// reciprocal, about 1 ulp each).  Saturates correctly: exp2 -> inf gives 0.
 __device__ __forceinline__ float gelu(float x) {
-  const float u = 1.5957691216057308f * fmaf(0.044715f * x, x * x, x);
-  return x / (1.0f + __expf(-u));
+  constexpr float c1 = -2.302208198144325f;         // -2 sqrt(2/pi) log2(e)
+  constexpr float c3 = c1 * 0.044715f;
+  const float t = x * fmaf(c3, x * x, c1);
+  return x * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(t));
 }
 // d gelu / dx for the same form: sig + x sig (1 - sig) u'
 __device__ __forceinline__ float gelu_grad(float x) {

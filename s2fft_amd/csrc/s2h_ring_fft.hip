@@ -67,6 +67,27 @@ __device__ void build_phase(cplx* hi, cplx* lo, int count, int m0, double phi0_o
 
 __device__ __forceinline__ cplx conj_if(cplx v, bool c) { return c ? cplx{v.x, -v.y} : v; }
 
+// for (p = tid; p < n; p += TPB) use(p, load(p)), with the loads of U
+// consecutive iterations issued before any of their uses.  Without this each
+// iteration waits out one global-memory latency, and a ring workgroup has too
+// few waves beside it to hide that.
+template <int TPB, int U, typename T, typename Load, typename Use>
+__device__ __forceinline__ void batched_for(int n, Load load, Use use) {
+  for (int p0 = threadIdx.x; p0 < n; p0 += U * TPB) {
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int p = p0 + u * TPB;
+      v[u] = p < n ? load(p) : T{};
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int p = p0 + u * TPB;
+      if (p < n) use(p, v[u]);
+    }
+  }
+}
+
 // Radix-R DFT in place.  Odd R uses the symmetric form: with s_k = v_k + v_{R-k},
 // d_k = v_k - v_{R-k} (k = 1..H, H = (R-1)/2) and c_m + i t_m = e^{sign 2 pi i m/R},
 //   X_j = v_0 + sum_k c_{jk} s_k + i t_{jk} d_k,  X_{R-j} = the same with -i,
@@ -243,10 +264,13 @@ __device__ void fft_t(double* re, double* im, int n, int sign, const Tab2& tw) {
   }
 }
 
-// LDS layout of a ring workgroup: re[N] im[N] | tables (576 complex).
+// LDS layout of a ring workgroup: tables (576 complex) | re[N] im[N], the
+// data region at least 2 TPB doubles (the inverse fold's partial sums).
 // Tables: FFT twiddles (hi <= 128, lo 64), chirp e^{-i pi k/n} for k < 2n
 // (hi <= 128, lo 64; Bluestein only), phase over <= 8192 values of m
 // (hi <= 128, lo 64).
+constexpr int kTabCplx = 576;
+
 struct RingLds {
   double* re;
   double* im;
@@ -260,9 +284,9 @@ struct RingLds {
 
 __device__ __forceinline__ RingLds ring_lds(double* smem, int N) {
   RingLds s;
-  s.re = smem;
-  s.im = smem + N;
-  cplx* t = reinterpret_cast<cplx*>(smem + 2 * N);
+  s.re = smem + 2 * kTabCplx;
+  s.im = s.re + N;
+  cplx* t = reinterpret_cast<cplx*>(smem);
   s.tw_hi = t;
   s.tw_lo = t + 128;
   s.ch_hi = t + 192;
@@ -298,11 +322,13 @@ __device__ void ring_transform(const RingLds& s, int n, int M, const cplx* chirp
   }
   __syncthreads();
   fft_t<TPB, MAXN, true, false>(s.re, s.im, M, -1, tw);
-  for (int k = threadIdx.x; k < M; k += TPB) {
-    const cplx a = cmul(cplx{s.re[k], s.im[k]}, chirp_hat[k]);
-    s.re[k] = a.x;
-    s.im[k] = a.y;
-  }
+  batched_for<TPB, 8, cplx>(
+      M, [&](int k) { return chirp_hat[k]; },
+      [&](int k, cplx c) {
+        const cplx a = cmul(cplx{s.re[k], s.im[k]}, c);
+        s.re[k] = a.x;
+        s.im[k] = a.y;
+      });
   __syncthreads();
   fft_t<TPB, MAXN, true, false>(s.re, s.im, M, +1, tw);
   for (int k = threadIdx.x; k < n; k += TPB) {
@@ -339,29 +365,31 @@ __global__ void __launch_bounds__(TPB) ring_fwd_kernel(RingFwdArgs a, int job0, 
   build_fft_tables<TPB, BLUE>(s, n, M);
   if (a.apply_phase) build_phase<TPB>(s.ph_hi, s.ph_lo, a.ncol, a.m_first, job.phi0_over_pi, -1.0);
 
+  const auto put = [&](int p, cplx v) {
+    s.re[p] = v.x;
+    s.im[p] = v.y;
+  };
   if (paired) {
     // real data: z = x_b0 + i x_b0+1
     if (a.f_complex) {
       const cplx* s0 = reinterpret_cast<const cplx*>(a.f) + b0 * a.map_stride + job.pix_offset;
-      for (int p = threadIdx.x; p < n; p += TPB) {
-        s.re[p] = s0[p].x;
-        s.im[p] = two ? s0[a.map_stride + p].x : 0.0;
-      }
+      batched_for<TPB, 8, cplx>(
+          n, [&](int p) { return cplx{s0[p].x, two ? s0[a.map_stride + p].x : 0.0}; }, put);
     } else {
       const double* s0 = reinterpret_cast<const double*>(a.f) + b0 * a.map_stride + job.pix_offset;
-      for (int p = threadIdx.x; p < n; p += TPB) {
-        s.re[p] = s0[p];
-        s.im[p] = two ? s0[a.map_stride + p] : 0.0;
-      }
+      batched_for<TPB, 8, cplx>(
+          n, [&](int p) { return cplx{s0[p], two ? s0[a.map_stride + p] : 0.0}; }, put);
     }
   } else {
     const bool real_only = a.real_only_all || (a.real_only_2L && n == 2 * a.L);
     const cplx* src = reinterpret_cast<const cplx*>(a.f) + b0 * a.map_stride + job.pix_offset;
-    for (int p = threadIdx.x; p < n; p += TPB) {
-      const cplx v = src[p];
-      s.re[p] = v.x;
-      s.im[p] = real_only ? 0.0 : v.y;
-    }
+    batched_for<TPB, 8, cplx>(
+        n,
+        [&](int p) {
+          const cplx v = src[p];
+          return cplx{v.x, real_only ? 0.0 : v.y};
+        },
+        put);
   }
   __syncthreads();
   ring_transform<TPB, MAXN, KIND>(s, n, M, a.chirp_pool + job.chirp_off, -1);
@@ -396,23 +424,115 @@ __global__ void __launch_bounds__(TPB) ring_fwd_kernel(RingFwdArgs a, int job0, 
 }
 
 // ------------------------------------------------------------------ inverse
-// Fold the (Hermitian-completed) spectrum of one map into F[q].
-__device__ __forceinline__ cplx fold_one(const cplx* src, const RingInvArgs& a, int q, int n,
-                                         int lo, int hi, bool herm, const Tab2& ph) {
-  int m = lo + (((q - lo) % n) + n) % n;
-  cplx acc = {0.0, 0.0};
-  for (; m <= hi; m += n) {
-    cplx v;
+// The folded spectrum of one ring, F[q] = w sum_{m = q mod n} e^{i m phi0} c(m),
+// for a map (z = c) or a pair of real maps (z = c_x + i c_y, whose inverse is
+// x + i y), m over [lo, hi].
+struct FoldSrc {
+  const cplx* x;  // column 0 of the ring's row (m = m_first)
+  const cplx* y;  // second map of a pair, or null
+  int m_first;
+  bool herm;      // m < 0 read as conj(c(-m))
+  bool phase;
+  Tab2 ph;        // e^{i (lo + k) phi0}
+  int lo;
+
+  __device__ __forceinline__ cplx one(const cplx* src, int m) const {
     if (herm && m < 0) {
-      const cplx t = src[-m - a.m_first];
-      v = {t.x, -t.y};
-    } else {
-      v = src[m - a.m_first];
+      const cplx t = src[-m - m_first];
+      return {t.x, -t.y};
     }
-    if (a.apply_phase) v = cmul(v, ph.at(m - lo));
-    acc = cadd(acc, v);
+    return src[m - m_first];
   }
-  return acc;
+  // z(m) with the phase applied
+  __device__ __forceinline__ cplx at(int m) const {
+    cplx z = one(x, m);
+    if (y) {
+      const cplx v = one(y, m);
+      z = {z.x - v.y, z.y + v.x};
+    }
+    return phase ? cmul(z, ph.at(m - lo)) : z;
+  }
+};
+
+// Writes F[q] (times w) to re/im[q], q < n: the sum over m = q (mod n), i.e.
+// m = lo + r + c n with r = (q - lo) mod n, c = 0, 1, ...  The terms of one q
+// are summed in increasing m, so the result does not depend on the launch.
+// Two layouts:
+//  n >= TPB: thread t owns q = t, t + TPB, ... and adds their terms in LDS;
+//  n <  TPB: G = TPB / n threads share a q (thread t: q = t % n, g = t / n
+//            takes the chunks c = g, g + G, ...), their partial sums meet in
+//            LDS (part, 2 TPB doubles) in order of g.
+// Both issue U loads before using them: the loads are independent, and a
+// ring workgroup has too few waves to hide one latency per term.
+template <int TPB>
+__device__ void fold_ring(const FoldSrc& src, int n, int hi, double w, double* re, double* im,
+                          double* part) {
+  constexpr int U = 8;
+  const int t = threadIdx.x;
+  const int lo = src.lo;
+  const int nm = hi - lo + 1;
+  const int nc = (nm + n - 1) / n;  // chunks of n consecutive m
+  const int lo_mod = ((-lo) % n + n) % n;
+  const auto r_of = [&](int q) { return q + lo_mod >= n ? q + lo_mod - n : q + lo_mod; };
+  if (n >= TPB) {
+    for (int q = t; q < n; q += TPB) {
+      re[q] = 0.0;
+      im[q] = 0.0;
+    }
+    const int nq = (n + TPB - 1) / TPB;
+    const int nslot = nq * nc;  // slot = (qi, c): q = t + qi TPB, m = lo + q + c n
+    for (int s0 = 0; s0 < nslot; s0 += U) {
+      cplx v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int sl = s0 + u, qi = sl / nc, q = t + qi * TPB;
+        const int m = lo + r_of(q) + (sl - qi * nc) * n;
+        v[u] = (sl < nslot && q < n && m <= hi) ? src.at(m) : cplx{0.0, 0.0};
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int sl = s0 + u, qi = sl / nc, q = t + qi * TPB;
+        const int m = lo + r_of(q) + (sl - qi * nc) * n;
+        if (sl < nslot && q < n && m <= hi) {
+          re[q] += v[u].x;
+          im[q] += v[u].y;
+        }
+      }
+    }
+    for (int q = t; q < n; q += TPB) {
+      re[q] *= w;
+      im[q] *= w;
+    }
+    return;
+  }
+  const int G = TPB / n;
+  const int q = t % n, g = t / n, r = r_of(q);
+  const int ns = (nc + G - 1) / G;  // chunks per thread
+  cplx acc = {0.0, 0.0};
+  if (g < G) {
+    for (int s0 = 0; s0 < ns; s0 += U) {
+      cplx v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int m = lo + r + (g + (s0 + u) * G) * n;
+        v[u] = (s0 + u < ns && m <= hi) ? src.at(m) : cplx{0.0, 0.0};
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) acc = cadd(acc, v[u]);
+    }
+    part[2 * t] = acc.x;
+    part[2 * t + 1] = acc.y;
+  }
+  __syncthreads();
+  cplx tot = {0.0, 0.0};
+  if (t < n) {
+    for (int gg = 0; gg < G; ++gg) tot = cadd(tot, cplx{part[2 * (gg * n + t)], part[2 * (gg * n + t) + 1]});
+  }
+  __syncthreads();  // part overlaps re/im
+  if (t < n) {
+    re[t] = tot.x * w;
+    im[t] = tot.y * w;
+  }
 }
 
 template <int TPB, int MAXN, int KIND>
@@ -433,22 +553,16 @@ __global__ void __launch_bounds__(TPB) ring_inv_kernel(RingInvArgs a, int job0, 
   if (a.apply_phase) build_phase<TPB>(s.ph_hi, s.ph_lo, hi - lo + 1, lo, job.phi0_over_pi, 1.0);
   __syncthreads();
 
-  const Tab2 ph = {s.ph_hi, s.ph_lo};
   const double w = a.apply_weight ? job.weight : 1.0;
-  const cplx* src0 = a.in + b0 * a.in_map_stride + (int64_t)job.row * a.ncol;
-  for (int q = threadIdx.x; q < n; q += TPB) {
-    const cplx X = fold_one(src0, a, q, n, lo, hi, herm, ph);
-    if (paired) {
-      // Z = X + i Y: the inverse of two Hermitian spectra gives x + i y
-      const cplx Y = two ? fold_one(src0 + a.in_map_stride, a, q, n, lo, hi, herm, ph)
-                         : cplx{0.0, 0.0};
-      s.re[q] = (X.x - Y.y) * w;
-      s.im[q] = (X.y + Y.x) * w;
-    } else {
-      s.re[q] = X.x * w;
-      s.im[q] = X.y * w;
-    }
-  }
+  FoldSrc fs;
+  fs.x = a.in + b0 * a.in_map_stride + (int64_t)job.row * a.ncol;
+  fs.y = two ? fs.x + a.in_map_stride : nullptr;
+  fs.m_first = a.m_first;
+  fs.herm = herm;
+  fs.phase = a.apply_phase;
+  fs.ph = {s.ph_hi, s.ph_lo};
+  fs.lo = lo;
+  fold_ring<TPB>(fs, n, hi, w, s.re, s.im, s.re);
   __syncthreads();
   ring_transform<TPB, MAXN, KIND>(s, n, M, a.chirp_pool + job.chirp_off, +1);
 
@@ -456,18 +570,29 @@ __global__ void __launch_bounds__(TPB) ring_inv_kernel(RingInvArgs a, int job0, 
   const int64_t off = b0 * a.map_stride + job.pix_offset;
   if (paired || !a.f_complex) {
     double* d0 = reinterpret_cast<double*>(a.f) + off;
-    const double* u0 = a.sub_from ? reinterpret_cast<const double*>(a.sub_from) + off : nullptr;
-    for (int p = threadIdx.x; p < n; p += TPB) {
-      d0[p] = u0 ? u0[p] - s.re[p] : s.re[p];
-      if (two) d0[a.map_stride + p] = u0 ? u0[a.map_stride + p] - s.im[p] : s.im[p];
+    if (a.sub_from) {
+      const double* u0 = reinterpret_cast<const double*>(a.sub_from) + off;
+      batched_for<TPB, 8, cplx>(
+          n, [&](int p) { return cplx{u0[p], two ? u0[a.map_stride + p] : 0.0}; },
+          [&](int p, cplx u) {
+            d0[p] = u.x - s.re[p];
+            if (two) d0[a.map_stride + p] = u.y - s.im[p];
+          });
+    } else {
+      for (int p = threadIdx.x; p < n; p += TPB) {
+        d0[p] = s.re[p];
+        if (two) d0[a.map_stride + p] = s.im[p];
+      }
     }
   } else {
     cplx* dst = reinterpret_cast<cplx*>(a.f) + off;
     const cplx* u0 = a.sub_from ? reinterpret_cast<const cplx*>(a.sub_from) + off : nullptr;
-    for (int p = threadIdx.x; p < n; p += TPB) {
-      const cplx v = {s.re[p], herm ? 0.0 : s.im[p]};
-      dst[p] = u0 ? csub(u0[p], v) : v;
-    }
+    batched_for<TPB, 8, cplx>(
+        n, [&](int p) { return u0 ? u0[p] : cplx{0.0, 0.0}; },
+        [&](int p, cplx u) {
+          const cplx v = {s.re[p], herm ? 0.0 : s.im[p]};
+          dst[p] = u0 ? csub(u, v) : v;
+        });
   }
 }
 
@@ -477,9 +602,13 @@ namespace {
 constexpr int kTpbPow2 = 512;
 constexpr int kTpbSmooth = 256;
 constexpr int kTpbBlue = 512;
-constexpr int kTabBytes = 576 * sizeof(cplx);
+constexpr int kTabBytes = kTabCplx * sizeof(cplx);
 
-size_t ring_lds_bytes(int N) { return 2 * sizeof(double) * (size_t)N + kTabBytes; }
+// tables + data region (2 N doubles, at least 2 TPB for the fold's partials)
+size_t ring_lds_bytes(int N, int tpb) {
+  const size_t nd = 2 * (size_t)(N > tpb ? N : tpb);
+  return nd * sizeof(double) + kTabBytes;
+}
 
 bool fwd_paired(const RingFwdArgs& a) { return !a.f_complex || a.real_only_all; }
 bool inv_paired(const RingInvArgs& a) { return a.herm_mode == 1 && !a.f_complex; }
@@ -496,14 +625,14 @@ hipError_t launch_ring_fft_forward(const RingFwdArgs& a0, const JobSet& js, int 
   const unsigned gy = paired ? (unsigned)((batch + 1) / 2) : (unsigned)batch;
   if (js.n_pow2)
     hipLaunchKernelGGL((ring_fwd_kernel<kTpbPow2, kMaxRingLen, kPow2>), dim3(js.n_pow2, gy),
-                       dim3(kTpbPow2), ring_lds_bytes(js.max_pow2), s, a, 0, paired);
+                       dim3(kTpbPow2), ring_lds_bytes(js.max_pow2, kTpbPow2), s, a, 0, paired);
   if (js.n_smooth)
     hipLaunchKernelGGL((ring_fwd_kernel<kTpbSmooth, kMaxRingLen, kSmooth>),
-                       dim3(js.n_smooth, gy), dim3(kTpbSmooth), ring_lds_bytes(js.max_smooth), s,
+                       dim3(js.n_smooth, gy), dim3(kTpbSmooth), ring_lds_bytes(js.max_smooth, kTpbSmooth), s,
                        a, js.n_pow2, paired);
   if (js.n_blue)
     hipLaunchKernelGGL((ring_fwd_kernel<kTpbBlue, kMaxConvLen, kBlue>), dim3(js.n_blue, gy),
-                       dim3(kTpbBlue), ring_lds_bytes(js.max_conv), s, a,
+                       dim3(kTpbBlue), ring_lds_bytes(js.max_conv, kTpbBlue), s, a,
                        js.n_pow2 + js.n_smooth, paired);
   return hipGetLastError();
 }
@@ -518,20 +647,20 @@ hipError_t launch_ring_fft_inverse(const RingInvArgs& a0, const JobSet& js, int 
   const unsigned gy = paired ? (unsigned)((batch + 1) / 2) : (unsigned)batch;
   if (js.n_pow2)
     hipLaunchKernelGGL((ring_inv_kernel<kTpbPow2, kMaxRingLen, kPow2>), dim3(js.n_pow2, gy),
-                       dim3(kTpbPow2), ring_lds_bytes(js.max_pow2), s, a, 0, paired);
+                       dim3(kTpbPow2), ring_lds_bytes(js.max_pow2, kTpbPow2), s, a, 0, paired);
   if (js.n_smooth)
     hipLaunchKernelGGL((ring_inv_kernel<kTpbSmooth, kMaxRingLen, kSmooth>),
-                       dim3(js.n_smooth, gy), dim3(kTpbSmooth), ring_lds_bytes(js.max_smooth), s,
+                       dim3(js.n_smooth, gy), dim3(kTpbSmooth), ring_lds_bytes(js.max_smooth, kTpbSmooth), s,
                        a, js.n_pow2, paired);
   if (js.n_blue)
     hipLaunchKernelGGL((ring_inv_kernel<kTpbBlue, kMaxConvLen, kBlue>), dim3(js.n_blue, gy),
-                       dim3(kTpbBlue), ring_lds_bytes(js.max_conv), s, a,
+                       dim3(kTpbBlue), ring_lds_bytes(js.max_conv, kTpbBlue), s, a,
                        js.n_pow2 + js.n_smooth, paired);
   return hipGetLastError();
 }
 
 hipError_t allow_large_lds_rings() {
-  const int bytes = (int)ring_lds_bytes(kMaxConvLen);
+  const int bytes = (int)ring_lds_bytes(kMaxConvLen, kTpbBlue);
   const void* fns[] = {(const void*)ring_fwd_kernel<kTpbPow2, kMaxRingLen, kPow2>,
                        (const void*)ring_fwd_kernel<kTpbSmooth, kMaxRingLen, kSmooth>,
                        (const void*)ring_fwd_kernel<kTpbBlue, kMaxConvLen, kBlue>,

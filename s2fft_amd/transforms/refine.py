@@ -117,13 +117,8 @@ def _judged_step(precond, h, r, f, flm, kw, stats, synth, ip, exact):
 
 
 def _norm(x, dims):
-    return (x.abs() ** 2).sum(dim=dims)
-
-
-def _real_if(x, like):
-    if not like.is_complex() and x.is_complex():
-        return x.real
-    return x
+    """Per-map 2-norm (one reduction pass over x)."""
+    return torch.linalg.vector_norm(x, ord=2, dim=dims)
 
 
 def _track(history, flm, flm_true):

@@ -336,7 +336,10 @@ def test_fused_residual_and_accumulate(rng, sampling, nside, L, reality):
         fr = f
     g = fr + 0.25 * torch.randn_like(fr)
     r = sph.inverse_hip(flm, L, 0, nside, sampling, reality, real_output=reality, sub_from=g)
-    assert max_rel(r.cpu().numpy(), (g - fr).cpu().numpy()) < 1e-14
+    # the real-output path pairs maps in one complex FFT: rounding at the
+    # level of |f|, so the error is measured against |g|, not the small |r|
+    err = (r - (g - fr)).abs().max() / g.abs().max()
+    assert float(err) < 1e-14
     base = sph.forward_hip(g, L, 0, nside, sampling, reality)
     acc = base.clone()
     sph.forward_hip(r, L, 0, nside, sampling, reality, accumulate_into=acc)
