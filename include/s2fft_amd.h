@@ -176,6 +176,14 @@ int s2h_gamma_dft(int N, int64_t npix, int mode, double scale, const void *in, v
  * (s2fft/transforms/wigner.py:523-526). */
 int s2h_wigner_reality_fill(void *flmn, int L, int N, s2h_stream stream);
 
+/* Per-map sums of squares out[b] = sum_{i < n} x[b * stride + i]^2 (doubles;
+ * a complex map is 2n doubles), with a fixed summation order.  No
+ * counterpart in the reference: the residual norms that judge the learned
+ * preconditioner's steps.  work: s2h_batch_sumsq_workspace_bytes(batch). */
+size_t s2h_batch_sumsq_workspace_bytes(int batch);
+int s2h_batch_sumsq(const double *x, int64_t n, int64_t stride, int batch, double *out,
+                    void *work, s2h_stream stream);
+
 /* Kernel timing for benchmarks (no counterpart in the reference).  While
  * enabled, s2h_forward / s2h_inverse record a HIP event pair on the launch
  * stream around each kernel; s2h_profile_query synchronises on them and

@@ -42,6 +42,7 @@ EXPORTED = (
     "s2h_lat_forward", "s2h_lat_inverse", "s2h_healpix_fft", "s2h_healpix_ifft",
     "healpix_fft_hip", "healpix_fft_hip_last_status", "s2h_gamma_dft",
     "s2h_profile_enable", "s2h_profile_query", "s2h_wigner_reality_fill",
+    "s2h_batch_sumsq_workspace_bytes", "s2h_batch_sumsq",
     "s2h_forward_adjoint", "s2h_inverse_adjoint", "s2h_probe_mfma_f64",
     "s2h_probe_mfma_f32", "s2h_forward_ex", "s2h_inverse_ex",
     "s2h_mlp_param_count", "s2h_mlp_precondition", "s2h_mlp_features",
@@ -88,6 +89,8 @@ def _declare(lib):
         "healpix_fft_hip_last_status": (i32, []),
         "s2h_gamma_dft": (i32, [i32, i64, i32, ctypes.c_double, vp, vp, vp]),
         "s2h_wigner_reality_fill": (i32, [vp, i32, i32, vp]),
+        "s2h_batch_sumsq_workspace_bytes": (ctypes.c_size_t, [i32]),
+        "s2h_batch_sumsq": (i32, [vp, i64, i64, i32, vp, vp, vp]),
         "s2h_profile_enable": (i32, [i32]),
         "s2h_profile_query": (i32, [i32, ctypes.POINTER(ctypes.c_double),
                                     ctypes.POINTER(ctypes.c_int)]),

@@ -639,6 +639,18 @@ extern "C" int s2h_wigner_reality_fill(void* flmn, int L, int N, s2h_stream stre
                                        static_cast<hipStream_t>(stream)));
 }
 
+extern "C" size_t s2h_batch_sumsq_workspace_bytes(int batch) {
+  return batch > 0 ? (size_t)kSumsqBlocks * (size_t)batch * sizeof(double) : 0;
+}
+
+extern "C" int s2h_batch_sumsq(const double* x, int64_t n, int64_t stride, int batch, double* out,
+                               void* work, s2h_stream stream) {
+  if (batch < 0 || n < 0 || (batch > 0 && (!out || !work || (n > 0 && !x) || stride < n)))
+    return S2H_ERR_INVALID;
+  return hip_status(launch_batch_sumsq(x, n, stride, batch, static_cast<double*>(work), out,
+                                       static_cast<hipStream_t>(stream)));
+}
+
 extern "C" int s2h_probe_mfma_f64(const double* a, const double* b, double* out,
                                   s2h_stream stream) {
   if (!a || !b || !out) return S2H_ERR_INVALID;

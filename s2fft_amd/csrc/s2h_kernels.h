@@ -132,6 +132,10 @@ struct PoleArgs {
 hipError_t launch_pole_inverse(const PoleArgs& a, hipStream_t s);
 hipError_t launch_pole_forward(const PoleArgs& a, hipStream_t s);
 hipError_t launch_zero(void* p, int64_t ndoubles, hipStream_t s);
+constexpr int kSumsqBlocks = 256;  // slices per map of launch_batch_sumsq
+// out[b] = sum_{i < n} x[b * stride + i]^2; partial holds kSumsqBlocks * batch doubles
+hipError_t launch_batch_sumsq(const double* x, int64_t n, int64_t stride, int batch,
+                              double* partial, double* out, hipStream_t s);
 hipError_t launch_reality_fill(cplx* flm, int64_t map_stride, int L, int batch, hipStream_t s);
 hipError_t launch_reality_fold(const cplx* g, cplx* out, int64_t map_stride, int L, int batch,
                                hipStream_t s);

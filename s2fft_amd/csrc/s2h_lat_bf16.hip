@@ -32,10 +32,24 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 using lat::m_of_block;
 using lat::start_value;
 
-__device__ __forceinline__ void split2(double v, __bf16& hi, __bf16& lo) {
-  const float f = (float)v;
+// FP64 -> FP32 -> BF16, each step rounded to nearest.  The empty asm keeps the
+// compiler from fusing the two conversions into one correctly rounded FP64 ->
+// BF16 conversion, which it expands to about ten instructions (round-to-odd
+// fix-up with two FP64 compares) per value; the double rounding it avoids is
+// far below the 2^-8 of a BF16 operand.
+__device__ __forceinline__ float to_f32(double v) {
+  float f = (float)v;
+  asm("" : "+v"(f));
+  return f;
+}
+
+__device__ __forceinline__ void split2f(float f, __bf16& hi, __bf16& lo) {
   hi = (__bf16)f;
   lo = (__bf16)(f - (float)hi);
+}
+
+__device__ __forceinline__ void split2(double v, __bf16& hi, __bf16& lo) {
+  split2f(to_f32(v), hi, lo);
 }
 
 __device__ __forceinline__ f32x4 mma(const bf16x8& a, const bf16x8& b, f32x4 c) {
@@ -64,11 +78,16 @@ constexpr int kFTiles = 2;      // 8-map tiles per workgroup (16 maps)
 template <bool PAIRED, int NS, int SPLIT>
 __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
   constexpr bool LO = SPLIT == 3;
-  // d tile per wave: [l_local 32][pair 64] hi (and lo)
-  __shared__ __bf16 dth[kFW][32 * 64];
-  __shared__ __bf16 dtl[LO ? kFW : 1][LO ? 32 * 64 : 1];
-  // per-wave results: [tile][parity][row 16][col 16]
-  __shared__ float ctile[kFW][kFTiles * 2 * 256];
+  // NI sets step together, sharing each coefficient load and giving the FP64
+  // recursion NI independent chains.
+  constexpr int NI = LO ? 1 : 2;
+  static_assert(NS % NI == 0, "set groups");
+  constexpr int kTileB = 32 * 64 * 2;             // one [l_local 32][pair 64] BF16 tile
+  constexpr int kWaveB = (NI + (LO ? 1 : 0)) * kTileB;
+  static_assert(kWaveB >= kFTiles * 2 * 256 * 4, "result tile fits the wave's d tiles");
+  // per wave: NI hi d tiles (+ one lo tile); after the chunk's products the
+  // same bytes hold the wave's results [tile][parity][row 16][col 16] (FP32)
+  __shared__ __attribute__((aligned(16))) unsigned char wbuf[kFW][kWaveB];
   __shared__ lat::ChunkCoef ccs[2][lat::kChunk];  // recursion coefficients, double-buffered
   const int m = m_of_block(blockIdx.x, a.nm, a.m_lo);
   const int am = m < 0 ? -m : m;
@@ -79,8 +98,9 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
   const MStart st = a.mstart[m - a.m_lo];
   const double bsign = m < 0 ? -1.0 : 1.0;
   const Coef* __restrict__ cf = a.coef + (int64_t)am * L;
-  __bf16* dh = dth[wave];
-  __bf16* dl = dtl[LO ? wave : 0];
+  __bf16* dh0 = reinterpret_cast<__bf16*>(wbuf[wave]);  // tile j at dh0 + j * 2048
+  __bf16* dl = reinterpret_cast<__bf16*>(wbuf[wave] + NI * kTileB);
+  float* ctw = reinterpret_cast<float*>(wbuf[wave]);
 
   // recursion state of this lane's pair in each set
   double x[NS], d[NS], dm1[NS];
@@ -151,61 +171,73 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
       acc[t][1] = f32x4{0.f, 0.f, 0.f, 0.f};
     }
 #pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      const bool live = ls[s] == 0;
+    for (int sg = 0; sg < NS; sg += NI) {
+      // 32 steps of sets sg .. sg + NI - 1; the values past the band-limit
+      // are zero (zero coefficients from l = L - 1 on)
 #pragma unroll
       for (int i = 0; i < 32; ++i) {
-        const int l = blk + i;
-        const double v = (live && l < L) ? d[s] : 0.0;
-        __bf16 h, lo;
-        split2(v, h, lo);
+        const lat::ChunkCoef c = cs[i];
         // even / odd l (relative to l0) in separate 16-column halves
         const int col = ((i & 1) << 4) | (i >> 1);
-        dh[col * 64 + lane] = h;
-        if constexpr (LO) dl[col * 64 + lane] = lo;
-        const double dn = lat::rec_step(cs[i], x[s], d[s], dm1[s]);
-        dm1[s] = d[s];
-        d[s] = dn;
+#pragma unroll
+        for (int j = 0; j < NI; ++j) {
+          const int s = sg + j;
+          __bf16 h, lo;
+          const float f = to_f32(d[s]);
+          split2f(ls[s] == 0 ? f : 0.0f, h, lo);
+          dh0[j * 2048 + col * 64 + lane] = h;
+          if constexpr (LO) dl[col * 64 + lane] = lo;
+          const double dn = lat::rec_step(c, x[s], d[s], dm1[s]);
+          dm1[s] = d[s];
+          d[s] = dn;
+        }
       }
-      lat::rescale(d[s], dm1[s], ls[s]);
+#pragma unroll
+      for (int j = 0; j < NI; ++j) lat::rescale(d[sg + j], dm1[sg + j], ls[sg + j]);
       lat::wave_sync();
 #pragma unroll
-      for (int par = 0; par < 2; ++par) {
+      for (int j = 0; j < NI; ++j) {
+        const int s = sg + j;
+        const __bf16* dh = dh0 + j * 2048;
 #pragma unroll
-        for (int kb = 0; kb < 2; ++kb) {
-          // B[k][j]: d of pair 32 kb + 8 kg + e at column j = row (parity par)
-          const int off = (par * 16 + row) * 64 + 32 * kb + 8 * kg;
-          const bf16x8 bh = *reinterpret_cast<const bf16x8*>(dh + off);
-          if constexpr (LO) {
-            const bf16x8 bl = *reinterpret_cast<const bf16x8*>(dl + off);
+        for (int par = 0; par < 2; ++par) {
 #pragma unroll
-            for (int t = 0; t < kFTiles; ++t) {
-              if (par == 0)
-                acc[t][0] = mma3(seh[s][t][kb], sel[s][t][kb], bh, bl, acc[t][0]);
-              else
-                acc[t][1] = mma3(soh[s][t][kb], sol[s][t][kb], bh, bl, acc[t][1]);
-            }
-          } else {
+          for (int kb = 0; kb < 2; ++kb) {
+            // B[k][j]: d of pair 32 kb + 8 kg + e at column j = row (parity par)
+            const int off = (par * 16 + row) * 64 + 32 * kb + 8 * kg;
+            const bf16x8 bh = *reinterpret_cast<const bf16x8*>(dh + off);
+            if constexpr (LO) {
+              const bf16x8 bl = *reinterpret_cast<const bf16x8*>(dl + off);
 #pragma unroll
-            for (int t = 0; t < kFTiles; ++t) {
-              if (par == 0)
-                acc[t][0] = mma(seh[s][t][kb], bh, acc[t][0]);
-              else
-                acc[t][1] = mma(soh[s][t][kb], bh, acc[t][1]);
+              for (int t = 0; t < kFTiles; ++t) {
+                if (par == 0)
+                  acc[t][0] = mma3(seh[s][t][kb], sel[s][t][kb], bh, bl, acc[t][0]);
+                else
+                  acc[t][1] = mma3(soh[s][t][kb], sol[s][t][kb], bh, bl, acc[t][1]);
+              }
+            } else {
+#pragma unroll
+              for (int t = 0; t < kFTiles; ++t) {
+                if (par == 0)
+                  acc[t][0] = mma(seh[s][t][kb], bh, acc[t][0]);
+                else
+                  acc[t][1] = mma(soh[s][t][kb], bh, acc[t][1]);
+              }
             }
           }
         }
       }
       lat::wave_sync();
     }
-    // C[i][j]: lane j + 16 (i/4), register i%4
+    // C[i][j]: lane j + 16 (i/4), register i%4 -> this wave's result tile,
+    // over its own d tiles (their last reads are ordered before by wave_sync)
 #pragma unroll
     for (int t = 0; t < kFTiles; ++t)
 #pragma unroll
       for (int par = 0; par < 2; ++par)
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          ctile[wave][(t * 2 + par) * 256 + (4 * kg + r) * 16 + row] = acc[t][par][r];
+          ctw[(t * 2 + par) * 256 + (4 * kg + r) * 16 + row] = acc[t][par][r];
     if (more && threadIdx.x < lat::kChunk) ccs[buf ^ 1][threadIdx.x] = cn;
     __syncthreads();
     // element (t, parity, row i, col j) -> l = blk + 2 j + parity, map 8t + i/2
@@ -215,7 +247,7 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
       const int bm = 8 * t + (ri >> 1);
       double v = 0.0;
 #pragma unroll
-      for (int w = 0; w < kFW; ++w) v += (double)ctile[w][e];
+      for (int w = 0; w < kFW; ++w) v += (double)reinterpret_cast<const float*>(wbuf[w])[e];
       if (l < L && l >= a.lmin_out && b0 + bm < a.batch) {
         double* o = reinterpret_cast<double*>(out + (int64_t)bm * a.flm_map_stride +
                                               (int64_t)l * (2 * L - 1)) + (ri & 1);

@@ -115,6 +115,66 @@ hipError_t launch_zero(void* p, int64_t ndoubles, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Per-map sum of squares out[b] = sum_{i < n} x[b * stride + i]^2 in two
+// passes with a fixed summation order (the result does not depend on the
+// launch): kSumsqBlocks contiguous slices per map, four interleaved FP64
+// accumulators per thread, then the slices in order.
+__device__ __forceinline__ double block_sum(double v, double* red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int w = 0; w < kThreads / 64; ++w) t += red[w];
+  return t;
+}
+
+__global__ void __launch_bounds__(kThreads) batch_sumsq_partial_kernel(const double* __restrict__ x,
+                                                                        int64_t n, int64_t stride,
+                                                                        double* partial) {
+  __shared__ double red[kThreads / 64];
+  const double* xb = x + (int64_t)blockIdx.y * stride;
+  const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t lo = (int64_t)blockIdx.x * per;
+  const int64_t hi = lo + per < n ? lo + per : n;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int64_t i = lo + threadIdx.x;
+  for (; i + 3 * kThreads < hi; i += 4 * kThreads) {
+    const double v0 = xb[i], v1 = xb[i + kThreads], v2 = xb[i + 2 * kThreads],
+                 v3 = xb[i + 3 * kThreads];
+    a0 = fma(v0, v0, a0);
+    a1 = fma(v1, v1, a1);
+    a2 = fma(v2, v2, a2);
+    a3 = fma(v3, v3, a3);
+  }
+  for (; i < hi; i += kThreads) a0 = fma(xb[i], xb[i], a0);
+  const double t = block_sum((a0 + a1) + (a2 + a3), red);
+  if (threadIdx.x == 0) partial[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(kThreads) batch_sumsq_final_kernel(const double* partial,
+                                                                      int nblk, double* out) {
+  __shared__ double red[kThreads / 64];
+  double a = 0.0;
+  for (int k = threadIdx.x; k < nblk; k += kThreads) a += partial[(int64_t)blockIdx.x * nblk + k];
+  const double t = block_sum(a, red);
+  if (threadIdx.x == 0) out[blockIdx.x] = t;
+}
+
+hipError_t launch_batch_sumsq(const double* x, int64_t n, int64_t stride, int batch,
+                              double* partial, double* out, hipStream_t s) {
+  if (batch <= 0) return hipSuccess;
+  hipLaunchKernelGGL(batch_sumsq_partial_kernel, dim3(kSumsqBlocks, batch), dim3(kThreads), 0, s,
+                     x, n, stride, partial);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(batch_sumsq_final_kernel, dim3(batch), dim3(kThreads), 0, s, partial,
+                     kSumsqBlocks, out);
+  return hipGetLastError();
+}
+
 }  // namespace s2h
 
 namespace s2h {

@@ -21,6 +21,7 @@ accepted only where the FP64-tracked residual decreases.
 
 import torch
 
+from .. import _lib
 from . import spherical as sph
 
 
@@ -60,6 +61,7 @@ def refine_mixed(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=N
 
     flm = sph.forward_hip(f, precision=fp, **kw)
     r = synth(flm, sub=f)
+    rn = None  # per-map squared norm of r, when known
     _track(history, flm, flm_true)
     for k in range(n_iter):
         last = k + 1 == n_iter
@@ -70,23 +72,25 @@ def refine_mixed(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=N
         h = sph.forward_hip(r, precision=fp, **kw)
         refresh = bool(refresh_every and (k + 1) % refresh_every == 0)
         if precond is not None and k < learned_steps:
-            delta, r = _judged_step(precond, h, r, f, flm, kw, stats, synth, ip, refresh)
+            delta, r, rn = _judged_step(precond, h, r, rn, f, flm, kw, stats, synth, ip, refresh)
             flm += delta
         else:
             flm += h
             r = synth(flm, sub=f) if refresh else synth(h, ip, sub=r)
+            rn = None
         _track(history, flm, flm_true)
     return flm
 
 
-def _judged_step(precond, h, r, f, flm, kw, stats, synth, ip, exact):
+def _judged_step(precond, h, r, rn, f, flm, kw, stats, synth, ip, exact):
     """Learned step per map where it lowers the FP64-held residual, else Jacobi.
 
     The candidate delta = P(h) moves the residual to r - I~(delta) (or, on a
     refresh step, to the exact f - I(flm + delta)); a map keeps it only if
-    that residual's 2-norm is below the current one.  Rejected maps take the
-    Jacobi step delta = h instead (one more synthesis, for those maps only).
-    Returns (delta, new residual).
+    that residual's 2-norm is below the current one (rn: the current squared
+    norms, or None).  Rejected maps take the Jacobi step delta = h instead
+    (one more synthesis, for those maps only).
+    Returns (delta, new residual, its squared norms where known, else None).
     """
     L = kw["L"]
     lmin = max(abs(kw["spin"]), kw["L_lower"])
@@ -99,13 +103,13 @@ def _judged_step(precond, h, r, f, flm, kw, stats, synth, ip, exact):
 
     every = slice(None)
     r_new = residual(delta, every)
-    dims = tuple(range(1, r.dim()))
-    ok = _norm(r_new, dims) < _norm(r, dims)
+    new_n = _sumsq(r_new)
+    ok = new_n < (_sumsq(r) if rn is None else rn)
     if stats is not None:
         stats["accepted"] = stats.get("accepted", 0) + int(ok.sum())
         stats["steps"] = stats.get("steps", 0) + int(ok.numel())
     if bool(ok.all()):
-        return delta, r_new
+        return delta, r_new, new_n
     bad = (~ok).nonzero().flatten()
     hj = h[bad].contiguous()
     rj = residual(hj, bad)
@@ -113,12 +117,25 @@ def _judged_step(precond, h, r, f, flm, kw, stats, synth, ip, exact):
     r_new = r_new.clone()
     delta[bad] = hj
     r_new[bad] = rj
-    return delta, r_new
+    return delta, r_new, None
 
 
-def _norm(x, dims):
-    """Per-map 2-norm (one reduction pass over x)."""
-    return torch.linalg.vector_norm(x, ord=2, dim=dims)
+def _sumsq(x):
+    """Per-map sum of squares on the device (s2h_batch_sumsq: one pass, fixed order)."""
+    x = x.contiguous()
+    B = x.shape[0]
+    v = torch.view_as_real(x) if x.is_complex() else x
+    if v.dtype != torch.float64:
+        raise TypeError("residuals are float64 / complex128")
+    n = v.numel() // max(B, 1)
+    out = torch.empty(B, dtype=torch.float64, device=x.device)
+    lib = _lib.lib()
+    work = torch.empty(max(int(lib.s2h_batch_sumsq_workspace_bytes(B)), 8), dtype=torch.uint8,
+                       device=x.device)
+    _lib.check(lib.s2h_batch_sumsq(v.data_ptr(), n, n, B, out.data_ptr(), work.data_ptr(),
+                                   torch.cuda.current_stream(x.device).cuda_stream),
+               "batch sumsq")
+    return out
 
 
 def _track(history, flm, flm_true):
