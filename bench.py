@@ -31,8 +31,13 @@ import time
 
 METRIC = "maps/s at L=2048 HEALPix: mixed+learned vs plain FP64 GPU at ≤1e-7 rel err"
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
+FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 matrix
 BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch at this workload from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE, scaled by a 1 GiB device copy's readings (x2.000 and x1.000):
+# profiles/counters_healpix_nside1024_L2048_b16.md
+TRAFFIC_BYTES = {"lat_forward": 6.80e9, "lat_inverse": 16.48e9}
 
 
 def parse():
@@ -102,19 +107,25 @@ def cpu_baseline(nside, L, n_transforms, rings=512, msteps=256):
 
 
 def kernel_times(_lib):
+    """(total ms, launches) per kernel kind; the latitudinal kinds by arithmetic."""
     names = {"lat_forward": _lib.KERNEL_LAT_FORWARD, "lat_inverse": _lib.KERNEL_LAT_INVERSE,
+             "lat_forward_fp32": _lib.KERNEL_LAT_FORWARD_FP32,
+             "lat_inverse_fp32": _lib.KERNEL_LAT_INVERSE_FP32,
+             "lat_forward_bf16": _lib.KERNEL_LAT_FORWARD_BF16,
+             "lat_inverse_bf16": _lib.KERNEL_LAT_INVERSE_BF16,
              "ring_fft_forward": _lib.KERNEL_RING_FFT_FORWARD,
              "ring_fft_inverse": _lib.KERNEL_RING_FFT_INVERSE}
-    return {k: _lib.profile_query(v) for k, v in names.items()}
+    out = {k: _lib.profile_query(v) for k, v in names.items()}
+    return {k: v for k, v in out.items() if v[1] > 0}
 
 
-def roofline(kernels, nside, L, B, lat_precision):
+def roofline(kernels, nside, L, B):
     """Roofline entry of the dominant kernel kind (by total time).
 
     Latitudinal kernels: algorithmic flops per launch = ceil(ntheta/2) ring
     pairs x P = L(L+1)/2 coefficients x (5 + 4 B) (5 for the recursion step,
-    4 per map for one complex x real multiply-add).  FP64 launches are priced
-    against the FP64 peak, split-BF16 ones against the dense BF16 peak.
+    4 per map for one complex x real multiply-add), priced against the peak
+    of the launch's arithmetic (FP64, FP32 or dense BF16 matrix).
     Ring FFTs: algorithmic bytes per launch = B maps x (8 B per pixel + 16 B
     per ftm element, ntheta x L).
     """
@@ -124,16 +135,15 @@ def roofline(kernels, nside, L, B, lat_precision):
     if dom.startswith("lat"):
         flop = math.ceil(ntheta / 2) * (L * (L + 1) // 2) * (5 + 4 * B)
         achieved = flop / (avg_ms * 1e-3) / 1e12
-        bf = lat_precision.startswith("bf16")
-        peak = BF16_PEAK_TFLOPS if bf else FP64_PEAK_TFLOPS
+        arith = "bf16" if dom.endswith("bf16") else "fp32" if dom.endswith("fp32") else "fp64"
+        peak = {"bf16": BF16_PEAK_TFLOPS, "fp32": FP32_PEAK_TFLOPS}.get(arith, FP64_PEAK_TFLOPS)
         return {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": None, "kernel": dom,
-                "arith": lat_precision,
-                "flop_per_launch": flop, "avg_launch_ms": avg_ms}
+                "frac": achieved / peak, "traffic": TRAFFIC_BYTES.get(dom), "kernel": dom,
+                "arith": arith, "flop_per_launch": flop, "avg_launch_ms": avg_ms}
     nbytes = B * (8 * 12 * nside * nside + 16 * ntheta * L)
     achieved = nbytes / (avg_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES.get(dom), "kernel": dom,
             "bytes_per_launch": nbytes, "avg_launch_ms": avg_ms}
 
 
@@ -210,7 +220,7 @@ def main():
         el, err, kt = timed(lambda: refine.refine_fp64(f, L, 0, nside, "healpix", True, n_plain),
                             "plain")
         plain = {"elapsed": el, "err": err, "kernels": kt, "iters": n_plain, "calib": errs,
-                 "roofline": roofline(kt, nside, L, B, "fp64")}
+                 "roofline": roofline(kt, nside, L, B)}
 
     # ---------------------------------------------------------------- mixed
     precond, learned = None, False
@@ -238,7 +248,7 @@ def main():
         inv_precision=args.mixed_inv_precision, learned_steps=args.learned_steps),
         "mixed")
     mixed = {"elapsed": el, "err": err, "kernels": kt, "iters": n_mixed,
-             "roofline": roofline(kt, nside, L, B, prec)}
+             "roofline": roofline(kt, nside, L, B)}
 
     value = B * args.steps / mixed["elapsed"]
     plain_value = B * args.steps / plain["elapsed"] if plain else None

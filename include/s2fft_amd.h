@@ -188,13 +188,19 @@ int s2h_batch_sumsq(const double *x, int64_t n, int64_t stride, int batch, doubl
  * enabled, s2h_forward / s2h_inverse record a HIP event pair on the launch
  * stream around each kernel; s2h_profile_query synchronises on them and
  * returns the summed milliseconds and the launch count of one kernel kind.
- * s2h_profile_enable(0/1) clears the records.  Off by default. */
+ * s2h_profile_enable(0/1) clears the records.  Off by default.  The
+ * latitudinal kinds count FP64 launches; the _FP32 / _BF16 kinds count the
+ * reduced-precision ones (S2H_PREC_MIXED, and S2H_PREC_BF16 / _BF16X1). */
 enum s2h_kernel_kind {
   S2H_KERNEL_LAT_FORWARD = 0,
   S2H_KERNEL_LAT_INVERSE = 1,
   S2H_KERNEL_RING_FFT_FORWARD = 2,
   S2H_KERNEL_RING_FFT_INVERSE = 3,
-  S2H_KERNEL_RESAMPLE = 4
+  S2H_KERNEL_RESAMPLE = 4,
+  S2H_KERNEL_LAT_FORWARD_FP32 = 5,
+  S2H_KERNEL_LAT_INVERSE_FP32 = 6,
+  S2H_KERNEL_LAT_FORWARD_BF16 = 7,
+  S2H_KERNEL_LAT_INVERSE_BF16 = 8
 };
 int s2h_profile_enable(int on);
 int s2h_profile_query(int kernel, double *total_ms, int *count);

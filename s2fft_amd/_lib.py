@@ -51,6 +51,8 @@ EXPORTED = (
 
 KERNEL_LAT_FORWARD, KERNEL_LAT_INVERSE, KERNEL_RING_FFT_FORWARD = 0, 1, 2
 KERNEL_RING_FFT_INVERSE, KERNEL_RESAMPLE = 3, 4
+KERNEL_LAT_FORWARD_FP32, KERNEL_LAT_INVERSE_FP32 = 5, 6
+KERNEL_LAT_FORWARD_BF16, KERNEL_LAT_INVERSE_BF16 = 7, 8
 PREC_FP64, PREC_MIXED, PREC_BF16, PREC_BF16X1 = 0, 1, 2, 3
 FWD_ACCUMULATE = 1
 INV_REAL_OUTPUT, INV_SUBTRACT = 1, 2

@@ -231,7 +231,9 @@ extern "C" int s2h_forward_ex(const s2h_plan* p, const void* f, int f_is_complex
   a.lmin_out = std::max(std::abs(p->d.spin), p->d.L_lower);
   a.accumulate = accumulate ? 1 : 0;
   {
-    ProfScope ps(S2H_KERNEL_LAT_FORWARD, s);
+    ProfScope ps(bf16 ? S2H_KERNEL_LAT_FORWARD_BF16
+                      : mixed ? S2H_KERNEL_LAT_FORWARD_FP32 : S2H_KERNEL_LAT_FORWARD,
+                 s);
     // batches of 4 or more maps share each d-value across an 8-map MFMA tile;
     // the mixed-precision projection always runs on the FP32 matrix cores
     if (bf16)
@@ -300,7 +302,9 @@ extern "C" int s2h_inverse_ex(const s2h_plan* p, const void* flm, void* f, void*
   a.ftm_ncol = p->Mc;
   a.ftm_m_first = p->m_lo;
   {
-    ProfScope ps(S2H_KERNEL_LAT_INVERSE, s);
+    ProfScope ps(bf16 ? S2H_KERNEL_LAT_INVERSE_BF16
+                      : mixed ? S2H_KERNEL_LAT_INVERSE_FP32 : S2H_KERNEL_LAT_INVERSE,
+                 s);
     // batches of 4 or more maps: each d-value feeds 16 maps through the MFMA
     if (bf16)
       st = hip_status(launch_lat_inverse_bf16(a, p->inv_paired, s));

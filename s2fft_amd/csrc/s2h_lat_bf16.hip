@@ -65,6 +65,8 @@ __device__ __forceinline__ f32x4 mma3(const bf16x8& ah, const bf16x8& al, const 
 }
 
 constexpr int kFW = 8;          // forward: waves per workgroup
+constexpr int kFRow = 72;       // forward d tile row stride (BF16): 64 pairs + 8 pad, so the
+                                // 16-byte B-operand reads of 16 rows hit distinct banks
 constexpr int kFT = kFW * 64;
 constexpr int kFTiles = 2;      // 8-map tiles per workgroup (16 maps)
 
@@ -82,7 +84,8 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
   // recursion NI independent chains.
   constexpr int NI = LO ? 1 : 2;
   static_assert(NS % NI == 0, "set groups");
-  constexpr int kTileB = 32 * 64 * 2;             // one [l_local 32][pair 64] BF16 tile
+  constexpr int kTileE = 32 * kFRow;               // one [l_local 32][pair 64 + pad] BF16 tile
+  constexpr int kTileB = kTileE * 2;
   constexpr int kWaveB = (NI + (LO ? 1 : 0)) * kTileB;
   static_assert(kWaveB >= kFTiles * 2 * 256 * 4, "result tile fits the wave's d tiles");
   // per wave: NI hi d tiles (+ one lo tile); after the chunk's products the
@@ -98,7 +101,7 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
   const MStart st = a.mstart[m - a.m_lo];
   const double bsign = m < 0 ? -1.0 : 1.0;
   const Coef* __restrict__ cf = a.coef + (int64_t)am * L;
-  __bf16* dh0 = reinterpret_cast<__bf16*>(wbuf[wave]);  // tile j at dh0 + j * 2048
+  __bf16* dh0 = reinterpret_cast<__bf16*>(wbuf[wave]);  // tile j at dh0 + j * kTileE
   __bf16* dl = reinterpret_cast<__bf16*>(wbuf[wave] + NI * kTileB);
   float* ctw = reinterpret_cast<float*>(wbuf[wave]);
 
@@ -185,8 +188,8 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
           __bf16 h, lo;
           const float f = to_f32(d[s]);
           split2f(ls[s] == 0 ? f : 0.0f, h, lo);
-          dh0[j * 2048 + col * 64 + lane] = h;
-          if constexpr (LO) dl[col * 64 + lane] = lo;
+          dh0[j * kTileE + col * kFRow + lane] = h;
+          if constexpr (LO) dl[col * kFRow + lane] = lo;
           const double dn = lat::rec_step(c, x[s], d[s], dm1[s]);
           dm1[s] = d[s];
           d[s] = dn;
@@ -198,13 +201,13 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
 #pragma unroll
       for (int j = 0; j < NI; ++j) {
         const int s = sg + j;
-        const __bf16* dh = dh0 + j * 2048;
+        const __bf16* dh = dh0 + j * kTileE;
 #pragma unroll
         for (int par = 0; par < 2; ++par) {
 #pragma unroll
           for (int kb = 0; kb < 2; ++kb) {
             // B[k][j]: d of pair 32 kb + 8 kg + e at column j = row (parity par)
-            const int off = (par * 16 + row) * 64 + 32 * kb + 8 * kg;
+            const int off = (par * 16 + row) * kFRow + 32 * kb + 8 * kg;
             const bf16x8 bh = *reinterpret_cast<const bf16x8*>(dh + off);
             if constexpr (LO) {
               const bf16x8 bl = *reinterpret_cast<const bf16x8*>(dl + off);
@@ -264,6 +267,8 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
 // ------------------------------------------------------------------ inverse
 namespace {
 constexpr int kIW = 4;          // inverse: waves per workgroup
+constexpr int kIRow = 40;       // inverse tile row stride (BF16): 32 l + 8 pad, so the 16-byte
+                                // operand reads and writes of 16 rows hit distinct banks
 constexpr int kIT = kIW * 64;
 constexpr int kIMaps = 16;      // maps per workgroup (two column tiles)
 }  // namespace
@@ -271,12 +276,12 @@ constexpr int kIMaps = 16;      // maps per workgroup (two column tiles)
 template <bool PAIRED>
 __global__ void __launch_bounds__(kIT) lat_inverse_bf16_kernel(LatArgs a) {
   // d tile per wave: [pair 64][l_local 32] hi / lo (A layout)
-  __shared__ __bf16 dth[kIW][64 * 32];
-  __shared__ __bf16 dtl[kIW][64 * 32];
+  __shared__ __attribute__((aligned(16))) __bf16 dth[kIW][64 * kIRow];
+  __shared__ __attribute__((aligned(16))) __bf16 dtl[kIW][64 * kIRow];
   // flm chunk: [parity][column 32 = 16 maps x re/im][l 32] hi / lo (B layout),
   // entries of the other parity zero
-  __shared__ __bf16 bth[2][32 * 32];
-  __shared__ __bf16 btl[2][32 * 32];
+  __shared__ __attribute__((aligned(16))) __bf16 bth[2][32 * kIRow];
+  __shared__ __attribute__((aligned(16))) __bf16 btl[2][32 * kIRow];
   __shared__ lat::ChunkCoef ccs[2][lat::kChunk];  // recursion coefficients, double-buffered
   const int m = m_of_block(blockIdx.x, a.nm, a.m_lo);
   const int am = m < 0 ? -m : m;
@@ -337,10 +342,10 @@ __global__ void __launch_bounds__(kIT) lat_inverse_bf16_kernel(LatArgs a) {
       __bf16 h, lo;
       split2(v[j], h, lo);
       const int par = i & 1;
-      bth[par][col * 32 + i] = h;
-      btl[par][col * 32 + i] = lo;
-      bth[par ^ 1][col * 32 + i] = (__bf16)0.0f;
-      btl[par ^ 1][col * 32 + i] = (__bf16)0.0f;
+      bth[par][col * kIRow + i] = h;
+      btl[par][col * kIRow + i] = lo;
+      bth[par ^ 1][col * kIRow + i] = (__bf16)0.0f;
+      btl[par ^ 1][col * kIRow + i] = (__bf16)0.0f;
     }
   };
   double fv[kStage];
@@ -375,8 +380,8 @@ __global__ void __launch_bounds__(kIT) lat_inverse_bf16_kernel(LatArgs a) {
         dm1 = d;
         d = dn;
       }
-      *reinterpret_cast<bf16x8*>(dh + lane * 32 + 8 * g8) = vh;
-      *reinterpret_cast<bf16x8*>(dl + lane * 32 + 8 * g8) = vl;
+      *reinterpret_cast<bf16x8*>(dh + lane * kIRow + 8 * g8) = vh;
+      *reinterpret_cast<bf16x8*>(dl + lane * kIRow + 8 * g8) = vl;
     }
     lat::rescale(d, dm1, ls);
     if (more && threadIdx.x < lat::kChunk) ccs[buf ^ 1][threadIdx.x] = cn;
@@ -384,7 +389,7 @@ __global__ void __launch_bounds__(kIT) lat_inverse_bf16_kernel(LatArgs a) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       // A[i][k]: pair 16 q + row, l = 8 kg + e
-      const int aoff = (16 * q + row) * 32 + 8 * kg;
+      const int aoff = (16 * q + row) * kIRow + 8 * kg;
       const bf16x8 ah = *reinterpret_cast<const bf16x8*>(dh + aoff);
       const bf16x8 al = *reinterpret_cast<const bf16x8*>(dl + aoff);
 #pragma unroll
@@ -392,7 +397,7 @@ __global__ void __launch_bounds__(kIT) lat_inverse_bf16_kernel(LatArgs a) {
 #pragma unroll
         for (int par = 0; par < 2; ++par) {
           // B[k][j]: column 16 t + row, l = 8 kg + e
-          const int boff = (16 * t + row) * 32 + 8 * kg;
+          const int boff = (16 * t + row) * kIRow + 8 * kg;
           const bf16x8 bh = *reinterpret_cast<const bf16x8*>(&bth[par][boff]);
           const bf16x8 bl = *reinterpret_cast<const bf16x8*>(&btl[par][boff]);
           acc[q][t][par] = mma3(ah, al, bh, bl, acc[q][t][par]);

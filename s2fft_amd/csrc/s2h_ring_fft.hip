@@ -27,6 +27,12 @@ namespace {
 
 constexpr int kTabLo = 64;
 
+// LDS word index of data point p: one pad word per 32 breaks the power-of-two
+// strides of the early Stockham passes across the LDS banks (their writes go
+// to stride R, then to runs of R at stride R^2).
+__device__ __forceinline__ int lx(int p) { return p + (p >> 5); }
+__host__ __device__ constexpr int lx_len(int n) { return n + (n >> 5) + 1; }
+
 // e^{i pi num/den}-based two-level table: value(k) = hi[k >> 6] * lo[k & 63],
 // value(k) = e^{i pi s k / d} for 0 <= k < count.
 struct Tab2 {
@@ -132,13 +138,14 @@ __device__ __forceinline__ void radix_dft(cplx* v, int sign, const cplx* cs) {
 // The last pass (Ns R = n) writes exactly the points each butterfly read, so
 // it streams one butterfly at a time; earlier passes hold their butterflies
 // in registers across the barrier between the reads and the writes.
-template <int R>
+// P2: Ns is a power of two (every pass of a power-of-two length).
+template <int R, bool P2>
 __device__ __forceinline__ void butterfly(cplx* v, const double* re, const double* im, int j,
                                           int nb, int Ns, int step, bool cj, int sign,
                                           const Tab2& tw, const cplx* cs) {
 #pragma unroll
-  for (int r = 0; r < R; ++r) v[r] = {re[j + r * nb], im[j + r * nb]};
-  const int jl = j % Ns;
+  for (int r = 0; r < R; ++r) v[r] = {re[lx(j + r * nb)], im[lx(j + r * nb)]};
+  const int jl = P2 ? (j & (Ns - 1)) : j % Ns;
   if (jl != 0) {
     const cplx w1 = conj_if(tw.at(jl * step), cj);
     cplx wr = w1;
@@ -151,7 +158,7 @@ __device__ __forceinline__ void butterfly(cplx* v, const double* re, const doubl
   radix_dft<R>(v, sign, cs);
 }
 
-template <int R, int TPB, int MAXN>
+template <int R, int TPB, int MAXN, bool P2>
 __device__ __forceinline__ void pass_t(double* re, double* im, int n, int Ns, int sign, const Tab2& tw) {
   constexpr int MAXB = (MAXN / R + TPB - 1) / TPB;
   constexpr int H = R / 2 + 1;
@@ -167,11 +174,11 @@ __device__ __forceinline__ void pass_t(double* re, double* im, int n, int Ns, in
   if (R == 2 || R == 4 || Ns * R == n) {
     for (int j = threadIdx.x; j < nb; j += TPB) {
       cplx v[R];
-      butterfly<R>(v, re, im, j, nb, Ns, step, cj, sign, tw, cs);
+      butterfly<R, P2>(v, re, im, j, nb, Ns, step, cj, sign, tw, cs);
 #pragma unroll
       for (int k = 0; k < R; ++k) {
-        re[j + k * Ns] = v[k].x;
-        im[j + k * Ns] = v[k].y;
+        re[lx(j + k * Ns)] = v[k].x;
+        im[lx(j + k * Ns)] = v[k].y;
       }
     }
     __syncthreads();
@@ -182,19 +189,19 @@ __device__ __forceinline__ void pass_t(double* re, double* im, int n, int Ns, in
 #pragma unroll
     for (int c = 0; c < MAXB; ++c) {
       const int j = threadIdx.x + c * TPB;
-      if (j < nb) butterfly<R>(v[c], re, im, j, nb, Ns, step, cj, sign, tw, cs);
+      if (j < nb) butterfly<R, P2>(v[c], re, im, j, nb, Ns, step, cj, sign, tw, cs);
     }
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < MAXB; ++c) {
       const int j = threadIdx.x + c * TPB;
       if (j < nb) {
-        const int jl = j % Ns;
+        const int jl = P2 ? (j & (Ns - 1)) : j % Ns;
         const int base = (j - jl) * R + jl;
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-          re[base + k * Ns] = v[c][k].x;
-          im[base + k * Ns] = v[c][k].y;
+          re[lx(base + k * Ns)] = v[c][k].x;
+          im[lx(base + k * Ns)] = v[c][k].y;
         }
       }
     }
@@ -204,36 +211,36 @@ __device__ __forceinline__ void pass_t(double* re, double* im, int n, int Ns, in
 
 // Out-of-line pass: the smooth and Bluestein kernels call their passes so
 // that each pass gets its own register allocation.
-template <int R, int TPB, int MAXN>
+template <int R, int TPB, int MAXN, bool P2>
 __device__ __noinline__ void pass_call(double* re, double* im, int n, int Ns, int sign,
                                        const Tab2 tw) {
-  pass_t<R, TPB, MAXN>(re, im, n, Ns, sign, tw);
+  pass_t<R, TPB, MAXN, P2>(re, im, n, Ns, sign, tw);
 }
 
-template <int R, int TPB, int MAXN, bool INL>
+template <int R, int TPB, int MAXN, bool INL, bool P2>
 __device__ __forceinline__ void pass_any(double* re, double* im, int n, int Ns, int sign,
                                          const Tab2& tw) {
   if constexpr (INL)
-    pass_t<R, TPB, MAXN>(re, im, n, Ns, sign, tw);
+    pass_t<R, TPB, MAXN, P2>(re, im, n, Ns, sign, tw);
   else
-    pass_call<R, TPB, MAXN>(re, im, n, Ns, sign, tw);
+    pass_call<R, TPB, MAXN, P2>(re, im, n, Ns, sign, tw);
 }
 
 template <int TPB, int MAXN, bool POW2, bool INL>
 __device__ __forceinline__ void run_pass(int R, double* re, double* im, int n, int Ns, int sign,
                                          const Tab2& tw) {
   switch (R) {
-    case 8: pass_any<8, TPB, MAXN, INL>(re, im, n, Ns, sign, tw); break;
-    case 4: pass_any<4, TPB, MAXN, INL>(re, im, n, Ns, sign, tw); break;
-    case 2: pass_any<2, TPB, MAXN, INL>(re, im, n, Ns, sign, tw); break;
+    case 8: pass_any<8, TPB, MAXN, INL, POW2>(re, im, n, Ns, sign, tw); break;
+    case 4: pass_any<4, TPB, MAXN, INL, POW2>(re, im, n, Ns, sign, tw); break;
+    case 2: pass_any<2, TPB, MAXN, INL, POW2>(re, im, n, Ns, sign, tw); break;
     default:
       if constexpr (!POW2) {
         switch (R) {
-          case 3: pass_any<3, TPB, MAXN, INL>(re, im, n, Ns, sign, tw); break;
-          case 5: pass_any<5, TPB, MAXN, INL>(re, im, n, Ns, sign, tw); break;
-          case 7: pass_any<7, TPB, MAXN, INL>(re, im, n, Ns, sign, tw); break;
-          case 11: pass_any<11, TPB, MAXN, INL>(re, im, n, Ns, sign, tw); break;
-          case 13: pass_any<13, TPB, MAXN, INL>(re, im, n, Ns, sign, tw); break;
+          case 3: pass_any<3, TPB, MAXN, INL, false>(re, im, n, Ns, sign, tw); break;
+          case 5: pass_any<5, TPB, MAXN, INL, false>(re, im, n, Ns, sign, tw); break;
+          case 7: pass_any<7, TPB, MAXN, INL, false>(re, im, n, Ns, sign, tw); break;
+          case 11: pass_any<11, TPB, MAXN, INL, false>(re, im, n, Ns, sign, tw); break;
+          case 13: pass_any<13, TPB, MAXN, INL, false>(re, im, n, Ns, sign, tw); break;
           default: break;
         }
       }
@@ -264,8 +271,9 @@ __device__ void fft_t(double* re, double* im, int n, int sign, const Tab2& tw) {
   }
 }
 
-// LDS layout of a ring workgroup: tables (576 complex) | re[N] im[N], the
-// data region at least 2 TPB doubles (the inverse fold's partial sums).
+// LDS layout of a ring workgroup: tables (576 complex) | re im, each
+// lx_len(N) words (points at lx(p)); the data region is at least 2 TPB
+// doubles (the inverse fold's partial sums).
 // Tables: FFT twiddles (hi <= 128, lo 64), chirp e^{-i pi k/n} for k < 2n
 // (hi <= 128, lo 64; Bluestein only), phase over <= 8192 values of m
 // (hi <= 128, lo 64).
@@ -285,7 +293,7 @@ struct RingLds {
 __device__ __forceinline__ RingLds ring_lds(double* smem, int N) {
   RingLds s;
   s.re = smem + 2 * kTabCplx;
-  s.im = s.re + N;
+  s.im = s.re + lx_len(N);
   cplx* t = reinterpret_cast<cplx*>(smem);
   s.tw_hi = t;
   s.tw_lo = t + 128;
@@ -311,13 +319,13 @@ __device__ void ring_transform(const RingLds& s, int n, int M, const cplx* chirp
   const bool cj = sign > 0;
   for (int p = threadIdx.x; p < M; p += TPB) {
     if (p < n) {
-      cplx x = {s.re[p], cj ? -s.im[p] : s.im[p]};
+      cplx x = {s.re[lx(p)], cj ? -s.im[lx(p)] : s.im[lx(p)]};
       x = cmul(x, ch.at((int)(((unsigned)p * (unsigned)p) % (unsigned)(2 * n))));
-      s.re[p] = x.x;
-      s.im[p] = x.y;
+      s.re[lx(p)] = x.x;
+      s.im[lx(p)] = x.y;
     } else {
-      s.re[p] = 0.0;
-      s.im[p] = 0.0;
+      s.re[lx(p)] = 0.0;
+      s.im[lx(p)] = 0.0;
     }
   }
   __syncthreads();
@@ -325,16 +333,16 @@ __device__ void ring_transform(const RingLds& s, int n, int M, const cplx* chirp
   batched_for<TPB, 8, cplx>(
       M, [&](int k) { return chirp_hat[k]; },
       [&](int k, cplx c) {
-        const cplx a = cmul(cplx{s.re[k], s.im[k]}, c);
-        s.re[k] = a.x;
-        s.im[k] = a.y;
+        const cplx a = cmul(cplx{s.re[lx(k)], s.im[lx(k)]}, c);
+        s.re[lx(k)] = a.x;
+        s.im[lx(k)] = a.y;
       });
   __syncthreads();
   fft_t<TPB, MAXN, true, false>(s.re, s.im, M, +1, tw);
   for (int k = threadIdx.x; k < n; k += TPB) {
-    const cplx a = cmul(cplx{s.re[k], s.im[k]}, ch.at((int)(((unsigned)k * (unsigned)k) % (unsigned)(2 * n))));
-    s.re[k] = a.x;
-    s.im[k] = cj ? -a.y : a.y;
+    const cplx a = cmul(cplx{s.re[lx(k)], s.im[lx(k)]}, ch.at((int)(((unsigned)k * (unsigned)k) % (unsigned)(2 * n))));
+    s.re[lx(k)] = a.x;
+    s.im[lx(k)] = cj ? -a.y : a.y;
   }
   __syncthreads();
   }
@@ -366,8 +374,8 @@ __global__ void __launch_bounds__(TPB) ring_fwd_kernel(RingFwdArgs a, int job0, 
   if (a.apply_phase) build_phase<TPB>(s.ph_hi, s.ph_lo, a.ncol, a.m_first, job.phi0_over_pi, -1.0);
 
   const auto put = [&](int p, cplx v) {
-    s.re[p] = v.x;
-    s.im[p] = v.y;
+    s.re[lx(p)] = v.x;
+    s.im[lx(p)] = v.y;
   };
   if (paired) {
     // real data: z = x_b0 + i x_b0+1
@@ -403,10 +411,10 @@ __global__ void __launch_bounds__(TPB) ring_fwd_kernel(RingFwdArgs a, int job0, 
     if (k < 0) k += n;
     const bool zero = a.zero_neg_2L && n == 2 * a.L && m < 0;
     const cplx f = a.apply_phase ? ph.at(col) : cplx{1.0, 0.0};
-    const cplx z = {s.re[k], s.im[k]};
+    const cplx z = {s.re[lx(k)], s.im[lx(k)]};
     if (paired) {
       const int k2 = k == 0 ? 0 : n - k;
-      const cplx zc = {s.re[k2], -s.im[k2]};
+      const cplx zc = {s.re[lx(k2)], -s.im[lx(k2)]};
       // X = (z + zc)/2, Y = (z - zc)/(2i)
       const cplx X = {0.5 * (z.x + zc.x), 0.5 * (z.y + zc.y)};
       const cplx Y = {0.5 * (z.y - zc.y), -0.5 * (z.x - zc.x)};
@@ -476,8 +484,8 @@ __device__ void fold_ring(const FoldSrc& src, int n, int hi, double w, double* r
   const auto r_of = [&](int q) { return q + lo_mod >= n ? q + lo_mod - n : q + lo_mod; };
   if (n >= TPB) {
     for (int q = t; q < n; q += TPB) {
-      re[q] = 0.0;
-      im[q] = 0.0;
+      re[lx(q)] = 0.0;
+      im[lx(q)] = 0.0;
     }
     const int nq = (n + TPB - 1) / TPB;
     const int nslot = nq * nc;  // slot = (qi, c): q = t + qi TPB, m = lo + q + c n
@@ -494,14 +502,14 @@ __device__ void fold_ring(const FoldSrc& src, int n, int hi, double w, double* r
         const int sl = s0 + u, qi = sl / nc, q = t + qi * TPB;
         const int m = lo + r_of(q) + (sl - qi * nc) * n;
         if (sl < nslot && q < n && m <= hi) {
-          re[q] += v[u].x;
-          im[q] += v[u].y;
+          re[lx(q)] += v[u].x;
+          im[lx(q)] += v[u].y;
         }
       }
     }
     for (int q = t; q < n; q += TPB) {
-      re[q] *= w;
-      im[q] *= w;
+      re[lx(q)] *= w;
+      im[lx(q)] *= w;
     }
     return;
   }
@@ -530,8 +538,8 @@ __device__ void fold_ring(const FoldSrc& src, int n, int hi, double w, double* r
   }
   __syncthreads();  // part overlaps re/im
   if (t < n) {
-    re[t] = tot.x * w;
-    im[t] = tot.y * w;
+    re[lx(t)] = tot.x * w;
+    im[lx(t)] = tot.y * w;
   }
 }
 
@@ -575,13 +583,13 @@ __global__ void __launch_bounds__(TPB) ring_inv_kernel(RingInvArgs a, int job0, 
       batched_for<TPB, 8, cplx>(
           n, [&](int p) { return cplx{u0[p], two ? u0[a.map_stride + p] : 0.0}; },
           [&](int p, cplx u) {
-            d0[p] = u.x - s.re[p];
-            if (two) d0[a.map_stride + p] = u.y - s.im[p];
+            d0[p] = u.x - s.re[lx(p)];
+            if (two) d0[a.map_stride + p] = u.y - s.im[lx(p)];
           });
     } else {
       for (int p = threadIdx.x; p < n; p += TPB) {
-        d0[p] = s.re[p];
-        if (two) d0[a.map_stride + p] = s.im[p];
+        d0[p] = s.re[lx(p)];
+        if (two) d0[a.map_stride + p] = s.im[lx(p)];
       }
     }
   } else {
@@ -590,7 +598,7 @@ __global__ void __launch_bounds__(TPB) ring_inv_kernel(RingInvArgs a, int job0, 
     batched_for<TPB, 8, cplx>(
         n, [&](int p) { return u0 ? u0[p] : cplx{0.0, 0.0}; },
         [&](int p, cplx u) {
-          const cplx v = {s.re[p], herm ? 0.0 : s.im[p]};
+          const cplx v = {s.re[lx(p)], herm ? 0.0 : s.im[lx(p)]};
           dst[p] = u0 ? csub(u, v) : v;
         });
   }
@@ -604,9 +612,10 @@ constexpr int kTpbSmooth = 256;
 constexpr int kTpbBlue = 512;
 constexpr int kTabBytes = kTabCplx * sizeof(cplx);
 
-// tables + data region (2 N doubles, at least 2 TPB for the fold's partials)
+// tables + data region (2 lx_len(N) doubles, at least 2 TPB for the fold's partials)
 size_t ring_lds_bytes(int N, int tpb) {
-  const size_t nd = 2 * (size_t)(N > tpb ? N : tpb);
+  const int w = lx_len(N);
+  const size_t nd = 2 * (size_t)(w > tpb ? w : tpb);
   return nd * sizeof(double) + kTabBytes;
 }
 
