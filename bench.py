@@ -4,18 +4,22 @@ Workload (one step): the forward spin-0 transform of a batch of 16 real
 HEALPix maps at nside=1024 (L=2048), refined until the max-relative error
 against the seeded coefficients is <= 1e-7.  Maps are band-limited:
 f = I(flm) for flm drawn with seeds 300..315 (the reference's
-signal-generator distribution).  Two pipelines are timed on the same maps:
+signal-generator distribution).  Three pipelines are timed on the same maps:
 
 * plain   -- the reference's Jacobi refinement flm += F(f - I(flm)) on the
-             FP64 kernels (the "plain FP64 GPU" baseline of the metric);
+             FP64 kernels, residual on the pixels (two ring FFTs per
+             iteration): the "plain FP64 GPU" baseline of the metric;
+* plain spectral -- the same FP64 iteration with the residual on the ring
+             spectra (no ring FFT inside the loop), to separate that
+             algorithmic gain from the precision gain;
 * mixed   -- the mixed-precision refinement (FP64 recursion and residual,
-             split-BF16 matrix-core projection / synthesis, FP64 residual
-             refresh) with the learned preconditioner when its trained
-             weights are present (else its Jacobi fallback; the JSON says
-             which).
+             BF16 matrix-core projection / synthesis, FP64 residual refresh,
+             residual on the ring spectra) with the learned preconditioner
+             when its trained weights are present (else its Jacobi fallback;
+             the JSON says which).
 
 Iteration counts are fixed before timing by running each pipeline once.
-`value` is the mixed pipeline's maps/s; the plain one and the ratio sit in
+`value` is the mixed pipeline's maps/s; the plain ones and the ratios sit in
 `config`.  The roofline is reported for the plain pipeline's dominant kernel
 (timed live with HIP events on its launch stream) and, under
 `config.mixed_roofline`, for the mixed pipeline's.  The CPU baseline is the
@@ -56,6 +60,9 @@ def parse():
     ap.add_argument("--learned-steps", type=int, default=2)
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--pixel-residual", action="store_true",
+                    help="mixed leg: residuals on the pixels (two ring FFTs per step) instead of "
+                         "on the ring spectra")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     return ap.parse_args()
 
@@ -114,7 +121,8 @@ def kernel_times(_lib):
              "lat_forward_bf16": _lib.KERNEL_LAT_FORWARD_BF16,
              "lat_inverse_bf16": _lib.KERNEL_LAT_INVERSE_BF16,
              "ring_fft_forward": _lib.KERNEL_RING_FFT_FORWARD,
-             "ring_fft_inverse": _lib.KERNEL_RING_FFT_INVERSE}
+             "ring_fft_inverse": _lib.KERNEL_RING_FFT_INVERSE,
+             "ring_residual": _lib.KERNEL_RING_RESIDUAL}
     out = {k: _lib.profile_query(v) for k, v in names.items()}
     return {k: v for k, v in out.items() if v[1] > 0}
 
@@ -221,6 +229,10 @@ def main():
                             "plain")
         plain = {"elapsed": el, "err": err, "kernels": kt, "iters": n_plain, "calib": errs,
                  "roofline": roofline(kt, nside, L, B)}
+        # the same Jacobi iteration with its residuals on the ring spectra
+        el, err, kt = timed(lambda: refine.refine_fp64(f, L, 0, nside, "healpix", True, n_plain,
+                                                       spectral=True), "plain spectral")
+        plain["spectral"] = {"elapsed": el, "err": err, "kernels": kt}
 
     # ---------------------------------------------------------------- mixed
     precond, learned = None, False
@@ -234,10 +246,12 @@ def main():
     prec = args.mixed_precision
     log(f"[bench] mixed ({prec}, learned={learned}): calibrating the step count")
     hist, stats = [], {}
+    spectral = not args.pixel_residual
     refine.refine_mixed(f, L, 0, nside, "healpix", True, args.max_iter, plan=plan,
                         precond=precond, refresh_every=args.refresh_every, history=hist,
                         flm_true=flm_true, stats=stats, precision=prec,
-                        inv_precision=args.mixed_inv_precision, learned_steps=args.learned_steps)
+                        inv_precision=args.mixed_inv_precision, learned_steps=args.learned_steps,
+                        spectral=spectral)
     ok = [i for i, e in enumerate(hist) if e <= args.tol]
     n_mixed = ok[0] if ok else args.max_iter
     log(f"[bench]   mixed errors {['%.1e' % e for e in hist[:n_mixed + 2]]} -> {n_mixed} steps")
@@ -245,8 +259,8 @@ def main():
     el, err, kt = timed(lambda: refine.refine_mixed(
         f, L, 0, nside, "healpix", True, n_mixed, plan=plan, precond=precond,
         refresh_every=args.refresh_every, precision=prec,
-        inv_precision=args.mixed_inv_precision, learned_steps=args.learned_steps),
-        "mixed")
+        inv_precision=args.mixed_inv_precision, learned_steps=args.learned_steps,
+        spectral=spectral), "mixed")
     mixed = {"elapsed": el, "err": err, "kernels": kt, "iters": n_mixed,
              "roofline": roofline(kt, nside, L, B)}
 
@@ -272,6 +286,8 @@ def main():
             "method": (f"mixed: FP64 recursion, {prec} matrix-core projection, "
                        f"{args.mixed_inv_precision} matrix-core synthesis, "
                        f"FP64 residual refreshed every {args.refresh_every} steps, "
+                       + ("residuals on the ring spectra (no ring FFT in the loop), "
+                          if spectral else "residuals on the pixels, ")
                        + (f"learned MLP preconditioner (Jacobi-judged) on the first "
                           f"{args.learned_steps} steps, Jacobi after" if learned
                           else "Jacobi")),
@@ -287,9 +303,15 @@ def main():
         "roofline": head,
     }
     if plain:
+        spec_value = B * args.steps / plain["spectral"]["elapsed"]
         result["config"].update({
             "plain_fp64_maps_per_s": plain_value,
             "speedup_vs_plain_fp64": value / plain_value,
+            "plain_fp64_spectral_maps_per_s": spec_value,
+            "speedup_vs_plain_fp64_spectral": value / spec_value,
+            "plain_fp64_spectral_max_rel_err": plain["spectral"]["err"],
+            "plain_fp64_spectral_kernel_ms": {k: {"total_ms": v[0], "launches": v[1]}
+                                              for k, v in plain["spectral"]["kernels"].items()},
             "plain_ms_per_step": 1e3 * plain["elapsed"] / args.steps,
             "plain_refine_iters": plain["iters"], "plain_max_rel_err": plain["err"],
             "plain_calibration_errs": plain["calib"],

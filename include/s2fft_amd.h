@@ -176,6 +176,28 @@ int s2h_gamma_dft(int N, int64_t npix, int mode, double scale, const void *in, v
  * (s2fft/transforms/wigner.py:523-526). */
 int s2h_wigner_reality_fill(void *flmn, int L, int N, s2h_stream stream);
 
+/* The transforms split at the ring spectra (no counterpart in the reference;
+ * for refinement in spectral space).  For plans whose forward and inverse
+ * share one ring grid with neither pole rows nor resampling (HEALPix, GL;
+ * others return S2H_ERR_UNSUPPORTED):
+ *   s2h_forward = s2h_forward_rings . s2h_ring_analysis,
+ *   F(f - I(x)) = s2h_forward_rings(s2h_ring_residual(G, s2h_inverse_rings(x)))
+ * with G = s2h_ring_analysis(f).  Spectra: batch x (rows, Mc) complex128,
+ * s2h_ring_spectra_bytes(plan, batch) bytes.  s2h_ring_residual writes
+ * R = G - A(Y), A = ring FFT . ring iFFT evaluated without the FFTs (a fold of
+ * each ring's m-spectrum onto its n_t bins and the periodic extension back,
+ * with weights and ring phases); R may be G, not Y.  s2h_forward_rings takes
+ * S2H_FWD_ACCUMULATE; precision as in s2h_forward_ex. */
+size_t s2h_ring_spectra_bytes(const s2h_plan *plan, int batch);
+int s2h_ring_analysis(const s2h_plan *plan, const void *f, int f_is_complex, void *G, int batch,
+                      s2h_stream stream);
+int s2h_forward_rings(const s2h_plan *plan, const void *G, void *flm, int batch, int precision,
+                      int flags, s2h_stream stream);
+int s2h_inverse_rings(const s2h_plan *plan, const void *flm, void *Y, int batch, int precision,
+                      s2h_stream stream);
+int s2h_ring_residual(const s2h_plan *plan, const void *G, const void *Y, void *R, int batch,
+                      s2h_stream stream);
+
 /* Per-map sums of squares out[b] = sum_{i < n} x[b * stride + i]^2 (doubles;
  * a complex map is 2n doubles), with a fixed summation order.  No
  * counterpart in the reference: the residual norms that judge the learned
@@ -200,7 +222,8 @@ enum s2h_kernel_kind {
   S2H_KERNEL_LAT_FORWARD_FP32 = 5,
   S2H_KERNEL_LAT_INVERSE_FP32 = 6,
   S2H_KERNEL_LAT_FORWARD_BF16 = 7,
-  S2H_KERNEL_LAT_INVERSE_BF16 = 8
+  S2H_KERNEL_LAT_INVERSE_BF16 = 8,
+  S2H_KERNEL_RING_RESIDUAL = 9
 };
 int s2h_profile_enable(int on);
 int s2h_profile_query(int kernel, double *total_ms, int *count);

@@ -43,6 +43,8 @@ EXPORTED = (
     "healpix_fft_hip", "healpix_fft_hip_last_status", "s2h_gamma_dft",
     "s2h_profile_enable", "s2h_profile_query", "s2h_wigner_reality_fill",
     "s2h_batch_sumsq_workspace_bytes", "s2h_batch_sumsq",
+    "s2h_ring_spectra_bytes", "s2h_ring_analysis", "s2h_forward_rings", "s2h_inverse_rings",
+    "s2h_ring_residual",
     "s2h_forward_adjoint", "s2h_inverse_adjoint", "s2h_probe_mfma_f64",
     "s2h_probe_mfma_f32", "s2h_forward_ex", "s2h_inverse_ex",
     "s2h_mlp_param_count", "s2h_mlp_precondition", "s2h_mlp_features",
@@ -53,6 +55,7 @@ KERNEL_LAT_FORWARD, KERNEL_LAT_INVERSE, KERNEL_RING_FFT_FORWARD = 0, 1, 2
 KERNEL_RING_FFT_INVERSE, KERNEL_RESAMPLE = 3, 4
 KERNEL_LAT_FORWARD_FP32, KERNEL_LAT_INVERSE_FP32 = 5, 6
 KERNEL_LAT_FORWARD_BF16, KERNEL_LAT_INVERSE_BF16 = 7, 8
+KERNEL_RING_RESIDUAL = 9
 PREC_FP64, PREC_MIXED, PREC_BF16, PREC_BF16X1 = 0, 1, 2, 3
 FWD_ACCUMULATE = 1
 INV_REAL_OUTPUT, INV_SUBTRACT = 1, 2
@@ -93,6 +96,11 @@ def _declare(lib):
         "s2h_wigner_reality_fill": (i32, [vp, i32, i32, vp]),
         "s2h_batch_sumsq_workspace_bytes": (ctypes.c_size_t, [i32]),
         "s2h_batch_sumsq": (i32, [vp, i64, i64, i32, vp, vp, vp]),
+        "s2h_ring_spectra_bytes": (sz, [vp, i32]),
+        "s2h_ring_analysis": (i32, [vp, vp, i32, vp, i32, vp]),
+        "s2h_forward_rings": (i32, [vp, vp, vp, i32, i32, i32, vp]),
+        "s2h_inverse_rings": (i32, [vp, vp, vp, i32, i32, vp]),
+        "s2h_ring_residual": (i32, [vp, vp, vp, vp, i32, vp]),
         "s2h_profile_enable": (i32, [i32]),
         "s2h_profile_query": (i32, [i32, ctypes.POINTER(ctypes.c_double),
                                     ctypes.POINTER(ctypes.c_int)]),

@@ -142,33 +142,18 @@ extern "C" int s2h_forward(const s2h_plan* p, const void* f, int f_is_complex, v
   return s2h_forward_ex(p, f, f_is_complex, flm, work, batch, S2H_PREC_FP64, 0, stream);
 }
 
-extern "C" int s2h_forward_ex(const s2h_plan* p, const void* f, int f_is_complex, void* flm,
-                              void* work, int batch, int precision, int flags,
-                              s2h_stream stream) {
-  if (!p || !f || !flm || batch < 0 || (batch > 0 && !work)) return S2H_ERR_INVALID;
-  if (precision < S2H_PREC_FP64 || precision > S2H_PREC_BF16X1) return S2H_ERR_INVALID;
-  if (flags & ~S2H_FWD_ACCUMULATE) return S2H_ERR_INVALID;
-  const bool accumulate = (flags & S2H_FWD_ACCUMULATE) != 0;
-  const bool mixed = precision == S2H_PREC_MIXED;
-  const bool bf16 = precision == S2H_PREC_BF16 || precision == S2H_PREC_BF16X1;
-  const int split = precision == S2H_PREC_BF16X1 ? 1 : 3;
-  if (batch == 0) return S2H_OK;
-  int st = check_device(p);
-  if (st) return st;
-  if (p->d.norm == S2H_NORM_NONE) return S2H_ERR_INVALID;
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  const int L = p->d.L;
+namespace {
+
+bool valid_precision(int precision) {
+  return precision >= S2H_PREC_FP64 && precision <= S2H_PREC_BF16X1;
+}
+
+// The ring FFTs of f (weights and ring phases fused) into `out`: the plan's
+// forward grid, or its input rows when the forward resamples.
+int ring_analysis_stage(const s2h_plan* p, const void* f, int f_is_complex, cplx* out, int batch,
+                        hipStream_t s) {
   const bool real = p->d.reality != 0;
-  cplx* ftm = static_cast<cplx*>(work);
-  const int64_t fs = flm_stride(p);
-
-  if (!accumulate && (st = hip_status(launch_zero(flm, 2 * fs * batch, s)))) return st;
   const bool resample = p->fwd_kind != 0;
-  // resampled forward: phi-spectrum of the input rows first, then the
-  // resampled MWSS-at-2L ftm after it in the workspace
-  cplx* spec = ftm;
-  if (resample) ftm = spec + (int64_t)batch * p->fwd_in_rows * p->Mc;
-
   RingFwdArgs r{};
   r.f = f;
   r.f_complex = f_is_complex;
@@ -183,48 +168,34 @@ extern "C" int s2h_forward_ex(const s2h_plan* p, const void* f, int f_is_complex
       r.real_only_all = 1;
   }
   r.map_stride = p->npix;
-  r.out = resample ? spec : ftm;
+  r.out = out;
   r.out_map_stride = (int64_t)(resample ? p->fwd_in_rows : p->fwd_rows) * p->Mc;
   r.ncol = p->Mc;
   r.m_first = p->m_lo;
   r.apply_weight = resample ? 0 : 1;
   r.apply_phase = p->d.sampling == S2H_HEALPIX;
   r.zero_neg_2L = 0;
-  r.L = L;
+  r.L = p->d.L;
   r.chirp_pool = p->chirp;
-  {
-    ProfScope ps(S2H_KERNEL_RING_FFT_FORWARD, s);
-    if ((st = hip_status(launch_ring_fft_forward(r, p->fwd_set, batch, s)))) return st;
-  }
-  const int64_t ftm_stride = (int64_t)p->fwd_rows * p->Mc;
-  if (resample) {
-    ResampleArgs ra{};
-    ra.in = spec;
-    ra.in_map_stride = r.out_map_stride;
-    ra.out = ftm;
-    ra.out_map_stride = ftm_stride;
-    ra.ncol = p->Mc;
-    ra.m_lo = p->m_lo;
-    ra.nm = p->nm;
-    ra.L = L;
-    ra.spin = p->d.spin;
-    ra.mw = p->fwd_kind == 1;
-    ra.nyq_split = real ? 1 : 0;
-    ra.weights = p->fwd_weights;
-    ra.phi_factor = ra.mw ? (2.0 * L) / (2.0 * L - 1.0) : 1.0;
-    ra.lds = p->rs_info[3];
-    for (int i = 0; i < 3; ++i) {
-      ra.conv[i] = p->rs_info[i];
-      ra.chirp_off[i] = p->rs_chirp[i];
-    }
-    ra.pool = p->chirp;
-    ProfScope ps(S2H_KERNEL_RESAMPLE, s);
-    if ((st = hip_status(launch_resample(ra, batch, s)))) return st;
-  }
+  ProfScope ps(S2H_KERNEL_RING_FFT_FORWARD, s);
+  return hip_status(launch_ring_fft_forward(r, p->fwd_set, batch, s));
+}
 
+// Latitudinal projection of forward-grid ring spectra (+ pole rows and the
+// conjugate fill): flm += or = the forward transform of ftm.  flm must be
+// zero where the kernels do not write it unless accumulating.
+int lat_forward_stage(const s2h_plan* p, const cplx* ftm, void* flm, int batch, int precision,
+                      bool accumulate, hipStream_t s) {
+  const bool mixed = precision == S2H_PREC_MIXED;
+  const bool bf16 = precision == S2H_PREC_BF16 || precision == S2H_PREC_BF16X1;
+  const int split = precision == S2H_PREC_BF16X1 ? 1 : 3;
+  const int L = p->d.L;
+  const int64_t fs = flm_stride(p);
+  const int64_t ftm_stride = (int64_t)p->fwd_rows * p->Mc;
+  int st = S2H_OK;
   LatArgs a = lat_args(p, true, batch);
   a.flm = static_cast<cplx*>(flm);
-  a.ftm = ftm;
+  a.ftm = const_cast<cplx*>(ftm);
   a.ftm_map_stride = ftm_stride;
   a.ftm_ncol = p->Mc;
   a.ftm_m_first = p->m_lo;
@@ -249,7 +220,7 @@ extern "C" int s2h_forward_ex(const s2h_plan* p, const void* f, int f_is_complex
     PoleArgs pa{};
     pa.flm = static_cast<cplx*>(flm);
     pa.flm_map_stride = fs;
-    pa.ftm = ftm;
+    pa.ftm = const_cast<cplx*>(ftm);
     pa.ftm_map_stride = ftm_stride;
     pa.ftm_ncol = p->Mc;
     pa.ftm_m_first = p->m_lo;
@@ -263,38 +234,19 @@ extern "C" int s2h_forward_ex(const s2h_plan* p, const void* f, int f_is_complex
     if ((st = hip_status(launch_pole_forward(pa, s)))) return st;
   }
 
-  if (real) {
+  if (p->d.reality) {
     if ((st = hip_status(launch_reality_fill(static_cast<cplx*>(flm), fs, L, batch, s)))) return st;
   }
   return S2H_OK;
 }
 
-extern "C" int s2h_inverse(const s2h_plan* p, const void* flm, void* f, void* work, int batch,
-                           s2h_stream stream) {
-  return s2h_inverse_ex(p, flm, f, work, batch, S2H_PREC_FP64, 0, nullptr, stream);
-}
-
-extern "C" int s2h_inverse_ex(const s2h_plan* p, const void* flm, void* f, void* work, int batch,
-                              int precision, int flags, const void* sub_from,
-                              s2h_stream stream) {
-  if (!p || !flm || !f || batch < 0 || (batch > 0 && !work)) return S2H_ERR_INVALID;
-  if (flags & ~(S2H_INV_REAL_OUTPUT | S2H_INV_SUBTRACT)) return S2H_ERR_INVALID;
-  const bool real_out = (flags & S2H_INV_REAL_OUTPUT) != 0;
-  if (real_out && !p->d.reality) return S2H_ERR_INVALID;
-  if ((flags & S2H_INV_SUBTRACT) && !sub_from) return S2H_ERR_INVALID;
-  if (precision < S2H_PREC_FP64 || precision > S2H_PREC_BF16X1) return S2H_ERR_INVALID;
+// Latitudinal synthesis (+ pole rows) into inverse-grid ring spectra ftm.
+int lat_inverse_stage(const s2h_plan* p, const void* flm, cplx* ftm, int batch, int precision,
+                      hipStream_t s) {
   const bool mixed = precision == S2H_PREC_MIXED;
   const bool bf16 = precision == S2H_PREC_BF16 || precision == S2H_PREC_BF16X1;
-  const int split = precision == S2H_PREC_BF16X1 ? 1 : 3;
-  if (batch == 0) return S2H_OK;
-  int st = check_device(p);
-  if (st) return st;
-  if (p->d.norm == S2H_NORM_NONE) return S2H_ERR_INVALID;
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  const int L = p->d.L;
-  cplx* ftm = static_cast<cplx*>(work);
   const int64_t ftm_stride = (int64_t)p->inv_rows * p->Mc;
-
+  int st = S2H_OK;
   LatArgs a = lat_args(p, false, batch);
   a.flm = const_cast<cplx*>(static_cast<const cplx*>(flm));
   a.ftm = ftm;
@@ -323,7 +275,7 @@ extern "C" int s2h_inverse_ex(const s2h_plan* p, const void* flm, void* f, void*
     pa.ftm_map_stride = ftm_stride;
     pa.ftm_ncol = p->Mc;
     pa.ftm_m_first = p->m_lo;
-    pa.L = L;
+    pa.L = p->d.L;
     pa.spin = p->d.spin;
     pa.lmin = p->d.L_lower;
     pa.row_south = p->inv_row_south;
@@ -332,6 +284,94 @@ extern "C" int s2h_inverse_ex(const s2h_plan* p, const void* flm, void* f, void*
     pa.batch = batch;
     if ((st = hip_status(launch_pole_inverse(pa, s)))) return st;
   }
+  return S2H_OK;
+}
+
+// Plans whose forward and inverse share one ring grid without pole rows or
+// resampling (HEALPix, GL): their ring spectra compose directly.
+bool spectral_ok(const s2h_plan* p) {
+  return p->fwd_kind == 0 && p->fwd_rows == p->inv_rows && p->fwd_row_south < 0 &&
+         p->fwd_row_north < 0 && p->inv_row_south < 0 && p->inv_row_north < 0 &&
+         p->d.norm != S2H_NORM_NONE;
+}
+
+}  // namespace
+
+extern "C" int s2h_forward_ex(const s2h_plan* p, const void* f, int f_is_complex, void* flm,
+                              void* work, int batch, int precision, int flags,
+                              s2h_stream stream) {
+  if (!p || !f || !flm || batch < 0 || (batch > 0 && !work)) return S2H_ERR_INVALID;
+  if (!valid_precision(precision)) return S2H_ERR_INVALID;
+  if (flags & ~S2H_FWD_ACCUMULATE) return S2H_ERR_INVALID;
+  const bool accumulate = (flags & S2H_FWD_ACCUMULATE) != 0;
+  if (batch == 0) return S2H_OK;
+  int st = check_device(p);
+  if (st) return st;
+  if (p->d.norm == S2H_NORM_NONE) return S2H_ERR_INVALID;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int L = p->d.L;
+  const bool real = p->d.reality != 0;
+  cplx* ftm = static_cast<cplx*>(work);
+  const int64_t fs = flm_stride(p);
+
+  if (!accumulate && (st = hip_status(launch_zero(flm, 2 * fs * batch, s)))) return st;
+  const bool resample = p->fwd_kind != 0;
+  // resampled forward: phi-spectrum of the input rows first, then the
+  // resampled MWSS-at-2L ftm after it in the workspace
+  cplx* spec = ftm;
+  if (resample) ftm = spec + (int64_t)batch * p->fwd_in_rows * p->Mc;
+  if ((st = ring_analysis_stage(p, f, f_is_complex, resample ? spec : ftm, batch, s))) return st;
+  const int64_t ftm_stride = (int64_t)p->fwd_rows * p->Mc;
+  if (resample) {
+    ResampleArgs ra{};
+    ra.in = spec;
+    ra.in_map_stride = (int64_t)p->fwd_in_rows * p->Mc;
+    ra.out = ftm;
+    ra.out_map_stride = ftm_stride;
+    ra.ncol = p->Mc;
+    ra.m_lo = p->m_lo;
+    ra.nm = p->nm;
+    ra.L = L;
+    ra.spin = p->d.spin;
+    ra.mw = p->fwd_kind == 1;
+    ra.nyq_split = real ? 1 : 0;
+    ra.weights = p->fwd_weights;
+    ra.phi_factor = ra.mw ? (2.0 * L) / (2.0 * L - 1.0) : 1.0;
+    ra.lds = p->rs_info[3];
+    for (int i = 0; i < 3; ++i) {
+      ra.conv[i] = p->rs_info[i];
+      ra.chirp_off[i] = p->rs_chirp[i];
+    }
+    ra.pool = p->chirp;
+    ProfScope ps(S2H_KERNEL_RESAMPLE, s);
+    if ((st = hip_status(launch_resample(ra, batch, s)))) return st;
+  }
+  return lat_forward_stage(p, ftm, flm, batch, precision, accumulate, s);
+}
+
+extern "C" int s2h_inverse(const s2h_plan* p, const void* flm, void* f, void* work, int batch,
+                           s2h_stream stream) {
+  return s2h_inverse_ex(p, flm, f, work, batch, S2H_PREC_FP64, 0, nullptr, stream);
+}
+
+extern "C" int s2h_inverse_ex(const s2h_plan* p, const void* flm, void* f, void* work, int batch,
+                              int precision, int flags, const void* sub_from,
+                              s2h_stream stream) {
+  if (!p || !flm || !f || batch < 0 || (batch > 0 && !work)) return S2H_ERR_INVALID;
+  if (flags & ~(S2H_INV_REAL_OUTPUT | S2H_INV_SUBTRACT)) return S2H_ERR_INVALID;
+  const bool real_out = (flags & S2H_INV_REAL_OUTPUT) != 0;
+  if (real_out && !p->d.reality) return S2H_ERR_INVALID;
+  if ((flags & S2H_INV_SUBTRACT) && !sub_from) return S2H_ERR_INVALID;
+  if (!valid_precision(precision)) return S2H_ERR_INVALID;
+  if (batch == 0) return S2H_OK;
+  int st = check_device(p);
+  if (st) return st;
+  if (p->d.norm == S2H_NORM_NONE) return S2H_ERR_INVALID;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int L = p->d.L;
+  cplx* ftm = static_cast<cplx*>(work);
+  const int64_t ftm_stride = (int64_t)p->inv_rows * p->Mc;
+  if ((st = lat_inverse_stage(p, flm, ftm, batch, precision, s))) return st;
 
   RingInvArgs r{};
   r.in = ftm;
@@ -351,6 +391,72 @@ extern "C" int s2h_inverse_ex(const s2h_plan* p, const void* flm, void* f, void*
   r.sub_from = (flags & S2H_INV_SUBTRACT) ? sub_from : nullptr;
   ProfScope ps(S2H_KERNEL_RING_FFT_INVERSE, s);
   return hip_status(launch_ring_fft_inverse(r, p->inv_set, batch, s));
+}
+
+// ---------------------------------------------------------------- ring spectra
+// The transforms split at the ring spectra, for refinement in spectral space:
+// F(f - I(x)) = Lat_fwd(G - A(Lat_inv(x))) with G the ring spectra of f and A
+// the ring FFT of the ring iFFT, applied without either FFT.
+extern "C" size_t s2h_ring_spectra_bytes(const s2h_plan* p, int batch) {
+  if (!p || batch < 0 || !spectral_ok(p)) return 0;
+  return (size_t)batch * (size_t)p->fwd_rows * (size_t)p->Mc * sizeof(cplx);
+}
+
+extern "C" int s2h_ring_analysis(const s2h_plan* p, const void* f, int f_is_complex, void* G,
+                                 int batch, s2h_stream stream) {
+  if (!p || !f || !G || batch < 0) return S2H_ERR_INVALID;
+  if (!spectral_ok(p)) return S2H_ERR_UNSUPPORTED;
+  if (batch == 0) return S2H_OK;
+  int st = check_device(p);
+  if (st) return st;
+  return ring_analysis_stage(p, f, f_is_complex, static_cast<cplx*>(G), batch,
+                             static_cast<hipStream_t>(stream));
+}
+
+extern "C" int s2h_forward_rings(const s2h_plan* p, const void* G, void* flm, int batch,
+                                 int precision, int flags, s2h_stream stream) {
+  if (!p || !G || !flm || batch < 0) return S2H_ERR_INVALID;
+  if (!valid_precision(precision) || (flags & ~S2H_FWD_ACCUMULATE)) return S2H_ERR_INVALID;
+  if (!spectral_ok(p)) return S2H_ERR_UNSUPPORTED;
+  if (batch == 0) return S2H_OK;
+  int st = check_device(p);
+  if (st) return st;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const bool accumulate = (flags & S2H_FWD_ACCUMULATE) != 0;
+  if (!accumulate && (st = hip_status(launch_zero(flm, 2 * flm_stride(p) * batch, s)))) return st;
+  return lat_forward_stage(p, static_cast<const cplx*>(G), flm, batch, precision, accumulate, s);
+}
+
+extern "C" int s2h_inverse_rings(const s2h_plan* p, const void* flm, void* Y, int batch,
+                                 int precision, s2h_stream stream) {
+  if (!p || !flm || !Y || batch < 0 || !valid_precision(precision)) return S2H_ERR_INVALID;
+  if (!spectral_ok(p)) return S2H_ERR_UNSUPPORTED;
+  if (batch == 0) return S2H_OK;
+  int st = check_device(p);
+  if (st) return st;
+  return lat_inverse_stage(p, flm, static_cast<cplx*>(Y), batch, precision,
+                           static_cast<hipStream_t>(stream));
+}
+
+extern "C" int s2h_ring_residual(const s2h_plan* p, const void* G, const void* Y, void* R,
+                                 int batch, s2h_stream stream) {
+  if (!p || !G || !Y || !R || batch < 0 || Y == R) return S2H_ERR_INVALID;
+  if (!spectral_ok(p)) return S2H_ERR_UNSUPPORTED;
+  if (batch == 0) return S2H_OK;
+  int st = check_device(p);
+  if (st) return st;
+  RingAliasArgs a{};
+  a.y = static_cast<const cplx*>(Y);
+  a.g = static_cast<const cplx*>(G);
+  a.r = static_cast<cplx*>(R);
+  a.map_stride = (int64_t)p->fwd_rows * p->Mc;
+  a.ncol = p->Mc;
+  a.m_first = p->m_lo;
+  a.herm = p->d.reality ? 1 : 0;
+  a.apply_phase = p->d.sampling == S2H_HEALPIX;
+  a.L = p->d.L;
+  ProfScope ps(S2H_KERNEL_RING_RESIDUAL, static_cast<hipStream_t>(stream));
+  return hip_status(launch_ring_alias(a, p->fwd_set, batch, static_cast<hipStream_t>(stream)));
 }
 
 // Adjoint of s2h_forward (the VJP of a complex-linear map, torch convention):

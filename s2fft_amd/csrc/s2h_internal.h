@@ -31,12 +31,19 @@ struct RingJob {
 // Ring jobs of one transform on the device, in three consecutive classes:
 // n_pow2 power-of-two lengths, n_smooth other lengths with prime factors
 // <= 13, n_blue Bluestein rings; each class sorted by decreasing length.
+// Bluestein rings whose convolution is longer than this launch separately:
+// the shorter ones then fit two workgroups' LDS per CU.
+constexpr int kBlueSplit = 4096;
+
 struct JobSet {
   RingJob* jobs = nullptr;
   int n_pow2 = 0, n_smooth = 0, n_blue = 0;
+  int n_blue_long = 0;  // Bluestein rings with convolution > kBlueSplit (listed first)
   int max_pow2 = 0;     // longest power-of-two ring
   int max_smooth = 0;   // longest other mixed-radix ring
   int max_conv = 0;     // longest Bluestein convolution
+  int max_conv_short = 0;  // longest Bluestein convolution <= kBlueSplit
+  int max_nphi = 0;     // longest ring
 };
 
 // Start of the ell-recursion for one signed m.

@@ -43,6 +43,23 @@ struct RingInvArgs {
   const void* sub_from;     // non-null: f = sub_from - (transform), same layout as f
 };
 
+// R = G - A(Y) on ring spectra, A = (ring FFT) . (ring iFFT): per ring, the
+// fold of Y's m-spectrum onto n_t bins with e^{+i m phi0} (as the inverse
+// FFT reads it), then the periodic extension times n_t w_t e^{-i m phi0} (as
+// the forward FFT writes it).  Same layouts as RingInvArgs.in / RingFwdArgs.out.
+struct RingAliasArgs {
+  const RingJob* jobs;
+  const cplx* y;            // spectra to alias (m in [-(L-1), L-1], Hermitian if herm)
+  const cplx* g;            // subtrahend G (may equal r)
+  cplx* r;
+  int64_t map_stride;       // rows * ncol
+  int ncol;                 // columns of every spectrum, m = m_first + col
+  int m_first;
+  int herm;                 // y stores m >= 0 of Hermitian spectra
+  int apply_phase;
+  int L;
+};
+
 struct LatArgs {
   const LatRing* rings;
   int nrings;               // entries of rings (pairs count as one)
@@ -70,6 +87,7 @@ hipError_t launch_ring_fft_forward(const RingFwdArgs& a, const JobSet& js, int b
 hipError_t launch_ring_fft_inverse(const RingInvArgs& a, const JobSet& js, int batch,
                                    hipStream_t s);
 hipError_t allow_large_lds_rings();
+hipError_t launch_ring_alias(const RingAliasArgs& a, const JobSet& js, int batch, hipStream_t s);
 hipError_t launch_chirp_setup(const int* lengths, const int* conv_lens, const int64_t* offs,
                               cplx* pool, int count, int max_conv, hipStream_t s);
 hipError_t launch_gamma_dft(int N, int64_t npix, int mode, double scale, const void* in,

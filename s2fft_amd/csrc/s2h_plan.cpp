@@ -339,6 +339,7 @@ int make_set(s2h_plan* p, std::vector<RingJob> jobs, JobSet* out) {
   });
   JobSet js;
   for (auto& j : jobs) {
+    js.max_nphi = std::max(js.max_nphi, j.nphi);
     switch (ring_class(j)) {
       case 0:
         ++js.n_pow2;
@@ -351,6 +352,10 @@ int make_set(s2h_plan* p, std::vector<RingJob> jobs, JobSet* out) {
       default:
         ++js.n_blue;
         js.max_conv = std::max(js.max_conv, j.conv_len);
+        if (j.conv_len > kBlueSplit)
+          ++js.n_blue_long;
+        else
+          js.max_conv_short = std::max(js.max_conv_short, j.conv_len);
     }
   }
   const int st = upload<RingJob>(p, jobs, &js.jobs);

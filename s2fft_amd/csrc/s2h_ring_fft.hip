@@ -604,12 +604,130 @@ __global__ void __launch_bounds__(TPB) ring_inv_kernel(RingInvArgs a, int job0, 
   }
 }
 
+// ------------------------------------------------------------------ aliasing
+// R = G - A(Y) for one (ring, map), A = (ring FFT) . (ring iFFT) with the
+// FFTs cancelled: with m = q + j_m n (q = m mod n) and w = e^{i n phi0},
+//   A(Y)[m] = w_t n e^{-i m phi0} sum_{m' = q mod n} e^{i m' phi0} c(m')
+//           = w_t n w^{-j_m} T[q],   T[q] = sum_j w^j c(q + j n),
+// m' = q + j n over [-(L-1), L-1] (c(m') = conj c(-m') for Hermitian spectra).
+// On the supported grids n phi0 / pi is an integer, so w = +-1 (sgn).
+// Rings with n >= 2L - 1 have one m' per q: R = G - w_t n Y elementwise.
+// Otherwise thread t owns the residue q = t (mod n): with n >= TPB it sums
+// T[q] itself; with n < TPB the G = TPB / n threads of a residue split the j
+// (j = g, g + G, ... from the first) and meet in LDS in order of g.  Each
+// thread then writes the outputs of its residue (the same split over j).
+__device__ __forceinline__ cplx alias_term(const cplx* y, int m, int m_first, bool herm) {
+  if (herm && m < 0) {
+    const cplx t = y[-m - m_first];
+    return {t.x, -t.y};
+  }
+  return y[m - m_first];
+}
+
+// floor(a / n) for n > 0
+__device__ __forceinline__ int floor_div(int a, int n) { return a >= 0 ? a / n : -((n - 1 - a) / n); }
+
+template <int TPB>
+__global__ void __launch_bounds__(TPB) ring_alias_kernel(RingAliasArgs a) {
+  __shared__ double part[2 * TPB];
+  const RingJob job = a.jobs[blockIdx.x];
+  const int n = job.nphi;
+  const int64_t off = (int64_t)blockIdx.y * a.map_stride + (int64_t)job.row * a.ncol;
+  const cplx* __restrict__ y = a.y + off;
+  const cplx* g = a.g + off;
+  cplx* r = a.r + off;
+  const double wn = job.weight * n;
+  const int lo = -(a.L - 1), hi = a.L - 1;
+  const bool herm = a.herm != 0;
+  const int t = threadIdx.x;
+  if (n > hi - lo) {
+    // no aliasing: the column's own m is the only term
+    batched_for<TPB, 8, cplx>(
+        a.ncol, [&](int col) { return g[col]; },
+        [&](int col, cplx gv) {
+          const int m = a.m_first + col;
+          cplx v = {0.0, 0.0};
+          if (m >= lo && m <= hi) v = alias_term(y, m, a.m_first, herm);
+          r[col] = {gv.x - wn * v.x, gv.y - wn * v.y};
+        });
+    return;
+  }
+  const int e = (int)rint(n * job.phi0_over_pi);
+  const double sgn = (a.apply_phase && (e & 1)) ? -1.0 : 1.0;
+  const int G = n >= TPB ? 1 : TPB / n;
+  const int nq = n >= TPB ? (n + TPB - 1) / TPB : 1;  // residues per thread
+  const int gi = n >= TPB ? 0 : t / n;
+  const bool active = gi < G;
+  const int m_last = a.m_first + a.ncol - 1;
+  for (int qi = 0; qi < nq; ++qi) {
+    const int q = n >= TPB ? t + qi * TPB : t % n;
+    const bool own = active && q < n;
+    // T[q]: j from j0 = ceil((lo - q) / n) to floor((hi - q) / n), this
+    // thread's j = j0 + gi + k G
+    cplx acc = {0.0, 0.0};
+    int j0 = 0;
+    if (own) {
+      j0 = -floor_div(q - lo, n);
+      const int j1 = floor_div(hi - q, n);
+      const int cnt = j1 - j0 + 1;
+      for (int k0 = gi; k0 < cnt; k0 += 4 * G) {
+        cplx v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int k = k0 + u * G;
+          v[u] = k < cnt ? alias_term(y, q + (j0 + k) * n, a.m_first, herm) : cplx{0.0, 0.0};
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int k = k0 + u * G;
+          const double sj = ((j0 + k) & 1) && sgn < 0 ? -1.0 : 1.0;
+          acc = {fma(sj, v[u].x, acc.x), fma(sj, v[u].y, acc.y)};
+        }
+      }
+    }
+    if (G > 1) {
+      if (own) {
+        part[2 * t] = acc.x;
+        part[2 * t + 1] = acc.y;
+      }
+      __syncthreads();
+      cplx tot = {0.0, 0.0};
+      if (own)
+        for (int gg = 0; gg < G; ++gg)
+          tot = cadd(tot, cplx{part[2 * (gg * n + q)], part[2 * (gg * n + q) + 1]});
+      __syncthreads();
+      acc = tot;
+    }
+    if (!own) continue;
+    // outputs m = q + j n in [m_first, m_last], this thread's j = jo + gi + k G
+    const int jo = -floor_div(q - a.m_first, n);
+    const int cnt = floor_div(m_last - q, n) - jo + 1;
+    for (int k0 = gi; k0 < cnt; k0 += 4 * G) {
+      cplx gv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int k = k0 + u * G;
+        gv[u] = k < cnt ? g[q + (jo + k) * n - a.m_first] : cplx{0.0, 0.0};
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int k = k0 + u * G;
+        if (k < cnt) {
+          const double c = (((jo + k) & 1) && sgn < 0 ? -wn : wn);
+          r[q + (jo + k) * n - a.m_first] = {gv[u].x - c * acc.x, gv[u].y - c * acc.y};
+        }
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------------ launchers
 namespace {
 
 constexpr int kTpbPow2 = 512;
 constexpr int kTpbSmooth = 256;
 constexpr int kTpbBlue = 512;
+constexpr int kTpbAlias = 256;
 constexpr int kTabBytes = kTabCplx * sizeof(cplx);
 
 // tables + data region (2 lx_len(N) doubles, at least 2 TPB for the fold's partials)
@@ -639,10 +757,16 @@ hipError_t launch_ring_fft_forward(const RingFwdArgs& a0, const JobSet& js, int 
     hipLaunchKernelGGL((ring_fwd_kernel<kTpbSmooth, kMaxRingLen, kSmooth>),
                        dim3(js.n_smooth, gy), dim3(kTpbSmooth), ring_lds_bytes(js.max_smooth, kTpbSmooth), s,
                        a, js.n_pow2, paired);
-  if (js.n_blue)
-    hipLaunchKernelGGL((ring_fwd_kernel<kTpbBlue, kMaxConvLen, kBlue>), dim3(js.n_blue, gy),
-                       dim3(kTpbBlue), ring_lds_bytes(js.max_conv, kTpbBlue), s, a,
-                       js.n_pow2 + js.n_smooth, paired);
+  // Bluestein: the long convolutions (one workgroup per CU), then the short
+  // ones with their own, smaller LDS size
+  const int b0 = js.n_pow2 + js.n_smooth, nshort = js.n_blue - js.n_blue_long;
+  if (js.n_blue_long)
+    hipLaunchKernelGGL((ring_fwd_kernel<kTpbBlue, kMaxConvLen, kBlue>), dim3(js.n_blue_long, gy),
+                       dim3(kTpbBlue), ring_lds_bytes(js.max_conv, kTpbBlue), s, a, b0, paired);
+  if (nshort)
+    hipLaunchKernelGGL((ring_fwd_kernel<kTpbBlue, kMaxConvLen, kBlue>), dim3(nshort, gy),
+                       dim3(kTpbBlue), ring_lds_bytes(js.max_conv_short, kTpbBlue), s, a,
+                       b0 + js.n_blue_long, paired);
   return hipGetLastError();
 }
 
@@ -661,10 +785,23 @@ hipError_t launch_ring_fft_inverse(const RingInvArgs& a0, const JobSet& js, int 
     hipLaunchKernelGGL((ring_inv_kernel<kTpbSmooth, kMaxRingLen, kSmooth>),
                        dim3(js.n_smooth, gy), dim3(kTpbSmooth), ring_lds_bytes(js.max_smooth, kTpbSmooth), s,
                        a, js.n_pow2, paired);
-  if (js.n_blue)
-    hipLaunchKernelGGL((ring_inv_kernel<kTpbBlue, kMaxConvLen, kBlue>), dim3(js.n_blue, gy),
-                       dim3(kTpbBlue), ring_lds_bytes(js.max_conv, kTpbBlue), s, a,
-                       js.n_pow2 + js.n_smooth, paired);
+  const int b0 = js.n_pow2 + js.n_smooth, nshort = js.n_blue - js.n_blue_long;
+  if (js.n_blue_long)
+    hipLaunchKernelGGL((ring_inv_kernel<kTpbBlue, kMaxConvLen, kBlue>), dim3(js.n_blue_long, gy),
+                       dim3(kTpbBlue), ring_lds_bytes(js.max_conv, kTpbBlue), s, a, b0, paired);
+  if (nshort)
+    hipLaunchKernelGGL((ring_inv_kernel<kTpbBlue, kMaxConvLen, kBlue>), dim3(nshort, gy),
+                       dim3(kTpbBlue), ring_lds_bytes(js.max_conv_short, kTpbBlue), s, a,
+                       b0 + js.n_blue_long, paired);
+  return hipGetLastError();
+}
+
+hipError_t launch_ring_alias(const RingAliasArgs& a, const JobSet& js, int batch, hipStream_t s) {
+  const int njobs = js.n_pow2 + js.n_smooth + js.n_blue;
+  if (batch == 0 || njobs == 0) return hipSuccess;
+  RingAliasArgs b = a;
+  b.jobs = js.jobs;
+  hipLaunchKernelGGL(ring_alias_kernel<kTpbAlias>, dim3(njobs, batch), dim3(kTpbAlias), 0, s, b);
   return hipGetLastError();
 }
 

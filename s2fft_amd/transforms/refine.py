@@ -17,6 +17,12 @@ the incremental residual carries an error below eps sum_k |I delta_k|, which
 is about eps rho / (1 - rho) of |f| because the first residual is exact.
 P is the identity (Jacobi) or a learned per-(l, m) preconditioner that is
 accepted only where the FP64-tracked residual decreases.
+
+With ``spectral`` (plans with one ring grid: HEALPix) the residual lives on
+the ring spectra instead of the pixels: r = G - A(Lat_inv(flm)) with
+G the ring FFT of f (computed once) and A = (ring FFT) . (ring iFFT) applied
+without either FFT (s2h_ring_residual), so F~(r) = Lat_fwd~(r) and the loop
+runs no ring FFT at all.  The iterates are the same up to rounding.
 """
 
 import torch
@@ -25,14 +31,17 @@ from .. import _lib
 from . import spherical as sph
 
 
-def refine_fp64(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=None):
-    """The reference's Jacobi refinement on the FP64 kernels (n_iter steps)."""
-    return sph.forward_iterative_hip(f, L, spin, nside, sampling, reality, L_lower, n_iter)
+def refine_fp64(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=None,
+                spectral=False):
+    """The reference's Jacobi refinement on the FP64 kernels (n_iter steps);
+    ``spectral``: residuals on the ring spectra (no ring FFT per step)."""
+    return sph.forward_iterative_hip(f, L, spin, nside, sampling, reality, L_lower, n_iter,
+                                     spectral=spectral)
 
 
 def refine_mixed(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=None,
                  precond=None, refresh_every=0, history=None, flm_true=None, stats=None,
-                 precision="mixed", inv_precision=None, learned_steps=None):
+                 precision="mixed", inv_precision=None, learned_steps=None, spectral=False):
     """Mixed-precision refinement with an FP64 residual (see module docstring).
 
     precond: None (Jacobi) or an object with ``apply(h, L, reality, lmin)``
@@ -46,6 +55,8 @@ def refine_mixed(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=N
     ``flm_true`` is given.  stats: optional dict counting accepted learned steps.
     precision / inv_precision: arithmetic of F~ / I~ ("mixed" FP32 or "bf16"
     split-BF16 matrix cores; inv_precision defaults to precision).
+    spectral: residuals on the ring spectra (see the module docstring); needs
+    a plan with one ring grid.
     """
     s = sampling.lower()
     plan = plan or sph.get_plan(s, L, nside, spin, L_lower, reality, device=f.device)
@@ -55,21 +66,33 @@ def refine_mixed(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=N
     if learned_steps is None:
         learned_steps = max(n_iter - 1, 0)
     real_out = bool(reality) and not f.is_complex()
+    if spectral:
+        # f is replaced by its ring spectra G: every residual is G - A(I~(x))
+        f = sph.ring_analysis_hip(f, plan)
 
-    def synth(x, prec="fp64", sub=None):
-        return sph.inverse_hip(x, precision=prec, real_output=real_out, sub_from=sub, **kw)
+        def synth(x, prec="fp64", sub=None):
+            return sph.ring_residual_hip(sub, sph.inverse_rings_hip(x, plan, prec), plan)
 
-    flm = sph.forward_hip(f, precision=fp, **kw)
+        def proj(x, prec, acc=None):
+            return sph.forward_rings_hip(x, L, plan, prec, accumulate_into=acc)
+    else:
+        def synth(x, prec="fp64", sub=None):
+            return sph.inverse_hip(x, precision=prec, real_output=real_out, sub_from=sub, **kw)
+
+        def proj(x, prec, acc=None):
+            return sph.forward_hip(x, precision=prec, accumulate_into=acc, **kw)
+
+    flm = proj(f, fp)
     r = synth(flm, sub=f)
     rn = None  # per-map squared norm of r, when known
     _track(history, flm, flm_true)
     for k in range(n_iter):
         last = k + 1 == n_iter
         if last:
-            sph.forward_hip(r, precision=fp, accumulate_into=flm, **kw)
+            proj(r, fp, acc=flm)
             _track(history, flm, flm_true)
             break
-        h = sph.forward_hip(r, precision=fp, **kw)
+        h = proj(r, fp)
         refresh = bool(refresh_every and (k + 1) % refresh_every == 0)
         if precond is not None and k < learned_steps:
             delta, r, rn = _judged_step(precond, h, r, rn, f, flm, kw, stats, synth, ip, refresh)

@@ -249,19 +249,96 @@ def inverse_diff(flm, L, spin=0, nside=None, sampling="mw", reality=False, L_low
     return inverse_hip(flm, L, spin, nside, sampling, reality, L_lower)
 
 
+# ------------------------------------------------------------------ ring spectra
+# The transforms split at the ring spectra (s2h_ring_* in the C ABI): with
+# G = ring_analysis_hip(f), forward = forward_rings_hip(G) and
+# F(f - I(x)) = forward_rings_hip(ring_residual_hip(G, inverse_rings_hip(x))),
+# where the residual applies (ring FFT . ring iFFT) without either FFT.
+def spectra_supported(plan):
+    """Whether the plan's forward and inverse share one ring grid (HEALPix, GL)."""
+    return int(_lib.lib().s2h_ring_spectra_bytes(plan.handle, 1)) > 0
+
+
+def _new_spectra(plan, B, device):
+    nbytes = int(_lib.lib().s2h_ring_spectra_bytes(plan.handle, B))
+    if nbytes == 0 and B > 0:
+        raise ValueError("ring spectra need a plan whose forward and inverse share one ring grid")
+    return torch.empty((B, nbytes // (16 * max(B, 1))), dtype=torch.complex128, device=device)
+
+
+def ring_analysis_hip(f, plan):
+    """Ring spectra G (B, rows * Mc) complex128 of the maps f (B, ...)."""
+    f = f.contiguous()
+    G = _new_spectra(plan, f.shape[0], f.device)
+    _lib.check(_lib.lib().s2h_ring_analysis(plan.handle, f.data_ptr(), int(f.is_complex()),
+                                            G.data_ptr(), f.shape[0], _stream(f.device)),
+               "ring analysis")
+    return G
+
+
+def forward_rings_hip(G, L, plan, precision="fp64", accumulate_into=None):
+    """flm (B, L, 2L-1) from ring spectra; ``accumulate_into``: flm += in place."""
+    B = G.shape[0]
+    flags = 0
+    if accumulate_into is not None:
+        flm = accumulate_into
+        if (flm.dtype != torch.complex128 or not flm.is_contiguous()
+                or tuple(flm.shape) != (B, L, 2 * L - 1)):
+            raise ValueError("accumulate_into must be a contiguous (B, L, 2L-1) complex128 tensor")
+        flags = _lib.FWD_ACCUMULATE
+    else:
+        flm = torch.empty((B, L, 2 * L - 1), dtype=torch.complex128, device=G.device)
+    _lib.check(_lib.lib().s2h_forward_rings(plan.handle, G.contiguous().data_ptr(),
+                                            flm.data_ptr(), B, _precision(precision), flags,
+                                            _stream(G.device)), "forward rings")
+    return flm
+
+
+def inverse_rings_hip(flm, plan, precision="fp64"):
+    """Ring spectra (B, rows * Mc) of the latitudinal synthesis of flm."""
+    flm = flm.contiguous()
+    Y = _new_spectra(plan, flm.shape[0], flm.device)
+    _lib.check(_lib.lib().s2h_inverse_rings(plan.handle, flm.data_ptr(), Y.data_ptr(),
+                                            flm.shape[0], _precision(precision),
+                                            _stream(flm.device)), "inverse rings")
+    return Y
+
+
+def ring_residual_hip(G, Y, plan, out=None):
+    """R = G - A(Y) (out may be G)."""
+    R = torch.empty_like(G) if out is None else out
+    _lib.check(_lib.lib().s2h_ring_residual(plan.handle, G.contiguous().data_ptr(),
+                                            Y.contiguous().data_ptr(), R.data_ptr(), G.shape[0],
+                                            _stream(G.device)), "ring residual")
+    return R
+
+
 def forward_iterative_hip(f, L, spin=0, nside=None, sampling="mw", reality=False, L_lower=0,
-                          n_iter=0):
+                          n_iter=0, spectral=True):
     """flm = F(f); repeat n_iter: flm += F(f - I(flm)).
 
     s2fft/utils/iterative_refinement.py:9-46 as used by spherical.py:453-462;
-    the residual and the update stay in FP64 on the device.  Without autograd
-    the residual comes out of the synthesis pass (f - I(flm)) and the update
-    is accumulated by the projection pass (flm += F(r)).
+    the residual and the update stay in FP64 on the device.  Without autograd,
+    on a plan with one ring grid (HEALPix) and ``spectral``, the residual is
+    formed on the ring spectra, F(f - I(flm)) = Lat_fwd(G - A(Lat_inv(flm))),
+    so each iteration runs no ring FFT; otherwise the residual comes out of the
+    synthesis pass (f - I(flm)) and the update is accumulated by the
+    projection pass (flm += F(r)).
     """
-    flm = forward_diff(f, L, spin, nside, sampling, reality, L_lower)
     if n_iter and not (torch.is_grad_enabled() and f.requires_grad):
         s = sampling.lower()
         plan = get_plan(s, L, nside, spin, L_lower, reality, device=f.device)
+        f = f.contiguous()
+        if not f.is_complex():
+            f = f.to(torch.float64)
+        if spectral and spectra_supported(plan):
+            G = ring_analysis_hip(f, plan)
+            flm = forward_rings_hip(G, L, plan)
+            for _ in range(n_iter):
+                R = ring_residual_hip(G, inverse_rings_hip(flm, plan), plan)
+                forward_rings_hip(R, L, plan, accumulate_into=flm)
+            return flm
+        flm = forward_hip(f, L, spin, nside, s, reality, L_lower, plan=plan)
         real_out = bool(reality) and not f.is_complex()
         for _ in range(n_iter):
             r = inverse_hip(flm, L, spin, nside, s, reality, L_lower, plan=plan,
@@ -269,6 +346,7 @@ def forward_iterative_hip(f, L, spin=0, nside=None, sampling="mw", reality=False
             forward_hip(r, L, spin, nside, s, reality, L_lower, plan=plan,
                         accumulate_into=flm)
         return flm
+    flm = forward_diff(f, L, spin, nside, sampling, reality, L_lower)
     for _ in range(n_iter):
         f_rec = inverse_diff(flm, L, spin, nside, sampling, reality, L_lower)
         flm = flm + forward_diff(f - f_rec, L, spin, nside, sampling, reality, L_lower)
