@@ -57,6 +57,8 @@ def parse():
     ap.add_argument("--mixed-precision", default="bf16x1", choices=["bf16x1", "bf16", "mixed"])
     ap.add_argument("--mixed-inv-precision", default="bf16", choices=["bf16", "mixed"])
     ap.add_argument("--refresh-every", type=int, default=3)
+    ap.add_argument("--init-precision", default="fp64", choices=["fp64", "bf16", "mixed"],
+                    help="mixed leg: arithmetic of the synthesis forming the first residual")
     ap.add_argument("--learned-steps", type=int, default=2)
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -251,7 +253,7 @@ def main():
                         precond=precond, refresh_every=args.refresh_every, history=hist,
                         flm_true=flm_true, stats=stats, precision=prec,
                         inv_precision=args.mixed_inv_precision, learned_steps=args.learned_steps,
-                        spectral=spectral)
+                        spectral=spectral, init_precision=args.init_precision)
     ok = [i for i, e in enumerate(hist) if e <= args.tol]
     n_mixed = ok[0] if ok else args.max_iter
     log(f"[bench]   mixed errors {['%.1e' % e for e in hist[:n_mixed + 2]]} -> {n_mixed} steps")
@@ -260,7 +262,7 @@ def main():
         f, L, 0, nside, "healpix", True, n_mixed, plan=plan, precond=precond,
         refresh_every=args.refresh_every, precision=prec,
         inv_precision=args.mixed_inv_precision, learned_steps=args.learned_steps,
-        spectral=spectral), "mixed")
+        spectral=spectral, init_precision=args.init_precision), "mixed")
     mixed = {"elapsed": el, "err": err, "kernels": kt, "iters": n_mixed,
              "roofline": roofline(kt, nside, L, B)}
 
@@ -285,6 +287,7 @@ def main():
                          f"refined to <= {args.tol:g} max-rel error vs the seeded flm"),
             "method": (f"mixed: FP64 recursion, {prec} matrix-core projection, "
                        f"{args.mixed_inv_precision} matrix-core synthesis, "
+                       f"{args.init_precision} first residual, "
                        f"FP64 residual refreshed every {args.refresh_every} steps, "
                        + ("residuals on the ring spectra (no ring FFT in the loop), "
                           if spectral else "residuals on the pixels, ")

@@ -182,8 +182,9 @@ int ring_analysis_stage(const s2h_plan* p, const void* f, int f_is_complex, cplx
 }
 
 // Latitudinal projection of forward-grid ring spectra (+ pole rows and the
-// conjugate fill): flm += or = the forward transform of ftm.  flm must be
-// zero where the kernels do not write it unless accumulating.
+// conjugate fill): flm = or += the forward transform of ftm.  The BF16
+// kernels with one ring group per m write every entry themselves (zeros and
+// the m < 0 mirror); otherwise flm is zeroed first and filled after.
 int lat_forward_stage(const s2h_plan* p, const cplx* ftm, void* flm, int batch, int precision,
                       bool accumulate, hipStream_t s) {
   const bool mixed = precision == S2H_PREC_MIXED;
@@ -201,6 +202,11 @@ int lat_forward_stage(const s2h_plan* p, const cplx* ftm, void* flm, int batch, 
   a.ftm_m_first = p->m_lo;
   a.lmin_out = std::max(std::abs(p->d.spin), p->d.L_lower);
   a.accumulate = accumulate ? 1 : 0;
+  const bool poles = p->fwd_row_south >= 0 || p->fwd_row_north >= 0;
+  const bool fill = bf16 && !poles && lat_forward_bf16_groups(a, split) == 1;
+  a.fill = fill ? 1 : 0;
+  a.mirror = fill && p->d.reality ? 1 : 0;
+  if (!accumulate && !fill && (st = hip_status(launch_zero(flm, 2 * fs * batch, s)))) return st;
   {
     ProfScope ps(bf16 ? S2H_KERNEL_LAT_FORWARD_BF16
                       : mixed ? S2H_KERNEL_LAT_FORWARD_FP32 : S2H_KERNEL_LAT_FORWARD,
@@ -234,7 +240,7 @@ int lat_forward_stage(const s2h_plan* p, const cplx* ftm, void* flm, int batch, 
     if ((st = hip_status(launch_pole_forward(pa, s)))) return st;
   }
 
-  if (p->d.reality) {
+  if (p->d.reality && !a.mirror) {
     if ((st = hip_status(launch_reality_fill(static_cast<cplx*>(flm), fs, L, batch, s)))) return st;
   }
   return S2H_OK;
@@ -314,7 +320,6 @@ extern "C" int s2h_forward_ex(const s2h_plan* p, const void* f, int f_is_complex
   cplx* ftm = static_cast<cplx*>(work);
   const int64_t fs = flm_stride(p);
 
-  if (!accumulate && (st = hip_status(launch_zero(flm, 2 * fs * batch, s)))) return st;
   const bool resample = p->fwd_kind != 0;
   // resampled forward: phi-spectrum of the input rows first, then the
   // resampled MWSS-at-2L ftm after it in the workspace
@@ -423,7 +428,6 @@ extern "C" int s2h_forward_rings(const s2h_plan* p, const void* G, void* flm, in
   if (st) return st;
   hipStream_t s = static_cast<hipStream_t>(stream);
   const bool accumulate = (flags & S2H_FWD_ACCUMULATE) != 0;
-  if (!accumulate && (st = hip_status(launch_zero(flm, 2 * flm_stride(p) * batch, s)))) return st;
   return lat_forward_stage(p, static_cast<const cplx*>(G), flm, batch, precision, accumulate, s);
 }
 

@@ -80,6 +80,10 @@ struct LatArgs {
   int ftm_m_first;
   int groups;               // ring groups per m (forward); >1 accumulates atomically
   int accumulate;           // forward: add into flm instead of overwriting it
+  int fill;                 // forward, one ring group per m: also write zeros below
+                            // max(ell0, lmin_out), so flm needs no zeroing pass
+  int mirror;               // forward with fill, real maps: also write
+                            // flm[l, -m] = (-1)^m conj(flm[l, m]) (no conjugate fill pass)
 };
 
 hipError_t launch_ring_fft_forward(const RingFwdArgs& a, const JobSet& js, int batch,
@@ -101,6 +105,8 @@ hipError_t launch_lat_forward_mfma(const LatArgs& a, bool paired, bool fp32, hip
 hipError_t launch_mfma_f64_probe(const double* a, const double* b, double* out, hipStream_t s);
 hipError_t launch_lat_inverse_mma(const LatArgs& a, bool paired, bool fp32, hipStream_t s);
 hipError_t launch_lat_forward_bf16(const LatArgs& a, bool paired, int split, hipStream_t s);
+// ring groups per m of launch_lat_forward_bf16 (fill and mirror need 1)
+int lat_forward_bf16_groups(const LatArgs& a, int split);
 hipError_t launch_flm_to_mmajor(const cplx* flm, cplx* flmT, int L, int m_lo, int nm, int batch,
                                 hipStream_t s);
 hipError_t launch_mmajor_to_flm(const cplx* flmT, cplx* flm, int L, int m_lo, int nm, int batch,

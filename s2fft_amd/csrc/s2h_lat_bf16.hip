@@ -159,6 +159,18 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
   }
 
   cplx* out = a.flm + (int64_t)b0 * a.flm_map_stride + (L - 1 + m);
+  const bool mirror = a.mirror && m > 0;
+  if (a.fill && !a.accumulate) {
+    // structural zeros of columns m (and -m): l < ell0
+    const int l_end = st.ell0 < L ? st.ell0 : L;
+    for (int e = threadIdx.x; e < l_end * 8 * kFTiles; e += kFT) {
+      const int j = e / l_end, l = e - j * l_end;
+      if (b0 + j >= a.batch) continue;
+      cplx* o = out + (int64_t)j * a.flm_map_stride + (int64_t)l * (2 * L - 1);
+      *o = cplx{0.0, 0.0};
+      if (mirror) o[-2 * m] = cplx{0.0, 0.0};
+    }
+  }
   lat::ChunkCoef cn = {0.0, 0.0, 0.0, 0.0};
   if (threadIdx.x < lat::kChunk) ccs[0][threadIdx.x] = lat::load_coef(cf, st.ell0, threadIdx.x, L, bsign);
   __syncthreads();
@@ -251,13 +263,23 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
       double v = 0.0;
 #pragma unroll
       for (int w = 0; w < kFW; ++w) v += (double)reinterpret_cast<const float*>(wbuf[w])[e];
-      if (l < L && l >= a.lmin_out && b0 + bm < a.batch) {
+      if (l < L && b0 + bm < a.batch && (l >= a.lmin_out || a.fill)) {
+        if (l < a.lmin_out) v = 0.0;
         double* o = reinterpret_cast<double*>(out + (int64_t)bm * a.flm_map_stride +
                                               (int64_t)l * (2 * L - 1)) + (ri & 1);
         if (a.groups > 1 || a.accumulate)
           atomicAdd(o, v);
         else
           *o = v;
+        if (mirror) {
+          // column -m: (-1)^m conj, i.e. the imaginary part negated once more
+          const double mv = ((m & 1) ? -v : v) * ((ri & 1) ? -1.0 : 1.0);
+          double* om = o - 4 * m;
+          if (a.accumulate)
+            atomicAdd(om, mv);
+          else
+            *om = mv;
+        }
       }
     }
     __syncthreads();
@@ -441,6 +463,11 @@ static hipError_t lat_fwd_bf16(const LatArgs& a0, bool paired, hipStream_t s) {
   else
     hipLaunchKernelGGL((lat_forward_bf16_kernel<false, NS, SPLIT>), grid, dim3(kFT), 0, s, a);
   return hipGetLastError();
+}
+
+int lat_forward_bf16_groups(const LatArgs& a, int split) {
+  const int per_wg = kFW * (split == 1 ? 4 : 2) * 64;
+  return (a.nrings + per_wg - 1) / per_wg;
 }
 
 hipError_t launch_lat_forward_bf16(const LatArgs& a, bool paired, int split, hipStream_t s) {

@@ -41,7 +41,8 @@ def refine_fp64(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=No
 
 def refine_mixed(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=None,
                  precond=None, refresh_every=0, history=None, flm_true=None, stats=None,
-                 precision="mixed", inv_precision=None, learned_steps=None, spectral=False):
+                 precision="mixed", inv_precision=None, learned_steps=None, spectral=False,
+                 init_precision="fp64"):
     """Mixed-precision refinement with an FP64 residual (see module docstring).
 
     precond: None (Jacobi) or an object with ``apply(h, L, reality, lmin)``
@@ -57,6 +58,8 @@ def refine_mixed(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=N
     split-BF16 matrix cores; inv_precision defaults to precision).
     spectral: residuals on the ring spectra (see the module docstring); needs
     a plan with one ring grid.
+    init_precision: arithmetic of the synthesis that forms the first residual
+    (FP64 by default; a reduced one leaves its error in r until a refresh).
     """
     s = sampling.lower()
     plan = plan or sph.get_plan(s, L, nside, spin, L_lower, reality, device=f.device)
@@ -83,7 +86,7 @@ def refine_mixed(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=N
             return sph.forward_hip(x, precision=prec, accumulate_into=acc, **kw)
 
     flm = proj(f, fp)
-    r = synth(flm, sub=f)
+    r = synth(flm, init_precision, sub=f)
     rn = None  # per-map squared norm of r, when known
     _track(history, flm, flm_true)
     for k in range(n_iter):

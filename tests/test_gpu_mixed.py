@@ -133,3 +133,29 @@ def test_batch_sumsq(rng, shape, cplx):
     want = (np.abs(x.reshape(shape[0], -1)) ** 2).sum(axis=1)
     assert np.allclose(got, want, rtol=1e-13, atol=0.0)
     assert np.array_equal(refine._sumsq(xt).cpu().numpy(), got)
+
+
+@pytest.mark.parametrize("reality", [True, False])
+def test_bf16x1_forward_writes_every_entry(rng, reality):
+    """The single-product BF16 forward writes zeros below l0 and the m < 0
+    mirror itself (no zeroing or conjugate-fill pass): run it on memory the
+    caching allocator hands back full of NaN, and check accumulate_into."""
+    nside, L, B = 16, 32, 5
+    flm_true = torch.from_numpy(_flms(rng, B, L, 0, reality)).cuda()
+    f = sph.inverse_hip(flm_true, L, 0, nside, "healpix", reality)
+    if reality:
+        f = f.real.contiguous()
+    junk = torch.full((B, L, 2 * L - 1), float("nan"), dtype=torch.complex128, device="cuda")
+    del junk
+    got = sph.forward_hip(f, L, 0, nside, "healpix", reality, precision="bf16x1")
+    ref = sph.forward_hip(f, L, 0, nside, "healpix", reality)
+    assert bool(torch.isfinite(got).all())
+    e = max_rel(got.cpu().numpy(), ref.cpu().numpy())
+    assert 1e-12 < e < TOLS["bf16x1"]
+    # structural zeros exactly zero
+    mask = np.abs(np.arange(-(L - 1), L))[None, :] > np.arange(L)[:, None]
+    assert np.all(got.cpu().numpy()[:, mask] == 0)
+    base = torch.from_numpy(_flms(rng, B, L, 0, reality)).cuda()
+    acc = base.clone()
+    sph.forward_hip(f, L, 0, nside, "healpix", reality, precision="bf16x1", accumulate_into=acc)
+    assert max_rel(acc.cpu().numpy(), (base + got).cpu().numpy()) < 1e-15
