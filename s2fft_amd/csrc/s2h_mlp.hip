@@ -145,6 +145,10 @@ __global__ void __launch_bounds__(kInfWaves * 64) mlp_apply_kernel(MlpRowArgs a,
   const int64_t ntile = (a.P + 15) / 16;
   for (int64_t tile = (int64_t)blockIdx.x * kInfWaves + wave; tile < ntile;
        tile += (int64_t)gridDim.x * kInfWaves) {
+    // The weights are re-read from LDS every tile: kept in registers across
+    // tiles (the compiler's choice without this barrier) they take ~400
+    // registers and leave one wave per SIMD to hide every latency.
+    asm volatile("" ::: "memory");
     // features of row `row` (every lane of the 16-lane group computes them)
     const int64_t p = tile * 16 + row;
     int l = 0, m = 0;
