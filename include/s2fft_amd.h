@@ -194,7 +194,16 @@ int s2h_ring_analysis(const s2h_plan *plan, const void *f, int f_is_complex, voi
 int s2h_forward_rings(const s2h_plan *plan, const void *G, void *flm, int batch, int precision,
                       int flags, s2h_stream stream);
 int s2h_inverse_rings(const s2h_plan *plan, const void *flm, void *Y, int batch, int precision,
-                      s2h_stream stream);
+                      int flags, s2h_stream stream);
+/* S2H_FLM_PACKED (flags of s2h_forward_rings / s2h_inverse_rings, real maps
+ * only): flm in the packed m-major layout, batch x L(L+1)/2 complex128 with
+ * (l, m), l >= m >= 0, at m L - m(m-1)/2 + l - m -- the l of one m contiguous,
+ * so the latitudinal kernels read and write whole cache lines.  Converters
+ * to and from the reference layout (unpack writes the zeros and the m < 0
+ * half, (-1)^m conj). */
+enum { S2H_FLM_PACKED = 4 };
+int s2h_flm_pack(int L, const void *flm, void *flm_packed, int batch, s2h_stream stream);
+int s2h_flm_unpack(int L, const void *flm_packed, void *flm, int batch, s2h_stream stream);
 int s2h_ring_residual(const s2h_plan *plan, const void *G, const void *Y, void *R, int batch,
                       s2h_stream stream);
 
@@ -241,7 +250,8 @@ int s2h_mlp_param_count(void);
 /* delta = correction for every packed coefficient (m in [0, l] with reality,
  * whose m < 0 entries are filled by conjugate symmetry; else m in [-l, l]),
  * zero for l < lmin.  h, delta: batch x (L, 2L-1) complex128, distinct;
- * work: >= 8 * batch bytes. */
+ * reality = 2: real maps with h and delta in the S2H_FLM_PACKED layout
+ * (batch x L(L+1)/2).  work: >= 8 * batch bytes. */
 int s2h_mlp_precondition(const float *params, const void *h, void *delta, int L, int reality,
                          int lmin, int batch, void *work, s2h_stream stream);
 /* Training rows from a residual h and the error e it should predict:

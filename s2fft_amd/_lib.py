@@ -44,7 +44,7 @@ EXPORTED = (
     "s2h_profile_enable", "s2h_profile_query", "s2h_wigner_reality_fill",
     "s2h_batch_sumsq_workspace_bytes", "s2h_batch_sumsq",
     "s2h_ring_spectra_bytes", "s2h_ring_analysis", "s2h_forward_rings", "s2h_inverse_rings",
-    "s2h_ring_residual",
+    "s2h_ring_residual", "s2h_flm_pack", "s2h_flm_unpack",
     "s2h_forward_adjoint", "s2h_inverse_adjoint", "s2h_probe_mfma_f64",
     "s2h_probe_mfma_f32", "s2h_forward_ex", "s2h_inverse_ex",
     "s2h_mlp_param_count", "s2h_mlp_precondition", "s2h_mlp_features",
@@ -58,6 +58,7 @@ KERNEL_LAT_FORWARD_BF16, KERNEL_LAT_INVERSE_BF16 = 7, 8
 KERNEL_RING_RESIDUAL = 9
 PREC_FP64, PREC_MIXED, PREC_BF16, PREC_BF16X1 = 0, 1, 2, 3
 FWD_ACCUMULATE = 1
+FLM_PACKED = 4
 INV_REAL_OUTPUT, INV_SUBTRACT = 1, 2
 
 
@@ -99,7 +100,9 @@ def _declare(lib):
         "s2h_ring_spectra_bytes": (sz, [vp, i32]),
         "s2h_ring_analysis": (i32, [vp, vp, i32, vp, i32, vp]),
         "s2h_forward_rings": (i32, [vp, vp, vp, i32, i32, i32, vp]),
-        "s2h_inverse_rings": (i32, [vp, vp, vp, i32, i32, vp]),
+        "s2h_inverse_rings": (i32, [vp, vp, vp, i32, i32, i32, vp]),
+        "s2h_flm_pack": (i32, [i32, vp, vp, i32, vp]),
+        "s2h_flm_unpack": (i32, [i32, vp, vp, i32, vp]),
         "s2h_ring_residual": (i32, [vp, vp, vp, vp, i32, vp]),
         "s2h_profile_enable": (i32, [i32]),
         "s2h_profile_query": (i32, [i32, ctypes.POINTER(ctypes.c_double),

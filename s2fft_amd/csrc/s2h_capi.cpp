@@ -186,16 +186,18 @@ int ring_analysis_stage(const s2h_plan* p, const void* f, int f_is_complex, cplx
 // kernels with one ring group per m write every entry themselves (zeros and
 // the m < 0 mirror); otherwise flm is zeroed first and filled after.
 int lat_forward_stage(const s2h_plan* p, const cplx* ftm, void* flm, int batch, int precision,
-                      bool accumulate, hipStream_t s) {
+                      bool accumulate, hipStream_t s, bool packed = false) {
   const bool mixed = precision == S2H_PREC_MIXED;
   const bool bf16 = precision == S2H_PREC_BF16 || precision == S2H_PREC_BF16X1;
   const int split = precision == S2H_PREC_BF16X1 ? 1 : 3;
   const int L = p->d.L;
-  const int64_t fs = flm_stride(p);
+  const int64_t fs = packed ? (int64_t)L * (L + 1) / 2 : flm_stride(p);
   const int64_t ftm_stride = (int64_t)p->fwd_rows * p->Mc;
   int st = S2H_OK;
   LatArgs a = lat_args(p, true, batch);
   a.flm = static_cast<cplx*>(flm);
+  a.flm_packed = packed ? 1 : 0;
+  a.flm_map_stride = fs;
   a.ftm = const_cast<cplx*>(ftm);
   a.ftm_map_stride = ftm_stride;
   a.ftm_ncol = p->Mc;
@@ -205,7 +207,7 @@ int lat_forward_stage(const s2h_plan* p, const cplx* ftm, void* flm, int batch, 
   const bool poles = p->fwd_row_south >= 0 || p->fwd_row_north >= 0;
   const bool fill = bf16 && !poles && lat_forward_bf16_groups(a, split) == 1;
   a.fill = fill ? 1 : 0;
-  a.mirror = fill && p->d.reality ? 1 : 0;
+  a.mirror = fill && p->d.reality && !packed ? 1 : 0;
   if (!accumulate && !fill && (st = hip_status(launch_zero(flm, 2 * fs * batch, s)))) return st;
   {
     ProfScope ps(bf16 ? S2H_KERNEL_LAT_FORWARD_BF16
@@ -240,7 +242,7 @@ int lat_forward_stage(const s2h_plan* p, const cplx* ftm, void* flm, int batch, 
     if ((st = hip_status(launch_pole_forward(pa, s)))) return st;
   }
 
-  if (p->d.reality && !a.mirror) {
+  if (p->d.reality && !a.mirror && !packed) {
     if ((st = hip_status(launch_reality_fill(static_cast<cplx*>(flm), fs, L, batch, s)))) return st;
   }
   return S2H_OK;
@@ -248,13 +250,15 @@ int lat_forward_stage(const s2h_plan* p, const cplx* ftm, void* flm, int batch, 
 
 // Latitudinal synthesis (+ pole rows) into inverse-grid ring spectra ftm.
 int lat_inverse_stage(const s2h_plan* p, const void* flm, cplx* ftm, int batch, int precision,
-                      hipStream_t s) {
+                      hipStream_t s, bool packed = false) {
   const bool mixed = precision == S2H_PREC_MIXED;
   const bool bf16 = precision == S2H_PREC_BF16 || precision == S2H_PREC_BF16X1;
   const int64_t ftm_stride = (int64_t)p->inv_rows * p->Mc;
   int st = S2H_OK;
   LatArgs a = lat_args(p, false, batch);
   a.flm = const_cast<cplx*>(static_cast<const cplx*>(flm));
+  a.flm_packed = packed ? 1 : 0;
+  if (packed) a.flm_map_stride = (int64_t)p->d.L * (p->d.L + 1) / 2;
   a.ftm = ftm;
   a.ftm_map_stride = ftm_stride;
   a.ftm_ncol = p->Mc;
@@ -421,25 +425,46 @@ extern "C" int s2h_ring_analysis(const s2h_plan* p, const void* f, int f_is_comp
 extern "C" int s2h_forward_rings(const s2h_plan* p, const void* G, void* flm, int batch,
                                  int precision, int flags, s2h_stream stream) {
   if (!p || !G || !flm || batch < 0) return S2H_ERR_INVALID;
-  if (!valid_precision(precision) || (flags & ~S2H_FWD_ACCUMULATE)) return S2H_ERR_INVALID;
+  if (!valid_precision(precision) || (flags & ~(S2H_FWD_ACCUMULATE | S2H_FLM_PACKED)))
+    return S2H_ERR_INVALID;
+  const bool packed = (flags & S2H_FLM_PACKED) != 0;
+  if (packed && !p->d.reality) return S2H_ERR_INVALID;
   if (!spectral_ok(p)) return S2H_ERR_UNSUPPORTED;
   if (batch == 0) return S2H_OK;
   int st = check_device(p);
   if (st) return st;
   hipStream_t s = static_cast<hipStream_t>(stream);
   const bool accumulate = (flags & S2H_FWD_ACCUMULATE) != 0;
-  return lat_forward_stage(p, static_cast<const cplx*>(G), flm, batch, precision, accumulate, s);
+  return lat_forward_stage(p, static_cast<const cplx*>(G), flm, batch, precision, accumulate, s,
+                           packed);
 }
 
 extern "C" int s2h_inverse_rings(const s2h_plan* p, const void* flm, void* Y, int batch,
-                                 int precision, s2h_stream stream) {
+                                 int precision, int flags, s2h_stream stream) {
   if (!p || !flm || !Y || batch < 0 || !valid_precision(precision)) return S2H_ERR_INVALID;
+  if (flags & ~S2H_FLM_PACKED) return S2H_ERR_INVALID;
+  const bool packed = (flags & S2H_FLM_PACKED) != 0;
+  if (packed && !p->d.reality) return S2H_ERR_INVALID;
   if (!spectral_ok(p)) return S2H_ERR_UNSUPPORTED;
   if (batch == 0) return S2H_OK;
   int st = check_device(p);
   if (st) return st;
   return lat_inverse_stage(p, flm, static_cast<cplx*>(Y), batch, precision,
-                           static_cast<hipStream_t>(stream));
+                           static_cast<hipStream_t>(stream), packed);
+}
+
+extern "C" int s2h_flm_pack(int L, const void* flm, void* flm_p, int batch, s2h_stream stream) {
+  if (L < 1 || batch < 0 || (batch > 0 && (!flm || !flm_p)) || flm == flm_p)
+    return S2H_ERR_INVALID;
+  return hip_status(launch_flm_pack(L, static_cast<const cplx*>(flm), static_cast<cplx*>(flm_p),
+                                    batch, static_cast<hipStream_t>(stream)));
+}
+
+extern "C" int s2h_flm_unpack(int L, const void* flm_p, void* flm, int batch, s2h_stream stream) {
+  if (L < 1 || batch < 0 || (batch > 0 && (!flm || !flm_p)) || flm == flm_p)
+    return S2H_ERR_INVALID;
+  return hip_status(launch_flm_unpack(L, static_cast<const cplx*>(flm_p), static_cast<cplx*>(flm),
+                                      batch, static_cast<hipStream_t>(stream)));
 }
 
 extern "C" int s2h_ring_residual(const s2h_plan* p, const void* G, const void* Y, void* R,

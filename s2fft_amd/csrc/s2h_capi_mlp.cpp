@@ -7,19 +7,22 @@ namespace {
 
 int hip_status(hipError_t e) { return e == hipSuccess ? S2H_OK : S2H_ERR_HIP; }
 
+// reality: 0 complex maps, 1 real maps, 2 real maps in the packed m-major layout
 int row_args(const void* h, void* out, int L, int reality, int lmin, int batch, MlpRowArgs& a) {
-  if (!h || L < 1 || batch < 0 || lmin < 0 || lmin >= L) return S2H_ERR_INVALID;
+  if (!h || L < 1 || batch < 0 || lmin < 0 || lmin >= L || reality < 0 || reality > 2)
+    return S2H_ERR_INVALID;
   a = MlpRowArgs{};
   a.h = static_cast<const cplx*>(h);
   a.out = static_cast<cplx*>(out);
   a.L = L;
   a.reality = reality ? 1 : 0;
+  a.packed = reality == 2 ? 1 : 0;
   a.lmin = lmin;
   a.batch = batch;
   a.P = reality ? (int64_t)L * (L + 1) / 2 : (int64_t)L * L;
   const int64_t below = reality ? (int64_t)lmin * (lmin + 1) / 2 : (int64_t)lmin * lmin;
   a.count = a.P - below;
-  a.map_stride = (int64_t)L * (2 * L - 1);
+  a.map_stride = a.packed ? a.P : (int64_t)L * (2 * L - 1);
   return S2H_OK;
 }
 
@@ -38,9 +41,8 @@ extern "C" int s2h_mlp_precondition(const float* params, const void* h, void* de
   if (h == delta) return S2H_ERR_INVALID;
   hipStream_t s = static_cast<hipStream_t>(stream);
   double* sumsq = static_cast<double*>(work);
-  // entries outside the packed rows (m > l, and m < 0 without reality
-  // handled by the rows themselves) stay zero
-  if ((st = hip_status(launch_zero(delta, 2 * a.map_stride * batch, s)))) return st;
+  // entries outside the rows (m > l) stay zero; the packed layout has none
+  if (!a.packed && (st = hip_status(launch_zero(delta, 2 * a.map_stride * batch, s)))) return st;
   if ((st = hip_status(launch_mlp_sumsq(a, sumsq, s)))) return st;
   return hip_status(launch_mlp_apply(a, params, sumsq, s));
 }

@@ -84,7 +84,32 @@ struct LatArgs {
                             // max(ell0, lmin_out), so flm needs no zeroing pass
   int mirror;               // forward with fill, real maps: also write
                             // flm[l, -m] = (-1)^m conj(flm[l, m]) (no conjugate fill pass)
+  int flm_packed;           // flm in the packed m-major layout (real maps, m >= 0 only)
 };
+
+// Harmonic-side addressing.  Reference layout: flm[b][l][L-1+m].  Packed
+// m-major layout (real maps): the L(L+1)/2 coefficients l >= m >= 0 of a map
+// stored as m-blocks, flm_p[b][moff(m) + l - m], moff(m) = m L - m(m-1)/2, so
+// the l of one m are contiguous.  Element (l, m) sits at flm_col(m) + l * flm_row.
+__host__ __device__ __forceinline__ int64_t packed_moff(int L, int m) {
+  return (int64_t)m * L - (int64_t)m * (m - 1) / 2;
+}
+__host__ __device__ __forceinline__ int64_t flm_col(const LatArgs& a, int m) {
+  return a.flm_packed ? packed_moff(a.L, m) - m : (int64_t)(a.L - 1 + m);
+}
+__host__ __device__ __forceinline__ int64_t flm_row(const LatArgs& a) {
+  return a.flm_packed ? 1 : 2 * a.L - 1;
+}
+// (l, m) of packed row p
+__device__ __forceinline__ void packed_lm(int64_t p, int L, int& l, int& m) {
+  const double b = 2.0 * L + 1.0;
+  int mm = (int)((b - sqrt(b * b - 8.0 * (double)p)) * 0.5);
+  if (mm < 0) mm = 0;
+  while (mm > 0 && packed_moff(L, mm) > p) --mm;
+  while (mm + 1 < L && packed_moff(L, mm + 1) <= p) ++mm;
+  m = mm;
+  l = (int)(p - packed_moff(L, mm)) + mm;
+}
 
 hipError_t launch_ring_fft_forward(const RingFwdArgs& a, const JobSet& js, int batch,
                                    hipStream_t s);
@@ -116,13 +141,16 @@ hipError_t launch_mfma_f32_probe(const float* a, const float* b, float* out, hip
 
 // learned preconditioner (per-(l, m) rows of one or more flm arrays)
 struct MlpRowArgs {
-  const cplx* h;       // batch x (L, 2L-1)
-  cplx* out;           // batch x (L, 2L-1) (apply)
+  const cplx* h;       // batch x (L, 2L-1), or batch x L(L+1)/2 packed m-major
+  cplx* out;           // same layout as h (apply)
   int L, reality, lmin, batch;
-  int64_t P;           // packed rows per map: L(L+1)/2 (reality) or L^2
+  int packed;          // h / out in the packed m-major layout (reality)
+  int64_t P;           // rows per map: L(L+1)/2 (reality) or L^2
   int64_t count;       // rows with l >= lmin (for the RMS)
-  int64_t map_stride;  // L (2L-1)
+  int64_t map_stride;  // L (2L-1), or L(L+1)/2 packed
 };
+hipError_t launch_flm_pack(int L, const cplx* src, cplx* dst, int batch, hipStream_t s);
+hipError_t launch_flm_unpack(int L, const cplx* src, cplx* dst, int batch, hipStream_t s);
 hipError_t launch_mlp_sumsq(const MlpRowArgs& a, double* sumsq, hipStream_t s);
 hipError_t launch_mlp_apply(const MlpRowArgs& a, const float* w, const double* sumsq,
                             hipStream_t s);

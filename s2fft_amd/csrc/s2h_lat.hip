@@ -41,7 +41,7 @@ __global__ void __launch_bounds__(kThreads) lat_inverse_kernel(LatArgs a) {
   const double x = rg.x;
   const double bsign = m < 0 ? -1.0 : 1.0;
   const Coef* __restrict__ cf = a.coef + (int64_t)am * a.L;
-  const cplx* __restrict__ flm = a.flm + (int64_t)b0 * a.flm_map_stride + (a.L - 1 + m);
+  const cplx* __restrict__ flm = a.flm + (int64_t)b0 * a.flm_map_stride + flm_col(a, m);
 
   double d, dm1 = 0.0;
   int lscale;
@@ -68,7 +68,7 @@ __global__ void __launch_bounds__(kThreads) lat_inverse_kernel(LatArgs a) {
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
           if (j < nb) {
-            const cplx f = flm[(int64_t)j * a.flm_map_stride + (int64_t)l * (2 * L - 1)];
+            const cplx f = flm[(int64_t)j * a.flm_map_stride + (int64_t)l * flm_row(a)];
             if (odd) {
               ao[j].x = fma(d, f.x, ao[j].x);
               ao[j].y = fma(d, f.y, ao[j].y);
@@ -146,7 +146,7 @@ __global__ void __launch_bounds__(kThreads) lat_forward_kernel(LatArgs a) {
     }
   }
 
-  cplx* out = a.flm + b * a.flm_map_stride + (a.L - 1 + m);
+  cplx* out = a.flm + b * a.flm_map_stride + flm_col(a, m);
   for (int blk = st.ell0; blk < L; blk += kChunk) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -200,7 +200,7 @@ __global__ void __launch_bounds__(kThreads) lat_forward_kernel(LatArgs a) {
         cplx t = wsum[0][threadIdx.x];
 #pragma unroll
         for (int w = 1; w < kThreads / 64; ++w) t = cadd(t, wsum[w][threadIdx.x]);
-        cplx* o = out + (int64_t)l * (2 * L - 1);
+        cplx* o = out + (int64_t)l * flm_row(a);
         if (a.groups > 1 || a.accumulate) {
           atomicAdd(&o->x, t.x);
           atomicAdd(&o->y, t.y);

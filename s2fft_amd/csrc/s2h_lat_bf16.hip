@@ -158,15 +158,17 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
     }
   }
 
-  cplx* out = a.flm + (int64_t)b0 * a.flm_map_stride + (L - 1 + m);
+  cplx* out = a.flm + (int64_t)b0 * a.flm_map_stride + flm_col(a, m);
   const bool mirror = a.mirror && m > 0;
   if (a.fill && !a.accumulate) {
-    // structural zeros of columns m (and -m): l < ell0
+    // structural zeros of columns m (and -m): l < ell0 (from l = |m| when packed)
+    const int l_beg = a.flm_packed ? am : 0;
     const int l_end = st.ell0 < L ? st.ell0 : L;
-    for (int e = threadIdx.x; e < l_end * 8 * kFTiles; e += kFT) {
-      const int j = e / l_end, l = e - j * l_end;
+    const int nl = l_end > l_beg ? l_end - l_beg : 0;
+    for (int e = threadIdx.x; e < nl * 8 * kFTiles; e += kFT) {
+      const int j = e / nl, l = l_beg + e - j * nl;
       if (b0 + j >= a.batch) continue;
-      cplx* o = out + (int64_t)j * a.flm_map_stride + (int64_t)l * (2 * L - 1);
+      cplx* o = out + (int64_t)j * a.flm_map_stride + (int64_t)l * flm_row(a);
       *o = cplx{0.0, 0.0};
       if (mirror) o[-2 * m] = cplx{0.0, 0.0};
     }
@@ -266,7 +268,7 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
       if (l < L && b0 + bm < a.batch && (l >= a.lmin_out || a.fill)) {
         if (l < a.lmin_out) v = 0.0;
         double* o = reinterpret_cast<double*>(out + (int64_t)bm * a.flm_map_stride +
-                                              (int64_t)l * (2 * L - 1)) + (ri & 1);
+                                              (int64_t)l * flm_row(a)) + (ri & 1);
         if (a.groups > 1 || a.accumulate)
           atomicAdd(o, v);
         else
@@ -336,7 +338,7 @@ __global__ void __launch_bounds__(kIT) lat_inverse_bf16_kernel(LatArgs a) {
       acc[q][t][1] = f32x4{0.f, 0.f, 0.f, 0.f};
     }
 
-  const cplx* flm = a.flm + (int64_t)b0 * a.flm_map_stride + (L - 1 + m);
+  const cplx* flm = a.flm + (int64_t)b0 * a.flm_map_stride + flm_col(a, m);
   // flm chunk staging: thread t holds elements e = t + kIT * j (j < 4) of
   // [column 32][l_local 32]; the next chunk is loaded while this one runs
   constexpr int kStage = 32 * 32 / kIT;
@@ -349,7 +351,7 @@ __global__ void __launch_bounds__(kIT) lat_inverse_bf16_kernel(LatArgs a) {
       double x0 = 0.0;
       if (l < L && l >= a.lmin_out && b0 + bm < a.batch) {
         const double* f = reinterpret_cast<const double*>(
-            flm + (int64_t)bm * a.flm_map_stride + (int64_t)l * (2 * L - 1));
+            flm + (int64_t)bm * a.flm_map_stride + (int64_t)l * flm_row(a));
         x0 = f[col & 1];
       }
       v[j] = x0;

@@ -115,6 +115,60 @@ hipError_t launch_zero(void* p, int64_t ndoubles, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Reference layout flm[b][l][L-1+m] <-> packed m-major layout of real maps
+// (flm_p[b][moff(m) + l - m], l >= m >= 0).  Unpacking writes every entry:
+// zeros where |m| > l and (-1)^m conj(flm[l, -m]) for m < 0.
+__global__ void __launch_bounds__(kThreads) flm_unpack_kernel(int L, const cplx* src, cplx* dst) {
+  const int64_t b = blockIdx.y;
+  const int64_t n = (int64_t)L * (2 * L - 1);
+  const cplx* sp = src + b * ((int64_t)L * (L + 1) / 2);
+  cplx* dp = dst + b * n;
+  for (int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * kThreads) {
+    const int l = (int)(e / (2 * L - 1));
+    const int m = (int)(e - (int64_t)l * (2 * L - 1)) - (L - 1);
+    const int am = m < 0 ? -m : m;
+    cplx v = {0.0, 0.0};
+    if (am <= l) {
+      const cplx t = sp[packed_moff(L, am) + l - am];
+      const double s = (m < 0 && (am & 1)) ? -1.0 : 1.0;
+      v = m < 0 ? cplx{s * t.x, -s * t.y} : t;
+    }
+    dp[e] = v;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) flm_pack_kernel(int L, const cplx* src, cplx* dst) {
+  const int64_t b = blockIdx.y;
+  const int64_t P = (int64_t)L * (L + 1) / 2;
+  const cplx* sp = src + b * (int64_t)L * (2 * L - 1);
+  cplx* dp = dst + b * P;
+  for (int64_t p = (int64_t)blockIdx.x * kThreads + threadIdx.x; p < P;
+       p += (int64_t)gridDim.x * kThreads) {
+    int l, m;
+    packed_lm(p, L, l, m);
+    dp[p] = sp[(int64_t)l * (2 * L - 1) + L - 1 + m];
+  }
+}
+
+hipError_t launch_flm_unpack(int L, const cplx* src, cplx* dst, int batch, hipStream_t s) {
+  if (batch == 0) return hipSuccess;
+  const int64_t n = (int64_t)L * (2 * L - 1);
+  const int64_t blocks = (n + kThreads - 1) / kThreads;
+  hipLaunchKernelGGL(flm_unpack_kernel, dim3((unsigned)(blocks < 4096 ? blocks : 4096), batch),
+                     dim3(kThreads), 0, s, L, src, dst);
+  return hipGetLastError();
+}
+
+hipError_t launch_flm_pack(int L, const cplx* src, cplx* dst, int batch, hipStream_t s) {
+  if (batch == 0) return hipSuccess;
+  const int64_t P = (int64_t)L * (L + 1) / 2;
+  const int64_t blocks = (P + kThreads - 1) / kThreads;
+  hipLaunchKernelGGL(flm_pack_kernel, dim3((unsigned)(blocks < 4096 ? blocks : 4096), batch),
+                     dim3(kThreads), 0, s, L, src, dst);
+  return hipGetLastError();
+}
+
 // Per-map sum of squares out[b] = sum_{i < n} x[b * stride + i]^2 in two
 // passes with a fixed summation order (the result does not depend on the
 // launch): kSumsqBlocks contiguous slices per map, four interleaved FP64

@@ -71,6 +71,16 @@ __device__ __forceinline__ void lm_of(int64_t p, int reality, int& l, int& m) {
   }
 }
 
+// row p -> (l, m) and the element index of row p in h / out
+__device__ __forceinline__ int64_t row_index(const MlpRowArgs& a, int64_t p, int& l, int& m) {
+  if (a.packed) {
+    packed_lm(p, a.L, l, m);
+    return p;
+  }
+  lm_of(p, a.reality, l, m);
+  return (int64_t)l * (2 * a.L - 1) + a.L - 1 + m;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- statistics
@@ -82,9 +92,9 @@ __global__ void __launch_bounds__(kThreads) mlp_sumsq_kernel(MlpRowArgs a, doubl
   for (int64_t p = (int64_t)blockIdx.x * kThreads + threadIdx.x; p < a.P;
        p += (int64_t)gridDim.x * kThreads) {
     int l, m;
-    lm_of(p, a.reality, l, m);
+    const int64_t idx = row_index(a, p, l, m);
     if (l < a.lmin) continue;
-    const cplx v = a.h[(int64_t)b * a.map_stride + (int64_t)l * (2 * a.L - 1) + a.L - 1 + m];
+    const cplx v = a.h[(int64_t)b * a.map_stride + idx];
     acc += v.x * v.x + v.y * v.y;
   }
   for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
@@ -155,10 +165,10 @@ __global__ void __launch_bounds__(kInfWaves * 64) mlp_apply_kernel(MlpRowArgs a,
     float x0 = 0.f, x1 = 0.f, x2 = 0.f, x3 = 0.f;
     bool valid = false;
     if (p < a.P) {
-      lm_of(p, a.reality, l, m);
+      const int64_t idx = row_index(a, p, l, m);
       valid = l >= a.lmin;
       if (valid) {
-        const cplx v = h[(int64_t)l * (2 * a.L - 1) + a.L - 1 + m];
+        const cplx v = h[idx];
         x0 = (float)v.x * inv_sig;
         x1 = (float)v.y * inv_sig;
         x2 = (float)l * inv_L;
@@ -215,8 +225,7 @@ __global__ void __launch_bounds__(kInfWaves * 64) mlp_apply_kernel(MlpRowArgs a,
       const int64_t q = tile * 16 + 4 * kq + row;
       if (q < a.P) {
         int lq, mq;
-        lm_of(q, a.reality, lq, mq);
-        const int64_t idx = (int64_t)lq * (2 * a.L - 1) + a.L - 1 + mq;
+        const int64_t idx = row_index(a, q, lq, mq);
         if (lq < a.lmin) {
           out[idx] = cplx{0.0, 0.0};
         } else {
@@ -224,7 +233,7 @@ __global__ void __launch_bounds__(kInfWaves * 64) mlp_apply_kernel(MlpRowArgs a,
           double im = sig * (double)(o1 + b3[1]);
           if (a.reality && mq == 0) im = 0.0;
           out[idx] = cplx{re, im};
-          if (a.reality && mq > 0) {
+          if (a.reality && !a.packed && mq > 0) {
             const double sg = (mq & 1) ? -1.0 : 1.0;
             out[(int64_t)lq * (2 * a.L - 1) + a.L - 1 - mq] = cplx{sg * re, -sg * im};
           }
@@ -245,12 +254,12 @@ __global__ void __launch_bounds__(kThreads) mlp_features_kernel(MlpRowArgs a, co
   for (int64_t p = (int64_t)blockIdx.x * kThreads + threadIdx.x; p < a.P;
        p += (int64_t)gridDim.x * kThreads) {
     int l, m;
-    lm_of(p, a.reality, l, m);
+    const int64_t idx0 = row_index(a, p, l, m);
     const int64_t row = (int64_t)b * a.P + p;
     float4 xv = {0.f, 0.f, 0.f, 0.f};
     float2 yv = {0.f, 0.f};
     if (l >= a.lmin) {
-      const int64_t idx = (int64_t)b * a.map_stride + (int64_t)l * (2 * a.L - 1) + a.L - 1 + m;
+      const int64_t idx = (int64_t)b * a.map_stride + idx0;
       const cplx hv = a.h[idx], ev = e[idx];
       xv = float4{(float)(hv.x / sig), (float)(hv.y / sig), (float)l / a.L, (float)m / a.L};
       yv = float2{(float)(ev.x / sig), (float)(ev.y / sig)};

@@ -70,14 +70,16 @@ class MLPPreconditioner:
     def save(self, path=WEIGHTS):
         np.save(path, self.params.cpu().numpy())
 
-    def apply(self, h, L, reality, lmin=0):
-        """delta (B, L, 2L-1) complex128 for a residual h of the same shape."""
+    def apply(self, h, L, reality, lmin=0, packed=False):
+        """delta for a residual h of the same shape: (B, L, 2L-1) complex128, or
+        (B, L(L+1)/2) in the packed m-major layout (``packed``, real maps)."""
         h = h.contiguous()
         B = h.shape[0]
         out = torch.empty_like(h)
         work = torch.empty(max(B, 1), dtype=torch.float64, device=h.device)
+        mode = 2 if packed else int(bool(reality))
         _lib.check(_lib.lib().s2h_mlp_precondition(
-            self.params.data_ptr(), h.data_ptr(), out.data_ptr(), L, int(bool(reality)), lmin,
+            self.params.data_ptr(), h.data_ptr(), out.data_ptr(), L, mode, lmin,
             B, work.data_ptr(), _stream(h)), "mlp precondition")
         return out
 

@@ -22,7 +22,10 @@ With ``spectral`` (plans with one ring grid: HEALPix) the residual lives on
 the ring spectra instead of the pixels: r = G - A(Lat_inv(flm)) with
 G the ring FFT of f (computed once) and A = (ring FFT) . (ring iFFT) applied
 without either FFT (s2h_ring_residual), so F~(r) = Lat_fwd~(r) and the loop
-runs no ring FFT at all.  The iterates are the same up to rounding.
+runs no ring FFT at all.  The iterates are the same up to rounding.  With
+real maps the coefficients stay in the packed m-major layout inside the loop
+(the l of one m contiguous: whole cache lines for the latitudinal kernels,
+half the memory) and are unpacked once at the end.
 """
 
 import torch
@@ -69,15 +72,17 @@ def refine_mixed(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=N
     if learned_steps is None:
         learned_steps = max(n_iter - 1, 0)
     real_out = bool(reality) and not f.is_complex()
+    packed = False
     if spectral:
         # f is replaced by its ring spectra G: every residual is G - A(I~(x))
         f = sph.ring_analysis_hip(f, plan)
+        packed = bool(reality)
 
         def synth(x, prec="fp64", sub=None):
-            return sph.ring_residual_hip(sub, sph.inverse_rings_hip(x, plan, prec), plan)
+            return sph.ring_residual_hip(sub, sph.inverse_rings_hip(x, plan, prec, packed), plan)
 
         def proj(x, prec, acc=None):
-            return sph.forward_rings_hip(x, L, plan, prec, accumulate_into=acc)
+            return sph.forward_rings_hip(x, L, plan, prec, accumulate_into=acc, packed=packed)
     else:
         def synth(x, prec="fp64", sub=None):
             return sph.inverse_hip(x, precision=prec, real_output=real_out, sub_from=sub, **kw)
@@ -85,30 +90,32 @@ def refine_mixed(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=N
         def proj(x, prec, acc=None):
             return sph.forward_hip(x, precision=prec, accumulate_into=acc, **kw)
 
+    unpack = (lambda x: sph.flm_unpack_hip(x, L)) if packed else (lambda x: x)
     flm = proj(f, fp)
     r = synth(flm, init_precision, sub=f)
     rn = None  # per-map squared norm of r, when known
-    _track(history, flm, flm_true)
+    _track(history, flm, flm_true, unpack)
     for k in range(n_iter):
         last = k + 1 == n_iter
         if last:
             proj(r, fp, acc=flm)
-            _track(history, flm, flm_true)
+            _track(history, flm, flm_true, unpack)
             break
         h = proj(r, fp)
         refresh = bool(refresh_every and (k + 1) % refresh_every == 0)
         if precond is not None and k < learned_steps:
-            delta, r, rn = _judged_step(precond, h, r, rn, f, flm, kw, stats, synth, ip, refresh)
+            delta, r, rn = _judged_step(precond, h, r, rn, f, flm, kw, stats, synth, ip, refresh,
+                                        packed)
             flm += delta
         else:
             flm += h
             r = synth(flm, sub=f) if refresh else synth(h, ip, sub=r)
             rn = None
-        _track(history, flm, flm_true)
-    return flm
+        _track(history, flm, flm_true, unpack)
+    return unpack(flm)
 
 
-def _judged_step(precond, h, r, rn, f, flm, kw, stats, synth, ip, exact):
+def _judged_step(precond, h, r, rn, f, flm, kw, stats, synth, ip, exact, packed=False):
     """Learned step per map where it lowers the FP64-held residual, else Jacobi.
 
     The candidate delta = P(h) moves the residual to r - I~(delta) (or, on a
@@ -120,7 +127,7 @@ def _judged_step(precond, h, r, rn, f, flm, kw, stats, synth, ip, exact):
     """
     L = kw["L"]
     lmin = max(abs(kw["spin"]), kw["L_lower"])
-    delta = precond.apply(h, L, kw["reality"], lmin)
+    delta = precond.apply(h, L, kw["reality"], lmin, packed=packed)
 
     def residual(d, rows):
         if exact:
@@ -164,7 +171,7 @@ def _sumsq(x):
     return out
 
 
-def _track(history, flm, flm_true):
+def _track(history, flm, flm_true, unpack=lambda x: x):
     if history is None or flm_true is None:
         return
-    history.append(float((flm - flm_true).abs().max() / flm_true.abs().max()))
+    history.append(float((unpack(flm) - flm_true).abs().max() / flm_true.abs().max()))

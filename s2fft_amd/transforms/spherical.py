@@ -276,32 +276,60 @@ def ring_analysis_hip(f, plan):
     return G
 
 
-def forward_rings_hip(G, L, plan, precision="fp64", accumulate_into=None):
-    """flm (B, L, 2L-1) from ring spectra; ``accumulate_into``: flm += in place."""
+def _flm_shape(B, L, packed):
+    return (B, L * (L + 1) // 2) if packed else (B, L, 2 * L - 1)
+
+
+def forward_rings_hip(G, L, plan, precision="fp64", accumulate_into=None, packed=False):
+    """flm (B, L, 2L-1) from ring spectra; ``accumulate_into``: flm += in place.
+    ``packed``: flm in the packed m-major layout (B, L(L+1)/2) (real maps)."""
     B = G.shape[0]
-    flags = 0
+    flags = _lib.FLM_PACKED if packed else 0
+    shape = _flm_shape(B, L, packed)
     if accumulate_into is not None:
         flm = accumulate_into
         if (flm.dtype != torch.complex128 or not flm.is_contiguous()
-                or tuple(flm.shape) != (B, L, 2 * L - 1)):
-            raise ValueError("accumulate_into must be a contiguous (B, L, 2L-1) complex128 tensor")
-        flags = _lib.FWD_ACCUMULATE
+                or tuple(flm.shape) != shape):
+            raise ValueError(f"accumulate_into must be a contiguous {shape} complex128 tensor")
+        flags |= _lib.FWD_ACCUMULATE
     else:
-        flm = torch.empty((B, L, 2 * L - 1), dtype=torch.complex128, device=G.device)
+        flm = torch.empty(shape, dtype=torch.complex128, device=G.device)
     _lib.check(_lib.lib().s2h_forward_rings(plan.handle, G.contiguous().data_ptr(),
                                             flm.data_ptr(), B, _precision(precision), flags,
                                             _stream(G.device)), "forward rings")
     return flm
 
 
-def inverse_rings_hip(flm, plan, precision="fp64"):
-    """Ring spectra (B, rows * Mc) of the latitudinal synthesis of flm."""
+def inverse_rings_hip(flm, plan, precision="fp64", packed=False):
+    """Ring spectra (B, rows * Mc) of the latitudinal synthesis of flm
+    (``packed``: flm in the packed m-major layout)."""
     flm = flm.contiguous()
     Y = _new_spectra(plan, flm.shape[0], flm.device)
     _lib.check(_lib.lib().s2h_inverse_rings(plan.handle, flm.data_ptr(), Y.data_ptr(),
                                             flm.shape[0], _precision(precision),
+                                            _lib.FLM_PACKED if packed else 0,
                                             _stream(flm.device)), "inverse rings")
     return Y
+
+
+def flm_pack_hip(flm, L):
+    """(B, L, 2L-1) -> packed m-major (B, L(L+1)/2) (real maps: m >= 0 kept)."""
+    flm = flm.contiguous()
+    out = torch.empty(_flm_shape(flm.shape[0], L, True), dtype=torch.complex128,
+                      device=flm.device)
+    _lib.check(_lib.lib().s2h_flm_pack(L, flm.data_ptr(), out.data_ptr(), flm.shape[0],
+                                       _stream(flm.device)), "flm pack")
+    return out
+
+
+def flm_unpack_hip(flm_p, L):
+    """Packed m-major (B, L(L+1)/2) -> (B, L, 2L-1) with the m < 0 half."""
+    flm_p = flm_p.contiguous()
+    out = torch.empty(_flm_shape(flm_p.shape[0], L, False), dtype=torch.complex128,
+                      device=flm_p.device)
+    _lib.check(_lib.lib().s2h_flm_unpack(L, flm_p.data_ptr(), out.data_ptr(), flm_p.shape[0],
+                                         _stream(flm_p.device)), "flm unpack")
+    return out
 
 
 def ring_residual_hip(G, Y, plan, out=None):
@@ -332,12 +360,14 @@ def forward_iterative_hip(f, L, spin=0, nside=None, sampling="mw", reality=False
         if not f.is_complex():
             f = f.to(torch.float64)
         if spectral and spectra_supported(plan):
+            # real maps iterate on the packed m-major coefficients
+            pk = bool(reality)
             G = ring_analysis_hip(f, plan)
-            flm = forward_rings_hip(G, L, plan)
+            flm = forward_rings_hip(G, L, plan, packed=pk)
             for _ in range(n_iter):
-                R = ring_residual_hip(G, inverse_rings_hip(flm, plan), plan)
-                forward_rings_hip(R, L, plan, accumulate_into=flm)
-            return flm
+                R = ring_residual_hip(G, inverse_rings_hip(flm, plan, packed=pk), plan)
+                forward_rings_hip(R, L, plan, accumulate_into=flm, packed=pk)
+            return flm_unpack_hip(flm, L) if pk else flm
         flm = forward_hip(f, L, spin, nside, s, reality, L_lower, plan=plan)
         real_out = bool(reality) and not f.is_complex()
         for _ in range(n_iter):

@@ -92,3 +92,47 @@ def test_spectra_unsupported_for_resampled_forward():
     assert not sph.spectra_supported(plan)
     with pytest.raises(ValueError):
         sph.ring_analysis_hip(torch.zeros((1, 8, 15), dtype=torch.float64, device="cuda"), plan)
+
+
+@pytest.mark.parametrize("L", [1, 2, 7, 33])
+def test_flm_pack_unpack_round_trip(rng, L):
+    flm = torch.from_numpy(_flms(rng, 3, L, True)).cuda()
+    p = sph.flm_pack_hip(flm, L)
+    assert p.shape == (3, L * (L + 1) // 2)
+    assert torch.equal(sph.flm_unpack_hip(p, L), flm)
+    # packed order: (l, m) at m L - m (m - 1) / 2 + l - m
+    pn, fn = p.cpu().numpy(), flm.cpu().numpy()
+    for m in range(L):
+        for l in range(m, L):
+            assert pn[1, m * L - m * (m - 1) // 2 + l - m] == fn[1, l, L - 1 + m]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "bf16x1", "bf16", "mixed"])
+def test_packed_rings_match_full_layout(rng, precision):
+    """The latitudinal kernels give the same bits in the packed layout."""
+    nside, L = 16, 32
+    flm, f = _maps(rng, "healpix", nside, L, True, B=5)
+    plan = sph.get_plan("healpix", L, nside, 0, 0, True, device=f.device)
+    G = sph.ring_analysis_hip(f, plan)
+    full = sph.forward_rings_hip(G, L, plan, precision)
+    pk = sph.forward_rings_hip(G, L, plan, precision, packed=True)
+    assert torch.equal(sph.flm_pack_hip(full, L), pk)
+    assert torch.equal(sph.flm_unpack_hip(pk, L), full)
+    inv_prec = "bf16" if precision == "bf16x1" else precision
+    Yf = sph.inverse_rings_hip(flm, plan, inv_prec)
+    Yp = sph.inverse_rings_hip(sph.flm_pack_hip(flm, L), plan, inv_prec, packed=True)
+    assert torch.equal(Yf, Yp)
+    acc = pk.clone()
+    sph.forward_rings_hip(G, L, plan, precision, accumulate_into=acc, packed=True)
+    assert torch.allclose(acc, 2 * pk, rtol=1e-14, atol=0)
+
+
+def test_packed_mlp_matches_full_layout(rng):
+    from s2fft_amd.learned.precond import MLPPreconditioner
+    L, B = 40, 3
+    h = torch.from_numpy(_flms(rng, B, L, True)).cuda()
+    pc = MLPPreconditioner.load(device="cuda")
+    full = pc.apply(h, L, True, 0)
+    pk = pc.apply(sph.flm_pack_hip(h, L), L, True, 0, packed=True)
+    scale = float(full.abs().max())
+    assert float((sph.flm_pack_hip(full, L) - pk).abs().max()) <= 1e-3 * scale
