@@ -57,8 +57,9 @@ def parse():
     ap.add_argument("--mixed-precision", default="bf16x1", choices=["bf16x1", "bf16", "mixed"])
     ap.add_argument("--mixed-inv-precision", default="bf16", choices=["bf16", "mixed"])
     ap.add_argument("--refresh-every", type=int, default=3)
-    ap.add_argument("--init-precision", default="fp64", choices=["fp64", "bf16", "mixed"],
-                    help="mixed leg: arithmetic of the synthesis forming the first residual")
+    ap.add_argument("--init-precision", default="bf16", choices=["fp64", "bf16", "mixed"],
+                    help="mixed leg: arithmetic of the synthesis forming the first residual "
+                         "(its error is removed by the FP64 refresh)")
     ap.add_argument("--learned-steps", type=int, default=2)
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -127,6 +128,22 @@ def kernel_times(_lib):
              "ring_residual": _lib.KERNEL_RING_RESIDUAL}
     out = {k: _lib.profile_query(v) for k, v in names.items()}
     return {k: v for k, v in out.items() if v[1] > 0}
+
+
+def mfma_f64_ceiling(_lib, torch, blocks=4096, iters=250):
+    """Sustained V_MFMA_F64_16X16X4F64 TFLOP/s of this device (s2h_probe_mfma_f64_rate:
+    8 independent accumulators per wave, no memory traffic), timed with HIP
+    events on the launch stream."""
+    out = torch.empty(blocks * 256, dtype=torch.float64, device="cuda")
+    s = torch.cuda.current_stream()
+    _lib.check(_lib.lib().s2h_probe_mfma_f64_rate(blocks, 10, out.data_ptr(), s.cuda_stream))
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    _lib.check(_lib.lib().s2h_probe_mfma_f64_rate(blocks, iters, out.data_ptr(), s.cuda_stream))
+    e1.record(s)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1)
+    return blocks * 4 * iters * 8 * 2048.0 / (ms * 1e-3) / 1e12
 
 
 def roofline(kernels, nside, L, B):
@@ -266,6 +283,13 @@ def main():
     mixed = {"elapsed": el, "err": err, "kernels": kt, "iters": n_mixed,
              "roofline": roofline(kt, nside, L, B)}
 
+    ceiling = mfma_f64_ceiling(_lib, torch)
+    log(f"[bench] FP64 matrix-core rate probe: {ceiling:.1f} TFLOP/s")
+    for leg in ([plain] if plain else []) + [mixed]:
+        rf = leg["roofline"]
+        if rf.get("arith") == "fp64":
+            rf["measured_fp64_mfma_tflops"] = ceiling
+            rf["frac_of_measured_fp64_mfma"] = rf["achieved"] / ceiling
     value = B * args.steps / mixed["elapsed"]
     plain_value = B * args.steps / plain["elapsed"] if plain else None
     head = plain["roofline"] if plain else mixed["roofline"]

@@ -276,6 +276,10 @@ int s2h_mlp_adam(float *params, const float *grad, float *m1, float *m2, int ste
  * accumulator; out[4*lane + r] = result register r of lane (256 doubles).
  * Lets a test confirm the operand / result lane layouts the kernels assume. */
 int s2h_probe_mfma_f64(const double *a, const double *b, double *out, s2h_stream stream);
+/* Sustained FP64 matrix-core rate (no counterpart in the reference): blocks x
+ * 256 threads, each wave issuing 8 independent V_MFMA_F64_16X16X4F64 (2048
+ * flops each) per iteration, iters iterations; out: blocks * 256 doubles. */
+int s2h_probe_mfma_f64_rate(int blocks, int iters, double *out, s2h_stream stream);
 /* The same for V_MFMA_F32_16X16X4F32 (64 floats in each of a, b; 256 out). */
 int s2h_probe_mfma_f32(const float *a, const float *b, float *out, s2h_stream stream);
 /* The same for V_MFMA_F32_16X16X32_BF16: a, b hold 8 bf16 bit patterns per

@@ -45,7 +45,7 @@ EXPORTED = (
     "s2h_batch_sumsq_workspace_bytes", "s2h_batch_sumsq",
     "s2h_ring_spectra_bytes", "s2h_ring_analysis", "s2h_forward_rings", "s2h_inverse_rings",
     "s2h_ring_residual", "s2h_flm_pack", "s2h_flm_unpack",
-    "s2h_forward_adjoint", "s2h_inverse_adjoint", "s2h_probe_mfma_f64",
+    "s2h_forward_adjoint", "s2h_inverse_adjoint", "s2h_probe_mfma_f64", "s2h_probe_mfma_f64_rate",
     "s2h_probe_mfma_f32", "s2h_forward_ex", "s2h_inverse_ex",
     "s2h_mlp_param_count", "s2h_mlp_precondition", "s2h_mlp_features",
     "s2h_mlp_grad_workspace_bytes", "s2h_mlp_grad", "s2h_mlp_adam", "s2h_probe_mfma_bf16",
@@ -77,6 +77,7 @@ def _declare(lib):
         "s2h_inverse_ex": (i32, [vp, vp, vp, vp, i32, i32, i32, vp, vp]),
         "s2h_forward_adjoint": (i32, [vp, vp, vp, vp, i32, vp]),
         "s2h_probe_mfma_f64": (i32, [vp, vp, vp, vp]),
+        "s2h_probe_mfma_f64_rate": (i32, [i32, i32, vp, vp]),
         "s2h_probe_mfma_f32": (i32, [vp, vp, vp, vp]),
         "s2h_probe_mfma_bf16": (i32, [vp, vp, vp, vp]),
         "s2h_mlp_param_count": (i32, []),

@@ -796,6 +796,11 @@ extern "C" int s2h_probe_mfma_f64(const double* a, const double* b, double* out,
   return hip_status(launch_mfma_f64_probe(a, b, out, static_cast<hipStream_t>(stream)));
 }
 
+extern "C" int s2h_probe_mfma_f64_rate(int blocks, int iters, double* out, s2h_stream stream) {
+  if (blocks < 1 || iters < 1 || !out) return S2H_ERR_INVALID;
+  return hip_status(launch_mfma_f64_rate(blocks, iters, out, static_cast<hipStream_t>(stream)));
+}
+
 extern "C" int s2h_probe_mfma_f32(const float* a, const float* b, float* out, s2h_stream stream) {
   if (!a || !b || !out) return S2H_ERR_INVALID;
   return hip_status(launch_mfma_f32_probe(a, b, out, static_cast<hipStream_t>(stream)));

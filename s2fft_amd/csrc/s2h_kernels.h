@@ -128,6 +128,8 @@ hipError_t launch_lat_inverse(const LatArgs& a, bool paired, hipStream_t s);
 hipError_t launch_lat_forward(const LatArgs& a, bool paired, hipStream_t s);
 hipError_t launch_lat_forward_mfma(const LatArgs& a, bool paired, bool fp32, hipStream_t s);
 hipError_t launch_mfma_f64_probe(const double* a, const double* b, double* out, hipStream_t s);
+// out: blocks * 256 doubles; 8 * iters * blocks * 4 V_MFMA_F64_16X16X4F64
+hipError_t launch_mfma_f64_rate(int blocks, int iters, double* out, hipStream_t s);
 hipError_t launch_lat_inverse_mma(const LatArgs& a, bool paired, bool fp32, hipStream_t s);
 hipError_t launch_lat_forward_bf16(const LatArgs& a, bool paired, int split, hipStream_t s);
 // ring groups per m of launch_lat_forward_bf16 (fill and mirror need 1)
