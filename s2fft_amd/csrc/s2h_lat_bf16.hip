@@ -82,7 +82,7 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
   constexpr bool LO = SPLIT == 3;
   // NI sets step together, sharing each coefficient load and giving the FP64
   // recursion NI independent chains.
-  constexpr int NI = LO ? 1 : 2;
+  constexpr int NI = LO ? 1 : 4;
   static_assert(NS % NI == 0, "set groups");
   constexpr int kTileE = 32 * kFRow;               // one [l_local 32][pair 64 + pad] BF16 tile
   constexpr int kTileB = kTileE * 2;
