@@ -207,6 +207,37 @@ int s2h_flm_unpack(int L, const void *flm_packed, void *flm, int batch, s2h_stre
 int s2h_ring_residual(const s2h_plan *plan, const void *G, const void *Y, void *R, int batch,
                       s2h_stream stream);
 
+/* Conjugate-gradient refinement on the ring spectra (no counterpart in the
+ * reference, whose refinement is Jacobi, s2fft/utils/iterative_refinement.py:9-46).
+ * F I is Hermitian positive definite on the coefficients (F = w I^H with the
+ * HEALPix / GL quadrature), so the forward is refined by CG on F I x = F f.
+ *   s2h_ring_residual_scaled: R = G - alpha[b] A(Y) (alpha: batch doubles on
+ *     the device; null = 1; R may be G, not Y).
+ *   s2h_ring_dot: pq[b] = <p, F A(Y)> for Y = s2h_inverse_rings(p), i.e.
+ *     sum over rings and m of c_m Re(conj(Y) A(Y)), c_m = 2 for m > 0 of real
+ *     maps, else 1 (the CG curvature, without projecting A(Y)); with alpha,
+ *     alpha[b] = num[b] / pq[b].  pq or alpha may be null (not both).
+ *     work: s2h_ring_dot_workspace_bytes(plan, batch).
+ *   s2h_flm_dots: out[i * batch + b] = <z_b, v_i,b> = sum_j w_j Re(conj(z) v_i)
+ *     over n coefficients per map (maps contiguous), 1 <= nv <= 4 (v: host
+ *     array of device pointers); w_j = 1 for j < packed_L and 2 beyond (the
+ *     packed layout of real maps: the m = 0 block first), all 1 when
+ *     packed_L = 0.  Fixed summation order.  work: s2h_flm_dots_workspace_bytes.
+ *   s2h_cg_update: x += alpha[b] p; then, if p_new is given,
+ *     p_new = z - g1[b] p - g2[b] p_prev (p_prev and g2 optional; p_new may
+ *     be z, not p or x). */
+int s2h_ring_residual_scaled(const s2h_plan *plan, const void *G, const void *Y,
+                             const double *alpha, void *R, int batch, s2h_stream stream);
+size_t s2h_ring_dot_workspace_bytes(const s2h_plan *plan, int batch);
+int s2h_ring_dot(const s2h_plan *plan, const void *Y, const double *num, double *pq,
+                 double *alpha, void *work, int batch, s2h_stream stream);
+size_t s2h_flm_dots_workspace_bytes(int nv, int batch);
+int s2h_flm_dots(const void *z, const void *const *v, int nv, int64_t n, int packed_L, int batch,
+                 double *out, void *work, s2h_stream stream);
+int s2h_cg_update(void *x, const void *p, const double *alpha, void *p_new, const void *z,
+                  const double *g1, const void *p_prev, const double *g2, int64_t n, int batch,
+                  s2h_stream stream);
+
 /* Per-map sums of squares out[b] = sum_{i < n} x[b * stride + i]^2 (doubles;
  * a complex map is 2n doubles), with a fixed summation order.  No
  * counterpart in the reference: the residual norms that judge the learned

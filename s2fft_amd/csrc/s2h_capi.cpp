@@ -467,13 +467,9 @@ extern "C" int s2h_flm_unpack(int L, const void* flm_p, void* flm, int batch, s2
                                       batch, static_cast<hipStream_t>(stream)));
 }
 
-extern "C" int s2h_ring_residual(const s2h_plan* p, const void* G, const void* Y, void* R,
-                                 int batch, s2h_stream stream) {
-  if (!p || !G || !Y || !R || batch < 0 || Y == R) return S2H_ERR_INVALID;
-  if (!spectral_ok(p)) return S2H_ERR_UNSUPPORTED;
-  if (batch == 0) return S2H_OK;
-  int st = check_device(p);
-  if (st) return st;
+namespace {
+
+RingAliasArgs alias_args(const s2h_plan* p, const void* G, const void* Y, void* R) {
   RingAliasArgs a{};
   a.y = static_cast<const cplx*>(Y);
   a.g = static_cast<const cplx*>(G);
@@ -484,8 +480,47 @@ extern "C" int s2h_ring_residual(const s2h_plan* p, const void* G, const void* Y
   a.herm = p->d.reality ? 1 : 0;
   a.apply_phase = p->d.sampling == S2H_HEALPIX;
   a.L = p->d.L;
+  return a;
+}
+
+}  // namespace
+
+extern "C" int s2h_ring_residual_scaled(const s2h_plan* p, const void* G, const void* Y,
+                                        const double* alpha, void* R, int batch,
+                                        s2h_stream stream) {
+  if (!p || !G || !Y || !R || batch < 0 || Y == R) return S2H_ERR_INVALID;
+  if (!spectral_ok(p)) return S2H_ERR_UNSUPPORTED;
+  if (batch == 0) return S2H_OK;
+  int st = check_device(p);
+  if (st) return st;
+  RingAliasArgs a = alias_args(p, G, Y, R);
+  a.alpha = alpha;
   ProfScope ps(S2H_KERNEL_RING_RESIDUAL, static_cast<hipStream_t>(stream));
   return hip_status(launch_ring_alias(a, p->fwd_set, batch, static_cast<hipStream_t>(stream)));
+}
+
+extern "C" int s2h_ring_residual(const s2h_plan* p, const void* G, const void* Y, void* R,
+                                 int batch, s2h_stream stream) {
+  return s2h_ring_residual_scaled(p, G, Y, nullptr, R, batch, stream);
+}
+
+extern "C" size_t s2h_ring_dot_workspace_bytes(const s2h_plan* p, int batch) {
+  if (!p || batch <= 0 || !spectral_ok(p)) return 0;
+  return (size_t)batch * (size_t)ring_dot_jobs(p->fwd_set) * sizeof(double);
+}
+
+extern "C" int s2h_ring_dot(const s2h_plan* p, const void* Y, const double* num, double* pq,
+                            double* alpha, void* work, int batch, s2h_stream stream) {
+  if (!p || !Y || batch < 0 || (!pq && !alpha) || (alpha && !num)) return S2H_ERR_INVALID;
+  if (!spectral_ok(p)) return S2H_ERR_UNSUPPORTED;
+  if (batch == 0) return S2H_OK;
+  if (!work) return S2H_ERR_INVALID;
+  int st = check_device(p);
+  if (st) return st;
+  const RingAliasArgs a = alias_args(p, Y, Y, nullptr);
+  ProfScope ps(S2H_KERNEL_RING_RESIDUAL, static_cast<hipStream_t>(stream));
+  return hip_status(launch_ring_dot(a, p->fwd_set, batch, static_cast<double*>(work), num, pq,
+                                    alpha, static_cast<hipStream_t>(stream)));
 }
 
 // Adjoint of s2h_forward (the VJP of a complex-linear map, torch convention):
@@ -788,6 +823,53 @@ extern "C" int s2h_batch_sumsq(const double* x, int64_t n, int64_t stride, int b
     return S2H_ERR_INVALID;
   return hip_status(launch_batch_sumsq(x, n, stride, batch, static_cast<double*>(work), out,
                                        static_cast<hipStream_t>(stream)));
+}
+
+extern "C" size_t s2h_flm_dots_workspace_bytes(int nv, int batch) {
+  if (nv < 1 || nv > kFlmDotMax || batch <= 0) return 0;
+  return (size_t)nv * (size_t)batch * kFlmDotBlocks * sizeof(double);
+}
+
+extern "C" int s2h_flm_dots(const void* z, const void* const* v, int nv, int64_t n, int packed_L,
+                            int batch, double* out, void* work, s2h_stream stream) {
+  if (batch < 0 || n < 0 || nv < 1 || nv > kFlmDotMax || packed_L < 0) return S2H_ERR_INVALID;
+  if (batch == 0) return S2H_OK;
+  if (!out || !work || !v || (n > 0 && !z)) return S2H_ERR_INVALID;
+  FlmDotArgs a{};
+  a.z = static_cast<const cplx*>(z);
+  for (int i = 0; i < nv; ++i) {
+    if (n > 0 && !v[i]) return S2H_ERR_INVALID;
+    a.v[i] = static_cast<const cplx*>(v[i]);
+  }
+  a.nv = nv;
+  a.n = n;
+  a.stride = n;
+  a.packed_L = packed_L;
+  a.partial = static_cast<double*>(work);
+  a.out = out;
+  return hip_status(launch_flm_dots(a, batch, static_cast<hipStream_t>(stream)));
+}
+
+extern "C" int s2h_cg_update(void* x, const void* p, const double* alpha, void* p_new,
+                             const void* z, const double* g1, const void* p_prev,
+                             const double* g2, int64_t n, int batch, s2h_stream stream) {
+  if (batch < 0 || n < 0) return S2H_ERR_INVALID;
+  if (batch == 0 || n == 0) return S2H_OK;
+  if (!x || !p || !alpha || (p_new && (!z || !g1)) || (p_prev && (!g2 || !p_new)) || p_new == p ||
+      p_new == x)
+    return S2H_ERR_INVALID;
+  CgUpdateArgs a{};
+  a.x = static_cast<cplx*>(x);
+  a.p = static_cast<const cplx*>(p);
+  a.alpha = alpha;
+  a.p_new = static_cast<cplx*>(p_new);
+  a.z = static_cast<const cplx*>(z);
+  a.g1 = g1;
+  a.p_prev = static_cast<const cplx*>(p_prev);
+  a.g2 = g2;
+  a.n = n;
+  a.stride = n;
+  return hip_status(launch_cg_update(a, batch, static_cast<hipStream_t>(stream)));
 }
 
 extern "C" int s2h_probe_mfma_f64(const double* a, const double* b, double* out,

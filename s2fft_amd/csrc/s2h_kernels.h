@@ -58,6 +58,8 @@ struct RingAliasArgs {
   int herm;                 // y stores m >= 0 of Hermitian spectra
   int apply_phase;
   int L;
+  const double* alpha;      // per-map scale of A(Y) in R = G - alpha A(Y) (null: 1)
+  double* part;             // dot mode: [map][ring job] partial sums
 };
 
 struct LatArgs {
@@ -117,6 +119,11 @@ hipError_t launch_ring_fft_inverse(const RingInvArgs& a, const JobSet& js, int b
                                    hipStream_t s);
 hipError_t allow_large_lds_rings();
 hipError_t launch_ring_alias(const RingAliasArgs& a, const JobSet& js, int batch, hipStream_t s);
+// pq[b] = sum_m c_m Re(conj(Y) A(Y)) over every ring of map b (part: batch x
+// ring_dot_jobs(js) doubles); alpha[b] = num[b] / pq[b] when alpha is given
+int ring_dot_jobs(const JobSet& js);
+hipError_t launch_ring_dot(const RingAliasArgs& a, const JobSet& js, int batch, double* part,
+                           const double* num, double* pq, double* alpha, hipStream_t s);
 hipError_t launch_chirp_setup(const int* lengths, const int* conv_lens, const int64_t* offs,
                               cplx* pool, int count, int max_conv, hipStream_t s);
 hipError_t launch_gamma_dft(int N, int64_t npix, int mode, double scale, const void* in,
@@ -186,6 +193,32 @@ struct PoleArgs {
 hipError_t launch_pole_inverse(const PoleArgs& a, hipStream_t s);
 hipError_t launch_pole_forward(const PoleArgs& a, hipStream_t s);
 hipError_t launch_zero(void* p, int64_t ndoubles, hipStream_t s);
+constexpr int kFlmDotMax = 4;      // vectors per launch_flm_dots
+constexpr int kFlmDotBlocks = 256; // slices per map of launch_flm_dots
+struct FlmDotArgs {
+  const cplx* z;
+  const cplx* v[kFlmDotMax];
+  int nv;
+  int64_t n;                // coefficients per map
+  int64_t stride;           // between maps
+  int packed_L;             // > 0: packed m-major layout of real maps (weights 1 / 2)
+  double* partial;          // nv * batch * kFlmDotBlocks
+  double* out;              // [i][b]
+};
+hipError_t launch_flm_dots(const FlmDotArgs& a, int batch, hipStream_t s);
+struct CgUpdateArgs {
+  cplx* x;
+  const cplx* p;
+  const double* alpha;
+  cplx* p_new;              // null: x only
+  const cplx* z;
+  const double* g1;
+  const cplx* p_prev;       // optional second direction
+  const double* g2;
+  int64_t n;
+  int64_t stride;
+};
+hipError_t launch_cg_update(const CgUpdateArgs& a, int batch, hipStream_t s);
 constexpr int kSumsqBlocks = 256;  // slices per map of launch_batch_sumsq
 // out[b] = sum_{i < n} x[b * stride + i]^2; partial holds kSumsqBlocks * batch doubles
 hipError_t launch_batch_sumsq(const double* x, int64_t n, int64_t stride, int batch,

@@ -229,6 +229,95 @@ hipError_t launch_batch_sumsq(const double* x, int64_t n, int64_t stride, int ba
   return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ Krylov helpers
+// Per-map inner products <z, v_i> = sum_j w_j Re(conj(z_j) v_{i,j}) of
+// coefficient vectors, w_j = 1 for j < packed_L (the m = 0 block of the
+// packed m-major layout) and 2 beyond it, or 1 everywhere when packed_L = 0
+// (full layouts hold both signs of m).  Slices of n per workgroup, then a
+// fixed-order sum of the slices: the result does not depend on scheduling.
+__global__ void __launch_bounds__(kThreads) flm_dots_partial_kernel(FlmDotArgs a) {
+  __shared__ double red[kThreads / 64];
+  const int64_t base = (int64_t)blockIdx.y * a.stride;
+  const cplx* __restrict__ z = a.z + base;
+  const int64_t per = (a.n + gridDim.x - 1) / gridDim.x;
+  const int64_t lo = (int64_t)blockIdx.x * per;
+  const int64_t hi = lo + per < a.n ? lo + per : a.n;
+  double acc[kFlmDotMax];
+#pragma unroll
+  for (int i = 0; i < kFlmDotMax; ++i) acc[i] = 0.0;
+  for (int64_t j = lo + threadIdx.x; j < hi; j += kThreads) {
+    const cplx zv = z[j];
+    const double w = (a.packed_L > 0 && j >= a.packed_L) ? 2.0 : 1.0;
+#pragma unroll
+    for (int i = 0; i < kFlmDotMax; ++i) {
+      if (i < a.nv) {
+        const cplx v = a.v[i][base + j];
+        acc[i] = fma(w, fma(zv.x, v.x, zv.y * v.y), acc[i]);
+      }
+    }
+  }
+  for (int i = 0; i < a.nv; ++i) {
+    const double t = block_sum(acc[i], red);
+    if (threadIdx.x == 0)
+      a.partial[((int64_t)i * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) flm_dots_final_kernel(const double* partial, int nblk,
+                                                                  double* out) {
+  __shared__ double red[kThreads / 64];
+  double a = 0.0;
+  for (int k = threadIdx.x; k < nblk; k += kThreads) a += partial[(int64_t)blockIdx.x * nblk + k];
+  const double t = block_sum(a, red);
+  if (threadIdx.x == 0) out[blockIdx.x] = t;
+}
+
+hipError_t launch_flm_dots(const FlmDotArgs& a, int batch, hipStream_t s) {
+  if (batch <= 0 || a.nv <= 0) return hipSuccess;
+  hipLaunchKernelGGL(flm_dots_partial_kernel, dim3(kFlmDotBlocks, batch), dim3(kThreads), 0, s, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(flm_dots_final_kernel, dim3(a.nv * batch), dim3(kThreads), 0, s, a.partial,
+                     kFlmDotBlocks, a.out);
+  return hipGetLastError();
+}
+
+// x += alpha p; p_new = z - g1 p - g2 p_prev (p_new may be z; p_prev, g2 optional;
+// p_new null: only x)
+__global__ void __launch_bounds__(kThreads) cg_update_kernel(CgUpdateArgs a) {
+  const int b = blockIdx.y;
+  const int64_t base = (int64_t)b * a.stride;
+  const double al = a.alpha[b];
+  const double g1 = a.p_new ? a.g1[b] : 0.0;
+  const double g2 = a.p_prev ? a.g2[b] : 0.0;
+  for (int64_t j = (int64_t)blockIdx.x * kThreads + threadIdx.x; j < a.n;
+       j += (int64_t)gridDim.x * kThreads) {
+    const cplx pv = a.p[base + j];
+    cplx xv = a.x[base + j];
+    xv.x = fma(al, pv.x, xv.x);
+    xv.y = fma(al, pv.y, xv.y);
+    a.x[base + j] = xv;
+    if (a.p_new) {
+      const cplx zv = a.z[base + j];
+      cplx nv = {fma(-g1, pv.x, zv.x), fma(-g1, pv.y, zv.y)};
+      if (a.p_prev) {
+        const cplx qv = a.p_prev[base + j];
+        nv.x = fma(-g2, qv.x, nv.x);
+        nv.y = fma(-g2, qv.y, nv.y);
+      }
+      a.p_new[base + j] = nv;
+    }
+  }
+}
+
+hipError_t launch_cg_update(const CgUpdateArgs& a, int batch, hipStream_t s) {
+  if (batch <= 0 || a.n <= 0) return hipSuccess;
+  const int64_t want = (a.n + kThreads * 4 - 1) / (kThreads * 4);
+  const unsigned gx = (unsigned)(want < 1024 ? want : 1024);
+  hipLaunchKernelGGL(cg_update_kernel, dim3(gx, batch), dim3(kThreads), 0, s, a);
+  return hipGetLastError();
+}
+
 }  // namespace s2h
 
 namespace s2h {

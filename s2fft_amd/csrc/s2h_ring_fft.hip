@@ -627,7 +627,14 @@ __device__ __forceinline__ cplx alias_term(const cplx* y, int m, int m_first, bo
 // floor(a / n) for n > 0
 __device__ __forceinline__ int floor_div(int a, int n) { return a >= 0 ? a / n : -((n - 1 - a) / n); }
 
-template <int TPB>
+// DOT: instead of writing R, the workgroup adds up
+//   sum_m c_m Re(conj(Y[m]) A(Y)[m]),   c_m = 2 for m > 0 of Hermitian spectra, else 1,
+// and stores it in a.part[map * jobs + job]: summed over rings this is
+// <p, Lat_fwd(A(Lat_inv p))> for Y = Lat_inv(p) (the latitudinal transforms
+// are transposes of each other), the curvature a conjugate-gradient step needs
+// without projecting A(Y).  With a.alpha the written residual is
+// R = G - alpha[map] A(Y).
+template <int TPB, bool DOT>
 __global__ void __launch_bounds__(TPB) ring_alias_kernel(RingAliasArgs a) {
   __shared__ double part[2 * TPB];
   const RingJob job = a.jobs[blockIdx.x];
@@ -636,88 +643,136 @@ __global__ void __launch_bounds__(TPB) ring_alias_kernel(RingAliasArgs a) {
   const cplx* __restrict__ y = a.y + off;
   const cplx* g = a.g + off;
   cplx* r = a.r + off;
-  const double wn = job.weight * n;
+  const double wn = job.weight * n * (a.alpha && !DOT ? a.alpha[blockIdx.y] : 1.0);
   const int lo = -(a.L - 1), hi = a.L - 1;
   const bool herm = a.herm != 0;
   const int t = threadIdx.x;
+  double dsum = 0.0;
   if (n > hi - lo) {
     // no aliasing: the column's own m is the only term
-    batched_for<TPB, 8, cplx>(
-        a.ncol, [&](int col) { return g[col]; },
-        [&](int col, cplx gv) {
-          const int m = a.m_first + col;
-          cplx v = {0.0, 0.0};
-          if (m >= lo && m <= hi) v = alias_term(y, m, a.m_first, herm);
-          r[col] = {gv.x - wn * v.x, gv.y - wn * v.y};
-        });
-    return;
-  }
-  const int e = (int)rint(n * job.phi0_over_pi);
-  const double sgn = (a.apply_phase && (e & 1)) ? -1.0 : 1.0;
-  const int G = n >= TPB ? 1 : TPB / n;
-  const int nq = n >= TPB ? (n + TPB - 1) / TPB : 1;  // residues per thread
-  const int gi = n >= TPB ? 0 : t / n;
-  const bool active = gi < G;
-  const int m_last = a.m_first + a.ncol - 1;
-  for (int qi = 0; qi < nq; ++qi) {
-    const int q = n >= TPB ? t + qi * TPB : t % n;
-    const bool own = active && q < n;
-    // T[q]: j from j0 = ceil((lo - q) / n) to floor((hi - q) / n), this
-    // thread's j = j0 + gi + k G
-    cplx acc = {0.0, 0.0};
-    int j0 = 0;
-    if (own) {
-      j0 = -floor_div(q - lo, n);
-      const int j1 = floor_div(hi - q, n);
-      const int cnt = j1 - j0 + 1;
-      for (int k0 = gi; k0 < cnt; k0 += 4 * G) {
-        cplx v[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int k = k0 + u * G;
-          v[u] = k < cnt ? alias_term(y, q + (j0 + k) * n, a.m_first, herm) : cplx{0.0, 0.0};
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int k = k0 + u * G;
-          const double sj = ((j0 + k) & 1) && sgn < 0 ? -1.0 : 1.0;
-          acc = {fma(sj, v[u].x, acc.x), fma(sj, v[u].y, acc.y)};
-        }
-      }
+    if constexpr (DOT) {
+      batched_for<TPB, 8, cplx>(
+          a.ncol, [&](int col) { return y[col]; },
+          [&](int col, cplx yv) {
+            const int m = a.m_first + col;
+            if (m >= lo && m <= hi)
+              dsum += (herm && m > 0 ? 2.0 * wn : wn) * (yv.x * yv.x + yv.y * yv.y);
+          });
+    } else {
+      batched_for<TPB, 8, cplx>(
+          a.ncol, [&](int col) { return g[col]; },
+          [&](int col, cplx gv) {
+            const int m = a.m_first + col;
+            cplx v = {0.0, 0.0};
+            if (m >= lo && m <= hi) v = alias_term(y, m, a.m_first, herm);
+            r[col] = {gv.x - wn * v.x, gv.y - wn * v.y};
+          });
+      return;
     }
-    if (G > 1) {
+  } else {
+    const int e = (int)rint(n * job.phi0_over_pi);
+    const double sgn = (a.apply_phase && (e & 1)) ? -1.0 : 1.0;
+    const int G = n >= TPB ? 1 : TPB / n;
+    const int nq = n >= TPB ? (n + TPB - 1) / TPB : 1;  // residues per thread
+    const int gi = n >= TPB ? 0 : t / n;
+    const bool active = gi < G;
+    const int m_last = a.m_first + a.ncol - 1;
+    for (int qi = 0; qi < nq; ++qi) {
+      const int q = n >= TPB ? t + qi * TPB : t % n;
+      const bool own = active && q < n;
+      // T[q]: j from j0 = ceil((lo - q) / n) to floor((hi - q) / n), this
+      // thread's j = j0 + gi + k G
+      cplx acc = {0.0, 0.0};
+      int j0 = 0;
       if (own) {
-        part[2 * t] = acc.x;
-        part[2 * t + 1] = acc.y;
-      }
-      __syncthreads();
-      cplx tot = {0.0, 0.0};
-      if (own)
-        for (int gg = 0; gg < G; ++gg)
-          tot = cadd(tot, cplx{part[2 * (gg * n + q)], part[2 * (gg * n + q) + 1]});
-      __syncthreads();
-      acc = tot;
-    }
-    if (!own) continue;
-    // outputs m = q + j n in [m_first, m_last], this thread's j = jo + gi + k G
-    const int jo = -floor_div(q - a.m_first, n);
-    const int cnt = floor_div(m_last - q, n) - jo + 1;
-    for (int k0 = gi; k0 < cnt; k0 += 4 * G) {
-      cplx gv[4];
+        j0 = -floor_div(q - lo, n);
+        const int j1 = floor_div(hi - q, n);
+        const int cnt = j1 - j0 + 1;
+        for (int k0 = gi; k0 < cnt; k0 += 4 * G) {
+          cplx v[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int k = k0 + u * G;
-        gv[u] = k < cnt ? g[q + (jo + k) * n - a.m_first] : cplx{0.0, 0.0};
-      }
+          for (int u = 0; u < 4; ++u) {
+            const int k = k0 + u * G;
+            v[u] = k < cnt ? alias_term(y, q + (j0 + k) * n, a.m_first, herm) : cplx{0.0, 0.0};
+          }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int k = k0 + u * G;
-        if (k < cnt) {
-          const double c = (((jo + k) & 1) && sgn < 0 ? -wn : wn);
-          r[q + (jo + k) * n - a.m_first] = {gv[u].x - c * acc.x, gv[u].y - c * acc.y};
+          for (int u = 0; u < 4; ++u) {
+            const int k = k0 + u * G;
+            const double sj = ((j0 + k) & 1) && sgn < 0 ? -1.0 : 1.0;
+            acc = {fma(sj, v[u].x, acc.x), fma(sj, v[u].y, acc.y)};
+          }
+        }
+      }
+      if (G > 1) {
+        if (own) {
+          part[2 * t] = acc.x;
+          part[2 * t + 1] = acc.y;
+        }
+        __syncthreads();
+        cplx tot = {0.0, 0.0};
+        if (own)
+          for (int gg = 0; gg < G; ++gg)
+            tot = cadd(tot, cplx{part[2 * (gg * n + q)], part[2 * (gg * n + q) + 1]});
+        __syncthreads();
+        acc = tot;
+      }
+      if (!own) continue;
+      // outputs m = q + j n in [m_first, m_last], this thread's j = jo + gi + k G
+      const int jo = -floor_div(q - a.m_first, n);
+      const int cnt = floor_div(m_last - q, n) - jo + 1;
+      const cplx* src = DOT ? y : g;
+      for (int k0 = gi; k0 < cnt; k0 += 4 * G) {
+        cplx gv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int k = k0 + u * G;
+          gv[u] = k < cnt ? src[q + (jo + k) * n - a.m_first] : cplx{0.0, 0.0};
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int k = k0 + u * G;
+          if (k < cnt) {
+            const double c = (((jo + k) & 1) && sgn < 0 ? -wn : wn);
+            if constexpr (DOT) {
+              const int m = q + (jo + k) * n;
+              dsum += (herm && m > 0 ? 2.0 * c : c) * (gv[u].x * acc.x + gv[u].y * acc.y);
+            } else {
+              r[q + (jo + k) * n - a.m_first] = {gv[u].x - c * acc.x, gv[u].y - c * acc.y};
+            }
+          }
         }
       }
     }
+  }
+  if constexpr (DOT) {
+    // fixed-order workgroup sum: waves by shuffles, then wave 0 over the waves
+    for (int o = 32; o > 0; o >>= 1) dsum += __shfl_xor(dsum, o);
+    __syncthreads();
+    if ((t & 63) == 0) part[t >> 6] = dsum;
+    __syncthreads();
+    if (t == 0) {
+      double tot = 0.0;
+      for (int w = 0; w < TPB / 64; ++w) tot += part[w];
+      a.part[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = tot;
+    }
+  }
+}
+
+// pq[b] = sum over the rings' partial sums in ring order; alpha[b] = num[b] / pq[b]
+__global__ void __launch_bounds__(256) ring_dot_final_kernel(const double* part, int njobs,
+                                                             const double* num, double* pq,
+                                                             double* alpha) {
+  __shared__ double red[4];
+  const int b = blockIdx.x;
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < njobs; k += 256) acc += part[(int64_t)b * njobs + k];
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double tot = (red[0] + red[1]) + (red[2] + red[3]);
+    if (pq) pq[b] = tot;
+    if (alpha) alpha[b] = (num && tot != 0.0) ? num[b] / tot : 0.0;
   }
 }
 
@@ -801,7 +856,28 @@ hipError_t launch_ring_alias(const RingAliasArgs& a, const JobSet& js, int batch
   if (batch == 0 || njobs == 0) return hipSuccess;
   RingAliasArgs b = a;
   b.jobs = js.jobs;
-  hipLaunchKernelGGL(ring_alias_kernel<kTpbAlias>, dim3(njobs, batch), dim3(kTpbAlias), 0, s, b);
+  b.part = nullptr;
+  hipLaunchKernelGGL((ring_alias_kernel<kTpbAlias, false>), dim3(njobs, batch), dim3(kTpbAlias), 0,
+                     s, b);
+  return hipGetLastError();
+}
+
+int ring_dot_jobs(const JobSet& js) { return js.n_pow2 + js.n_smooth + js.n_blue; }
+
+hipError_t launch_ring_dot(const RingAliasArgs& a, const JobSet& js, int batch, double* part,
+                           const double* num, double* pq, double* alpha, hipStream_t s) {
+  const int njobs = ring_dot_jobs(js);
+  if (batch == 0 || njobs == 0) return hipSuccess;
+  RingAliasArgs b = a;
+  b.jobs = js.jobs;
+  b.part = part;
+  b.alpha = nullptr;
+  hipLaunchKernelGGL((ring_alias_kernel<kTpbAlias, true>), dim3(njobs, batch), dim3(kTpbAlias), 0,
+                     s, b);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(ring_dot_final_kernel, dim3(batch), dim3(256), 0, s, part, njobs, num, pq,
+                     alpha);
   return hipGetLastError();
 }
 

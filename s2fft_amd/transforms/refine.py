@@ -115,6 +115,118 @@ def refine_mixed(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=N
     return unpack(flm)
 
 
+def refine_cg(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=None, precond=None,
+              learned_steps=0, init_precision="bf16", proj_precision=("bf16", "bf16", "bf16x1"),
+              synth_precision="bf16", history=None, flm_true=None, stats=None):
+    """Conjugate-gradient refinement of the forward transform on the ring spectra.
+
+    F I is Hermitian positive definite on the coefficients (the HEALPix and
+    GL forwards are the quadrature-weighted adjoints of the inverses), so
+    flm solves F I flm = F f, and CG reaches a given error in fewer
+    transform pairs than the reference's Jacobi iteration
+    (s2fft/utils/iterative_refinement.py:9-46).  With G the ring spectra of f:
+
+        x_0 = F~(G);  R = G - A(I(x_0))            (FP64 synthesis)
+        r_0 = F~(R);  z_0 = P(r_0) or r_0;  p_0 = z_0
+        step k:  Y = I~(p_k);  pq_k = <p_k, F A(Y)>  (a ring-space sum)
+                 alpha_k = <r_k, z_k> / pq_k;  x += alpha_k p_k
+                 R -= alpha_k A(Y);  r_{k+1} = F~(R);  z = P(r_{k+1}) or r_{k+1}
+                 p_{k+1} = z - sum_j <z, q_j> / pq_j p_j   (j = k, k - 1),
+                 q_j = F I p_j = (r_j - r_{j+1}) / alpha_j
+
+    R stays FP64 and takes each direction's synthesis, so every r_k is the
+    projection of a residual that is accurate to the reduced synthesis' error
+    times |alpha p| (not a running sum of reduced-precision products).  The
+    directions are orthogonalised against the last two (flexible CG, for a
+    nonlinear P).  The last step forms no residual.
+
+    F~ / I~ run at proj_precision (one entry per residual r_0, r_1, ...; the
+    last repeats) / synth_precision; x_0 at init_precision.  precond: the
+    learned preconditioner for the first ``learned_steps`` residuals; a
+    learned direction is kept for a map only where that step lowers the norm
+    of its FP64 ring residual R, else the map redoes the step along its CG
+    direction (stats counts accepted / judged).  Needs a plan with one ring
+    grid (HEALPix, GL).  Returns flm (B, L, 2L-1).
+    """
+    s = sampling.lower()
+    plan = plan or sph.get_plan(s, L, nside, spin, L_lower, reality, device=f.device)
+    if not sph.spectra_supported(plan):
+        raise ValueError("refine_cg needs a sampling whose forward and inverse share one ring grid")
+    packed = bool(reality)
+    pL = L if packed else 0
+    unpack = (lambda x: sph.flm_unpack_hip(x, L)) if packed else (lambda x: x)
+    lmin = max(abs(spin), L_lower)
+
+    def proj(R, k):
+        prec = proj_precision[min(k, len(proj_precision) - 1)]
+        return sph.forward_rings_hip(R, L, plan, prec, packed=packed)
+
+    def precondition(r, k):
+        if precond is not None and k < learned_steps:
+            return precond.apply(r, L, reality, lmin, packed=packed)
+        return r
+
+    R = sph.ring_analysis_hip(f, plan)
+    x = sph.forward_rings_hip(R, L, plan, init_precision, packed=packed)
+    _track(history, x, flm_true, unpack)
+    if n_iter <= 0:
+        return unpack(x)
+    # R = G - A(I(x_0)) in place of G
+    sph.ring_residual_hip(R, sph.inverse_rings_hip(x, plan, "fp64", packed), plan, out=R)
+    r = proj(R, 0)
+    z = precondition(r, 0)
+    rz = sph.flm_dots_hip(z, [r], pL)[0]
+    rs, ps, rz_prev = [r], [z.clone() if z is r else z], None
+    for k in range(n_iter):
+        p = ps[-1]
+        Y = sph.inverse_rings_hip(p, plan, synth_precision, packed)
+        pq, alpha = sph.ring_dot_hip(Y, plan, num=rz)
+        if k + 1 == n_iter:
+            sph.cg_update_hip(x, p, alpha)
+            _track(history, x, flm_true, unpack)
+            break
+        judged = precond is not None and k < learned_steps
+        n_old = _sumsq(R) if judged else None
+        sph.ring_residual_hip(R, Y, plan, out=R, alpha=alpha)
+        if judged:
+            ok = _sumsq(R) < n_old
+            if stats is not None:
+                stats["accepted"] = stats.get("accepted", 0) + int(ok.sum())
+                stats["steps"] = stats.get("steps", 0) + int(ok.numel())
+            if not bool(ok.all()):
+                _redo_plain((~ok).nonzero().flatten(), R, Y, p, alpha, rz, rs[-1], plan,
+                            synth_precision, packed, pL)
+        r_new = proj(R, k + 1)
+        z = precondition(r_new, k + 1)
+        # d = <z, r_k>, <z, r_{k+1}> [, <z, r_{k-1}>]; alpha_j pq_j = <r_j, z_j>
+        d = sph.flm_dots_hip(z, [rs[-1], r_new] + rs[-2:-1], pL)
+        g1 = (d[0] - d[1]) / rz
+        g2 = (d[2] - d[0]) / rz_prev if len(rs) > 1 else None
+        p_new = ps[0] if len(ps) == 3 else torch.empty_like(p)
+        sph.cg_update_hip(x, p, alpha, p_new, z, g1, ps[-2] if len(ps) > 1 else None, g2)
+        _track(history, x, flm_true, unpack)
+        rz_prev, rz = rz, d[1]
+        rs = rs[-1:] + [r_new]
+        ps = ps[-2:] + [p_new]
+    return unpack(x)
+
+
+def _redo_plain(bad, R, Y, p, alpha, rz, r, plan, synth_precision, packed, pL):
+    """Judge's fallback of refine_cg: maps ``bad`` take back their learned
+    step's residual update (R += alpha A(Y)) and step along r instead; R and
+    the maps' rows of p, alpha and rz are replaced in place (x is updated
+    afterwards with the replaced alpha and p)."""
+    Rb = R[bad].contiguous()
+    sph.ring_residual_hip(Rb, Y[bad].contiguous(), plan, out=Rb, alpha=-alpha[bad])
+    rb = r[bad].contiguous()
+    rzb = sph.flm_dots_hip(rb, [rb], pL)[0]
+    Yb = sph.inverse_rings_hip(rb, plan, synth_precision, packed)
+    _, ab = sph.ring_dot_hip(Yb, plan, num=rzb)
+    sph.ring_residual_hip(Rb, Yb, plan, out=Rb, alpha=ab)
+    R[bad], p[bad] = Rb, rb
+    alpha[bad], rz[bad] = ab, rzb
+
+
 def _judged_step(precond, h, r, rn, f, flm, kw, stats, synth, ip, exact, packed=False):
     """Learned step per map where it lowers the FP64-held residual, else Jacobi.
 

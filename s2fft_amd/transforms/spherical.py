@@ -10,6 +10,7 @@ tensors (the result stays on the tensor's device); a leading batch axis is
 accepted and transformed in one launch per kernel.
 """
 
+import ctypes
 import threading
 
 import numpy as np
@@ -332,13 +333,76 @@ def flm_unpack_hip(flm_p, L):
     return out
 
 
-def ring_residual_hip(G, Y, plan, out=None):
-    """R = G - A(Y) (out may be G)."""
+def ring_residual_hip(G, Y, plan, out=None, alpha=None):
+    """R = G - A(Y), or G - alpha[b] A(Y) with a per-map float64 device tensor
+    ``alpha`` (out may be G)."""
     R = torch.empty_like(G) if out is None else out
-    _lib.check(_lib.lib().s2h_ring_residual(plan.handle, G.contiguous().data_ptr(),
-                                            Y.contiguous().data_ptr(), R.data_ptr(), G.shape[0],
-                                            _stream(G.device)), "ring residual")
+    a = 0
+    if alpha is not None:
+        if alpha.dtype != torch.float64 or alpha.numel() != G.shape[0] or alpha.device != G.device:
+            raise ValueError("alpha: one float64 per map on the spectra's device")
+        a = alpha.contiguous().data_ptr()
+    _lib.check(_lib.lib().s2h_ring_residual_scaled(plan.handle, G.contiguous().data_ptr(),
+                                                   Y.contiguous().data_ptr(), a, R.data_ptr(),
+                                                   G.shape[0], _stream(G.device)),
+               "ring residual")
     return R
+
+
+def ring_dot_hip(Y, plan, num=None):
+    """pq[b] = <p, F A(Y)> for Y = inverse_rings_hip(p) (the curvature of a
+    conjugate-gradient step); with ``num`` also alpha = num / pq.
+    Returns pq, or (pq, alpha)."""
+    B = Y.shape[0]
+    lib = _lib.lib()
+    pq = torch.empty(B, dtype=torch.float64, device=Y.device)
+    alpha = torch.empty_like(pq) if num is not None else None
+    work = torch.empty(max(int(lib.s2h_ring_dot_workspace_bytes(plan.handle, B)), 8),
+                       dtype=torch.uint8, device=Y.device)
+    _lib.check(lib.s2h_ring_dot(plan.handle, Y.contiguous().data_ptr(),
+                                num.contiguous().data_ptr() if num is not None else None,
+                                pq.data_ptr(), alpha.data_ptr() if alpha is not None else None,
+                                work.data_ptr(), B, _stream(Y.device)), "ring dot")
+    return pq if num is None else (pq, alpha)
+
+
+def flm_dots_hip(z, vs, packed_L=0):
+    """(len(vs), B) per-map inner products sum w Re(conj(z) v) of coefficient
+    tensors (B, ...) complex128; ``packed_L`` > 0: the packed layout of real
+    maps (weight 2 beyond the m = 0 block)."""
+    z = z.contiguous()
+    B = z.shape[0]
+    n = z[0].numel() if B else 0
+    vs = [v.contiguous() for v in vs]
+    for v in vs:
+        if v.shape != z.shape or v.dtype != torch.complex128:
+            raise ValueError("flm_dots: vectors must match z (complex128)")
+    lib = _lib.lib()
+    out = torch.empty((len(vs), B), dtype=torch.float64, device=z.device)
+    work = torch.empty(max(int(lib.s2h_flm_dots_workspace_bytes(len(vs), B)), 8),
+                       dtype=torch.uint8, device=z.device)
+    ptrs = (ctypes.c_void_p * len(vs))(*[v.data_ptr() for v in vs])
+    _lib.check(lib.s2h_flm_dots(z.data_ptr(), ptrs, len(vs), n, int(packed_L), B,
+                                out.data_ptr(), work.data_ptr(), _stream(z.device)), "flm dots")
+    return out
+
+
+def cg_update_hip(x, p, alpha, p_new=None, z=None, g1=None, p_prev=None, g2=None):
+    """x += alpha p, and p_new = z - g1 p - g2 p_prev (per-map float64 scalars
+    on the device; p_prev optional; p_new may be z)."""
+    B = x.shape[0]
+    n = x[0].numel() if B else 0
+    for t in (x, p, p_new, z, p_prev):
+        if t is not None and (not t.is_contiguous() or t.shape != x.shape):
+            raise ValueError("cg_update: contiguous tensors of one shape")
+
+    def ptr(t):
+        return t.data_ptr() if t is not None else None
+
+    _lib.check(_lib.lib().s2h_cg_update(ptr(x), ptr(p), ptr(alpha), ptr(p_new), ptr(z), ptr(g1),
+                                        ptr(p_prev), ptr(g2), n, B, _stream(x.device)),
+               "cg update")
+    return x
 
 
 def forward_iterative_hip(f, L, spin=0, nside=None, sampling="mw", reality=False, L_lower=0,
