@@ -4,7 +4,7 @@ Workload (one step): the forward spin-0 transform of a batch of 16 real
 HEALPix maps at nside=1024 (L=2048), refined until the max-relative error
 against the seeded coefficients is <= 1e-7.  Maps are band-limited:
 f = I(flm) for flm drawn with seeds 300..315 (the reference's
-signal-generator distribution).  Three pipelines are timed on the same maps:
+signal-generator distribution).  These pipelines are timed on the same maps:
 
 * plain   -- the reference's Jacobi refinement flm += F(f - I(flm)) on the
              FP64 kernels, residual on the pixels (two ring FFTs per
@@ -12,15 +12,20 @@ signal-generator distribution).  Three pipelines are timed on the same maps:
 * plain spectral -- the same FP64 iteration with the residual on the ring
              spectra (no ring FFT inside the loop), to separate that
              algorithmic gain from the precision gain;
-* mixed   -- the mixed-precision refinement (FP64 recursion and residual,
-             BF16 matrix-core projection / synthesis, FP64 residual refresh,
-             residual on the ring spectra) with the learned preconditioner
-             when its trained weights are present (else its Jacobi fallback;
-             the JSON says which).
+* mixed CG + learned -- the headline: conjugate gradients on F I x = F f
+             (F I is Hermitian positive definite for the HEALPix quadrature)
+             with the residual held in FP64 on the ring spectra, FP64
+             recursion, split-BF16 / BF16 matrix-core projections and
+             syntheses, one FP64 synthesis for the first residual, and the
+             learned MLP preconditioner on the first residual (its direction
+             kept per map only where the FP64 ring residual drops);
+* mixed CG -- the same without the network;
+* mixed Jacobi + learned -- Jacobi steps with the learned preconditioner on
+             the first two, BF16 matrix cores and an FP64 residual refresh.
 
 Iteration counts are fixed before timing by running each pipeline once.
-`value` is the mixed pipeline's maps/s; the plain ones and the ratios sit in
-`config`.  The roofline is reported for the plain pipeline's dominant kernel
+`value` is the mixed CG + learned pipeline's maps/s; the others and the ratios
+sit in `config` (`config.legs`).  The roofline is reported for the plain pipeline's dominant kernel
 (timed live with HIP events on its launch stream) and, under
 `config.mixed_roofline`, for the mixed pipeline's.  The CPU baseline is the
 NumPy oracle on a bounded sample of the same workload.
@@ -60,7 +65,15 @@ def parse():
     ap.add_argument("--init-precision", default="bf16", choices=["fp64", "bf16", "mixed"],
                     help="mixed leg: arithmetic of the synthesis forming the first residual "
                          "(its error is removed by the FP64 refresh)")
-    ap.add_argument("--learned-steps", type=int, default=2)
+    ap.add_argument("--learned-steps", type=int, default=2,
+                    help="Jacobi leg: learned steps")
+    ap.add_argument("--cg-proj", default="bf16,bf16,bf16x1",
+                    help="CG legs: projection arithmetic of the residuals r_0, r_1, ... "
+                         "(the last repeats)")
+    ap.add_argument("--cg-init", default="bf16", choices=["fp64", "bf16", "bf16x1", "mixed"])
+    ap.add_argument("--cg-learned-steps", type=int, default=1)
+    ap.add_argument("--no-jacobi-leg", action="store_true",
+                    help="skip the mixed Jacobi (+ learned) leg")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--pixel-residual", action="store_true",
@@ -261,31 +274,55 @@ def main():
             precond = lp.MLPPreconditioner.load(device=dev)
             learned = True
         else:
-            log("[bench] no trained preconditioner weights: the mixed leg runs Jacobi")
-    prec = args.mixed_precision
-    log(f"[bench] mixed ({prec}, learned={learned}): calibrating the step count")
-    hist, stats = [], {}
+            log("[bench] no trained preconditioner weights: the mixed legs run without it")
     spectral = not args.pixel_residual
-    refine.refine_mixed(f, L, 0, nside, "healpix", True, args.max_iter, plan=plan,
-                        precond=precond, refresh_every=args.refresh_every, history=hist,
-                        flm_true=flm_true, stats=stats, precision=prec,
-                        inv_precision=args.mixed_inv_precision, learned_steps=args.learned_steps,
-                        spectral=spectral, init_precision=args.init_precision)
-    ok = [i for i, e in enumerate(hist) if e <= args.tol]
-    n_mixed = ok[0] if ok else args.max_iter
-    log(f"[bench]   mixed errors {['%.1e' % e for e in hist[:n_mixed + 2]]} -> {n_mixed} steps")
-    torch.cuda.empty_cache()
-    el, err, kt = timed(lambda: refine.refine_mixed(
-        f, L, 0, nside, "healpix", True, n_mixed, plan=plan, precond=precond,
-        refresh_every=args.refresh_every, precision=prec,
-        inv_precision=args.mixed_inv_precision, learned_steps=args.learned_steps,
-        spectral=spectral, init_precision=args.init_precision), "mixed")
-    mixed = {"elapsed": el, "err": err, "kernels": kt, "iters": n_mixed,
-             "roofline": roofline(kt, nside, L, B)}
+    cg_proj = tuple(args.cg_proj.split(","))
+
+    def calibrate(label, run):
+        hist, stats = [], {}
+        run(args.max_iter, hist, stats)
+        ok = [i for i, e in enumerate(hist) if e <= args.tol]
+        n = ok[0] if ok else args.max_iter
+        log(f"[bench]   {label} errors {['%.1e' % e for e in hist[:n + 2]]} -> {n} steps")
+        torch.cuda.empty_cache()
+        return n, hist, stats
+
+    def cg_run(pc, lsteps):
+        def run(n, hist=None, stats=None):
+            return refine.refine_cg(f, L, 0, nside, "healpix", True, n, plan=plan, precond=pc,
+                                    learned_steps=lsteps, init_precision=args.cg_init,
+                                    proj_precision=cg_proj, synth_precision="bf16",
+                                    history=hist, flm_true=flm_true if hist is not None else None,
+                                    stats=stats)
+        return run
+
+    def jacobi_run(n, hist=None, stats=None):
+        return refine.refine_mixed(
+            f, L, 0, nside, "healpix", True, n, plan=plan, precond=precond,
+            refresh_every=args.refresh_every, history=hist,
+            flm_true=flm_true if hist is not None else None, stats=stats,
+            precision=args.mixed_precision, inv_precision=args.mixed_inv_precision,
+            learned_steps=args.learned_steps, spectral=spectral,
+            init_precision=args.init_precision)
+
+    legs = {}
+    specs = [("mixed_cg_learned", cg_run(precond, args.cg_learned_steps))] if learned else []
+    specs.append(("mixed_cg", cg_run(None, 0)))
+    if not args.no_jacobi_leg:
+        specs.append(("mixed_jacobi_learned" if learned else "mixed_jacobi", jacobi_run))
+    for label, run in specs:
+        log(f"[bench] {label}: calibrating the step count")
+        n, hist, stats = calibrate(label, run)
+        el, err, kt = timed(lambda: run(n), label)
+        legs[label] = {"elapsed": el, "err": err, "kernels": kt, "iters": n, "calib": hist[:n + 1],
+                       "stats": stats, "roofline": roofline(kt, nside, L, B)}
+    head_label = "mixed_cg_learned" if learned else "mixed_cg"
+    mixed = legs[head_label]
+    n_mixed = mixed["iters"]
 
     ceiling = mfma_f64_ceiling(_lib, torch)
     log(f"[bench] FP64 matrix-core rate probe: {ceiling:.1f} TFLOP/s")
-    for leg in ([plain] if plain else []) + [mixed]:
+    for leg in ([plain] if plain else []) + list(legs.values()):
         rf = leg["roofline"]
         if rf.get("arith") == "fp64":
             rf["measured_fp64_mfma_tflops"] = ceiling
@@ -309,23 +346,30 @@ def main():
         "config": {
             "workload": (f"HEALPix nside={nside} L={L} real spin-0 forward of a {B}-map batch "
                          f"refined to <= {args.tol:g} max-rel error vs the seeded flm"),
-            "method": (f"mixed: FP64 recursion, {prec} matrix-core projection, "
-                       f"{args.mixed_inv_precision} matrix-core synthesis, "
-                       f"{args.init_precision} first residual, "
-                       f"FP64 residual refreshed every {args.refresh_every} steps, "
-                       + ("residuals on the ring spectra (no ring FFT in the loop), "
-                          if spectral else "residuals on the pixels, ")
-                       + (f"learned MLP preconditioner (Jacobi-judged) on the first "
-                          f"{args.learned_steps} steps, Jacobi after" if learned
-                          else "Jacobi")),
+            "method": (f"mixed: conjugate gradients on F I x = F f with the residual held in "
+                       f"FP64 on the ring spectra (no ring FFT in the loop); FP64 recursion, "
+                       f"{args.cg_init} first projection, {'/'.join(cg_proj)} matrix-core "
+                       f"projections of the residuals, split-BF16 matrix-core syntheses of the "
+                       f"directions, one FP64 synthesis"
+                       + (f"; learned MLP preconditioner on the first {args.cg_learned_steps} "
+                          f"residual(s), kept per map only where the FP64 ring residual drops"
+                          if learned else "")),
             "nside": nside, "L": L, "batch": B, "learned": learned,
             "mixed_steps": n_mixed, "mixed_max_rel_err": mixed["err"],
-            "mixed_calibration_errs": hist[:n_mixed + 1],
-            "learned_steps_accepted": stats.get("accepted"),
-            "learned_steps_judged": stats.get("steps"),
+            "mixed_calibration_errs": mixed["calib"],
+            "learned_steps_accepted": mixed["stats"].get("accepted"),
+            "learned_steps_judged": mixed["stats"].get("steps"),
             "mixed_kernel_ms": {k: {"total_ms": v[0], "launches": v[1]}
                                 for k, v in mixed["kernels"].items()},
             "mixed_roofline": mixed["roofline"],
+            "legs": {k: {"maps_per_s": B * args.steps / v["elapsed"],
+                         "ms_per_step": 1e3 * v["elapsed"] / args.steps, "steps": v["iters"],
+                         "max_rel_err": v["err"], "calibration_errs": v["calib"],
+                         "learned_accepted": v["stats"].get("accepted"),
+                         "learned_judged": v["stats"].get("steps"),
+                         "kernel_ms": {kk: {"total_ms": vv[0], "launches": vv[1]}
+                                       for kk, vv in v["kernels"].items()}}
+                     for k, v in legs.items()},
         },
         "roofline": head,
     }
@@ -345,6 +389,8 @@ def main():
             "plain_kernel_ms": {k: {"total_ms": v[0], "launches": v[1]}
                                 for k, v in plain["kernels"].items()},
         })
+        for k, v in result["config"]["legs"].items():
+            v["speedup_vs_plain_fp64"] = v["maps_per_s"] / plain_value
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle sample)")
         n_tr = 1 + 2 * (plain["iters"] if plain else n_mixed)

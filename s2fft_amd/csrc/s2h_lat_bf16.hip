@@ -82,13 +82,13 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
   constexpr bool LO = SPLIT == 3;
   // NI sets step together, sharing each coefficient load and giving the FP64
   // recursion NI independent chains.
-  constexpr int NI = LO ? 1 : 4;
+  constexpr int NI = LO ? 2 : 4;
   static_assert(NS % NI == 0, "set groups");
   constexpr int kTileE = 32 * kFRow;               // one [l_local 32][pair 64 + pad] BF16 tile
   constexpr int kTileB = kTileE * 2;
-  constexpr int kWaveB = (NI + (LO ? 1 : 0)) * kTileB;
+  constexpr int kWaveB = (LO ? 2 * NI : NI) * kTileB;
   static_assert(kWaveB >= kFTiles * 2 * 256 * 4, "result tile fits the wave's d tiles");
-  // per wave: NI hi d tiles (+ one lo tile); after the chunk's products the
+  // per wave: NI hi d tiles (+ NI lo tiles); after the chunk's products the
   // same bytes hold the wave's results [tile][parity][row 16][col 16] (FP32)
   __shared__ __attribute__((aligned(16))) unsigned char wbuf[kFW][kWaveB];
   __shared__ lat::ChunkCoef ccs[2][lat::kChunk];  // recursion coefficients, double-buffered
@@ -203,7 +203,7 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
           const float f = to_f32(d[s]);
           split2f(ls[s] == 0 ? f : 0.0f, h, lo);
           dh0[j * kTileE + col * kFRow + lane] = h;
-          if constexpr (LO) dl[col * kFRow + lane] = lo;
+          if constexpr (LO) dl[j * kTileE + col * kFRow + lane] = lo;
           const double dn = lat::rec_step(c, x[s], d[s], dm1[s]);
           dm1[s] = d[s];
           d[s] = dn;
@@ -216,6 +216,7 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
       for (int j = 0; j < NI; ++j) {
         const int s = sg + j;
         const __bf16* dh = dh0 + j * kTileE;
+        const __bf16* dlj = dl + j * kTileE;
 #pragma unroll
         for (int par = 0; par < 2; ++par) {
 #pragma unroll
@@ -224,7 +225,7 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
             const int off = (par * 16 + row) * kFRow + 32 * kb + 8 * kg;
             const bf16x8 bh = *reinterpret_cast<const bf16x8*>(dh + off);
             if constexpr (LO) {
-              const bf16x8 bl = *reinterpret_cast<const bf16x8*>(dl + off);
+              const bf16x8 bl = *reinterpret_cast<const bf16x8*>(dlj + off);
 #pragma unroll
               for (int t = 0; t < kFTiles; ++t) {
                 if (par == 0)
