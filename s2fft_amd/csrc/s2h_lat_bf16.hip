@@ -52,6 +52,24 @@ __device__ __forceinline__ void split2(double v, __bf16& hi, __bf16& lo) {
   split2f(to_f32(v), hi, lo);
 }
 
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// split2f of two values into elements e, e + 1 of hi / lo: one paired
+// conversion for the high parts, their exact FP32 values from the bits, one
+// paired conversion of the remainders
+__device__ __forceinline__ void split_pair(float a, float b, bf16x8& hi, bf16x8& lo, int e) {
+  const bf16x2 h = __builtin_convertvector(f32x2{a, b}, bf16x2);
+  const unsigned bits = __builtin_bit_cast(unsigned, h);
+  const float ha = __builtin_bit_cast(float, bits << 16);
+  const float hb = __builtin_bit_cast(float, bits & 0xffff0000u);
+  const bf16x2 l = __builtin_convertvector(f32x2{a - ha, b - hb}, bf16x2);
+  hi[e] = h[0];
+  hi[e + 1] = h[1];
+  lo[e] = l[0];
+  lo[e + 1] = l[1];
+}
+
 __device__ __forceinline__ f32x4 mma(const bf16x8& a, const bf16x8& b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
@@ -173,13 +191,13 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
       if (mirror) o[-2 * m] = cplx{0.0, 0.0};
     }
   }
-  lat::ChunkCoef cn = {0.0, 0.0, 0.0, 0.0};
+  Coef cn = {0.0, 0.0, 0.0, 0.0};
   if (threadIdx.x < lat::kChunk) ccs[0][threadIdx.x] = lat::load_coef(cf, st.ell0, threadIdx.x, L, bsign);
   __syncthreads();
   int buf = 0;
   for (int blk = st.ell0; blk < L; blk += 32, buf ^= 1) {
     const bool more = blk + 32 < L;
-    if (more && threadIdx.x < lat::kChunk) cn = lat::load_coef(cf, blk + 32, threadIdx.x, L, bsign);
+    if (more && threadIdx.x < lat::kChunk) cn = lat::load_coef_raw(cf, blk + 32, threadIdx.x, L);
     const lat::ChunkCoef* cs = ccs[buf];
     f32x4 acc[kFTiles][2];
 #pragma unroll
@@ -256,7 +274,7 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
 #pragma unroll
         for (int r = 0; r < 4; ++r)
           ctw[(t * 2 + par) * 256 + (4 * kg + r) * 16 + row] = acc[t][par][r];
-    if (more && threadIdx.x < lat::kChunk) ccs[buf ^ 1][threadIdx.x] = cn;
+    if (more && threadIdx.x < lat::kChunk) ccs[buf ^ 1][threadIdx.x] = lat::finish_coef(cn, bsign);
     __syncthreads();
     // element (t, parity, row i, col j) -> l = blk + 2 j + parity, map 8t + i/2
     for (int e = threadIdx.x; e < kFTiles * 2 * 256; e += kFT) {
@@ -374,7 +392,7 @@ __global__ void __launch_bounds__(kIT) lat_inverse_bf16_kernel(LatArgs a) {
     }
   };
   double fv[kStage];
-  lat::ChunkCoef cn = {0.0, 0.0, 0.0, 0.0};
+  Coef cn = {0.0, 0.0, 0.0, 0.0};
   load_flm(st.ell0, fv);
   store_flm(fv);
   if (threadIdx.x < lat::kChunk) ccs[0][threadIdx.x] = lat::load_coef(cf, st.ell0, threadIdx.x, L, bsign);
@@ -385,31 +403,58 @@ __global__ void __launch_bounds__(kIT) lat_inverse_bf16_kernel(LatArgs a) {
     const bool more = nxt < L;
     if (more) {
       load_flm(nxt, fv);
-      if (threadIdx.x < lat::kChunk) cn = lat::load_coef(cf, nxt, threadIdx.x, L, bsign);
+      if (threadIdx.x < lat::kChunk) cn = lat::load_coef_raw(cf, nxt, threadIdx.x, L);
     }
-    // 32 recursion steps into the A tile, 8 l per 16-byte store
+    // 32 recursion steps into the A tile, 8 l per 16-byte store.  The
+    // coefficients from l = L - 1 on are zero, so d vanishes past the
+    // band-limit.  Each 4-step group's coefficients are read from LDS while
+    // the previous group steps, keeping the LDS latency off the FP64 chain.
     const bool live = ls == 0;
     const lat::ChunkCoef* cs = ccs[buf];
+    double ca[4], cb[4], cc[4], na[4], nb[4], nc[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      ca[e] = cs[e].a;
+      cb[e] = cs[e].nb;
+      cc[e] = cs[e].c;
+    }
 #pragma unroll
     for (int g8 = 0; g8 < 4; ++g8) {
+      float v[8];
+#pragma unroll
+      for (int h4 = 0; h4 < 2; ++h4) {
+        const int g = 2 * g8 + h4;
+        if (g < 7) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            na[e] = cs[4 * (g + 1) + e].a;
+            nb[e] = cs[4 * (g + 1) + e].nb;
+            nc[e] = cs[4 * (g + 1) + e].c;
+          }
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const float f = to_f32(d);
+          v[4 * h4 + e] = live ? f : 0.0f;
+          const double dn = fma(fma(ca[e], x, cb[e]), d, -cc[e] * dm1);
+          dm1 = d;
+          d = dn;
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          ca[e] = na[e];
+          cb[e] = nb[e];
+          cc[e] = nc[e];
+        }
+      }
       bf16x8 vh, vl;
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int l = blk + 8 * g8 + e;
-        const double v = (live && l < L) ? d : 0.0;
-        __bf16 h, lo;
-        split2(v, h, lo);
-        vh[e] = h;
-        vl[e] = lo;
-        const double dn = lat::rec_step(cs[8 * g8 + e], x, d, dm1);
-        dm1 = d;
-        d = dn;
-      }
+      for (int e = 0; e < 8; e += 2) split_pair(v[e], v[e + 1], vh, vl, e);
       *reinterpret_cast<bf16x8*>(dh + lane * kIRow + 8 * g8) = vh;
       *reinterpret_cast<bf16x8*>(dl + lane * kIRow + 8 * g8) = vl;
     }
     lat::rescale(d, dm1, ls);
-    if (more && threadIdx.x < lat::kChunk) ccs[buf ^ 1][threadIdx.x] = cn;
+    if (more && threadIdx.x < lat::kChunk) ccs[buf ^ 1][threadIdx.x] = lat::finish_coef(cn, bsign);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -434,6 +479,23 @@ __global__ void __launch_bounds__(kIT) lat_inverse_bf16_kernel(LatArgs a) {
   // C[i][j]: pair 16 q + 4 kg + r, column 16 t + row -> map t*8 + row/2
   // Parity is relative to l = blk offsets, i.e. to l0 = ell0, whose parity is
   // that of |m| for spin 0 (the paired case).
+  // ring rows of the 16 pairs this lane writes, loaded before any store (the
+  // stores could alias the ring table, which would serialise the loads)
+  int ta[16], tb[16];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int pp = pbase + 16 * q + 4 * kg + r;
+      int ra = -1, rb = -1;
+      if (pp < a.nrings) {
+        const LatRing rg = a.rings[pp];
+        ra = rg.ta;
+        if (PAIRED && rg.tb >= 0 && rg.tb != rg.ta) rb = rg.tb;
+      }
+      ta[4 * q + r] = ra;
+      tb[4 * q + r] = rb;
+    }
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
     const int bm = b0 + t * 8 + (row >> 1);
@@ -444,12 +506,10 @@ __global__ void __launch_bounds__(kIT) lat_inverse_bf16_kernel(LatArgs a) {
     for (int q = 0; q < 4; ++q)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int pp = pbase + 16 * q + 4 * kg + r;
-        if (pp >= a.nrings) continue;
-        const LatRing rg = a.rings[pp];
+        const int ra = ta[4 * q + r], rb = tb[4 * q + r];
         const double e = (double)acc[q][t][0][r], o = (double)acc[q][t][1][r];
-        out[2 * (int64_t)rg.ta * a.ftm_ncol] = e + o;
-        if (PAIRED && rg.tb >= 0 && rg.tb != rg.ta) out[2 * (int64_t)rg.tb * a.ftm_ncol] = e - o;
+        if (ra >= 0) out[2 * (int64_t)ra * a.ftm_ncol] = e + o;
+        if (rb >= 0) out[2 * (int64_t)rb * a.ftm_ncol] = e - o;
       }
   }
 }

@@ -88,16 +88,25 @@ struct ChunkCoef {
 };
 constexpr int kChunk = 32;
 
+// Chunk `blk`'s coefficient for lane i (< 32) as stored (zero from l = L - 1
+// on).  A prefetch keeps it in this form until it is written to LDS
+// (finish_coef there): arithmetic on the loaded value right after the load
+// would make the wave wait for the memory latency at once.
+__device__ __forceinline__ Coef load_coef_raw(const Coef* __restrict__ cf, int blk, int i, int L) {
+  const int l = blk + i;
+  Coef v = {0.0, 0.0, 0.0, 0.0};
+  if (l + 1 < L) v = cf[l];
+  return v;
+}
+
+__device__ __forceinline__ ChunkCoef finish_coef(const Coef& c, double bsign) {
+  return {c.a, -c.b * bsign, c.c, 0.0};
+}
+
 // Loads chunk `blk`'s coefficient for lane i (< 32) into registers.
 __device__ __forceinline__ ChunkCoef load_coef(const Coef* __restrict__ cf, int blk, int i, int L,
                                                double bsign) {
-  const int l = blk + i;
-  ChunkCoef v = {0.0, 0.0, 0.0, 0.0};
-  if (l + 1 < L) {
-    const Coef c = cf[l];
-    v = {c.a, -c.b * bsign, c.c, 0.0};
-  }
-  return v;
+  return finish_coef(load_coef_raw(cf, blk, i, L), bsign);
 }
 
 // One recursion step with staged coefficients: returns d_{l+1}.
