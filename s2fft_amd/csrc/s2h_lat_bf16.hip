@@ -208,23 +208,67 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
 #pragma unroll
     for (int sg = 0; sg < NS; sg += NI) {
       // 32 steps of sets sg .. sg + NI - 1; the values past the band-limit
-      // are zero (zero coefficients from l = L - 1 on)
+      // are zero (zero coefficients from l = L - 1 on).  The coefficients of
+      // each GS-step group are read from LDS while the previous group steps;
+      // steps i, i + 1 share one paired BF16 conversion.
+      constexpr int GS = LO ? 4 : 2;
+      double ca[GS], cb[GS], cc[GS], na[GS], nb[GS], nc[GS];
 #pragma unroll
-      for (int i = 0; i < 32; ++i) {
-        const lat::ChunkCoef c = cs[i];
+      for (int e = 0; e < GS; ++e) {
+        ca[e] = cs[e].a;
+        cb[e] = cs[e].nb;
+        cc[e] = cs[e].c;
+      }
+#pragma unroll
+      for (int g = 0; g < 32 / GS; ++g) {
+        if (g + 1 < 32 / GS) {
+#pragma unroll
+          for (int e = 0; e < GS; ++e) {
+            na[e] = cs[GS * (g + 1) + e].a;
+            nb[e] = cs[GS * (g + 1) + e].nb;
+            nc[e] = cs[GS * (g + 1) + e].c;
+          }
+        }
+        float fv[NI][GS];
+#pragma unroll
+        for (int e = 0; e < GS; ++e) {
+#pragma unroll
+          for (int j = 0; j < NI; ++j) {
+            const int s = sg + j;
+            const float f = to_f32(d[s]);
+            fv[j][e] = ls[s] == 0 ? f : 0.0f;
+            const double dn = fma(fma(ca[e], x[s], cb[e]), d[s], -cc[e] * dm1[s]);
+            dm1[s] = d[s];
+            d[s] = dn;
+          }
+        }
         // even / odd l (relative to l0) in separate 16-column halves
-        const int col = ((i & 1) << 4) | (i >> 1);
 #pragma unroll
         for (int j = 0; j < NI; ++j) {
-          const int s = sg + j;
-          __bf16 h, lo;
-          const float f = to_f32(d[s]);
-          split2f(ls[s] == 0 ? f : 0.0f, h, lo);
-          dh0[j * kTileE + col * kFRow + lane] = h;
-          if constexpr (LO) dl[j * kTileE + col * kFRow + lane] = lo;
-          const double dn = lat::rec_step(c, x[s], d[s], dm1[s]);
-          dm1[s] = d[s];
-          d[s] = dn;
+#pragma unroll
+          for (int e = 0; e < GS; e += 2) {
+            const int i0 = GS * g + e;  // even: column i0 / 2; i0 + 1: column 16 + i0 / 2
+            __bf16* th = dh0 + j * kTileE + (i0 >> 1) * kFRow + lane;
+            const bf16x2 h = __builtin_convertvector(f32x2{fv[j][e], fv[j][e + 1]}, bf16x2);
+            th[0] = h[0];
+            th[16 * kFRow] = h[1];
+            if constexpr (LO) {
+              const unsigned bits = __builtin_bit_cast(unsigned, h);
+              const float h0 = __builtin_bit_cast(float, bits << 16);
+              const float h1 = __builtin_bit_cast(float, bits & 0xffff0000u);
+              const bf16x2 lo =
+                  __builtin_convertvector(f32x2{fv[j][e] - h0, fv[j][e + 1] - h1}, bf16x2);
+              __bf16* tl = dl + j * kTileE + (i0 >> 1) * kFRow + lane;
+              tl[0] = lo[0];
+              tl[16 * kFRow] = lo[1];
+            }
+          }
+        }
+#pragma unroll
+        for (int e = 0; e < GS; ++e) {
+          ca[e] = na[e];
+          cb[e] = nb[e];
+          cc[e] = nc[e];
         }
       }
 #pragma unroll
