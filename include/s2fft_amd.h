@@ -212,7 +212,9 @@ int s2h_ring_residual(const s2h_plan *plan, const void *G, const void *Y, void *
  * F I is Hermitian positive definite on the coefficients (F = w I^H with the
  * HEALPix / GL quadrature), so the forward is refined by CG on F I x = F f.
  *   s2h_ring_residual_scaled: R = G - alpha[b] A(Y) (alpha: batch doubles on
- *     the device; null = 1; R may be G, not Y).
+ *     the device; null = 1; R may be G, not Y); with sumsq (batch doubles)
+ *     also sumsq[b] = sum |R|^2 over the map's spectra, fixed order
+ *     (work: s2h_ring_dot_workspace_bytes; sumsq and work may be null).
  *   s2h_ring_dot: pq[b] = <p, F A(Y)> for Y = s2h_inverse_rings(p), i.e.
  *     sum over rings and m of c_m Re(conj(Y) A(Y)), c_m = 2 for m > 0 of real
  *     maps, else 1 (the CG curvature, without projecting A(Y)); with alpha,
@@ -227,7 +229,8 @@ int s2h_ring_residual(const s2h_plan *plan, const void *G, const void *Y, void *
  *     p_new = z - g1[b] p - g2[b] p_prev (p_prev and g2 optional; p_new may
  *     be z, not p or x). */
 int s2h_ring_residual_scaled(const s2h_plan *plan, const void *G, const void *Y,
-                             const double *alpha, void *R, int batch, s2h_stream stream);
+                             const double *alpha, void *R, double *sumsq, void *work, int batch,
+                             s2h_stream stream);
 size_t s2h_ring_dot_workspace_bytes(const s2h_plan *plan, int batch);
 int s2h_ring_dot(const s2h_plan *plan, const void *Y, const double *num, double *pq,
                  double *alpha, void *work, int batch, s2h_stream stream);

@@ -107,7 +107,7 @@ def _declare(lib):
         "s2h_flm_pack": (i32, [i32, vp, vp, i32, vp]),
         "s2h_flm_unpack": (i32, [i32, vp, vp, i32, vp]),
         "s2h_ring_residual": (i32, [vp, vp, vp, vp, i32, vp]),
-        "s2h_ring_residual_scaled": (i32, [vp, vp, vp, vp, vp, i32, vp]),
+        "s2h_ring_residual_scaled": (i32, [vp, vp, vp, vp, vp, vp, vp, i32, vp]),
         "s2h_ring_dot_workspace_bytes": (sz, [vp, i32]),
         "s2h_ring_dot": (i32, [vp, vp, vp, vp, vp, vp, i32, vp]),
         "s2h_flm_dots_workspace_bytes": (sz, [i32, i32]),

@@ -486,22 +486,27 @@ RingAliasArgs alias_args(const s2h_plan* p, const void* G, const void* Y, void* 
 }  // namespace
 
 extern "C" int s2h_ring_residual_scaled(const s2h_plan* p, const void* G, const void* Y,
-                                        const double* alpha, void* R, int batch,
-                                        s2h_stream stream) {
+                                        const double* alpha, void* R, double* sumsq, void* work,
+                                        int batch, s2h_stream stream) {
   if (!p || !G || !Y || !R || batch < 0 || Y == R) return S2H_ERR_INVALID;
   if (!spectral_ok(p)) return S2H_ERR_UNSUPPORTED;
   if (batch == 0) return S2H_OK;
+  if (sumsq && !work) return S2H_ERR_INVALID;
   int st = check_device(p);
   if (st) return st;
   RingAliasArgs a = alias_args(p, G, Y, R);
   a.alpha = alpha;
   ProfScope ps(S2H_KERNEL_RING_RESIDUAL, static_cast<hipStream_t>(stream));
-  return hip_status(launch_ring_alias(a, p->fwd_set, batch, static_cast<hipStream_t>(stream)));
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  if (sumsq)
+    return hip_status(launch_ring_alias_sumsq(a, p->fwd_set, batch, static_cast<double*>(work),
+                                              sumsq, s));
+  return hip_status(launch_ring_alias(a, p->fwd_set, batch, s));
 }
 
 extern "C" int s2h_ring_residual(const s2h_plan* p, const void* G, const void* Y, void* R,
                                  int batch, s2h_stream stream) {
-  return s2h_ring_residual_scaled(p, G, Y, nullptr, R, batch, stream);
+  return s2h_ring_residual_scaled(p, G, Y, nullptr, R, nullptr, nullptr, batch, stream);
 }
 
 extern "C" size_t s2h_ring_dot_workspace_bytes(const s2h_plan* p, int batch) {

@@ -59,7 +59,7 @@ struct RingAliasArgs {
   int apply_phase;
   int L;
   const double* alpha;      // per-map scale of A(Y) in R = G - alpha A(Y) (null: 1)
-  double* part;             // dot mode: [map][ring job] partial sums
+  double* part;             // [map][ring job] partial sums: the dot (dot mode) or |R|^2
 };
 
 struct LatArgs {
@@ -122,6 +122,10 @@ hipError_t launch_ring_alias(const RingAliasArgs& a, const JobSet& js, int batch
 // pq[b] = sum_m c_m Re(conj(Y) A(Y)) over every ring of map b (part: batch x
 // ring_dot_jobs(js) doubles); alpha[b] = num[b] / pq[b] when alpha is given
 int ring_dot_jobs(const JobSet& js);
+// launch_ring_alias that also writes sumsq[b] = sum |R|^2 of map b (part as
+// for launch_ring_dot)
+hipError_t launch_ring_alias_sumsq(const RingAliasArgs& a, const JobSet& js, int batch,
+                                   double* part, double* sumsq, hipStream_t s);
 hipError_t launch_ring_dot(const RingAliasArgs& a, const JobSet& js, int batch, double* part,
                            const double* num, double* pq, double* alpha, hipStream_t s);
 hipError_t launch_chirp_setup(const int* lengths, const int* conv_lens, const int64_t* offs,

@@ -118,23 +118,35 @@ hipError_t launch_zero(void* p, int64_t ndoubles, hipStream_t s) {
 // Reference layout flm[b][l][L-1+m] <-> packed m-major layout of real maps
 // (flm_p[b][moff(m) + l - m], l >= m >= 0).  Unpacking writes every entry:
 // zeros where |m| > l and (-1)^m conj(flm[l, -m]) for m < 0.
+// One 32 x 32 tile of (l, m >= 0) per workgroup through LDS: the packed
+// reads run along l (contiguous for one m), the reference-layout writes
+// along m, for both signs of m.
 __global__ void __launch_bounds__(kThreads) flm_unpack_kernel(int L, const cplx* src, cplx* dst) {
-  const int64_t b = blockIdx.y;
-  const int64_t n = (int64_t)L * (2 * L - 1);
+  __shared__ cplx tile[32][33];  // [m_local][l_local]
+  const int l0 = blockIdx.x * 32, m0 = blockIdx.y * 32;
+  const int64_t b = blockIdx.z;
   const cplx* sp = src + b * ((int64_t)L * (L + 1) / 2);
-  cplx* dp = dst + b * n;
-  for (int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x; e < n;
-       e += (int64_t)gridDim.x * kThreads) {
-    const int l = (int)(e / (2 * L - 1));
-    const int m = (int)(e - (int64_t)l * (2 * L - 1)) - (L - 1);
-    const int am = m < 0 ? -m : m;
+  cplx* dp = dst + b * ((int64_t)L * (2 * L - 1));
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int m = m0 + ty + 8 * k, l = l0 + tx;
     cplx v = {0.0, 0.0};
-    if (am <= l) {
-      const cplx t = sp[packed_moff(L, am) + l - am];
-      const double s = (m < 0 && (am & 1)) ? -1.0 : 1.0;
-      v = m < 0 ? cplx{s * t.x, -s * t.y} : t;
+    if (l < L && m <= l) v = sp[packed_moff(L, m) + l - m];
+    tile[ty + 8 * k][tx] = v;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int l = l0 + ty + 8 * k, m = m0 + tx;
+    if (l >= L || m >= L) continue;
+    const cplx t = tile[tx][ty + 8 * k];
+    cplx* row = dp + (int64_t)l * (2 * L - 1) + (L - 1);
+    row[m] = t;
+    if (m > 0) {
+      const double sg = (m & 1) ? -1.0 : 1.0;
+      row[-m] = cplx{sg * t.x, -sg * t.y};
     }
-    dp[e] = v;
   }
 }
 
@@ -153,10 +165,8 @@ __global__ void __launch_bounds__(kThreads) flm_pack_kernel(int L, const cplx* s
 
 hipError_t launch_flm_unpack(int L, const cplx* src, cplx* dst, int batch, hipStream_t s) {
   if (batch == 0) return hipSuccess;
-  const int64_t n = (int64_t)L * (2 * L - 1);
-  const int64_t blocks = (n + kThreads - 1) / kThreads;
-  hipLaunchKernelGGL(flm_unpack_kernel, dim3((unsigned)(blocks < 4096 ? blocks : 4096), batch),
-                     dim3(kThreads), 0, s, L, src, dst);
+  const unsigned t = (unsigned)((L + 31) / 32);
+  hipLaunchKernelGGL(flm_unpack_kernel, dim3(t, t, batch), dim3(kThreads), 0, s, L, src, dst);
   return hipGetLastError();
 }
 

@@ -633,7 +633,8 @@ __device__ __forceinline__ int floor_div(int a, int n) { return a >= 0 ? a / n :
 // <p, Lat_fwd(A(Lat_inv p))> for Y = Lat_inv(p) (the latitudinal transforms
 // are transposes of each other), the curvature a conjugate-gradient step needs
 // without projecting A(Y).  With a.alpha the written residual is
-// R = G - alpha[map] A(Y).
+// R = G - alpha[map] A(Y); with a.part in that mode the workgroup also stores
+// the sum of squares of the R it wrote (re^2 + im^2 over its columns).
 template <int TPB, bool DOT>
 __global__ void __launch_bounds__(TPB) ring_alias_kernel(RingAliasArgs a) {
   __shared__ double part[2 * TPB];
@@ -665,9 +666,11 @@ __global__ void __launch_bounds__(TPB) ring_alias_kernel(RingAliasArgs a) {
             const int m = a.m_first + col;
             cplx v = {0.0, 0.0};
             if (m >= lo && m <= hi) v = alias_term(y, m, a.m_first, herm);
-            r[col] = {gv.x - wn * v.x, gv.y - wn * v.y};
+            const cplx rv = {gv.x - wn * v.x, gv.y - wn * v.y};
+            r[col] = rv;
+            dsum = fma(rv.x, rv.x, fma(rv.y, rv.y, dsum));
           });
-      return;
+      if (!a.part) return;
     }
   } else {
     const int e = (int)rint(n * job.phi0_over_pi);
@@ -737,14 +740,18 @@ __global__ void __launch_bounds__(TPB) ring_alias_kernel(RingAliasArgs a) {
               const int m = q + (jo + k) * n;
               dsum += (herm && m > 0 ? 2.0 * c : c) * (gv[u].x * acc.x + gv[u].y * acc.y);
             } else {
-              r[q + (jo + k) * n - a.m_first] = {gv[u].x - c * acc.x, gv[u].y - c * acc.y};
+              const cplx rv = {gv[u].x - c * acc.x, gv[u].y - c * acc.y};
+              r[q + (jo + k) * n - a.m_first] = rv;
+              dsum = fma(rv.x, rv.x, fma(rv.y, rv.y, dsum));
             }
           }
         }
       }
     }
   }
-  if constexpr (DOT) {
+  // residue-owned columns: the written values of R in the no-alias branch
+  // when a.part is set (write mode) fall through to the same reduction
+  if (DOT || a.part) {
     // fixed-order workgroup sum: waves by shuffles, then wave 0 over the waves
     for (int o = 32; o > 0; o >>= 1) dsum += __shfl_xor(dsum, o);
     __syncthreads();
@@ -856,9 +863,24 @@ hipError_t launch_ring_alias(const RingAliasArgs& a, const JobSet& js, int batch
   if (batch == 0 || njobs == 0) return hipSuccess;
   RingAliasArgs b = a;
   b.jobs = js.jobs;
-  b.part = nullptr;
   hipLaunchKernelGGL((ring_alias_kernel<kTpbAlias, false>), dim3(njobs, batch), dim3(kTpbAlias), 0,
                      s, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_ring_alias_sumsq(const RingAliasArgs& a, const JobSet& js, int batch,
+                                   double* part, double* sumsq, hipStream_t s) {
+  const int njobs = ring_dot_jobs(js);
+  if (batch == 0 || njobs == 0) return hipSuccess;
+  RingAliasArgs b = a;
+  b.jobs = js.jobs;
+  b.part = part;
+  hipLaunchKernelGGL((ring_alias_kernel<kTpbAlias, false>), dim3(njobs, batch), dim3(kTpbAlias), 0,
+                     s, b);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(ring_dot_final_kernel, dim3(batch), dim3(256), 0, s, part, njobs, nullptr,
+                     sumsq, nullptr);
   return hipGetLastError();
 }
 

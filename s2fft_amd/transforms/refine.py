@@ -171,12 +171,21 @@ def refine_cg(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=None
     _track(history, x, flm_true, unpack)
     if n_iter <= 0:
         return unpack(x)
-    # R = G - A(I(x_0)) in place of G
-    sph.ring_residual_hip(R, sph.inverse_rings_hip(x, plan, "fp64", packed), plan, out=R)
+    def judged(k):
+        return precond is not None and k < learned_steps
+
+    # R = G - A(I(x_0)) in place of G; |R|^2 per map where a judged step follows
+    Y = sph.inverse_rings_hip(x, plan, "fp64", packed)
+    n_R = None
+    if judged(0):
+        _, n_R = sph.ring_residual_hip(R, Y, plan, out=R, sumsq=True)
+    else:
+        sph.ring_residual_hip(R, Y, plan, out=R)
     r = proj(R, 0)
     z = precondition(r, 0)
     rz = sph.flm_dots_hip(z, [r], pL)[0]
-    rs, ps, rz_prev = [r], [z.clone() if z is r else z], None
+    # p_0 may be r_0 itself: no direction buffer is written before r_0 is dropped
+    rs, ps, rz_prev = [r], [z], None
     for k in range(n_iter):
         p = ps[-1]
         Y = sph.inverse_rings_hip(p, plan, synth_precision, packed)
@@ -185,17 +194,20 @@ def refine_cg(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=None
             sph.cg_update_hip(x, p, alpha)
             _track(history, x, flm_true, unpack)
             break
-        judged = precond is not None and k < learned_steps
-        n_old = _sumsq(R) if judged else None
-        sph.ring_residual_hip(R, Y, plan, out=R, alpha=alpha)
-        if judged:
-            ok = _sumsq(R) < n_old
+        n_new = None
+        if judged(k) or judged(k + 1):
+            _, n_new = sph.ring_residual_hip(R, Y, plan, out=R, alpha=alpha, sumsq=True)
+        else:
+            sph.ring_residual_hip(R, Y, plan, out=R, alpha=alpha)
+        if judged(k):
+            ok = n_new < n_R
             if stats is not None:
                 stats["accepted"] = stats.get("accepted", 0) + int(ok.sum())
                 stats["steps"] = stats.get("steps", 0) + int(ok.numel())
             if not bool(ok.all()):
                 _redo_plain((~ok).nonzero().flatten(), R, Y, p, alpha, rz, rs[-1], plan,
-                            synth_precision, packed, pL)
+                            synth_precision, packed, pL, n_new)
+        n_R = n_new
         r_new = proj(R, k + 1)
         z = precondition(r_new, k + 1)
         # d = <z, r_k>, <z, r_{k+1}> [, <z, r_{k-1}>]; alpha_j pq_j = <r_j, z_j>
@@ -211,20 +223,20 @@ def refine_cg(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=None
     return unpack(x)
 
 
-def _redo_plain(bad, R, Y, p, alpha, rz, r, plan, synth_precision, packed, pL):
+def _redo_plain(bad, R, Y, p, alpha, rz, r, plan, synth_precision, packed, pL, n_R):
     """Judge's fallback of refine_cg: maps ``bad`` take back their learned
     step's residual update (R += alpha A(Y)) and step along r instead; R and
-    the maps' rows of p, alpha and rz are replaced in place (x is updated
-    afterwards with the replaced alpha and p)."""
+    the maps' rows of p, alpha, rz and the residual norms n_R are replaced in
+    place (x is updated afterwards with the replaced alpha and p)."""
     Rb = R[bad].contiguous()
     sph.ring_residual_hip(Rb, Y[bad].contiguous(), plan, out=Rb, alpha=-alpha[bad])
     rb = r[bad].contiguous()
     rzb = sph.flm_dots_hip(rb, [rb], pL)[0]
     Yb = sph.inverse_rings_hip(rb, plan, synth_precision, packed)
     _, ab = sph.ring_dot_hip(Yb, plan, num=rzb)
-    sph.ring_residual_hip(Rb, Yb, plan, out=Rb, alpha=ab)
+    _, nb = sph.ring_residual_hip(Rb, Yb, plan, out=Rb, alpha=ab, sumsq=True)
     R[bad], p[bad] = Rb, rb
-    alpha[bad], rz[bad] = ab, rzb
+    alpha[bad], rz[bad], n_R[bad] = ab, rzb, nb
 
 
 def _judged_step(precond, h, r, rn, f, flm, kw, stats, synth, ip, exact, packed=False):

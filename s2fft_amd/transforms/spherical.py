@@ -333,20 +333,29 @@ def flm_unpack_hip(flm_p, L):
     return out
 
 
-def ring_residual_hip(G, Y, plan, out=None, alpha=None):
+def ring_residual_hip(G, Y, plan, out=None, alpha=None, sumsq=False):
     """R = G - A(Y), or G - alpha[b] A(Y) with a per-map float64 device tensor
-    ``alpha`` (out may be G)."""
+    ``alpha`` (out may be G).  ``sumsq``: also return the per-map sums of
+    |R|^2 (one pass, fixed order) as (R, sumsq)."""
+    B = G.shape[0]
     R = torch.empty_like(G) if out is None else out
-    a = 0
+    a = None
     if alpha is not None:
-        if alpha.dtype != torch.float64 or alpha.numel() != G.shape[0] or alpha.device != G.device:
+        if alpha.dtype != torch.float64 or alpha.numel() != B or alpha.device != G.device:
             raise ValueError("alpha: one float64 per map on the spectra's device")
         a = alpha.contiguous().data_ptr()
-    _lib.check(_lib.lib().s2h_ring_residual_scaled(plan.handle, G.contiguous().data_ptr(),
-                                                   Y.contiguous().data_ptr(), a, R.data_ptr(),
-                                                   G.shape[0], _stream(G.device)),
-               "ring residual")
-    return R
+    sq = work = None
+    lib = _lib.lib()
+    if sumsq:
+        sq = torch.empty(B, dtype=torch.float64, device=G.device)
+        work = torch.empty(max(int(lib.s2h_ring_dot_workspace_bytes(plan.handle, B)), 8),
+                           dtype=torch.uint8, device=G.device)
+    _lib.check(lib.s2h_ring_residual_scaled(plan.handle, G.contiguous().data_ptr(),
+                                            Y.contiguous().data_ptr(), a, R.data_ptr(),
+                                            sq.data_ptr() if sq is not None else None,
+                                            work.data_ptr() if work is not None else None,
+                                            B, _stream(G.device)), "ring residual")
+    return (R, sq) if sumsq else R
 
 
 def ring_dot_hip(Y, plan, num=None):

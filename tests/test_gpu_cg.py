@@ -79,6 +79,11 @@ def test_scaled_residual_is_linear(rng):
     assert torch.equal(sph.ring_residual_hip(G, Y, plan, alpha=torch.ones_like(alpha)), R1)
     with pytest.raises(ValueError):
         sph.ring_residual_hip(G, Y, plan, alpha=alpha.float())
+    # the fused per-map sum of squares of the written residual
+    Rs, sq = sph.ring_residual_hip(G, Y, plan, alpha=alpha, sumsq=True)
+    assert torch.equal(Rs, Ra)
+    ref = (Ra.real ** 2 + Ra.imag ** 2).sum(1)
+    assert torch.allclose(sq, ref, rtol=1e-13, atol=0)
 
 
 @pytest.mark.parametrize("packed_L", [0, 7])
