@@ -299,10 +299,11 @@ int lat_inverse_stage(const s2h_plan* p, const void* flm, cplx* ftm, int batch, 
 
 // Plans whose forward and inverse share one ring grid without pole rows or
 // resampling (HEALPix, GL): their ring spectra compose directly.
+// The alias kernel's residue arithmetic needs ring length + 2L < 2^16.
 bool spectral_ok(const s2h_plan* p) {
   return p->fwd_kind == 0 && p->fwd_rows == p->inv_rows && p->fwd_row_south < 0 &&
          p->fwd_row_north < 0 && p->inv_row_south < 0 && p->inv_row_north < 0 &&
-         p->d.norm != S2H_NORM_NONE;
+         p->d.norm != S2H_NORM_NONE && p->fwd_set.max_nphi + 2 * p->d.L < (1 << 16);
 }
 
 }  // namespace

@@ -624,8 +624,18 @@ __device__ __forceinline__ cplx alias_term(const cplx* y, int m, int m_first, bo
   return y[m - m_first];
 }
 
-// floor(a / n) for n > 0
-__device__ __forceinline__ int floor_div(int a, int n) { return a >= 0 ? a / n : -((n - 1 - a) / n); }
+// floor(a / n) by a multiply-high with mag = floor((2^32 - 1) / n) + 1: exact
+// for 0 <= a < 2^16 and 0 < n < 2^16 (the error a (n mag - 2^32) / 2^32 n stays
+// below 1/n), so one division per thread replaces one per residue
+struct FastDiv {
+  int n;
+  unsigned mag;
+  __device__ explicit FastDiv(int n_) : n(n_), mag(n_ > 1 ? 0xFFFFFFFFu / (unsigned)n_ + 1u : 0u) {}
+  __device__ __forceinline__ int floor(int a) const {
+    if (n == 1) return a;
+    return a >= 0 ? (int)__umulhi((unsigned)a, mag) : -(int)__umulhi((unsigned)(n - 1 - a), mag);
+  }
+};
 
 // DOT: instead of writing R, the workgroup adds up
 //   sum_m c_m Re(conj(Y[m]) A(Y)[m]),   c_m = 2 for m > 0 of Hermitian spectra, else 1,
@@ -675,6 +685,9 @@ __global__ void __launch_bounds__(TPB) ring_alias_kernel(RingAliasArgs a) {
   } else {
     const int e = (int)rint(n * job.phi0_over_pi);
     const double sgn = (a.apply_phase && (e & 1)) ? -1.0 : 1.0;
+    // the division's operands: |q - lo|, |hi - q|, |q - m_first| and
+    // |m_last - q| are below n + 2L (the host keeps n + 2L < 2^16)
+    const FastDiv dv(n);
     const int G = n >= TPB ? 1 : TPB / n;
     const int nq = n >= TPB ? (n + TPB - 1) / TPB : 1;  // residues per thread
     const int gi = n >= TPB ? 0 : t / n;
@@ -688,8 +701,8 @@ __global__ void __launch_bounds__(TPB) ring_alias_kernel(RingAliasArgs a) {
       cplx acc = {0.0, 0.0};
       int j0 = 0;
       if (own) {
-        j0 = -floor_div(q - lo, n);
-        const int j1 = floor_div(hi - q, n);
+        j0 = -dv.floor(q - lo);
+        const int j1 = dv.floor(hi - q);
         const int cnt = j1 - j0 + 1;
         for (int k0 = gi; k0 < cnt; k0 += 4 * G) {
           cplx v[4];
@@ -721,8 +734,8 @@ __global__ void __launch_bounds__(TPB) ring_alias_kernel(RingAliasArgs a) {
       }
       if (!own) continue;
       // outputs m = q + j n in [m_first, m_last], this thread's j = jo + gi + k G
-      const int jo = -floor_div(q - a.m_first, n);
-      const int cnt = floor_div(m_last - q, n) - jo + 1;
+      const int jo = -dv.floor(q - a.m_first);
+      const int cnt = dv.floor(m_last - q) - jo + 1;
       const cplx* src = DOT ? y : g;
       for (int k0 = gi; k0 < cnt; k0 += 4 * G) {
         cplx gv[4];
