@@ -67,7 +67,7 @@ def parse():
                          "(its error is removed by the FP64 refresh)")
     ap.add_argument("--learned-steps", type=int, default=2,
                     help="Jacobi leg: learned steps")
-    ap.add_argument("--cg-proj", default="bf16,bf16,bf16x1",
+    ap.add_argument("--cg-proj", default="bf16,bf16x1",
                     help="CG legs: projection arithmetic of the residuals r_0, r_1, ... "
                          "(the last repeats)")
     ap.add_argument("--cg-init", default="bf16", choices=["fp64", "bf16", "bf16x1", "mixed"])

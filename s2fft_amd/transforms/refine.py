@@ -116,7 +116,7 @@ def refine_mixed(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=N
 
 
 def refine_cg(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=None, precond=None,
-              learned_steps=0, init_precision="bf16", proj_precision=("bf16", "bf16", "bf16x1"),
+              learned_steps=0, init_precision="bf16", proj_precision=("bf16", "bf16x1"),
               synth_precision="bf16", history=None, flm_true=None, stats=None):
     """Conjugate-gradient refinement of the forward transform on the ring spectra.
 
@@ -131,14 +131,15 @@ def refine_cg(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=None
         step k:  Y = I~(p_k);  pq_k = <p_k, F A(Y)>  (a ring-space sum)
                  alpha_k = <r_k, z_k> / pq_k;  x += alpha_k p_k
                  R -= alpha_k A(Y);  r_{k+1} = F~(R);  z = P(r_{k+1}) or r_{k+1}
-                 p_{k+1} = z - sum_j <z, q_j> / pq_j p_j   (j = k, k - 1),
+                 p_{k+1} = z - sum_j <z, q_j> / pq_j p_j   (j = k, and k - 1
+                 when p_{k-1} was a learned direction),
                  q_j = F I p_j = (r_j - r_{j+1}) / alpha_j
 
     R stays FP64 and takes each direction's synthesis, so every r_k is the
     projection of a residual that is accurate to the reduced synthesis' error
-    times |alpha p| (not a running sum of reduced-precision products).  The
-    directions are orthogonalised against the last two (flexible CG, for a
-    nonlinear P).  The last step forms no residual.
+    times |alpha p| (not a running sum of reduced-precision products).  A
+    direction that follows a learned one is orthogonalised against both
+    (flexible CG, for a nonlinear P).  The last step forms no residual.
 
     F~ / I~ run at proj_precision (one entry per residual r_0, r_1, ...; the
     last repeats) / synth_precision; x_0 at init_precision.  precond: the
@@ -186,6 +187,8 @@ def refine_cg(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=None
     rz = sph.flm_dots_hip(z, [r], pL)[0]
     # p_0 may be r_0 itself: no direction buffer is written before r_0 is dropped
     rs, ps, rz_prev = [r], [z], None
+    # per map: whether direction p_{k-1} / p_k is a kept learned direction
+    kept_prev = kept = None
     for k in range(n_iter):
         p = ps[-1]
         Y = sph.inverse_rings_hip(p, plan, synth_precision, packed)
@@ -207,16 +210,23 @@ def refine_cg(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=None
             if not bool(ok.all()):
                 _redo_plain((~ok).nonzero().flatten(), R, Y, p, alpha, rz, rs[-1], plan,
                             synth_precision, packed, pL, n_new)
+            kept = ok
         n_R = n_new
         r_new = proj(R, k + 1)
         z = precondition(r_new, k + 1)
-        # d = <z, r_k>, <z, r_{k+1}> [, <z, r_{k-1}>]; alpha_j pq_j = <r_j, z_j>
-        d = sph.flm_dots_hip(z, [rs[-1], r_new] + rs[-2:-1], pL)
+        # d = <z, r_k>, <z, r_{k+1}> [, <z, r_{k-1}>]; alpha_j pq_j = <r_j, z_j>.
+        # p_{k-1} enters only when it was a learned direction: with z = r the
+        # two-term recurrence already keeps it conjugate, and the extra term
+        # would add the reduced projections' error of r_{k-1} (at single BF16
+        # it costs an order of magnitude by the third step).
+        back2 = len(rs) > 1 and kept_prev is not None and bool(kept_prev.any())
+        d = sph.flm_dots_hip(z, [rs[-1], r_new] + (rs[-2:-1] if back2 else []), pL)
         g1 = (d[0] - d[1]) / rz
-        g2 = (d[2] - d[0]) / rz_prev if len(rs) > 1 else None
+        g2 = (d[2] - d[0]) / rz_prev * kept_prev if back2 else None
         p_new = ps[0] if len(ps) == 3 else torch.empty_like(p)
-        sph.cg_update_hip(x, p, alpha, p_new, z, g1, ps[-2] if len(ps) > 1 else None, g2)
+        sph.cg_update_hip(x, p, alpha, p_new, z, g1, ps[-2] if back2 else None, g2)
         _track(history, x, flm_true, unpack)
+        kept_prev, kept = kept, None
         rz_prev, rz = rz, d[1]
         rs = rs[-1:] + [r_new]
         ps = ps[-2:] + [p_new]
