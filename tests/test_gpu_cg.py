@@ -245,3 +245,61 @@ def test_cg_rejects_resampled_sampling():
     f = torch.zeros((1, 8, 15), dtype=torch.float64, device="cuda")
     with pytest.raises(ValueError):
         refine.refine_cg(f, 8, 0, None, "mw", True, 2)
+
+
+@pytest.mark.parametrize("n_iter", [0, 1, 2])
+def test_short_runs_follow_textbook_cg(rng, n_iter):
+    """The first steps' bookkeeping (no direction history yet) in FP64."""
+    nside, L, B = 16, 32, 2
+    flm, f = _maps(rng, nside, L, B)
+    plan = sph.get_plan("healpix", L, nside, 0, 0, True, device=f.device)
+    a = refine.refine_cg(f, L, 0, nside, "healpix", True, n_iter, plan=plan,
+                         init_precision="fp64", proj_precision=("fp64",), synth_precision="fp64")
+    if n_iter == 0:
+        b = sph.forward_hip(f, L, 0, nside, "healpix", True, plan=plan)
+    else:
+        b = _torch_cg(f, L, nside, plan, n_iter)
+    assert float((a - b).abs().max() / b.abs().max()) < 1e-11
+
+
+def test_two_learned_directions_are_judged(rng):
+    """learned_steps = 2: both directions judged, the third orthogonalised
+    against both learned ones."""
+    from s2fft_amd.learned.precond import MLPPreconditioner
+    nside, L, B = 64, 128, 3
+    flm, f = _maps(rng, nside, L, B)
+    plan = sph.get_plan("healpix", L, nside, 0, 0, True, device=f.device)
+    stats, hist = {}, []
+    refine.refine_cg(f, L, 0, nside, "healpix", True, 5, plan=plan,
+                     precond=MLPPreconditioner.load(device="cuda"), learned_steps=2, stats=stats,
+                     history=hist, flm_true=flm, proj_precision=("bf16",))
+    print(stats, hist)
+    assert stats["steps"] == 2 * B
+    assert hist[-1] < 1e-3 * hist[0]
+
+
+def test_cg_with_band_floor_and_gl(rng):
+    """L_lower > 0 on HEALPix (coefficients below the floor stay zero) and the
+    GL grid (one ring grid, no aliasing)."""
+    nside, L, L_lower, B = 16, 32, 4, 2
+    flm = torch.from_numpy(np.stack([signals.generate_flm(rng, L, L_lower, 0, True)
+                                     for _ in range(B)])).cuda()
+    plan = sph.get_plan("healpix", L, nside, 0, L_lower, True, device=flm.device)
+    f = sph.inverse_hip(flm, L, 0, nside, "healpix", True, L_lower=L_lower,
+                        plan=plan).real.contiguous()
+    hist = []
+    out = refine.refine_cg(f, L, 0, nside, "healpix", True, 4, L_lower=L_lower, plan=plan,
+                           init_precision="fp64", proj_precision=("fp64",),
+                           synth_precision="fp64", history=hist, flm_true=flm)
+    assert float(out[:, :L_lower].abs().max()) == 0.0
+    assert hist[-1] < 1e-3 * hist[0]
+    L2 = 24
+    flm2 = torch.from_numpy(np.stack([signals.generate_flm(rng, L2, 0, 0, True)
+                                      for _ in range(B)])).cuda()
+    f2 = sph.inverse_hip(flm2, L2, 0, None, "gl", True)
+    if f2.is_complex():
+        f2 = f2.real.contiguous()
+    out2 = refine.refine_cg(f2, L2, 0, None, "gl", True, 1, init_precision="fp64",
+                            proj_precision=("fp64",), synth_precision="fp64")
+    # GL is exact: the first projection already is the answer, CG keeps it
+    assert float((out2 - flm2).abs().max() / flm2.abs().max()) < 1e-12
