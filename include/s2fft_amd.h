@@ -99,9 +99,9 @@ int s2h_inverse(const s2h_plan *plan, const void *flm, void *f, void *work, int 
  *   bits), products hi*hi + hi*lo + lo*hi on the BF16 matrix cores with FP32
  *   accumulation (forward: over one wave's ring pairs; inverse: over all l);
  *   relative error ~1e-5.
- * S2H_PREC_BF16X1: the forward with BF16 operands and one product (~4e-3
- *   relative), the inverse as S2H_PREC_BF16: the correction transform of a
- *   refinement, whose contraction it changes by its own relative error.
+ * S2H_PREC_BF16X1: BF16 operands and one product (~4e-3 relative): the
+ *   correction transform of a refinement, whose contraction it changes by
+ *   its own relative error (or, for the synthesis, a step length).
  * All are meant for iterative refinement whose residual is FP64. */
 enum s2h_precision {
   S2H_PREC_FP64 = 0,

@@ -269,7 +269,8 @@ int lat_inverse_stage(const s2h_plan* p, const void* flm, cplx* ftm, int batch, 
                  s);
     // batches of 4 or more maps: each d-value feeds 16 maps through the MFMA
     if (bf16)
-      st = hip_status(launch_lat_inverse_bf16(a, p->inv_paired, s));
+      st = hip_status(launch_lat_inverse_bf16(a, p->inv_paired,
+                                              precision == S2H_PREC_BF16X1 ? 1 : 3, s));
     else if (mixed || batch >= kMfmaMinBatch)
       st = hip_status(launch_lat_inverse_mma(a, p->inv_paired, mixed, s));
     else

@@ -149,7 +149,8 @@ hipError_t launch_flm_to_mmajor(const cplx* flm, cplx* flmT, int L, int m_lo, in
                                 hipStream_t s);
 hipError_t launch_mmajor_to_flm(const cplx* flmT, cplx* flm, int L, int m_lo, int nm, int batch,
                                 hipStream_t s);
-hipError_t launch_lat_inverse_bf16(const LatArgs& a, bool paired, hipStream_t s);
+// split: 3 = hi/lo operands (three products), 1 = single BF16 operands
+hipError_t launch_lat_inverse_bf16(const LatArgs& a, bool paired, int split, hipStream_t s);
 hipError_t launch_mfma_f32_probe(const float* a, const float* b, float* out, hipStream_t s);
 
 // learned preconditioner (per-(l, m) rows of one or more flm arrays)

@@ -360,15 +360,18 @@ constexpr int kIT = kIW * 64;
 constexpr int kIMaps = 16;      // maps per workgroup (two column tiles)
 }  // namespace
 
-template <bool PAIRED>
-__global__ void __launch_bounds__(kIT) lat_inverse_bf16_kernel(LatArgs a) {
+// LO: split operands (hi*hi + hi*lo + lo*hi, ~1e-5 relative); else one BF16
+// product (~4e-3 relative) and no lo tiles.
+template <bool PAIRED, bool LO>
+__global__ void __launch_bounds__(kIT, LO ? 1 : 3) lat_inverse_bf16_kernel(LatArgs a) {
+  constexpr int kLoW = LO ? kIW : 1, kLoE = LO ? 64 * kIRow : 8, kLoB = LO ? 32 * kIRow : 8;
   // d tile per wave: [pair 64][l_local 32] hi / lo (A layout)
   __shared__ __attribute__((aligned(16))) __bf16 dth[kIW][64 * kIRow];
-  __shared__ __attribute__((aligned(16))) __bf16 dtl[kIW][64 * kIRow];
+  __shared__ __attribute__((aligned(16))) __bf16 dtl[kLoW][kLoE];
   // flm chunk: [parity][column 32 = 16 maps x re/im][l 32] hi / lo (B layout),
   // entries of the other parity zero
   __shared__ __attribute__((aligned(16))) __bf16 bth[2][32 * kIRow];
-  __shared__ __attribute__((aligned(16))) __bf16 btl[2][32 * kIRow];
+  __shared__ __attribute__((aligned(16))) __bf16 btl[2][kLoB];
   __shared__ lat::ChunkCoef ccs[2][lat::kChunk];  // recursion coefficients, double-buffered
   const int m = m_of_block(blockIdx.x, a.nm, a.m_lo);
   const int am = m < 0 ? -m : m;
@@ -380,7 +383,7 @@ __global__ void __launch_bounds__(kIT) lat_inverse_bf16_kernel(LatArgs a) {
   const double bsign = m < 0 ? -1.0 : 1.0;
   const Coef* __restrict__ cf = a.coef + (int64_t)am * L;
   __bf16* dh = dth[wave];
-  __bf16* dl = dtl[wave];
+  __bf16* dl = dtl[LO ? wave : 0];
 
   const int pbase = (blockIdx.y * kIW + wave) * 64;
   const int p = pbase + lane;
@@ -430,9 +433,11 @@ __global__ void __launch_bounds__(kIT) lat_inverse_bf16_kernel(LatArgs a) {
       split2(v[j], h, lo);
       const int par = i & 1;
       bth[par][col * kIRow + i] = h;
-      btl[par][col * kIRow + i] = lo;
       bth[par ^ 1][col * kIRow + i] = (__bf16)0.0f;
-      btl[par ^ 1][col * kIRow + i] = (__bf16)0.0f;
+      if constexpr (LO) {
+        btl[par][col * kIRow + i] = lo;
+        btl[par ^ 1][col * kIRow + i] = (__bf16)0.0f;
+      }
     }
   };
   double fv[kStage];
@@ -492,10 +497,19 @@ __global__ void __launch_bounds__(kIT) lat_inverse_bf16_kernel(LatArgs a) {
         }
       }
       bf16x8 vh, vl;
+      if constexpr (LO) {
 #pragma unroll
-      for (int e = 0; e < 8; e += 2) split_pair(v[e], v[e + 1], vh, vl, e);
+        for (int e = 0; e < 8; e += 2) split_pair(v[e], v[e + 1], vh, vl, e);
+        *reinterpret_cast<bf16x8*>(dl + lane * kIRow + 8 * g8) = vl;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; e += 2) {
+          const bf16x2 h = __builtin_convertvector(f32x2{v[e], v[e + 1]}, bf16x2);
+          vh[e] = h[0];
+          vh[e + 1] = h[1];
+        }
+      }
       *reinterpret_cast<bf16x8*>(dh + lane * kIRow + 8 * g8) = vh;
-      *reinterpret_cast<bf16x8*>(dl + lane * kIRow + 8 * g8) = vl;
     }
     lat::rescale(d, dm1, ls);
     if (more && threadIdx.x < lat::kChunk) ccs[buf ^ 1][threadIdx.x] = lat::finish_coef(cn, bsign);
@@ -505,7 +519,6 @@ __global__ void __launch_bounds__(kIT) lat_inverse_bf16_kernel(LatArgs a) {
       // A[i][k]: pair 16 q + row, l = 8 kg + e
       const int aoff = (16 * q + row) * kIRow + 8 * kg;
       const bf16x8 ah = *reinterpret_cast<const bf16x8*>(dh + aoff);
-      const bf16x8 al = *reinterpret_cast<const bf16x8*>(dl + aoff);
 #pragma unroll
       for (int t = 0; t < 2; ++t)
 #pragma unroll
@@ -513,8 +526,13 @@ __global__ void __launch_bounds__(kIT) lat_inverse_bf16_kernel(LatArgs a) {
           // B[k][j]: column 16 t + row, l = 8 kg + e
           const int boff = (16 * t + row) * kIRow + 8 * kg;
           const bf16x8 bh = *reinterpret_cast<const bf16x8*>(&bth[par][boff]);
-          const bf16x8 bl = *reinterpret_cast<const bf16x8*>(&btl[par][boff]);
-          acc[q][t][par] = mma3(ah, al, bh, bl, acc[q][t][par]);
+          if constexpr (LO) {
+            const bf16x8 al = *reinterpret_cast<const bf16x8*>(dl + aoff);
+            const bf16x8 bl = *reinterpret_cast<const bf16x8*>(&btl[par][boff]);
+            acc[q][t][par] = mma3(ah, al, bh, bl, acc[q][t][par]);
+          } else {
+            acc[q][t][par] = mma(ah, bh, acc[q][t][par]);
+          }
         }
     }
     __syncthreads();
@@ -582,14 +600,19 @@ hipError_t launch_lat_forward_bf16(const LatArgs& a, bool paired, int split, hip
   return split == 1 ? lat_fwd_bf16<4, 1>(a, paired, s) : lat_fwd_bf16<2, 3>(a, paired, s);
 }
 
-hipError_t launch_lat_inverse_bf16(const LatArgs& a, bool paired, hipStream_t s) {
-  if (a.nm == 0 || a.nrings == 0 || a.batch == 0) return hipSuccess;
+template <bool LO>
+static hipError_t lat_inv_bf16(const LatArgs& a, bool paired, hipStream_t s) {
   dim3 grid(a.nm, (a.nrings + kIT - 1) / kIT, (a.batch + kIMaps - 1) / kIMaps);
   if (paired)
-    hipLaunchKernelGGL(lat_inverse_bf16_kernel<true>, grid, dim3(kIT), 0, s, a);
+    hipLaunchKernelGGL((lat_inverse_bf16_kernel<true, LO>), grid, dim3(kIT), 0, s, a);
   else
-    hipLaunchKernelGGL(lat_inverse_bf16_kernel<false>, grid, dim3(kIT), 0, s, a);
+    hipLaunchKernelGGL((lat_inverse_bf16_kernel<false, LO>), grid, dim3(kIT), 0, s, a);
   return hipGetLastError();
+}
+
+hipError_t launch_lat_inverse_bf16(const LatArgs& a, bool paired, int split, hipStream_t s) {
+  if (a.nm == 0 || a.nrings == 0 || a.batch == 0) return hipSuccess;
+  return split == 1 ? lat_inv_bf16<false>(a, paired, s) : lat_inv_bf16<true>(a, paired, s);
 }
 
 }  // namespace s2h

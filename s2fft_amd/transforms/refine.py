@@ -117,7 +117,8 @@ def refine_mixed(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=N
 
 def refine_cg(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=None, precond=None,
               learned_steps=0, init_precision="bf16", proj_precision=("bf16", "bf16x1"),
-              synth_precision="bf16", history=None, flm_true=None, stats=None):
+              synth_precision="bf16", last_synth_precision="bf16x1", history=None,
+              flm_true=None, stats=None):
     """Conjugate-gradient refinement of the forward transform on the ring spectra.
 
     F I is Hermitian positive definite on the coefficients (the HEALPix and
@@ -142,7 +143,9 @@ def refine_cg(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=None
     (flexible CG, for a nonlinear P).  The last step forms no residual.
 
     F~ / I~ run at proj_precision (one entry per residual r_0, r_1, ...; the
-    last repeats) / synth_precision; x_0 at init_precision.  precond: the
+    last repeats) / synth_precision; x_0 at init_precision.  The last step's
+    synthesis only sets its step length (no residual is formed from it), so
+    it runs at last_synth_precision.  precond: the
     learned preconditioner for the first ``learned_steps`` residuals; a
     learned direction is kept for a map only where that step lowers the norm
     of its FP64 ring residual R, else the map redoes the step along its CG
@@ -191,9 +194,11 @@ def refine_cg(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=None
     kept_prev = kept = None
     for k in range(n_iter):
         p = ps[-1]
-        Y = sph.inverse_rings_hip(p, plan, synth_precision, packed)
+        last = k + 1 == n_iter
+        Y = sph.inverse_rings_hip(p, plan, last_synth_precision if last else synth_precision,
+                                  packed)
         pq, alpha = sph.ring_dot_hip(Y, plan, num=rz)
-        if k + 1 == n_iter:
+        if last:
             sph.cg_update_hip(x, p, alpha)
             _track(history, x, flm_true, unpack)
             break

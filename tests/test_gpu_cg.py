@@ -165,7 +165,8 @@ def test_fp64_cg_follows_textbook_cg(rng):
     plan = sph.get_plan("healpix", L, nside, 0, 0, True, device=f.device)
     for n in (1, 2, 3):
         a = refine.refine_cg(f, L, 0, nside, "healpix", True, n, plan=plan, init_precision="fp64",
-                             proj_precision=("fp64",), synth_precision="fp64")
+                             proj_precision=("fp64",), synth_precision="fp64",
+                             last_synth_precision="fp64")
         b = _torch_cg(f, L, nside, plan, n)
         err = float((a - b).abs().max() / b.abs().max())
         assert err < 1e-11, (n, err)
@@ -181,7 +182,8 @@ def test_cg_beats_jacobi(rng):
     errs_cg, errs_j = [], []
     for n in (1, 2, 3):
         a = refine.refine_cg(f, L, 0, nside, "healpix", True, n, plan=plan, init_precision="fp64",
-                             proj_precision=("fp64",), synth_precision="fp64")
+                             proj_precision=("fp64",), synth_precision="fp64",
+                             last_synth_precision="fp64")
         b = sph.forward_iterative_hip(f, L, 0, nside, "healpix", True, 0, n + 1)
         errs_cg.append(float((a - flm).abs().max()) / scale)
         errs_j.append(float((b - flm).abs().max()) / scale)
@@ -201,7 +203,8 @@ def test_complex_maps_cg(rng):
     plan = sph.get_plan("healpix", L, nside, 0, 0, False, device=f.device)
     hist = []
     refine.refine_cg(f, L, 0, nside, "healpix", False, 4, plan=plan, init_precision="fp64",
-                     proj_precision=("fp64",), synth_precision="fp64", history=hist,
+                     proj_precision=("fp64",), synth_precision="fp64",
+                             last_synth_precision="fp64", history=hist,
                      flm_true=flm)
     print(hist)
     assert hist[-1] < 1e-3 * hist[0]
@@ -254,7 +257,8 @@ def test_short_runs_follow_textbook_cg(rng, n_iter):
     flm, f = _maps(rng, nside, L, B)
     plan = sph.get_plan("healpix", L, nside, 0, 0, True, device=f.device)
     a = refine.refine_cg(f, L, 0, nside, "healpix", True, n_iter, plan=plan,
-                         init_precision="fp64", proj_precision=("fp64",), synth_precision="fp64")
+                         init_precision="fp64", proj_precision=("fp64",), synth_precision="fp64",
+                             last_synth_precision="fp64")
     if n_iter == 0:
         b = sph.forward_hip(f, L, 0, nside, "healpix", True, plan=plan)
     else:
@@ -290,7 +294,8 @@ def test_cg_with_band_floor_and_gl(rng):
     hist = []
     out = refine.refine_cg(f, L, 0, nside, "healpix", True, 4, L_lower=L_lower, plan=plan,
                            init_precision="fp64", proj_precision=("fp64",),
-                           synth_precision="fp64", history=hist, flm_true=flm)
+                           synth_precision="fp64",
+                             last_synth_precision="fp64", history=hist, flm_true=flm)
     assert float(out[:, :L_lower].abs().max()) == 0.0
     assert hist[-1] < 1e-3 * hist[0]
     L2 = 24
@@ -300,6 +305,7 @@ def test_cg_with_band_floor_and_gl(rng):
     if f2.is_complex():
         f2 = f2.real.contiguous()
     out2 = refine.refine_cg(f2, L2, 0, None, "gl", True, 1, init_precision="fp64",
-                            proj_precision=("fp64",), synth_precision="fp64")
+                            proj_precision=("fp64",), synth_precision="fp64",
+                             last_synth_precision="fp64")
     # GL is exact: the first projection already is the answer, CG keeps it
     assert float((out2 - flm2).abs().max() / flm2.abs().max()) < 1e-12
