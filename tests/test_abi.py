@@ -1,5 +1,6 @@
 """C ABI and host-side checks that need no GPU."""
 
+import ctypes
 import re
 from pathlib import Path
 
@@ -83,3 +84,32 @@ def test_precompute_api_errors_and_kernel_handles():
     assert k.reality is False and k.spin == 2 and k.L == 8
     with pytest.raises(ValueError):
         pspin._check_kernel(k, 9, 2, "mw", None, False, True)
+
+
+def test_cg_entry_points_reject_bad_arguments_without_gpu():
+    lib = _lib.lib()
+    inv = _lib.S2H_ERR_INVALID
+    assert lib.s2h_ring_residual_scaled(None, None, None, None, None, None, None, 1, None) == inv
+    assert lib.s2h_ring_dot(None, None, None, None, None, None, 1, None) == inv
+    assert lib.s2h_ring_dot_workspace_bytes(None, 4) == 0
+    assert lib.s2h_flm_dots_workspace_bytes(0, 4) == 0
+    assert lib.s2h_flm_dots_workspace_bytes(5, 4) == 0
+    assert lib.s2h_flm_dots_workspace_bytes(2, 0) == 0
+    assert lib.s2h_flm_dots_workspace_bytes(2, 3) > 0
+    vecs = (ctypes.c_void_p * 1)(None)
+    assert lib.s2h_flm_dots(None, vecs, 0, 10, 0, 1, None, None, None) == inv  # nv < 1
+    assert lib.s2h_flm_dots(None, vecs, 5, 10, 0, 1, None, None, None) == inv  # nv > 4
+    assert lib.s2h_flm_dots(None, vecs, 1, 10, 0, 1, None, None, None) == inv  # null z / out
+    assert lib.s2h_flm_dots(None, vecs, 1, 10, 0, 0, None, None, None) == _lib.S2H_OK  # no maps
+    assert lib.s2h_cg_update(None, None, None, None, None, None, None, None, 10, 1, None) == inv
+    assert lib.s2h_cg_update(None, None, None, None, None, None, None, None, 0, 1, None) == _lib.S2H_OK
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import subprocess
+    import sys
+    root = Path(__file__).resolve().parents[1]
+    r = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"], cwd=root,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and not r.stdout.strip()
