@@ -225,9 +225,14 @@ int s2h_ring_residual(const s2h_plan *plan, const void *G, const void *Y, void *
  *     array of device pointers); w_j = 1 for j < packed_L and 2 beyond (the
  *     packed layout of real maps: the m = 0 block first), all 1 when
  *     packed_L = 0.  Fixed summation order.  work: s2h_flm_dots_workspace_bytes.
- *   s2h_cg_update: x += alpha[b] p; then, if p_new is given,
- *     p_new = z - g1[b] p - g2[b] p_prev (p_prev and g2 optional; p_new may
- *     be z, not p or x). */
+ *   s2h_cg_update: x += alpha[b] p (x may be null); then, if p_new is
+ *     given, p_new = z - g1[b] p - g2[b] p_prev (p_prev and g2 optional;
+ *     p_new may be z, not p or x).
+ *   s2h_cg_finish: x + sum_{j < nd} alpha_j[b] p_j (nd <= 4, the terms in
+ *     the order j = 0, 1, ...), into x, or with flm_out (packed real maps,
+ *     n = L(L+1)/2) unpacked to the reference layout batch x (L, 2L-1) as
+ *     s2h_flm_unpack does: the deferred coefficient updates and the unpack
+ *     in one pass. */
 int s2h_ring_residual_scaled(const s2h_plan *plan, const void *G, const void *Y,
                              const double *alpha, void *R, double *sumsq, void *work, int batch,
                              s2h_stream stream);
@@ -240,6 +245,8 @@ int s2h_flm_dots(const void *z, const void *const *v, int nv, int64_t n, int pac
 int s2h_cg_update(void *x, const void *p, const double *alpha, void *p_new, const void *z,
                   const double *g1, const void *p_prev, const double *g2, int64_t n, int batch,
                   s2h_stream stream);
+int s2h_cg_finish(void *x, const void *const *p, const double *const *alpha, int nd, int64_t n,
+                  int L, void *flm_out, int batch, s2h_stream stream);
 
 /* Per-map sums of squares out[b] = sum_{i < n} x[b * stride + i]^2 (doubles;
  * a complex map is 2n doubles), with a fixed summation order.  No

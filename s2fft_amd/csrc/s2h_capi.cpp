@@ -862,8 +862,8 @@ extern "C" int s2h_cg_update(void* x, const void* p, const double* alpha, void* 
                              const double* g2, int64_t n, int batch, s2h_stream stream) {
   if (batch < 0 || n < 0) return S2H_ERR_INVALID;
   if (batch == 0 || n == 0) return S2H_OK;
-  if (!x || !p || !alpha || (p_new && (!z || !g1)) || (p_prev && (!g2 || !p_new)) || p_new == p ||
-      p_new == x)
+  if (!p || (!x && !p_new) || (x && !alpha) || (p_new && (!z || !g1)) ||
+      (p_prev && (!g2 || !p_new)) || (p_new && (p_new == p || p_new == x)))
     return S2H_ERR_INVALID;
   CgUpdateArgs a{};
   a.x = static_cast<cplx*>(x);
@@ -877,6 +877,26 @@ extern "C" int s2h_cg_update(void* x, const void* p, const double* alpha, void* 
   a.n = n;
   a.stride = n;
   return hip_status(launch_cg_update(a, batch, static_cast<hipStream_t>(stream)));
+}
+
+extern "C" int s2h_cg_finish(void* x, const void* const* p, const double* const* alpha, int nd,
+                             int64_t n, int L, void* flm_out, int batch, s2h_stream stream) {
+  if (batch < 0 || n < 0 || nd < 0 || nd > kCgFinishMax) return S2H_ERR_INVALID;
+  if (flm_out && (L < 1 || n != (int64_t)L * (L + 1) / 2)) return S2H_ERR_INVALID;
+  if (batch == 0 || n == 0) return S2H_OK;
+  if (!x || (nd > 0 && (!p || !alpha)) || flm_out == x) return S2H_ERR_INVALID;
+  CgFinishArgs a{};
+  a.x = static_cast<cplx*>(x);
+  for (int j = 0; j < nd; ++j) {
+    if (!p[j] || !alpha[j]) return S2H_ERR_INVALID;
+    a.p[j] = static_cast<const cplx*>(p[j]);
+    a.alpha[j] = alpha[j];
+  }
+  a.nd = nd;
+  a.n = n;
+  a.L = L;
+  a.dst = static_cast<cplx*>(flm_out);
+  return hip_status(launch_cg_finish(a, batch, static_cast<hipStream_t>(stream)));
 }
 
 extern "C" int s2h_probe_mfma_f64(const double* a, const double* b, double* out,

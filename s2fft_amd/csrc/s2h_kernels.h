@@ -212,7 +212,7 @@ struct FlmDotArgs {
 };
 hipError_t launch_flm_dots(const FlmDotArgs& a, int batch, hipStream_t s);
 struct CgUpdateArgs {
-  cplx* x;
+  cplx* x;                  // optional
   const cplx* p;
   const double* alpha;
   cplx* p_new;              // null: x only
@@ -224,6 +224,17 @@ struct CgUpdateArgs {
   int64_t stride;
 };
 hipError_t launch_cg_update(const CgUpdateArgs& a, int batch, hipStream_t s);
+constexpr int kCgFinishMax = 4;    // directions per launch_cg_finish
+struct CgFinishArgs {
+  cplx* x;                  // n per map; updated in place when dst is null
+  const cplx* p[kCgFinishMax];
+  const double* alpha[kCgFinishMax];
+  int nd;
+  int64_t n;
+  int L;                    // dst: packed real maps of band-limit L, n = L(L+1)/2
+  cplx* dst;                // optional: x + sum alpha p unpacked to (L, 2L-1)
+};
+hipError_t launch_cg_finish(const CgFinishArgs& a, int batch, hipStream_t s);
 constexpr int kSumsqBlocks = 256;  // slices per map of launch_batch_sumsq
 // out[b] = sum_{i < n} x[b * stride + i]^2; partial holds kSumsqBlocks * batch doubles
 hipError_t launch_batch_sumsq(const double* x, int64_t n, int64_t stride, int batch,

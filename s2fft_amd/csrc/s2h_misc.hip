@@ -292,21 +292,23 @@ hipError_t launch_flm_dots(const FlmDotArgs& a, int batch, hipStream_t s) {
   return hipGetLastError();
 }
 
-// x += alpha p; p_new = z - g1 p - g2 p_prev (p_new may be z; p_prev, g2 optional;
-// p_new null: only x)
+// x += alpha p (x optional); p_new = z - g1 p - g2 p_prev (p_new may be z;
+// p_prev, g2 optional; p_new null: only x)
 __global__ void __launch_bounds__(kThreads) cg_update_kernel(CgUpdateArgs a) {
   const int b = blockIdx.y;
   const int64_t base = (int64_t)b * a.stride;
-  const double al = a.alpha[b];
+  const double al = a.x ? a.alpha[b] : 0.0;
   const double g1 = a.p_new ? a.g1[b] : 0.0;
   const double g2 = a.p_prev ? a.g2[b] : 0.0;
   for (int64_t j = (int64_t)blockIdx.x * kThreads + threadIdx.x; j < a.n;
        j += (int64_t)gridDim.x * kThreads) {
     const cplx pv = a.p[base + j];
-    cplx xv = a.x[base + j];
-    xv.x = fma(al, pv.x, xv.x);
-    xv.y = fma(al, pv.y, xv.y);
-    a.x[base + j] = xv;
+    if (a.x) {
+      cplx xv = a.x[base + j];
+      xv.x = fma(al, pv.x, xv.x);
+      xv.y = fma(al, pv.y, xv.y);
+      a.x[base + j] = xv;
+    }
     if (a.p_new) {
       const cplx zv = a.z[base + j];
       cplx nv = {fma(-g1, pv.x, zv.x), fma(-g1, pv.y, zv.y)};
@@ -325,6 +327,71 @@ hipError_t launch_cg_update(const CgUpdateArgs& a, int batch, hipStream_t s) {
   const int64_t want = (a.n + kThreads * 4 - 1) / (kThreads * 4);
   const unsigned gx = (unsigned)(want < 1024 ? want : 1024);
   hipLaunchKernelGGL(cg_update_kernel, dim3(gx, batch), dim3(kThreads), 0, s, a);
+  return hipGetLastError();
+}
+
+// x + sum_j alpha_j[b] p_j, the terms added in the order j = 0, 1, ... (the
+// same roundings as one x += alpha_j p_j per step)
+__device__ __forceinline__ cplx cg_sum(const CgFinishArgs& a, int b, int64_t idx) {
+  cplx v = a.x[idx];
+  for (int j = 0; j < a.nd; ++j) {
+    const double al = a.alpha[j][b];
+    const cplx pv = a.p[j][idx];
+    v.x = fma(al, pv.x, v.x);
+    v.y = fma(al, pv.y, v.y);
+  }
+  return v;
+}
+
+__global__ void __launch_bounds__(kThreads) cg_finish_kernel(CgFinishArgs a) {
+  const int b = blockIdx.y;
+  const int64_t base = (int64_t)b * a.n;
+  for (int64_t j = (int64_t)blockIdx.x * kThreads + threadIdx.x; j < a.n;
+       j += (int64_t)gridDim.x * kThreads)
+    a.x[base + j] = cg_sum(a, b, base + j);
+}
+
+// the tiles of flm_unpack_kernel, reading x + sum alpha_j p_j
+__global__ void __launch_bounds__(kThreads) cg_finish_unpack_kernel(CgFinishArgs a) {
+  __shared__ cplx tile[32][33];  // [m_local][l_local]
+  const int L = a.L;
+  const int l0 = blockIdx.x * 32, m0 = blockIdx.y * 32;
+  const int b = blockIdx.z;
+  const int64_t base = (int64_t)b * a.n;
+  cplx* dp = a.dst + (int64_t)b * ((int64_t)L * (2 * L - 1));
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int m = m0 + ty + 8 * k, l = l0 + tx;
+    cplx v = {0.0, 0.0};
+    if (l < L && m <= l) v = cg_sum(a, b, base + packed_moff(L, m) + l - m);
+    tile[ty + 8 * k][tx] = v;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int l = l0 + ty + 8 * k, m = m0 + tx;
+    if (l >= L || m >= L) continue;
+    const cplx t = tile[tx][ty + 8 * k];
+    cplx* row = dp + (int64_t)l * (2 * L - 1) + (L - 1);
+    row[m] = t;
+    if (m > 0) {
+      const double sg = (m & 1) ? -1.0 : 1.0;
+      row[-m] = cplx{sg * t.x, -sg * t.y};
+    }
+  }
+}
+
+hipError_t launch_cg_finish(const CgFinishArgs& a, int batch, hipStream_t s) {
+  if (batch <= 0 || a.n <= 0) return hipSuccess;
+  if (a.dst) {
+    const unsigned t = (unsigned)((a.L + 31) / 32);
+    hipLaunchKernelGGL(cg_finish_unpack_kernel, dim3(t, t, batch), dim3(kThreads), 0, s, a);
+  } else {
+    const int64_t want = (a.n + kThreads * 4 - 1) / (kThreads * 4);
+    const unsigned gx = (unsigned)(want < 1024 ? want : 1024);
+    hipLaunchKernelGGL(cg_finish_kernel, dim3(gx, batch), dim3(kThreads), 0, s, a);
+  }
   return hipGetLastError();
 }
 

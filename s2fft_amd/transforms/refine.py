@@ -192,6 +192,10 @@ def refine_cg(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=None
     rs, ps, rz_prev = [r], [z], None
     # per map: whether direction p_{k-1} / p_k is a kept learned direction
     kept_prev = kept = None
+    # x += alpha_k p_k is deferred (one pass over x for several directions,
+    # fused with the unpack at the end) unless every iterate is tracked
+    eager = history is not None
+    pending = []
     for k in range(n_iter):
         p = ps[-1]
         last = k + 1 == n_iter
@@ -199,9 +203,13 @@ def refine_cg(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=None
                                   packed)
         pq, alpha = sph.ring_dot_hip(Y, plan, num=rz)
         if last:
-            sph.cg_update_hip(x, p, alpha)
-            _track(history, x, flm_true, unpack)
-            break
+            if eager:
+                sph.cg_update_hip(x, p, alpha)
+                _track(history, x, flm_true, unpack)
+                break
+            pending.append((p, alpha))
+            return sph.cg_finish_hip(x, [q for q, _ in pending], [a for _, a in pending],
+                                     unpack_L=L if packed else None)
         n_new = None
         if judged(k) or judged(k + 1):
             _, n_new = sph.ring_residual_hip(R, Y, plan, out=R, alpha=alpha, sumsq=True)
@@ -229,7 +237,14 @@ def refine_cg(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=None
         g1 = (d[0] - d[1]) / rz
         g2 = (d[2] - d[0]) / rz_prev * kept_prev if back2 else None
         p_new = ps[0] if len(ps) == 3 else torch.empty_like(p)
-        sph.cg_update_hip(x, p, alpha, p_new, z, g1, ps[-2] if back2 else None, g2)
+        if not eager and len(ps) == 3:
+            # ps[0] is reused below: its pending update goes into x first
+            sph.cg_finish_hip(x, [q for q, _ in pending], [a for _, a in pending])
+            pending = []
+        sph.cg_update_hip(x if eager else None, p, alpha, p_new, z, g1,
+                          ps[-2] if back2 else None, g2)
+        if not eager:
+            pending.append((p, alpha))
         _track(history, x, flm_true, unpack)
         kept_prev, kept = kept, None
         rz_prev, rz = rz, d[1]

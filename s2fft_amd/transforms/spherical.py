@@ -397,21 +397,43 @@ def flm_dots_hip(z, vs, packed_L=0):
 
 
 def cg_update_hip(x, p, alpha, p_new=None, z=None, g1=None, p_prev=None, g2=None):
-    """x += alpha p, and p_new = z - g1 p - g2 p_prev (per-map float64 scalars
-    on the device; p_prev optional; p_new may be z)."""
-    B = x.shape[0]
-    n = x[0].numel() if B else 0
+    """x += alpha p (x may be None), and p_new = z - g1 p - g2 p_prev (per-map
+    float64 scalars on the device; p_prev optional; p_new may be z)."""
+    B = p.shape[0]
+    n = p[0].numel() if B else 0
     for t in (x, p, p_new, z, p_prev):
-        if t is not None and (not t.is_contiguous() or t.shape != x.shape):
+        if t is not None and (not t.is_contiguous() or t.shape != p.shape):
             raise ValueError("cg_update: contiguous tensors of one shape")
 
     def ptr(t):
         return t.data_ptr() if t is not None else None
 
     _lib.check(_lib.lib().s2h_cg_update(ptr(x), ptr(p), ptr(alpha), ptr(p_new), ptr(z), ptr(g1),
-                                        ptr(p_prev), ptr(g2), n, B, _stream(x.device)),
+                                        ptr(p_prev), ptr(g2), n, B, _stream(p.device)),
                "cg update")
     return x
+
+
+def cg_finish_hip(x, ps, alphas, unpack_L=None):
+    """x + sum_j alphas[j] ps[j] (per-map float64 scalars, at most 4 terms, added
+    in order): in place into x, or, with ``unpack_L``, from the packed layout of
+    real maps into a new (B, L, 2L-1) tensor in the reference layout."""
+    B = x.shape[0]
+    n = x[0].numel() if B else 0
+    if len(ps) != len(alphas) or len(ps) > 4:
+        raise ValueError("cg_finish: up to 4 (direction, step) pairs")
+    for t in ps:
+        if not t.is_contiguous() or t.shape != x.shape:
+            raise ValueError("cg_finish: contiguous directions shaped like x")
+    out = None
+    if unpack_L is not None:
+        out = torch.empty(_flm_shape(B, unpack_L, False), dtype=torch.complex128, device=x.device)
+    pp = (ctypes.c_void_p * max(len(ps), 1))(*[t.data_ptr() for t in ps])
+    aa = (ctypes.c_void_p * max(len(ps), 1))(*[a.contiguous().data_ptr() for a in alphas])
+    _lib.check(_lib.lib().s2h_cg_finish(x.data_ptr(), pp, aa, len(ps), n,
+                                        int(unpack_L or 0), out.data_ptr() if out is not None
+                                        else None, B, _stream(x.device)), "cg finish")
+    return out if out is not None else x
 
 
 def forward_iterative_hip(f, L, spin=0, nside=None, sampling="mw", reality=False, L_lower=0,

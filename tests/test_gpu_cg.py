@@ -309,3 +309,29 @@ def test_cg_with_band_floor_and_gl(rng):
                              last_synth_precision="fp64")
     # GL is exact: the first projection already is the answer, CG keeps it
     assert float((out2 - flm2).abs().max() / flm2.abs().max()) < 1e-12
+
+
+@pytest.mark.parametrize("nd", [0, 1, 3, 4])
+def test_cg_finish_matches_torch(rng, nd):
+    """x + sum alpha_j p_j in place, and unpacked to the reference layout."""
+    L, B = 37, 3
+    flm = torch.from_numpy(np.stack([signals.generate_flm(rng, L, 0, 0, True)
+                                     for _ in range(B)])).cuda()
+    x = sph.flm_pack_hip(flm, L)
+    g = torch.Generator(device="cuda").manual_seed(nd)
+    ps = [sph.flm_pack_hip(torch.from_numpy(np.stack([signals.generate_flm(rng, L, 0, 0, True)
+                                                      for _ in range(B)])).cuda(), L)
+          for _ in range(nd)]
+    als = [torch.randn(B, generator=g, device="cuda", dtype=torch.float64) for _ in range(nd)]
+    ref = x.clone()
+    for pj, aj in zip(ps, als):
+        ref = ref + aj[:, None] * pj
+    full = sph.cg_finish_hip(x, ps, als, unpack_L=L)
+    assert torch.allclose(full, sph.flm_unpack_hip(ref, L), rtol=1e-14, atol=1e-14)
+    xi = x.clone()
+    sph.cg_finish_hip(xi, ps, als)
+    assert torch.allclose(xi, ref, rtol=1e-14, atol=1e-14)
+    with pytest.raises(ValueError):
+        sph.cg_finish_hip(x, ps + [x] * (5 - nd), als + [als[0] if als else
+                                                         torch.ones(B, dtype=torch.float64,
+                                                                    device="cuda")] * (5 - nd))
