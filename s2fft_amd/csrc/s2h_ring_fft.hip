@@ -682,6 +682,30 @@ __global__ void __launch_bounds__(TPB) ring_alias_kernel(RingAliasArgs a) {
           });
       if (!a.part) return;
     }
+  } else if (herm && a.m_first == 0 && n >= a.L) {
+    // Hermitian spectra stored for m in [0, L) on a ring with L <= n < 2L - 1:
+    // the only alias of m is m - n, whose coefficient is conj Y[n - m], so
+    // A(Y)[m] = w n (Y[m] + w^{-1} conj Y[n - m]) for m >= n - L + 1, else w n Y[m]
+    const int e = (int)rint(n * job.phi0_over_pi);
+    const double sgn = (a.apply_phase && (e & 1)) ? -1.0 : 1.0;
+    const int m_pair = n - a.L + 1;
+    for (int m = t; m < a.ncol; m += TPB) {
+      const cplx yv = y[m];
+      cplx av = yv;
+      if (m >= m_pair) {
+        const cplx yp = y[n - m];
+        av = {fma(sgn, yp.x, av.x), fma(-sgn, yp.y, av.y)};
+      }
+      if constexpr (DOT) {
+        dsum += (m > 0 ? 2.0 * wn : wn) * (yv.x * av.x + yv.y * av.y);
+      } else {
+        const cplx gv = g[m];
+        const cplx rv = {gv.x - wn * av.x, gv.y - wn * av.y};
+        r[m] = rv;
+        dsum = fma(rv.x, rv.x, fma(rv.y, rv.y, dsum));
+      }
+    }
+    if (!DOT && !a.part) return;
   } else {
     const int e = (int)rint(n * job.phi0_over_pi);
     const double sgn = (a.apply_phase && (e & 1)) ? -1.0 : 1.0;
