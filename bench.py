@@ -80,6 +80,9 @@ def parse():
                     help="mixed leg: residuals on the pixels (two ring FFTs per step) instead of "
                          "on the ring spectra")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kernels-once", action="store_true",
+                    help="run each transform kind once at the bench workload, plus a 1 GiB "
+                         "device copy, and exit (for rocprofv3 --pmc counter passes)")
     return ap.parse_args()
 
 
@@ -187,6 +190,46 @@ def roofline(kernels, nside, L, B):
             "bytes_per_launch": nbytes, "avg_launch_ms": avg_ms}
 
 
+def kernels_once(nside, B):
+    """One launch of each transform kind at the bench workload, for counters.
+
+    FP64, FP32 ("mixed"), split-BF16 and single-BF16 forward and inverse, the
+    learned preconditioner, the ring-spectra pieces, one learned CG refinement
+    (3 steps) and a plain 1 GiB device copy, whose FETCH_SIZE / WRITE_SIZE
+    readings calibrate the HBM byte counts.
+    """
+    import torch
+
+    from s2fft_amd.learned.precond import MLPPreconditioner
+    from s2fft_amd.transforms import refine
+    from s2fft_amd.transforms import spherical as sph
+
+    L = 2 * nside
+    dev = torch.device("cuda:0")
+    plan = sph.get_plan("healpix", L, nside, 0, 0, True, device=dev)
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    f = torch.randn((B, 12 * nside * nside), generator=g, dtype=torch.float64, device=dev)
+    flm = sph.forward_hip(f, L, 0, nside, "healpix", True, plan=plan)
+    sph.inverse_hip(flm, L, 0, nside, "healpix", True, plan=plan)
+    for prec in ("mixed", "bf16", "bf16x1"):
+        sph.forward_hip(f, L, 0, nside, "healpix", True, plan=plan, precision=prec)
+    for prec in ("mixed", "bf16", "bf16x1"):
+        sph.inverse_hip(flm, L, 0, nside, "healpix", True, plan=plan, precision=prec)
+    pc = MLPPreconditioner.load(device=dev)
+    pc.apply(flm, L, True, 0)
+    G = sph.ring_analysis_hip(f, plan)
+    R = sph.ring_residual_hip(G, sph.inverse_rings_hip(flm, plan), plan)
+    sph.forward_rings_hip(R, L, plan, "bf16x1")
+    sph.inverse_hip(flm, L, 0, nside, "healpix", True, plan=plan, real_output=True)
+    refine.refine_cg(f, L, 0, nside, "healpix", True, 3, plan=plan, precond=pc, learned_steps=1)
+    src = torch.empty(1 << 27, dtype=torch.float64, device=dev)  # 1 GiB
+    src.fill_(1.0)
+    torch.empty_like(src).copy_(src)
+    torch.cuda.synchronize()
+    print("bench.py --kernels-once: done", flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -206,6 +249,9 @@ def main():
     L = 2 * nside
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.kernels_once:
+        kernels_once(nside, B)
+        return
 
     log(f"[bench] generating {B} maps nside={nside} L={L}")
     flm_np = np.stack([generate_flm(np.random.default_rng(args.seed0 + i), L, reality=True)
