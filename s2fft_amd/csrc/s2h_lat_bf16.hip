@@ -373,7 +373,8 @@ __global__ void __launch_bounds__(kIT, LO ? 1 : 3) lat_inverse_bf16_kernel(LatAr
   __shared__ __attribute__((aligned(16))) __bf16 bth[2][32 * kIRow];
   __shared__ __attribute__((aligned(16))) __bf16 btl[2][kLoB];
   __shared__ lat::ChunkCoef ccs[2][lat::kChunk];  // recursion coefficients, double-buffered
-  const int m = m_of_block(blockIdx.x, a.nm, a.m_lo);
+  int grp;
+  const int m = lat::m_group_of_block(blockIdx.x, a.nm, (a.nrings + kIT - 1) / kIT, a.m_lo, grp);
   const int am = m < 0 ? -m : m;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int row = lane & 15, kg = lane >> 4;
@@ -385,7 +386,7 @@ __global__ void __launch_bounds__(kIT, LO ? 1 : 3) lat_inverse_bf16_kernel(LatAr
   __bf16* dh = dth[wave];
   __bf16* dl = dtl[LO ? wave : 0];
 
-  const int pbase = (blockIdx.y * kIW + wave) * 64;
+  const int pbase = (grp * kIW + wave) * 64;
   const int p = pbase + lane;
   double x = 0.0, d = 0.0, dm1 = 0.0;
   int ls = 0;
@@ -602,7 +603,7 @@ hipError_t launch_lat_forward_bf16(const LatArgs& a, bool paired, int split, hip
 
 template <bool LO>
 static hipError_t lat_inv_bf16(const LatArgs& a, bool paired, hipStream_t s) {
-  dim3 grid(a.nm, (a.nrings + kIT - 1) / kIT, (a.batch + kIMaps - 1) / kIMaps);
+  dim3 grid(a.nm * ((a.nrings + kIT - 1) / kIT), 1, (a.batch + kIMaps - 1) / kIMaps);
   if (paired)
     hipLaunchKernelGGL((lat_inverse_bf16_kernel<true, LO>), grid, dim3(kIT), 0, s, a);
   else

@@ -79,6 +79,29 @@ __device__ __forceinline__ int m_of_block(int i, int nm, int m_lo) {
   return (idx & 1) ? -((idx + 1) >> 1) : (idx >> 1);
 }
 
+// Block i of a launch over nm m-values x G ring groups (grid.x = nm G) ->
+// (m, ring group g).  Blocks are dispatched round-robin over the 8 XCDs by
+// linear index, so i & 7 is the XCD and i >> 3 the order on it: the G groups
+// of one m are consecutive blocks of one XCD.  They run together and read the
+// m's harmonic and recursion coefficients through that XCD's L2, not HBM.  The m-values
+// follow m_of_block.  A bijection of [0, nm G) onto [0, nm) x [0, G).
+__device__ __forceinline__ int m_group_of_block(int i, int nm, int G, int m_lo, int& g) {
+  const int nm8 = nm & ~7;
+  int ip;
+  if (i < nm8 * G) {
+    const int xcd = i & 7, j = i >> 3;
+    const int k = j / G;
+    g = j - k * G;
+    ip = 8 * k + xcd;
+  } else {
+    const int r = i - nm8 * G;
+    const int k = r / G;
+    g = r - k * G;
+    ip = nm8 + k;
+  }
+  return m_of_block(ip, nm, m_lo);
+}
+
 // Recursion coefficients of one 32-step chunk, staged in LDS so that every
 // step reads them with a broadcast LDS load instead of a dependent global or
 // scalar load.  (a, -b * bsign, c) for l = blk + i; zero from l = L - 1 on,
