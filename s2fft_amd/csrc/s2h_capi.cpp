@@ -29,6 +29,7 @@ LatArgs lat_args(const s2h_plan* p, bool forward, int batch) {
   a.nrings = forward ? p->n_fwd_rings : p->n_inv_rings;
   a.mstart = forward ? p->fwd_mstart : p->inv_mstart;
   a.coef = forward ? p->fwd_coef : p->inv_coef;
+  a.ucoef = forward ? p->fwd_ucoef : p->inv_ucoef;
   a.L = p->d.L;
   a.m_lo = p->m_lo;
   a.nm = p->nm;

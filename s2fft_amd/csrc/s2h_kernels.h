@@ -67,6 +67,7 @@ struct LatArgs {
   int nrings;               // entries of rings (pairs count as one)
   const MStart* mstart;     // indexed by m - m_lo
   const Coef* coef;         // [|m|][l]
+  const Coef* ucoef;        // [|m|][l], unit-c form {a', b', sigma, 0} (BF16 kernels)
   int L;
   int m_lo;                 // first m handled (0 or -(L-1))
   int nm;                   // number of m values handled

@@ -18,6 +18,11 @@
 //   Each lane writes its d-values as [pair][l], the A layout; the flm chunk
 //   is staged once per workgroup as [column][l], the B layout.  Even and odd
 //   l (relative to m) accumulate separately to serve theta and pi - theta.
+//
+// Both run the recursion in its unit-c form (the plan's unit coefficients):
+// e_{l+1} = (a'_l x - b'_l) e_l - e_{l-1}, two FP64 operations per step
+// instead of three, with d_l = sigma_l e_l.  sigma_l depends on l only, so
+// the forward scales its FP64 sums by it and the inverse its staged flm.
 #include "s2h_fft_lds.h"
 #include "s2h_kernels.h"
 #include "s2h_lat_common.h"
@@ -118,7 +123,7 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
   const int b0 = blockIdx.z * (8 * kFTiles);
   const MStart st = a.mstart[m - a.m_lo];
   const double bsign = m < 0 ? -1.0 : 1.0;
-  const Coef* __restrict__ cf = a.coef + (int64_t)am * L;
+  const Coef* __restrict__ cf = a.ucoef + (int64_t)am * L;  // unit-c: d_l = sigma_l e_l
   __bf16* dh0 = reinterpret_cast<__bf16*>(wbuf[wave]);  // tile j at dh0 + j * kTileE
   __bf16* dl = reinterpret_cast<__bf16*>(wbuf[wave] + NI * kTileB);
   float* ctw = reinterpret_cast<float*>(wbuf[wave]);
@@ -192,12 +197,13 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
     }
   }
   Coef cn = {0.0, 0.0, 0.0, 0.0};
-  if (threadIdx.x < lat::kChunk) ccs[0][threadIdx.x] = lat::load_coef(cf, st.ell0, threadIdx.x, L, bsign);
+  if (threadIdx.x < lat::kChunk)
+    ccs[0][threadIdx.x] = lat::finish_coef(lat::load_ucoef_raw(cf, st.ell0, threadIdx.x, L), bsign);
   __syncthreads();
   int buf = 0;
   for (int blk = st.ell0; blk < L; blk += 32, buf ^= 1) {
     const bool more = blk + 32 < L;
-    if (more && threadIdx.x < lat::kChunk) cn = lat::load_coef_raw(cf, blk + 32, threadIdx.x, L);
+    if (more && threadIdx.x < lat::kChunk) cn = lat::load_ucoef_raw(cf, blk + 32, threadIdx.x, L);
     const lat::ChunkCoef* cs = ccs[buf];
     f32x4 acc[kFTiles][2];
 #pragma unroll
@@ -207,17 +213,17 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
     }
 #pragma unroll
     for (int sg = 0; sg < NS; sg += NI) {
-      // 32 steps of sets sg .. sg + NI - 1; the values past the band-limit
-      // are zero (zero coefficients from l = L - 1 on).  The coefficients of
-      // each GS-step group are read from LDS while the previous group steps;
-      // steps i, i + 1 share one paired BF16 conversion.
+      // 32 unit-c steps e_{l+1} = (a' x - b') e_l - e_{l-1} of sets sg ..
+      // sg + NI - 1; the values past the band-limit are zero (zero
+      // coefficients from l = L - 1 on).  The coefficients of each GS-step
+      // group are read from LDS while the previous group steps; steps i, i + 1
+      // share one paired BF16 conversion.
       constexpr int GS = LO ? 4 : 2;
-      double ca[GS], cb[GS], cc[GS], na[GS], nb[GS], nc[GS];
+      double ca[GS], cb[GS], na[GS], nb[GS];
 #pragma unroll
       for (int e = 0; e < GS; ++e) {
         ca[e] = cs[e].a;
         cb[e] = cs[e].nb;
-        cc[e] = cs[e].c;
       }
 #pragma unroll
       for (int g = 0; g < 32 / GS; ++g) {
@@ -226,7 +232,6 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
           for (int e = 0; e < GS; ++e) {
             na[e] = cs[GS * (g + 1) + e].a;
             nb[e] = cs[GS * (g + 1) + e].nb;
-            nc[e] = cs[GS * (g + 1) + e].c;
           }
         }
         float fv[NI][GS];
@@ -237,7 +242,7 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
             const int s = sg + j;
             const float f = to_f32(d[s]);
             fv[j][e] = ls[s] == 0 ? f : 0.0f;
-            const double dn = fma(fma(ca[e], x[s], cb[e]), d[s], -cc[e] * dm1[s]);
+            const double dn = fma(fma(ca[e], x[s], cb[e]), d[s], -dm1[s]);
             dm1[s] = d[s];
             d[s] = dn;
           }
@@ -268,7 +273,6 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
         for (int e = 0; e < GS; ++e) {
           ca[e] = na[e];
           cb[e] = nb[e];
-          cc[e] = nc[e];
         }
       }
 #pragma unroll
@@ -320,7 +324,8 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
           ctw[(t * 2 + par) * 256 + (4 * kg + r) * 16 + row] = acc[t][par][r];
     if (more && threadIdx.x < lat::kChunk) ccs[buf ^ 1][threadIdx.x] = lat::finish_coef(cn, bsign);
     __syncthreads();
-    // element (t, parity, row i, col j) -> l = blk + 2 j + parity, map 8t + i/2
+    // element (t, parity, row i, col j) -> l = blk + 2 j + parity, map 8t + i/2;
+    // sigma_l (still in this chunk's coefficient buffer) turns e_l into d_l
     for (int e = threadIdx.x; e < kFTiles * 2 * 256; e += kFT) {
       const int t = e >> 9, par = (e >> 8) & 1, ri = (e >> 4) & 15, cj = e & 15;
       const int l = blk + 2 * cj + par;
@@ -328,6 +333,7 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
       double v = 0.0;
 #pragma unroll
       for (int w = 0; w < kFW; ++w) v += (double)reinterpret_cast<const float*>(wbuf[w])[e];
+      v *= cs[2 * cj + par].c;
       if (l < L && b0 + bm < a.batch && (l >= a.lmin_out || a.fill)) {
         if (l < a.lmin_out) v = 0.0;
         double* o = reinterpret_cast<double*>(out + (int64_t)bm * a.flm_map_stride +
@@ -382,7 +388,7 @@ __global__ void __launch_bounds__(kIT, LO ? 1 : 3) lat_inverse_bf16_kernel(LatAr
   const int b0 = blockIdx.z * kIMaps;
   const MStart st = a.mstart[m - a.m_lo];
   const double bsign = m < 0 ? -1.0 : 1.0;
-  const Coef* __restrict__ cf = a.coef + (int64_t)am * L;
+  const Coef* __restrict__ cf = a.ucoef + (int64_t)am * L;  // unit-c: d_l = sigma_l e_l
   __bf16* dh = dth[wave];
   __bf16* dl = dtl[LO ? wave : 0];
 
@@ -424,14 +430,15 @@ __global__ void __launch_bounds__(kIT, LO ? 1 : 3) lat_inverse_bf16_kernel(LatAr
       v[j] = x0;
     }
   };
-  // by parity of i, the other parity's entry zero
-  auto store_flm = [&](const double* v) {
+  // by parity of i, the other parity's entry zero; scaled by sigma_l (from
+  // the chunk's staged coefficients), since the A tile holds e_l = d_l / sigma_l
+  auto store_flm = [&](const double* v, const lat::ChunkCoef* cc) {
 #pragma unroll
     for (int j = 0; j < kStage; ++j) {
       const int e = threadIdx.x + kIT * j;
       const int col = e >> 5, i = e & 31;
       __bf16 h, lo;
-      split2(v[j], h, lo);
+      split2(v[j] * cc[i].c, h, lo);
       const int par = i & 1;
       bth[par][col * kIRow + i] = h;
       bth[par ^ 1][col * kIRow + i] = (__bf16)0.0f;
@@ -444,8 +451,10 @@ __global__ void __launch_bounds__(kIT, LO ? 1 : 3) lat_inverse_bf16_kernel(LatAr
   double fv[kStage];
   Coef cn = {0.0, 0.0, 0.0, 0.0};
   load_flm(st.ell0, fv);
-  store_flm(fv);
-  if (threadIdx.x < lat::kChunk) ccs[0][threadIdx.x] = lat::load_coef(cf, st.ell0, threadIdx.x, L, bsign);
+  if (threadIdx.x < lat::kChunk)
+    ccs[0][threadIdx.x] = lat::finish_coef(lat::load_ucoef_raw(cf, st.ell0, threadIdx.x, L), bsign);
+  __syncthreads();
+  store_flm(fv, ccs[0]);
   __syncthreads();
   int buf = 0;
   for (int blk = st.ell0; blk < L; blk += 32, buf ^= 1) {
@@ -453,20 +462,19 @@ __global__ void __launch_bounds__(kIT, LO ? 1 : 3) lat_inverse_bf16_kernel(LatAr
     const bool more = nxt < L;
     if (more) {
       load_flm(nxt, fv);
-      if (threadIdx.x < lat::kChunk) cn = lat::load_coef_raw(cf, nxt, threadIdx.x, L);
+      if (threadIdx.x < lat::kChunk) cn = lat::load_ucoef_raw(cf, nxt, threadIdx.x, L);
     }
-    // 32 recursion steps into the A tile, 8 l per 16-byte store.  The
-    // coefficients from l = L - 1 on are zero, so d vanishes past the
-    // band-limit.  Each 4-step group's coefficients are read from LDS while
+    // 32 unit-c recursion steps (e_l = d_l / sigma_l) into the A tile, 8 l
+    // per 16-byte store.  The coefficients from l = L - 1 on are zero, so e
+    // vanishes past the band-limit.  Each 4-step group's coefficients are read from LDS while
     // the previous group steps, keeping the LDS latency off the FP64 chain.
     const bool live = ls == 0;
     const lat::ChunkCoef* cs = ccs[buf];
-    double ca[4], cb[4], cc[4], na[4], nb[4], nc[4];
+    double ca[4], cb[4], na[4], nb[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       ca[e] = cs[e].a;
       cb[e] = cs[e].nb;
-      cc[e] = cs[e].c;
     }
 #pragma unroll
     for (int g8 = 0; g8 < 4; ++g8) {
@@ -479,14 +487,13 @@ __global__ void __launch_bounds__(kIT, LO ? 1 : 3) lat_inverse_bf16_kernel(LatAr
           for (int e = 0; e < 4; ++e) {
             na[e] = cs[4 * (g + 1) + e].a;
             nb[e] = cs[4 * (g + 1) + e].nb;
-            nc[e] = cs[4 * (g + 1) + e].c;
           }
         }
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const float f = to_f32(d);
           v[4 * h4 + e] = live ? f : 0.0f;
-          const double dn = fma(fma(ca[e], x, cb[e]), d, -cc[e] * dm1);
+          const double dn = fma(fma(ca[e], x, cb[e]), d, -dm1);
           dm1 = d;
           d = dn;
         }
@@ -494,7 +501,6 @@ __global__ void __launch_bounds__(kIT, LO ? 1 : 3) lat_inverse_bf16_kernel(LatAr
         for (int e = 0; e < 4; ++e) {
           ca[e] = na[e];
           cb[e] = nb[e];
-          cc[e] = nc[e];
         }
       }
       bf16x8 vh, vl;
@@ -537,7 +543,7 @@ __global__ void __launch_bounds__(kIT, LO ? 1 : 3) lat_inverse_bf16_kernel(LatAr
         }
     }
     __syncthreads();
-    if (more) store_flm(fv);
+    if (more) store_flm(fv, ccs[buf ^ 1]);
   }
   // C[i][j]: pair 16 q + 4 kg + r, column 16 t + row -> map t*8 + row/2
   // Parity is relative to l = blk offsets, i.e. to l0 = ell0, whose parity is

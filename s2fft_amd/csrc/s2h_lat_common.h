@@ -132,6 +132,16 @@ __device__ __forceinline__ ChunkCoef load_coef(const Coef* __restrict__ cf, int 
   return finish_coef(load_coef_raw(cf, blk, i, L), bsign);
 }
 
+// Unit-c coefficients {a', b', sigma, 0} (the plan's make_unit_coefs) of
+// chunk `blk` for lane i, read as stored up to l = L - 1 (a' and b' are
+// already zero there; sigma_{L-1} is needed), zero past it.
+__device__ __forceinline__ Coef load_ucoef_raw(const Coef* __restrict__ cf, int blk, int i, int L) {
+  const int l = blk + i;
+  Coef v = {0.0, 0.0, 0.0, 0.0};
+  if (l < L) v = cf[l];
+  return v;
+}
+
 // One recursion step with staged coefficients: returns d_{l+1}.
 __device__ __forceinline__ double rec_step(const ChunkCoef& c, double x, double d, double dm1) {
   return fma(fma(c.a, x, c.nb), d, -c.c * dm1);

@@ -252,6 +252,37 @@ void make_coefs(int L, int n, int kind, bool inverse, std::vector<Coef>& out) {
   }
 }
 
+// The same recursion with unit c: d_l = sigma_l e_l, where sigma_l0 =
+// sigma_{l0+1} = 1 and sigma_{l+1} = c_l sigma_{l-1}, turns
+//   d_{l+1} = (a_l x - b_l) d_l - c_l d_{l-1}
+// into e_{l+1} = (a'_l x - b'_l) e_l - e_{l-1} with (a', b') = (a, b) sigma_l /
+// sigma_{l+1}: one FP64 multiply less per step.  Entry l holds
+// {a'_l, b'_l, sigma_l, 0}; a' and b' are zero from l = L - 1 on (the steps
+// past the band-limit give zeros), sigma is kept up to l = L - 1.  c_l > 0
+// for l > l0, so sigma never vanishes; it varies slowly with l (the c_l are
+// near 1 away from l = |m|, |n|).
+void make_unit_coefs(int L, int n, const std::vector<Coef>& c, std::vector<Coef>& out) {
+  out.assign((size_t)L * L, Coef{0, 0, 0, 0});
+  for (int am = 0; am < L; ++am) {
+    const int l0 = std::max(am, std::abs(n));
+    if (l0 >= L) continue;
+    const Coef* cm = c.data() + (size_t)am * L;
+    Coef* um = out.data() + (size_t)am * L;
+    long double s_prev = 1.0L, s_cur = 1.0L;  // sigma_{l-1}, sigma_l at l = l0
+    for (int l = l0; l < L; ++l) {
+      const long double s_next = l == l0 ? 1.0L : (long double)cm[l].c * s_prev;
+      Coef u{0, 0, (double)s_cur, 0};
+      if (l + 1 < L) {
+        u.a = (double)((long double)cm[l].a * s_cur / s_next);
+        u.b = (double)((long double)cm[l].b * s_cur / s_next);
+      }
+      um[l] = u;
+      s_prev = s_cur;
+      s_cur = s_next;
+    }
+  }
+}
+
 // ---------------------------------------------------------------- FFT jobs
 int next_pow2(int v) {
   int p = 1;
@@ -556,6 +587,17 @@ static int plan_build(s2h_plan* p) {
     p->inv_coef = p->fwd_coef;
   } else if ((st = upload(p, ci, &p->inv_coef))) {
     return st;
+  }
+  {
+    std::vector<Coef> u;
+    make_unit_coefs(L, n, cf, u);
+    if ((st = upload(p, u, &p->fwd_ucoef))) return st;
+    if (same) {
+      p->inv_ucoef = p->fwd_ucoef;
+    } else {
+      make_unit_coefs(L, n, ci, u);
+      if ((st = upload(p, u, &p->inv_ucoef))) return st;
+    }
   }
   if ((st = upload(p, nf, &p->fwd_norm))) return st;
   if ((st = upload(p, ni, &p->inv_norm))) return st;

@@ -24,6 +24,7 @@ struct s2h_plan {
   int fwd_row_south = -1, fwd_row_north = -1;
   s2h::MStart* fwd_mstart = nullptr;
   s2h::Coef* fwd_coef = nullptr;
+  s2h::Coef* fwd_ucoef = nullptr;  // unit-c form (make_unit_coefs)
   double* fwd_norm = nullptr;
   double* fwd_weights = nullptr;  // MWSS-at-2L weights (kinds 1, 2)
   int rs_info[8] = {0};           // resampling DFT lengths / conv lengths
@@ -38,6 +39,7 @@ struct s2h_plan {
   int inv_row_south = -1, inv_row_north = -1;
   s2h::MStart* inv_mstart = nullptr;
   s2h::Coef* inv_coef = nullptr;
+  s2h::Coef* inv_ucoef = nullptr;
   double* inv_norm = nullptr;
   bool inv_out_complex = true;
 
