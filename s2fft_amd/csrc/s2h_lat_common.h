@@ -146,6 +146,10 @@ __device__ __forceinline__ Coef load_ucoef_raw(const Coef* __restrict__ cf, int 
 __device__ __forceinline__ double rec_step(const ChunkCoef& c, double x, double d, double dm1) {
   return fma(fma(c.a, x, c.nb), d, -c.c * dm1);
 }
+// The same step in unit-c form (unit coefficients staged): e_{l+1}.
+__device__ __forceinline__ double rec_step_unit(const ChunkCoef& c, double x, double e, double em1) {
+  return fma(fma(c.a, x, c.nb), e, -em1);
+}
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
