@@ -46,7 +46,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch at this workload from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE, scaled by a 1 GiB device copy's readings (x2.000 and x1.000):
 # profiles/counters_healpix_nside1024_L2048_b16.md
-TRAFFIC_BYTES = {"lat_forward": 6.80e9, "lat_inverse": 16.48e9}
+TRAFFIC_BYTES = {"lat_forward": 6.86e9, "lat_inverse": 5.72e9}
 
 
 def parse():
