@@ -108,11 +108,19 @@ __global__ void __launch_bounds__(kThreads) mlp_sumsq_kernel(MlpRowArgs a, doubl
 }
 
 // ---------------------------------------------------------------- inference
-// One wave handles 16 rows per pass: layer 1 on the vector ALUs straight into
-// the A-operand layout of the BF16 MFMA (A[i][k]: lane i + 16 (k/8 % 4),
-// element k % 8 of the 8-vector, for each 32-k block), layer 2 as 8 column
-// tiles x 4 k-blocks of V_MFMA_F32_16X16X32_BF16 with W2q staged in LDS in
-// the B-operand layout, layer 3 as a 16-lane shuffle reduction.
+// One wave handles 16 rows per pass, all three layers in registers:
+//  * layer 1 as 8 V_MFMA_F32_16X16X4F32, one per 16 hidden units (A = W1q,
+//    16 units x 4 features; B = the 16 rows' features, one per lane: lane
+//    r + 16 f holds feature f of row r), bias preloaded in the accumulator.
+//    The result holds unit 16 nt + 4 (lane/16) + i of row lane%16 in
+//    register i of tile nt.
+//  * layer 2 as 8 column tiles x 4 k-blocks of V_MFMA_F32_16X16X32_BF16.  Its
+//    A operand (A[i][k]: lane i + 16 (k/8 % 4), element k % 8 per 32-k
+//    block) is taken straight from layer 1's registers, which fixes the
+//    order of the hidden units along k: k = 32 kb + 8 q + e is unit
+//    32 kb + 16 (e/4) + 4 q + e%4.  W2q is staged in LDS in the B-operand
+//    layout with its columns in that order.
+//  * layer 3 as a 16-lane shuffle reduction.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
@@ -121,19 +129,27 @@ constexpr int kInfWaves = 4;
 __global__ void __launch_bounds__(kInfWaves * 64) mlp_apply_kernel(MlpRowArgs a,
                                                                    const float* __restrict__ w,
                                                                    const double* sumsq) {
-  // B operand of column tile ct, k-block kb for lane l: W2q[16 ct + l%16][32 kb + 8 (l/16) + e]
+  // B operand of column tile ct, k-block kb for lane l:
+  // W2q[16 ct + l%16][unit(32 kb + 8 (l/16) + e)], unit() as above
   __shared__ bf16x8 wb[8][4][64];
-  __shared__ float w1[kH * 4], b1[kH], b2[kH], w3[2 * kH];
+  // A operand of layer-1 tile nt for lane l: W1q[16 nt + l%16][l/16]
+  __shared__ float w1[8][64];
+  __shared__ __attribute__((aligned(16))) float b1[kH];
+  __shared__ float b2[kH], w3[2 * kH];
   __shared__ float b3[2];
   for (int e = threadIdx.x; e < 8 * 4 * 64; e += kInfWaves * 64) {
     const int ct = e >> 8, kb = (e >> 6) & 3, l = e & 63;
-    const int j = 16 * ct + (l & 15), k0 = 32 * kb + 8 * (l >> 4);
+    const int j = 16 * ct + (l & 15), q4 = 4 * (l >> 4);
     bf16x8 v;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) v[q] = (__bf16)w[oW2 + j * kH + k0 + q];
+    for (int q = 0; q < 8; ++q)
+      v[q] = (__bf16)w[oW2 + j * kH + 32 * kb + 16 * (q >> 2) + q4 + (q & 3)];
     wb[ct][kb][l] = v;
   }
-  for (int e = threadIdx.x; e < kH * 4; e += kInfWaves * 64) w1[e] = bf16_round(w[oW1 + e]);
+  for (int e = threadIdx.x; e < 8 * 64; e += kInfWaves * 64) {
+    const int nt = e >> 6, l = e & 63;
+    w1[nt][l] = bf16_round(w[oW1 + (16 * nt + (l & 15)) * 4 + (l >> 4)]);
+  }
   for (int e = threadIdx.x; e < kH; e += kInfWaves * 64) {
     b1[e] = w[oB1 + e];
     b2[e] = w[oB2 + e];
@@ -159,21 +175,36 @@ __global__ void __launch_bounds__(kInfWaves * 64) mlp_apply_kernel(MlpRowArgs a,
     // tiles (the compiler's choice without this barrier) they take ~400
     // registers and leave one wave per SIMD to hide every latency.
     asm volatile("" ::: "memory");
-    // features of row `row` (every lane of the 16-lane group computes them)
+    // (l, m) of row `row`; packed: from the tile's first row, stepping over
+    // the m-blocks (block m holds l = m .. L - 1)
     const int64_t p = tile * 16 + row;
     int l = 0, m = 0;
-    float x0 = 0.f, x1 = 0.f, x2 = 0.f, x3 = 0.f;
-    bool valid = false;
-    if (p < a.P) {
-      const int64_t idx = row_index(a, p, l, m);
-      valid = l >= a.lmin;
-      if (valid) {
-        const cplx v = h[idx];
-        x0 = (float)v.x * inv_sig;
-        x1 = (float)v.y * inv_sig;
-        x2 = (float)l * inv_L;
-        x3 = (float)m * inv_L;
+    int64_t idx = p;
+    if (a.packed) {
+      int l0, m0;
+      packed_lm(tile * 16, a.L, l0, m0);
+      l = l0 + row;
+      m = m0;
+      while (l >= a.L && m + 1 < a.L) {
+        l -= a.L - m - 1;
+        ++m;
       }
+    } else if (p < a.P) {
+      idx = row_index(a, p, l, m);
+    }
+    // feature kq of row `row`: (h_re / s, h_im / s, l / L, m / L)
+    float xf = 0.f;
+    if (p < a.P && l >= a.lmin) {
+      const cplx v = h[idx];
+      const float fl = (float)(kq == 2 ? l : m) * inv_L;
+      xf = kq == 0 ? (float)v.x * inv_sig : kq == 1 ? (float)v.y * inv_sig : fl;
+    }
+    // layer 1: z of unit 16 nt + 4 kq + i, row `row`, in register i of z1[nt]
+    f32x4 z1[8];
+#pragma unroll
+    for (int nt = 0; nt < 8; ++nt) {
+      z1[nt] = *reinterpret_cast<const f32x4*>(&b1[16 * nt + 4 * kq]);
+      z1[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(w1[nt][lane], xf, z1[nt], 0, 0, 0);
     }
     f32x4 acc[8];
 #pragma unroll
@@ -182,13 +213,7 @@ __global__ void __launch_bounds__(kInfWaves * 64) mlp_apply_kernel(MlpRowArgs a,
     for (int kb = 0; kb < 4; ++kb) {
       bf16x8 av;
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const int k = 32 * kb + 8 * kq + q;
-        const float z = fmaf(w1[4 * k + 3], x3,
-                             fmaf(w1[4 * k + 2], x2,
-                                  fmaf(w1[4 * k + 1], x1, fmaf(w1[4 * k], x0, b1[k]))));
-        av[q] = (__bf16)gelu(z);
-      }
+      for (int q = 0; q < 8; ++q) av[q] = (__bf16)gelu(z1[2 * kb + (q >> 2)][q & 3]);
 #pragma unroll
       for (int ct = 0; ct < 8; ++ct)
         acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, wb[ct][kb][lane], acc[ct], 0, 0, 0);
@@ -213,7 +238,10 @@ __global__ void __launch_bounds__(kInfWaves * 64) mlp_apply_kernel(MlpRowArgs a,
         y1[r] += __shfl_xor(y1[r], o);
       }
     }
-    // lane (kq, row = r) writes row 4 kq + r: pick register r == row
+    // lane (kq, row = r) writes row 4 kq + r: pick register r == row; its
+    // (l, m) come from lane 4 kq + r, which computed that row's features
+    const int src = 4 * kq + (row & 3);
+    const int lq = __shfl(l, src), mq = __shfl(m, src);
     if (row < 4) {
       float o0 = y0[0], o1 = y1[0];
 #pragma unroll
@@ -224,8 +252,7 @@ __global__ void __launch_bounds__(kInfWaves * 64) mlp_apply_kernel(MlpRowArgs a,
         }
       const int64_t q = tile * 16 + 4 * kq + row;
       if (q < a.P) {
-        int lq, mq;
-        const int64_t idx = row_index(a, q, lq, mq);
+        const int64_t idx = a.packed ? q : (int64_t)lq * (2 * a.L - 1) + a.L - 1 + mq;
         if (lq < a.lmin) {
           out[idx] = cplx{0.0, 0.0};
         } else {
