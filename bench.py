@@ -52,8 +52,8 @@ TRAFFIC_BYTES = {"lat_forward": 6.86e9, "lat_inverse": 5.72e9}
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--nside", type=int, default=1024)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--tol", type=float, default=1e-7)
