@@ -3,7 +3,8 @@
  *
  * Every entry point takes plain device pointers (FP64 / complex128 data laid
  * out exactly as the reference's NumPy arrays), a batch count and a HIP
- * stream.  Nothing on a launch path allocates or synchronises: the caller owns
+ * stream.  Nothing on a launch path allocates or synchronises (only plan
+ * creation and the explicit prepare / descriptor calls do): the caller owns
  * every input, output and workspace buffer; a plan owns only its constant
  * tables.  Plans are immutable after creation and may be shared by threads.
  * Every function returns an s2h status (0 = ok, negative = error; see
@@ -22,6 +23,8 @@
  *   healpix_fft_hip                lib/src/extensions.cc:26-100 (the XLA
  *                                  custom call healpix_fft_cuda) with the
  *                                  descriptor of lib/include/s2fft.h:29-59
+ *   s2h_build_healpix_fft_descriptor
+ *                                  lib/src/extensions.cc:115-149
  *   s2h_gamma_dft                  the FFT over gamma of
  *                                  s2fft/transforms/wigner.py:183-195,495-502
  *   s2h_wigner_reality_fill        s2fft/transforms/wigner.py:523-526
@@ -145,18 +148,54 @@ int s2h_lat_forward(const s2h_plan *plan, const void *ftm, void *flm, int batch,
 int s2h_lat_inverse(const s2h_plan *plan, const void *flm, void *ftm, int batch,
                     s2h_stream stream);
 
-/* HEALPix ring FFTs with the reference layouts: f (12 nside^2,) complex128,
- * ftm (4 nside - 1, 2L) complex128.  No plan needed (tables are cached). */
+/* HEALPix ring FFTs with the reference layouts: f (12 nside^2,) complex,
+ * ftm (4 nside - 1, 2L) complex; batch arrays back to back.
+ * s2h_healpix_fft_prepare builds the ring tables of (current device, nside,
+ * L) once -- it allocates and synchronises -- and every launch below only
+ * looks them up: S2H_ERR_INVALID if they were never prepared.
+ * s2h_healpix_fft / s2h_healpix_ifft: complex128, unnormalised in both
+ * directions (healpix_ffts.py:150-226 / 297-371, norm "backward" forward and
+ * "forward" inverse).  The _ex forms take the reference's fft_norm
+ * (lib/include/s2fft.h:3-6: the forward divides each ring by its length n
+ * under FORWARD, the inverse under BACKWARD, both by sqrt(n) under ORTHO,
+ * neither under NONE) and double_precision = 0 for complex64 buffers (the
+ * arithmetic stays FP64).  With reality the rings of length 2L use Re(f) and
+ * zero m < 0, as the NumPy/JAX paths do. */
+enum s2h_fft_norm {
+  S2H_FFT_NORM_FORWARD = 1,
+  S2H_FFT_NORM_BACKWARD = 2,
+  S2H_FFT_NORM_ORTHO = 3,
+  S2H_FFT_NORM_NONE = 4
+};
+int s2h_healpix_fft_prepare(int nside, int L);
 int s2h_healpix_fft(int nside, int L, int reality, const void *f, void *ftm, int batch,
                     s2h_stream stream);
 int s2h_healpix_ifft(int nside, int L, int reality, const void *ftm, void *f, int batch,
                      s2h_stream stream);
+int s2h_healpix_fft_ex(int nside, int L, int reality, int norm, int double_precision,
+                       const void *f, void *ftm, int batch, s2h_stream stream);
+int s2h_healpix_ifft_ex(int nside, int L, int reality, int norm, int double_precision,
+                        const void *ftm, void *f, int batch, s2h_stream stream);
 
-/* XLA custom-call compatible entry: buffers[0] = input, buffers[1] = output,
- * opaque = packed s2h_healpix_fft_descriptor. The input is never modified. */
+/* XLA custom-call entry (replaces healpix_fft_cuda, lib/src/extensions.cc:90):
+ * buffers[0] = input, buffers[1] = output, opaque = the bytes of an
+ * s2h_healpix_fft_descriptor made by s2h_build_healpix_fft_descriptor (which
+ * maps (forward, normalize) to a norm as build_healpix_fft_descriptor does,
+ * extensions.cc:115-149, and prepares the tables, so the call itself neither
+ * allocates nor synchronises).  The fields are those of the reference's
+ * s2fftDescriptor (lib/include/s2fft.h:29-59) as int32 plus a batch count
+ * (0 or 1: one array).  The input is never modified (the reference FFTs in
+ * place).  shift must be 1 (the reference always shifts). */
 typedef struct s2h_healpix_fft_descriptor {
-  int32_t nside, L, reality, forward, batch, reserved;
+  int32_t nside, L;          /* nside, harmonic_band_limit */
+  int32_t reality, forward;
+  int32_t norm;              /* enum s2h_fft_norm */
+  int32_t shift;
+  int32_t double_precision;  /* 1 complex128, 0 complex64 */
+  int32_t batch;
 } s2h_healpix_fft_descriptor;
+int s2h_build_healpix_fft_descriptor(int nside, int L, int reality, int forward, int normalize,
+                                     int double_precision, s2h_healpix_fft_descriptor *out);
 void healpix_fft_hip(void *stream, void **buffers, const char *opaque, size_t opaque_len);
 /* Status of the last healpix_fft_hip call on this thread (XLA's signature has
  * no return value). */

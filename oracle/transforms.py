@@ -46,6 +46,55 @@ def inverse(flm, L, spin=0, nside=None, sampling="mw", reality=False,
     return np.fft.ifft(np.fft.ifftshift(ftm, axes=1), axis=1, norm="forward")
 
 
+def inverse_rings(flm, L, rings, spin=0, nside=None, sampling="mw", reality=False):
+    """The inverse restricted to the rings ``rings``: their pixels, concatenated
+    in the order given (complex128 like ``inverse`` for HEALPix, and for
+    equiangular grids complex unless ``reality``).
+
+    The steps of s2fft/transforms/spherical.py:112-208 for a subset of rings;
+    every ring's latitudinal synthesis and azimuthal iDFT are independent of
+    the others (otf_recursions.py:15-171 recurses per theta;
+    healpix_ffts.py:335-371 folds and transforms per ring), so large grids
+    stay cheap on the CPU.  The MW / MWSS pole rows (set in closed form by the
+    reference) are not covered: rings must be interior rows there.
+    """
+    s = sampling.lower()
+    rings = np.asarray(rings)
+    if s == "mw":
+        assert np.all(rings < L - 1), "MW row L-1 is the south pole"
+    if s == "mwss":
+        assert np.all((rings > 0) & (rings < L)), "MWSS rows 0 and L are the poles"
+    beta = geo.thetas(L, s, nside)[rings]
+    off = 1 if s in ("mwss", "healpix") else 0
+    flm = np.array(flm, dtype=np.complex128, copy=True)
+    flm *= _norm(L, 0)[:, None]
+    ftm = pm.inverse_latitudinal_step(flm, beta, L, spin, nside, s, reality)
+    if s == "healpix":
+        start = L - 1 if reality else 0
+        ftm[:, start + off :] *= geo.ring_phase_shifts_hp(L, nside, False, reality)[rings]
+    ftm *= (-1.0) ** spin
+    out = []
+    if s != "healpix":
+        nphi = geo.nphi_equiang(L, s)
+        for k in range(len(rings)):
+            if reality:
+                out.append(np.fft.irfft(ftm[k, L - 1 + off :], nphi, norm="forward")
+                           .astype(np.complex128))
+            else:
+                out.append(np.fft.ifft(np.fft.ifftshift(ftm[k]), norm="forward"))
+        return np.concatenate(out)
+    if reality:
+        ftm[:, off : L - 1 + off] = np.flip(np.conj(ftm[:, L + off :]), axis=-1)
+    for k, t in enumerate(rings):
+        n = geo.nphi_ring(int(t), nside)
+        row = ftm[k] if n == 2 * L else ffts.folding_row(ftm[k], n, L)
+        if reality and n == 2 * L:
+            out.append(np.fft.irfft(row[n // 2 :], n, norm="forward").astype(np.complex128))
+        else:
+            out.append(np.fft.ifft(np.fft.ifftshift(row), norm="forward"))
+    return np.concatenate(out)
+
+
 def forward(f, L, spin=0, nside=None, sampling="mw", reality=False, precomps=None,
             L_lower=0):
     """Spin forward transform.  s2fft/transforms/spherical.py:467-588."""

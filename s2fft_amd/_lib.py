@@ -32,8 +32,13 @@ class Desc(ctypes.Structure):
 
 
 class HealpixFftDescriptor(ctypes.Structure):
+    """s2h_healpix_fft_descriptor: the reference's s2fftDescriptor fields as int32."""
     _fields_ = [(name, ctypes.c_int32) for name in
-                ("nside", "L", "reality", "forward", "batch", "reserved")]
+                ("nside", "L", "reality", "forward", "norm", "shift", "double_precision",
+                 "batch")]
+
+
+FFT_NORM_FORWARD, FFT_NORM_BACKWARD, FFT_NORM_ORTHO, FFT_NORM_NONE = 1, 2, 3, 4
 
 
 EXPORTED = (
@@ -41,6 +46,8 @@ EXPORTED = (
     "s2h_workspace_bytes", "s2h_inverse_output_is_complex", "s2h_forward", "s2h_inverse",
     "s2h_lat_forward", "s2h_lat_inverse", "s2h_healpix_fft", "s2h_healpix_ifft",
     "healpix_fft_hip", "healpix_fft_hip_last_status", "s2h_gamma_dft",
+    "s2h_healpix_fft_prepare", "s2h_healpix_fft_ex", "s2h_healpix_ifft_ex",
+    "s2h_build_healpix_fft_descriptor",
     "s2h_profile_enable", "s2h_profile_query", "s2h_wigner_reality_fill",
     "s2h_batch_sumsq_workspace_bytes", "s2h_batch_sumsq",
     "s2h_ring_spectra_bytes", "s2h_ring_analysis", "s2h_forward_rings", "s2h_inverse_rings",
@@ -94,6 +101,11 @@ def _declare(lib):
         "s2h_lat_inverse": (i32, [vp, vp, vp, i32, vp]),
         "s2h_healpix_fft": (i32, [i32, i32, i32, vp, vp, i32, vp]),
         "s2h_healpix_ifft": (i32, [i32, i32, i32, vp, vp, i32, vp]),
+        "s2h_healpix_fft_prepare": (i32, [i32, i32]),
+        "s2h_healpix_fft_ex": (i32, [i32, i32, i32, i32, i32, vp, vp, i32, vp]),
+        "s2h_healpix_ifft_ex": (i32, [i32, i32, i32, i32, i32, vp, vp, i32, vp]),
+        "s2h_build_healpix_fft_descriptor": (i32, [i32, i32, i32, i32, i32, i32,
+                                                   ctypes.POINTER(HealpixFftDescriptor)]),
         "healpix_fft_hip": (None, [vp, ctypes.POINTER(vp), ctypes.c_char_p, sz]),
         "healpix_fft_hip_last_status": (i32, []),
         "s2h_gamma_dft": (i32, [i32, i64, i32, ctypes.c_double, vp, vp, vp]),

@@ -740,18 +740,48 @@ extern "C" int s2h_lat_inverse(const s2h_plan* p, const void* flm, void* ftm, in
   return hip_status(launch_lat_inverse(a, p->inv_paired, s));
 }
 
-extern "C" int s2h_healpix_fft(int nside, int L, int reality, const void* f, void* ftm, int batch,
-                               s2h_stream stream) {
-  if (nside < 1 || !f || !ftm || batch < 0) return S2H_ERR_INVALID;
+namespace {
+
+int healpix_fft_checks(int nside, int L, int norm, int batch) {
+  if (nside < 1 || batch < 0) return S2H_ERR_INVALID;
+  if (norm < S2H_FFT_NORM_FORWARD || norm > S2H_FFT_NORM_NONE) return S2H_ERR_INVALID;
   if (L < 2 * nside) return S2H_ERR_BANDLIMIT;
   if (4 * nside > kMaxRingLen || L > kMaxL) return S2H_ERR_UNSUPPORTED;
+  return S2H_OK;
+}
+
+// The per-ring scale of a transform direction under a norm (the reference's
+// cuFFT store callbacks, lib/src/s2fft_callbacks.cu:256-308): the forward
+// divides by n under FORWARD, the backward under BACKWARD, both by sqrt(n)
+// under ORTHO; NONE scales neither.  0 none, 1 1/n, 2 1/sqrt(n).
+int norm_mode(int norm, bool forward) {
+  if (norm == S2H_FFT_NORM_ORTHO) return 2;
+  if (norm == (forward ? S2H_FFT_NORM_FORWARD : S2H_FFT_NORM_BACKWARD)) return 1;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int s2h_healpix_fft_prepare(int nside, int L) {
+  const int st = healpix_fft_checks(nside, L, S2H_FFT_NORM_NONE, 0);
+  if (st) return st;
+  return healpix_tables_prepare(nside, L);
+}
+
+extern "C" int s2h_healpix_fft_ex(int nside, int L, int reality, int norm, int double_precision,
+                                  const void* f, void* ftm, int batch, s2h_stream stream) {
+  int st = healpix_fft_checks(nside, L, norm, batch);
+  if (st) return st;
+  if (batch > 0 && (!f || !ftm)) return S2H_ERR_INVALID;
   if (batch == 0) return S2H_OK;
   HpTables tb;
-  int st = healpix_tables(nside, L, &tb);
-  if (st) return st;
+  if ((st = healpix_tables(nside, L, &tb))) return st;
   RingFwdArgs r{};
   r.f = f;
   r.f_complex = 1;
+  r.in_single = double_precision ? 0 : 1;
+  r.out_single = double_precision ? 0 : 1;
+  r.norm_mode = norm_mode(norm, true);
   r.real_only_2L = reality ? 1 : 0;
   r.map_stride = 12LL * nside * nside;
   r.out = static_cast<cplx*>(ftm);
@@ -765,17 +795,19 @@ extern "C" int s2h_healpix_fft(int nside, int L, int reality, const void* f, voi
       launch_ring_fft_forward(r, tb.set, batch, static_cast<hipStream_t>(stream)));
 }
 
-extern "C" int s2h_healpix_ifft(int nside, int L, int reality, const void* ftm, void* f,
-                                int batch, s2h_stream stream) {
-  if (nside < 1 || !f || !ftm || batch < 0) return S2H_ERR_INVALID;
-  if (L < 2 * nside) return S2H_ERR_BANDLIMIT;
-  if (4 * nside > kMaxRingLen || L > kMaxL) return S2H_ERR_UNSUPPORTED;
+extern "C" int s2h_healpix_ifft_ex(int nside, int L, int reality, int norm, int double_precision,
+                                   const void* ftm, void* f, int batch, s2h_stream stream) {
+  int st = healpix_fft_checks(nside, L, norm, batch);
+  if (st) return st;
+  if (batch > 0 && (!f || !ftm)) return S2H_ERR_INVALID;
   if (batch == 0) return S2H_OK;
   HpTables tb;
-  int st = healpix_tables(nside, L, &tb);
-  if (st) return st;
+  if ((st = healpix_tables(nside, L, &tb))) return st;
   RingInvArgs r{};
   r.in = static_cast<const cplx*>(ftm);
+  r.in_single = double_precision ? 0 : 1;
+  r.f_single = double_precision ? 0 : 1;
+  r.norm_mode = norm_mode(norm, false);
   r.in_map_stride = (int64_t)(4 * nside - 1) * 2 * L;
   r.ncol = 2 * L;
   r.m_first = -L;
@@ -791,6 +823,41 @@ extern "C" int s2h_healpix_ifft(int nside, int L, int reality, const void* ftm, 
       launch_ring_fft_inverse(r, tb.set, batch, static_cast<hipStream_t>(stream)));
 }
 
+extern "C" int s2h_healpix_fft(int nside, int L, int reality, const void* f, void* ftm, int batch,
+                               s2h_stream stream) {
+  return s2h_healpix_fft_ex(nside, L, reality, S2H_FFT_NORM_BACKWARD, 1, f, ftm, batch, stream);
+}
+
+extern "C" int s2h_healpix_ifft(int nside, int L, int reality, const void* ftm, void* f,
+                                int batch, s2h_stream stream) {
+  return s2h_healpix_ifft_ex(nside, L, reality, S2H_FFT_NORM_FORWARD, 1, ftm, f, batch, stream);
+}
+
+// (forward, normalize) -> norm as build_healpix_fft_descriptor does
+// (lib/src/extensions.cc:115-149); shift is always on; the tables are built
+// here, at descriptor time, so that the launches never allocate.
+extern "C" int s2h_build_healpix_fft_descriptor(int nside, int L, int reality, int forward,
+                                                int normalize, int double_precision,
+                                                s2h_healpix_fft_descriptor* out) {
+  if (!out) return S2H_ERR_INVALID;
+  s2h_healpix_fft_descriptor d{};
+  d.nside = nside;
+  d.L = L;
+  d.reality = reality ? 1 : 0;
+  d.forward = forward ? 1 : 0;
+  if (forward)
+    d.norm = normalize ? S2H_FFT_NORM_FORWARD : S2H_FFT_NORM_BACKWARD;
+  else
+    d.norm = normalize ? S2H_FFT_NORM_BACKWARD : S2H_FFT_NORM_FORWARD;
+  d.shift = 1;
+  d.double_precision = double_precision ? 1 : 0;
+  d.batch = 1;
+  const int st = s2h_healpix_fft_prepare(nside, L);
+  if (st) return st;
+  *out = d;
+  return S2H_OK;
+}
+
 extern "C" void healpix_fft_hip(void* stream, void** buffers, const char* opaque,
                                 size_t opaque_len) {
   if (!buffers || !opaque || opaque_len != sizeof(s2h_healpix_fft_descriptor)) {
@@ -799,10 +866,17 @@ extern "C" void healpix_fft_hip(void* stream, void** buffers, const char* opaque
   }
   s2h_healpix_fft_descriptor d;
   std::memcpy(&d, opaque, sizeof(d));
+  if (d.shift != 1 || d.batch < 0 || (d.double_precision != 0 && d.double_precision != 1)) {
+    // the reference always shifts (extensions.cc:131-132)
+    g_last_xla_status = d.shift != 1 ? S2H_ERR_UNSUPPORTED : S2H_ERR_INVALID;
+    return;
+  }
   const int batch = d.batch > 0 ? d.batch : 1;
   g_last_xla_status =
-      d.forward ? s2h_healpix_fft(d.nside, d.L, d.reality, buffers[0], buffers[1], batch, stream)
-                : s2h_healpix_ifft(d.nside, d.L, d.reality, buffers[0], buffers[1], batch, stream);
+      d.forward ? s2h_healpix_fft_ex(d.nside, d.L, d.reality, d.norm, d.double_precision,
+                                     buffers[0], buffers[1], batch, stream)
+                : s2h_healpix_ifft_ex(d.nside, d.L, d.reality, d.norm, d.double_precision,
+                                      buffers[0], buffers[1], batch, stream);
 }
 
 extern "C" int healpix_fft_hip_last_status(void) { return g_last_xla_status; }

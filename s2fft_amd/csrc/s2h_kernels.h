@@ -22,6 +22,9 @@ struct RingFwdArgs {
   int L;
   const cplx* chirp_pool;
   int batch;                // set by the launcher
+  int in_single;            // complex64 input (complex, unpaired maps only)
+  int out_single;           // complex64 output
+  int norm_mode;            // extra per-ring scale: 0 none, 1: 1/nphi, 2: 1/sqrt(nphi)
 };
 
 struct RingInvArgs {
@@ -41,6 +44,9 @@ struct RingInvArgs {
   int L;
   int batch;                // set by the launcher
   const void* sub_from;     // non-null: f = sub_from - (transform), same layout as f
+  int in_single;            // complex64 input spectra (unpaired maps only)
+  int f_single;             // complex64 output pixels (complex output only)
+  int norm_mode;            // extra per-ring scale: 0 none, 1: 1/nphi, 2: 1/sqrt(nphi)
 };
 
 // R = G - A(Y) on ring spectra, A = (ring FFT) . (ring iFFT): per ring, the

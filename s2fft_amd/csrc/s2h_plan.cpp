@@ -310,13 +310,12 @@ struct ChirpBuilder {
     max_conv = std::max(max_conv, M);
     return M;
   }
+  // Allocates the chirp pool and fills it on the device (synchronously: plan
+  // creation only).  On success the pool is recorded in allocs (freed with the
+  // plan) or, with allocs null, owned by the caller; on failure nothing leaks.
   int build(cplx** pool, std::vector<void*>* allocs) {
     *pool = nullptr;
     if (table.empty()) return S2H_OK;
-    void* ptr = nullptr;
-    if (hipMalloc(&ptr, sizeof(cplx) * total) != hipSuccess) return S2H_ERR_NOMEM;
-    if (allocs) allocs->push_back(ptr);
-    *pool = static_cast<cplx*>(ptr);
     std::vector<int> ns, Ms;
     std::vector<int64_t> offs;
     for (auto& kv : table) {
@@ -324,21 +323,32 @@ struct ChirpBuilder {
       Ms.push_back(kv.second.first);
       offs.push_back(kv.second.second);
     }
-    int* dn = nullptr;
-    int* dM = nullptr;
-    int64_t* doff = nullptr;
-    if (hipMalloc(&dn, sizeof(int) * ns.size()) != hipSuccess) return S2H_ERR_NOMEM;
-    if (hipMalloc(&dM, sizeof(int) * ns.size()) != hipSuccess) return S2H_ERR_NOMEM;
-    if (hipMalloc(&doff, sizeof(int64_t) * ns.size()) != hipSuccess) return S2H_ERR_NOMEM;
-    (void)hipMemcpy(dn, ns.data(), sizeof(int) * ns.size(), hipMemcpyHostToDevice);
-    (void)hipMemcpy(dM, Ms.data(), sizeof(int) * ns.size(), hipMemcpyHostToDevice);
-    (void)hipMemcpy(doff, offs.data(), sizeof(int64_t) * ns.size(), hipMemcpyHostToDevice);
-    hipError_t e = launch_chirp_setup(dn, dM, doff, *pool, (int)ns.size(), max_conv, 0);
+    // every temporary is freed on every path
+    struct Tmp {
+      void* p[4] = {nullptr, nullptr, nullptr, nullptr};
+      ~Tmp() {
+        for (void* q : p)
+          if (q) (void)hipFree(q);
+      }
+    } t;
+    const size_t cnt = ns.size();
+    if (hipMalloc(&t.p[0], sizeof(cplx) * total) != hipSuccess) return S2H_ERR_NOMEM;
+    if (hipMalloc(&t.p[1], sizeof(int) * cnt) != hipSuccess) return S2H_ERR_NOMEM;
+    if (hipMalloc(&t.p[2], sizeof(int) * cnt) != hipSuccess) return S2H_ERR_NOMEM;
+    if (hipMalloc(&t.p[3], sizeof(int64_t) * cnt) != hipSuccess) return S2H_ERR_NOMEM;
+    if (hipMemcpy(t.p[1], ns.data(), sizeof(int) * cnt, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(t.p[2], Ms.data(), sizeof(int) * cnt, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(t.p[3], offs.data(), sizeof(int64_t) * cnt, hipMemcpyHostToDevice) != hipSuccess)
+      return S2H_ERR_HIP;
+    hipError_t e = launch_chirp_setup(static_cast<int*>(t.p[1]), static_cast<int*>(t.p[2]),
+                                      static_cast<int64_t*>(t.p[3]), static_cast<cplx*>(t.p[0]),
+                                      (int)cnt, max_conv, 0);
     if (e == hipSuccess) e = hipDeviceSynchronize();
-    (void)hipFree(dn);
-    (void)hipFree(dM);
-    (void)hipFree(doff);
-    return e == hipSuccess ? S2H_OK : S2H_ERR_HIP;
+    if (e != hipSuccess) return S2H_ERR_HIP;
+    *pool = static_cast<cplx*>(t.p[0]);
+    t.p[0] = nullptr;  // kept
+    if (allocs) allocs->push_back(*pool);
+    return S2H_OK;
   }
 };
 
@@ -417,19 +427,20 @@ std::vector<LatRing> lat_rings(const std::vector<double>& th, int r0, int r1, bo
 
 }  // namespace
 
-// Reference-layout HEALPix ring tables, cached per (device, nside, L).
-int healpix_tables(int nside, int L, HpTables* out) {
-  static std::mutex mu;
-  static std::map<std::tuple<int, int, int>, HpTables> cache;
+// Reference-layout HEALPix ring tables, cached per (device, nside, L).  Built
+// by healpix_tables_prepare (allocates and synchronises once); the launch
+// paths only look them up.
+namespace {
+std::mutex g_hp_mu;
+std::map<std::tuple<int, int, int>, HpTables> g_hp_cache;
+}  // namespace
+
+int healpix_tables_prepare(int nside, int L) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return S2H_ERR_HIP;
-  std::lock_guard<std::mutex> lock(mu);
-  auto key = std::make_tuple(dev, nside, L);
-  auto it = cache.find(key);
-  if (it != cache.end()) {
-    *out = it->second;
-    return S2H_OK;
-  }
+  std::lock_guard<std::mutex> lock(g_hp_mu);
+  const auto key = std::make_tuple(dev, nside, L);
+  if (g_hp_cache.count(key)) return S2H_OK;
   if (allow_large_lds() != hipSuccess) return S2H_ERR_HIP;
   ChirpBuilder cb;
   std::vector<RingJob> jobs;
@@ -440,12 +451,25 @@ int healpix_tables(int nside, int L, HpTables* out) {
     off += n;
   }
   HpTables tb;
-  int st = make_set(nullptr, jobs, &tb.set);
-  if (st) return st;
-  st = cb.build(&tb.chirp, nullptr);
-  if (st) return st;
-  cache[key] = tb;
-  *out = tb;
+  std::vector<void*> own;  // the job table and the chirp pool (kept for the process)
+  s2h_plan tmp;            // make_set records its upload here
+  int st = make_set(&tmp, jobs, &tb.set);
+  if (st == S2H_OK) st = cb.build(&tb.chirp, &tmp.allocations);
+  if (st != S2H_OK) {
+    for (void* q : tmp.allocations) (void)hipFree(q);
+    return st;
+  }
+  g_hp_cache[key] = tb;
+  return S2H_OK;
+}
+
+int healpix_tables(int nside, int L, HpTables* out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return S2H_ERR_HIP;
+  std::lock_guard<std::mutex> lock(g_hp_mu);
+  const auto it = g_hp_cache.find(std::make_tuple(dev, nside, L));
+  if (it == g_hp_cache.end()) return S2H_ERR_INVALID;  // not prepared
+  *out = it->second;
   return S2H_OK;
 }
 

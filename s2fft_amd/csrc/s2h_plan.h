@@ -54,5 +54,9 @@ struct HpTables {
   JobSet set;
   cplx* chirp = nullptr;
 };
+// Builds and caches the tables of (current device, nside, L) once
+// (allocates, synchronises); healpix_tables only looks them up and returns
+// S2H_ERR_INVALID for tables that were never prepared.
+int healpix_tables_prepare(int nside, int L);
 int healpix_tables(int nside, int L, HpTables* out);
 }  // namespace s2h

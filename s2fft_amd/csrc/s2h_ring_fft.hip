@@ -357,6 +357,12 @@ __device__ void build_fft_tables(const RingLds& s, int n, int M) {
 
 }  // namespace
 
+// Extra scale of one ring's transform: 1, 1/n or 1/sqrt(n) (the reference's
+// fft_norm modes, lib/src/s2fft_callbacks.cu:256-308).
+__device__ __forceinline__ double norm_scale(int mode, int n) {
+  return mode == 1 ? 1.0 / (double)n : mode == 2 ? 1.0 / sqrt((double)n) : 1.0;
+}
+
 // ------------------------------------------------------------------ forward
 // gridDim.y: maps (complex input) or map pairs (real input, `paired`).
 template <int TPB, int MAXN, int KIND>
@@ -388,6 +394,16 @@ __global__ void __launch_bounds__(TPB) ring_fwd_kernel(RingFwdArgs a, int job0, 
       batched_for<TPB, 8, cplx>(
           n, [&](int p) { return cplx{s0[p], two ? s0[a.map_stride + p] : 0.0}; }, put);
     }
+  } else if (a.in_single) {
+    const bool real_only = a.real_only_all || (a.real_only_2L && n == 2 * a.L);
+    const float2* src = reinterpret_cast<const float2*>(a.f) + b0 * a.map_stride + job.pix_offset;
+    batched_for<TPB, 8, cplx>(
+        n,
+        [&](int p) {
+          const float2 v = src[p];
+          return cplx{(double)v.x, real_only ? 0.0 : (double)v.y};
+        },
+        put);
   } else {
     const bool real_only = a.real_only_all || (a.real_only_2L && n == 2 * a.L);
     const cplx* src = reinterpret_cast<const cplx*>(a.f) + b0 * a.map_stride + job.pix_offset;
@@ -402,9 +418,11 @@ __global__ void __launch_bounds__(TPB) ring_fwd_kernel(RingFwdArgs a, int job0, 
   __syncthreads();
   ring_transform<TPB, MAXN, KIND>(s, n, M, a.chirp_pool + job.chirp_off, -1);
 
-  const double w = a.apply_weight ? job.weight : 1.0;
+  const double w = (a.apply_weight ? job.weight : 1.0) * norm_scale(a.norm_mode, n);
   const Tab2 ph = {s.ph_hi, s.ph_lo};
-  cplx* d0 = a.out + b0 * a.out_map_stride + (int64_t)job.row * a.ncol;
+  const int64_t o0 = b0 * a.out_map_stride + (int64_t)job.row * a.ncol;
+  cplx* d0 = a.out + o0;
+  float2* d0s = reinterpret_cast<float2*>(a.out) + o0;
   for (int col = threadIdx.x; col < a.ncol; col += TPB) {
     const int m = a.m_first + col;
     int k = m % n;
@@ -425,8 +443,11 @@ __global__ void __launch_bounds__(TPB) ring_fwd_kernel(RingFwdArgs a, int job0, 
         d0[a.out_map_stride + col] = zero ? cplx{0.0, 0.0} : vy;
       }
     } else {
-      const cplx v = cmul(cplx{z.x * w, z.y * w}, f);
-      d0[col] = zero ? cplx{0.0, 0.0} : v;
+      const cplx v = zero ? cplx{0.0, 0.0} : cmul(cplx{z.x * w, z.y * w}, f);
+      if (a.out_single)
+        d0s[col] = make_float2((float)v.x, (float)v.y);
+      else
+        d0[col] = v;
     }
   }
 }
@@ -438,6 +459,7 @@ __global__ void __launch_bounds__(TPB) ring_fwd_kernel(RingFwdArgs a, int job0, 
 struct FoldSrc {
   const cplx* x;  // column 0 of the ring's row (m = m_first)
   const cplx* y;  // second map of a pair, or null
+  const float2* xs;  // complex64 row instead of x (unpaired maps), or null
   int m_first;
   bool herm;      // m < 0 read as conj(c(-m))
   bool phase;
@@ -451,9 +473,17 @@ struct FoldSrc {
     }
     return src[m - m_first];
   }
+  __device__ __forceinline__ cplx one_single(int m) const {
+    if (herm && m < 0) {
+      const float2 t = xs[-m - m_first];
+      return {(double)t.x, -(double)t.y};
+    }
+    const float2 t = xs[m - m_first];
+    return {(double)t.x, (double)t.y};
+  }
   // z(m) with the phase applied
   __device__ __forceinline__ cplx at(int m) const {
-    cplx z = one(x, m);
+    cplx z = xs ? one_single(m) : one(x, m);
     if (y) {
       const cplx v = one(y, m);
       z = {z.x - v.y, z.y + v.x};
@@ -561,10 +591,13 @@ __global__ void __launch_bounds__(TPB) ring_inv_kernel(RingInvArgs a, int job0, 
   if (a.apply_phase) build_phase<TPB>(s.ph_hi, s.ph_lo, hi - lo + 1, lo, job.phi0_over_pi, 1.0);
   __syncthreads();
 
-  const double w = a.apply_weight ? job.weight : 1.0;
+  const double w = (a.apply_weight ? job.weight : 1.0) * norm_scale(a.norm_mode, n);
   FoldSrc fs;
   fs.x = a.in + b0 * a.in_map_stride + (int64_t)job.row * a.ncol;
   fs.y = two ? fs.x + a.in_map_stride : nullptr;
+  fs.xs = a.in_single ? reinterpret_cast<const float2*>(a.in) + b0 * a.in_map_stride +
+                            (int64_t)job.row * a.ncol
+                      : nullptr;
   fs.m_first = a.m_first;
   fs.herm = herm;
   fs.phase = a.apply_phase;
@@ -592,6 +625,10 @@ __global__ void __launch_bounds__(TPB) ring_inv_kernel(RingInvArgs a, int job0, 
         if (two) d0[a.map_stride + p] = s.im[lx(p)];
       }
     }
+  } else if (a.f_single) {
+    float2* dst = reinterpret_cast<float2*>(a.f) + off;
+    for (int p = threadIdx.x; p < n; p += TPB)
+      dst[p] = make_float2((float)s.re[lx(p)], herm ? 0.0f : (float)s.im[lx(p)]);
   } else {
     cplx* dst = reinterpret_cast<cplx*>(a.f) + off;
     const cplx* u0 = a.sub_from ? reinterpret_cast<const cplx*>(a.sub_from) + off : nullptr;

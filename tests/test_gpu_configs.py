@@ -1,0 +1,232 @@
+"""The five benchmark configurations at their own sizes (needs a GPU).
+
+Each follows the reference's test strategy for its path
+(tests/test_spherical_transform.py:24-179, test_spherical_base.py:115-138,
+test_spherical_custom_grads.py, test_healpix_ffts.py): the HIP FP64 kernels
+against the CPU oracle where the oracle finishes in seconds, and
+size-independent properties (sampling-theorem round trips, refinement
+reaching the seeded coefficients, adjoint identities) at full size.
+
+* C1  MW L=128, seed 0, complex: inverse vs the oracle (1e-12), forward of
+      the oracle's map vs the oracle's forward (1e-12) and vs flm (the MW
+      round trip is exact, 1e-11).
+* C2  HEALPix nside 512, L=1024, real, seed 1, iter=3: the inverse vs the
+      oracle on a subset of rings and on the sampled pixels of the committed
+      fixture (tests/golden/healpix_nside512_L1024_iter3.npz); the refined
+      forward's error against flm matches the oracle's own error at each
+      iteration count, and its sampled coefficients match the oracle's.
+* C3  64 spin-2 MW maps, L=512, seeds 100..163: FP64 round trip exact for
+      every map; FP32 and split-BF16 forward and inverse against the FP64
+      kernels within the stated tolerances; two maps' inverse against the
+      oracle on a ring subset.
+* C4a Gauss-Legendre L=256, seeds 200 / 201: loss sum |forward(f) - t|^2;
+      central finite differences (h = 1e-6) and the adjoint identity
+      against the oracle's forward.
+* C5  HEALPix nside 1024, L=2048, 16 real maps, seeds 300..315: the FP64
+      inverse vs the oracle on a ring subset, the plain FP64 Jacobi
+      refinement and the mixed-precision CG refinement (with and without
+      the learned preconditioner) reach 1e-7 against the seeded flm.
+
+Ring-subset tolerance at L >= 1024: the oracle restates the reference's
+m-recursion, which itself drifts from an 80-bit evaluation by 4.5e-12 at
+L = 256 (tests/test_oracle_pins.py::test_oracle_pm_precision_at_nside_128)
+and more at higher L, so the bound there is the oracle's own accuracy, stated
+per test; the printed figures are the measured ones.
+"""
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import ffts as offts
+from oracle import signals, transforms as otr
+from tests.conftest import max_rel
+
+import s2fft_amd
+from s2fft_amd.transforms import refine, spherical as sph
+
+pytestmark = pytest.mark.gpu
+GOLDEN = Path(__file__).resolve().parent / "golden"
+TOLS = {"mixed": 2e-6, "bf16": 1e-4, "bf16x1": 2e-2}
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _ring_pixels(f, nside, rings):
+    offs, nph = offts.ring_offsets(nside)
+    return np.concatenate([f[offs[t]:offs[t] + nph[t]] for t in rings])
+
+
+# ------------------------------------------------------------------ C1
+def test_c1_mw_L128_round_trip():
+    L = 128
+    flm = signals.generate_flm(np.random.default_rng(0), L)
+    f_ref = otr.inverse(flm, L, 0, sampling="mw")
+    f = s2fft_amd.inverse(flm, L, 0, sampling="mw", method="hip")
+    e_inv = max_rel(f, f_ref)
+    g = s2fft_amd.forward(f_ref, L, 0, sampling="mw", method="hip")
+    e_fwd = max_rel(g, otr.forward(f_ref, L, 0, sampling="mw"))
+    e_rt = max_rel(g, flm)
+    _say(f"C1: inverse {e_inv:.2e}, forward {e_fwd:.2e}, round trip {e_rt:.2e}")
+    assert e_inv < 1e-12 and e_fwd < 1e-12 and e_rt < 1e-11
+
+
+# ------------------------------------------------------------------ C2
+def _c2_fixture():
+    path = GOLDEN / "healpix_nside512_L1024_iter3.npz"
+    if not path.exists():
+        pytest.skip("fixture not generated (python -m tests.golden.make_healpix_nside512_iter3)")
+    return np.load(path)
+
+
+def test_c2_healpix_nside512_inverse():
+    nside, L = 512, 1024
+    flm = signals.generate_flm(np.random.default_rng(1), L, reality=True)
+    f = s2fft_amd.inverse(flm, L, 0, nside, "healpix", "hip", True)
+    assert f.dtype == np.complex128 and np.max(np.abs(f.imag)) == 0.0
+    rings = [0, 1, 200, 511, 1023, 1500, 2045, 2046]
+    want = otr.inverse_rings(flm, L, rings, 0, nside, "healpix", True)
+    e_ring = max_rel(_ring_pixels(f, nside, rings), want)
+    fx = _c2_fixture()
+    e_pix = float(np.max(np.abs(f.real[fx["pix"]] - fx["f_pix"])) / fx["f_max"])
+    _say(f"C2 inverse: ring subset {e_ring:.2e}, fixture pixels {e_pix:.2e}")
+    assert e_ring < 1e-11 and e_pix < 1e-11
+
+
+def test_c2_healpix_nside512_iter3_error_matches_oracle():
+    fx = _c2_fixture()
+    nside, L = 512, 1024
+    flm = signals.generate_flm(np.random.default_rng(1), L, reality=True)
+    f = np.real(s2fft_amd.inverse(flm, L, 0, nside, "healpix", "hip", True))
+    errs = []
+    for it in range(4):
+        g = s2fft_amd.forward(f, L, 0, nside, "healpix", "hip", True, iter=it)
+        errs.append(max_rel(g, flm))
+    want = fx["err_by_iter"]
+    _say(f"C2 forward error by iter: HIP {['%.4e' % e for e in errs]}, "
+         f"oracle {['%.4e' % e for e in want]}")
+    # the refined forward reproduces the oracle's (reference's) own error
+    assert np.allclose(errs, want, rtol=1e-6, atol=0)
+    got = g[fx["ell"], L - 1 + fx["m"]]
+    e_coef = float(np.max(np.abs(got - fx["flm_iter3"])) / fx["flm_iter3_max"])
+    _say(f"C2 iter=3 sampled coefficients vs oracle: {e_coef:.2e}")
+    assert e_coef < 1e-11
+
+
+# ------------------------------------------------------------------ C3
+def test_c3_spin2_mw_L512_batch64():
+    L, spin, B = 512, 2, 64
+    dev = torch.device("cuda:0")
+    flms = np.stack([signals.generate_flm(np.random.default_rng(100 + i), L, spin=spin)
+                     for i in range(B)])
+    flm_t = torch.from_numpy(flms).to(dev)
+    f64 = sph.inverse_hip(flm_t, L, spin, None, "mw")
+    # two maps against the oracle on interior rings
+    rings = [0, 1, 255, 509, 510]
+    for i in (0, B - 1):
+        want = otr.inverse_rings(flms[i], L, rings, spin, None, "mw")
+        e = max_rel(f64[i, rings].cpu().numpy().reshape(-1), want)
+        _say(f"C3 map {i}: FP64 inverse vs oracle (ring subset) {e:.2e}")
+        assert e < 1e-12
+    g64 = sph.forward_hip(f64, L, spin, None, "mw")
+    e_rt = float(((g64 - flm_t).abs().amax(dim=(1, 2)) / flm_t.abs().amax(dim=(1, 2))).max())
+    _say(f"C3: FP64 round trip, worst of {B} maps {e_rt:.2e}")
+    assert e_rt < 1e-11
+    for prec in ("mixed", "bf16"):
+        gx = sph.forward_hip(f64, L, spin, None, "mw", precision=prec)
+        fx = sph.inverse_hip(flm_t, L, spin, None, "mw", precision=prec)
+        e_f = float(((gx - g64).abs().amax(dim=(1, 2)) / g64.abs().amax(dim=(1, 2))).max())
+        e_i = float(((fx - f64).abs().amax(dim=(1, 2)) / f64.abs().amax(dim=(1, 2))).max())
+        _say(f"C3 {prec}: forward {e_f:.2e}, inverse {e_i:.2e} (max-rel vs FP64, worst map)")
+        assert 1e-12 < e_f < TOLS[prec] and 1e-12 < e_i < TOLS[prec]
+        del gx, fx
+
+
+# ------------------------------------------------------------------ C4a
+def test_c4a_gl_L256_gradients():
+    L = 256
+    rng = np.random.default_rng(2024)
+    flm_f = signals.generate_flm(np.random.default_rng(200), L)
+    t = signals.generate_flm(np.random.default_rng(201), L)
+    f0 = otr.inverse(flm_f, L, 0, sampling="gl")
+    tt = torch.from_numpy(t).cuda()
+
+    def loss(ft):
+        return (s2fft_amd.forward(ft, L, 0, None, "gl", "hip") - tt).abs().pow(2).sum()
+
+    ft = torch.from_numpy(f0).cuda().requires_grad_(True)
+    loss(ft).backward()
+    grad = ft.grad.detach().cpu().numpy()
+    h = 1e-6
+    for _ in range(3):
+        v = rng.standard_normal(f0.shape) + 1j * rng.standard_normal(f0.shape)
+        vt = torch.from_numpy(v).cuda()
+        with torch.no_grad():
+            fd = (loss(ft + h * vt) - loss(ft - h * vt)).item() / (2 * h)
+        an = float(np.real(np.vdot(grad, v)))
+        _say(f"C4a: finite difference {fd:.10e}, adjoint {an:.10e}")
+        assert abs(fd - an) < 1e-6 * max(abs(an), 1.0)
+    # adjoint identity <y, F x> = <F^H y, x> with F from the oracle
+    x = rng.standard_normal(f0.shape) + 1j * rng.standard_normal(f0.shape)
+    y = rng.standard_normal((L, 2 * L - 1)) + 1j * rng.standard_normal((L, 2 * L - 1))
+    lhs = np.real(np.vdot(y, otr.forward(x, L, 0, sampling="gl")))
+    fhy = sph.forward_adjoint_hip(torch.from_numpy(y)[None].cuda(), L, 0, None, "gl")[0]
+    rhs = np.real(np.vdot(fhy.cpu().numpy(), x))
+    _say(f"C4a: adjoint identity {lhs:.12e} vs {rhs:.12e}")
+    assert abs(lhs - rhs) < 1e-11 * max(abs(lhs), 1.0)
+
+
+# ------------------------------------------------------------------ C5
+@pytest.fixture(scope="module")
+def c5():
+    nside, L, B = 1024, 2048, 16
+    dev = torch.device("cuda:0")
+    flms = np.stack([signals.generate_flm(np.random.default_rng(300 + i), L, reality=True)
+                     for i in range(B)])
+    flm_true = torch.from_numpy(flms).to(dev)
+    plan = sph.get_plan("healpix", L, nside, 0, 0, True)
+    f = sph.inverse_hip(flm_true, L, 0, nside, "healpix", True, plan=plan)
+    yield dict(nside=nside, L=L, B=B, flms=flms, flm_true=flm_true, f=f, plan=plan)
+    del f, flm_true
+    torch.cuda.empty_cache()
+
+
+def _err(x, ref):
+    return float((x - ref).abs().max() / ref.abs().max())
+
+
+def test_c5_inverse_ring_subset_vs_oracle(c5):
+    nside, L = c5["nside"], c5["L"]
+    rings = [0, 1, 700, 1023, 2047, 3000, 4093, 4094]
+    for i in (0, c5["B"] - 1):
+        want = otr.inverse_rings(c5["flms"][i], L, rings, 0, nside, "healpix", True)
+        got = _ring_pixels(c5["f"][i].cpu().numpy(), nside, rings)
+        e = max_rel(got, want)
+        _say(f"C5 map {i}: FP64 inverse vs oracle (ring subset) {e:.2e}")
+        assert e < 1e-10
+
+
+def test_c5_plain_fp64_jacobi_reaches_target(c5):
+    f = c5["f"].real.contiguous()
+    x = refine.refine_fp64(f, c5["L"], 0, c5["nside"], "healpix", True, 5, spectral=True)
+    e = _err(x, c5["flm_true"])
+    _say(f"C5 FP64 Jacobi, 5 steps: {e:.2e}")
+    assert e <= 1e-7
+
+
+@pytest.mark.parametrize("learned", [False, True])
+def test_c5_mixed_cg_reaches_target(c5, learned):
+    from s2fft_amd.learned.precond import MLPPreconditioner
+
+    f = c5["f"].real.contiguous()
+    pc = MLPPreconditioner.load(device=f.device) if learned else None
+    stats = {}
+    x = refine.refine_cg(f, c5["L"], 0, c5["nside"], "healpix", True, 3, plan=c5["plan"],
+                         precond=pc, learned_steps=1 if learned else 0, stats=stats)
+    e = _err(x, c5["flm_true"])
+    _say(f"C5 mixed CG (3 steps, learned={learned}): {e:.2e}, judge {stats}")
+    assert e <= 1e-7

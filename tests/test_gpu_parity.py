@@ -127,6 +127,7 @@ def test_healpix_ring_fft_entry(rng, nside, L_mult, reality):
     ft = torch.from_numpy(f).cuda()
     ftm = torch.empty((4 * nside - 1, 2 * L), dtype=torch.complex128, device="cuda")
     s = torch.cuda.current_stream().cuda_stream
+    _lib.check(_lib.lib().s2h_healpix_fft_prepare(nside, L))
     _lib.check(_lib.lib().s2h_healpix_fft(nside, L, int(reality), ft.data_ptr(), ftm.data_ptr(),
                                           1, s))
     assert max_rel(ftm.cpu().numpy(), ref) < TOL

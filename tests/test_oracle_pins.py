@@ -353,3 +353,22 @@ def test_generate_flmn_draw_order_and_symmetry():
                     (-1) ** (mm + n) * np.conj(flmn[N - 1 + n, l, L - 1 + mm]))
             if l < abs(n):
                 assert np.all(flmn[N - 1 + n, l] == 0)
+
+
+@pytest.mark.parametrize("sampling,nside,L,rings", [
+    ("healpix", 4, 9, [0, 3, 5, 7, 12, 14]), ("mw", None, 7, [0, 2, 5]),
+    ("mwss", None, 6, [1, 3, 5]), ("gl", None, 6, [0, 5]), ("dh", None, 5, [0, 4, 9])])
+@pytest.mark.parametrize("reality", [False, True])
+@pytest.mark.parametrize("spin", [0, 1])
+def test_inverse_rings_is_a_ring_subset_of_inverse(sampling, nside, L, rings, reality, spin):
+    if reality and spin:
+        pytest.skip("reality only for spin 0")
+    flm = signals.generate_flm(np.random.default_rng(5), L, 0, spin, reality)
+    full = np.asarray(T.inverse(flm, L, spin, nside, sampling, reality))
+    if sampling == "healpix":
+        offs, nph = ffts.ring_offsets(nside)
+        want = np.concatenate([full[offs[t]:offs[t] + nph[t]] for t in rings])
+    else:
+        want = full[rings].reshape(-1)
+    got = T.inverse_rings(flm, L, rings, spin, nside, sampling, reality)
+    assert max_rel(got, want) < 1e-14
