@@ -207,6 +207,10 @@ int healpix_fft_hip_last_status(void);
  *   mode 2: float64 f[g]           -> same, for a real signal
  *   mode 1: complex128 fban[k]     -> f[g] = scale sum_n fban[k] e^{+2 pi i n g/G}
  *   mode 3: complex128 fban[n >= 0] (N rows) -> float64 f[g], Hermitian in n
+ *   mode 4: complex128 x[n >= 0] (N rows) -> float64 f[g] = scale Re sum_{n >= 0}
+ *           x[n] e^{+2 pi i n g/G} (the adjoint of mode 2 kept to n >= 0)
+ *   mode 5: float64 f[g] -> complex128 y[n >= 0] (N rows) = w_n scale
+ *           sum_g f[g] e^{-2 pi i n g/G}, w_0 = 1, w_n = 2 (the adjoint of mode 3)
  * with n = k - (N-1).  in and out must not alias. */
 int s2h_gamma_dft(int N, int64_t npix, int mode, double scale, const void *in, void *out,
                   s2h_stream stream);
@@ -214,6 +218,36 @@ int s2h_gamma_dft(int N, int64_t npix, int mode, double scale, const void *in, v
  * flmn[N-1-n, l, L-1+m] = (-1)^(m+n) conj(flmn[N-1+n, l, L-1-m]), n = 1..N-1
  * (s2fft/transforms/wigner.py:523-526). */
 int s2h_wigner_reality_fill(void *flmn, int L, int N, s2h_stream stream);
+
+/* Wigner (SO(3)) transforms with every n-plane in one launch per stage
+ * (s2fft/transforms/wigner.py:13-107 inverse, 346-439 forward; the reference
+ * maps a spin transform over n with shared precomputes, price_mcewen.py:
+ * 309-374).  f: [gamma (2N-1)][beta, alpha] float64 (reality) or complex128,
+ * or [gamma][pixel] for HEALPix; flmn: (2N-1, L, 2L-1) complex128.  The
+ * gamma DFT, the ring FFTs, the theta resampling and the latitudinal steps
+ * each run once over all planes (each plane with its own spin -n recursion
+ * tables); with reality the n = 0 plane of the forward runs as its own real
+ * transform (the reference's reality semantics) and the n < 0 planes are
+ * filled by symmetry.  N <= 7 (N >= 8 returns S2H_ERR_SPIN).
+ * s2h_wigner_forward_adjoint / s2h_wigner_inverse_adjoint: the VJPs in the
+ * torch convention (grad_f = F^H g, grad_flmn = I^H g; float64 grad_f with
+ * reality), for autograd; no counterpart beyond the reference's JAX autodiff
+ * of wigner.py:536-645 through the spin custom VJPs. */
+typedef struct s2h_wigner_plan s2h_wigner_plan;
+typedef struct s2h_wigner_desc {
+  int32_t sampling, L, N, nside, L_lower, reality;
+} s2h_wigner_desc;
+int s2h_wigner_plan_create(s2h_wigner_plan **plan, const s2h_wigner_desc *desc);
+int s2h_wigner_plan_destroy(s2h_wigner_plan *plan);
+size_t s2h_wigner_workspace_bytes(const s2h_wigner_plan *plan);
+int s2h_wigner_forward(const s2h_wigner_plan *plan, const void *f, int f_is_complex, void *flmn,
+                       void *work, s2h_stream stream);
+int s2h_wigner_inverse(const s2h_wigner_plan *plan, const void *flmn, void *f, void *work,
+                       s2h_stream stream);
+int s2h_wigner_forward_adjoint(const s2h_wigner_plan *plan, const void *g, void *grad_f,
+                               void *work, s2h_stream stream);
+int s2h_wigner_inverse_adjoint(const s2h_wigner_plan *plan, const void *g, int g_is_complex,
+                               void *grad_flmn, void *work, s2h_stream stream);
 
 /* The transforms split at the ring spectra (no counterpart in the reference;
  * for refinement in spectral space).  For plans whose forward and inverse

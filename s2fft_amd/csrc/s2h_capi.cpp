@@ -187,7 +187,8 @@ int ring_analysis_stage(const s2h_plan* p, const void* f, int f_is_complex, cplx
 // kernels with one ring group per m write every entry themselves (zeros and
 // the m < 0 mirror); otherwise flm is zeroed first and filled after.
 int lat_forward_stage(const s2h_plan* p, const cplx* ftm, void* flm, int batch, int precision,
-                      bool accumulate, hipStream_t s, bool packed = false) {
+                      bool accumulate, hipStream_t s, bool packed = false,
+                      const PlaneSet* ps = nullptr) {
   const bool mixed = precision == S2H_PREC_MIXED;
   const bool bf16 = precision == S2H_PREC_BF16 || precision == S2H_PREC_BF16X1;
   const int split = precision == S2H_PREC_BF16X1 ? 1 : 3;
@@ -205,18 +206,22 @@ int lat_forward_stage(const s2h_plan* p, const cplx* ftm, void* flm, int batch, 
   a.ftm_m_first = p->m_lo;
   a.lmin_out = std::max(std::abs(p->d.spin), p->d.L_lower);
   a.accumulate = accumulate ? 1 : 0;
+  if (ps) a.planes = ps->fwd;
   const bool poles = p->fwd_row_south >= 0 || p->fwd_row_north >= 0;
-  const bool fill = bf16 && !poles && lat_forward_bf16_groups(a, split) == 1;
+  const bool fill = !ps && bf16 && !poles && lat_forward_bf16_groups(a, split) == 1;
   a.fill = fill ? 1 : 0;
   a.mirror = fill && p->d.reality && !packed ? 1 : 0;
   if (!accumulate && !fill && (st = hip_status(launch_zero(flm, 2 * fs * batch, s)))) return st;
   {
-    ProfScope ps(bf16 ? S2H_KERNEL_LAT_FORWARD_BF16
-                      : mixed ? S2H_KERNEL_LAT_FORWARD_FP32 : S2H_KERNEL_LAT_FORWARD,
-                 s);
+    ProfScope prof(bf16 ? S2H_KERNEL_LAT_FORWARD_BF16
+                        : mixed ? S2H_KERNEL_LAT_FORWARD_FP32 : S2H_KERNEL_LAT_FORWARD,
+                   s);
     // batches of 4 or more maps share each d-value across an 8-map MFMA tile;
     // the mixed-precision projection always runs on the FP32 matrix cores
-    if (bf16)
+    // multi-plane launches: the VALU kernel reads each map's own tables
+    if (ps)
+      st = hip_status(launch_lat_forward(a, false, s));
+    else if (bf16)
       st = hip_status(launch_lat_forward_bf16(a, p->fwd_paired, split, s));
     else if (mixed || batch >= kMfmaMinBatch)
       st = hip_status(launch_lat_forward_mfma(a, p->fwd_paired, mixed, s));
@@ -234,8 +239,10 @@ int lat_forward_stage(const s2h_plan* p, const cplx* ftm, void* flm, int batch, 
     pa.ftm_ncol = p->Mc;
     pa.ftm_m_first = p->m_lo;
     pa.L = L;
-    pa.spin = p->d.spin;
-    pa.lmin = a.lmin_out;
+    pa.spin = ps ? ps->spin0 : p->d.spin;
+    pa.spin_step = ps ? ps->spin_step : 0;
+    pa.lmin = ps ? p->d.L_lower : a.lmin_out;
+    pa.lmin_spin = ps ? 1 : 0;
     pa.row_south = p->fwd_row_south;
     pa.row_north = p->fwd_row_north;
     pa.norm = p->fwd_norm;
@@ -251,7 +258,7 @@ int lat_forward_stage(const s2h_plan* p, const cplx* ftm, void* flm, int batch, 
 
 // Latitudinal synthesis (+ pole rows) into inverse-grid ring spectra ftm.
 int lat_inverse_stage(const s2h_plan* p, const void* flm, cplx* ftm, int batch, int precision,
-                      hipStream_t s, bool packed = false) {
+                      hipStream_t s, bool packed = false, const PlaneSet* ps = nullptr) {
   const bool mixed = precision == S2H_PREC_MIXED;
   const bool bf16 = precision == S2H_PREC_BF16 || precision == S2H_PREC_BF16X1;
   const int64_t ftm_stride = (int64_t)p->inv_rows * p->Mc;
@@ -264,12 +271,16 @@ int lat_inverse_stage(const s2h_plan* p, const void* flm, cplx* ftm, int batch, 
   a.ftm_map_stride = ftm_stride;
   a.ftm_ncol = p->Mc;
   a.ftm_m_first = p->m_lo;
+  if (ps) a.planes = ps->inv;
   {
-    ProfScope ps(bf16 ? S2H_KERNEL_LAT_INVERSE_BF16
-                      : mixed ? S2H_KERNEL_LAT_INVERSE_FP32 : S2H_KERNEL_LAT_INVERSE,
-                 s);
-    // batches of 4 or more maps: each d-value feeds 16 maps through the MFMA
-    if (bf16)
+    ProfScope prof(bf16 ? S2H_KERNEL_LAT_INVERSE_BF16
+                        : mixed ? S2H_KERNEL_LAT_INVERSE_FP32 : S2H_KERNEL_LAT_INVERSE,
+                   s);
+    // batches of 4 or more maps: each d-value feeds 16 maps through the MFMA;
+    // multi-plane launches: the VALU kernel reads each map's own tables
+    if (ps)
+      st = hip_status(launch_lat_inverse(a, false, s));
+    else if (bf16)
       st = hip_status(launch_lat_inverse_bf16(a, p->inv_paired,
                                               precision == S2H_PREC_BF16X1 ? 1 : 3, s));
     else if (mixed || batch >= kMfmaMinBatch)
@@ -288,7 +299,8 @@ int lat_inverse_stage(const s2h_plan* p, const void* flm, cplx* ftm, int batch, 
     pa.ftm_ncol = p->Mc;
     pa.ftm_m_first = p->m_lo;
     pa.L = p->d.L;
-    pa.spin = p->d.spin;
+    pa.spin = ps ? ps->spin0 : p->d.spin;
+    pa.spin_step = ps ? ps->spin_step : 0;
     pa.lmin = p->d.L_lower;
     pa.row_south = p->inv_row_south;
     pa.row_north = p->inv_row_north;
@@ -297,6 +309,34 @@ int lat_inverse_stage(const s2h_plan* p, const void* flm, cplx* ftm, int batch, 
     if ((st = hip_status(launch_pole_inverse(pa, s)))) return st;
   }
   return S2H_OK;
+}
+
+// Resampling arguments (MW / MWSS forward): phi-spectra of the input rows in
+// `spec`, the MWSS-at-2L grid into `ftm`.
+ResampleArgs resample_args(const s2h_plan* p, const cplx* spec, cplx* ftm, const PlaneSet* ps) {
+  const int L = p->d.L;
+  ResampleArgs ra{};
+  ra.in = spec;
+  ra.in_map_stride = (int64_t)p->fwd_in_rows * p->Mc;
+  ra.out = ftm;
+  ra.out_map_stride = (int64_t)p->fwd_rows * p->Mc;
+  ra.ncol = p->Mc;
+  ra.m_lo = p->m_lo;
+  ra.nm = p->nm;
+  ra.L = L;
+  ra.spin = ps ? ps->spin0 : p->d.spin;
+  ra.spin_step = ps ? ps->spin_step : 0;
+  ra.mw = p->fwd_kind == 1;
+  ra.nyq_split = p->d.reality ? 1 : 0;
+  ra.weights = p->fwd_weights;
+  ra.phi_factor = ra.mw ? (2.0 * L) / (2.0 * L - 1.0) : 1.0;
+  ra.lds = p->rs_info[3];
+  for (int i = 0; i < 3; ++i) {
+    ra.conv[i] = p->rs_info[i];
+    ra.chirp_off[i] = p->rs_chirp[i];
+  }
+  ra.pool = p->chirp;
+  return ra;
 }
 
 // Plans whose forward and inverse share one ring grid without pole rows or
@@ -310,55 +350,43 @@ bool spectral_ok(const s2h_plan* p) {
 
 }  // namespace
 
-extern "C" int s2h_forward_ex(const s2h_plan* p, const void* f, int f_is_complex, void* flm,
-                              void* work, int batch, int precision, int flags,
-                              s2h_stream stream) {
-  if (!p || !f || !flm || batch < 0 || (batch > 0 && !work)) return S2H_ERR_INVALID;
-  if (!valid_precision(precision)) return S2H_ERR_INVALID;
-  if (flags & ~S2H_FWD_ACCUMULATE) return S2H_ERR_INVALID;
-  const bool accumulate = (flags & S2H_FWD_ACCUMULATE) != 0;
-  if (batch == 0) return S2H_OK;
-  int st = check_device(p);
-  if (st) return st;
-  if (p->d.norm == S2H_NORM_NONE) return S2H_ERR_INVALID;
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  const int L = p->d.L;
-  const bool real = p->d.reality != 0;
-  cplx* ftm = static_cast<cplx*>(work);
-  const int64_t fs = flm_stride(p);
+namespace {
 
+// The forward pipeline: ring FFTs (+ MW / MWSS resampling) and the
+// latitudinal projection; with ps, the batch is a set of n-planes.
+int forward_impl(const s2h_plan* p, const PlaneSet* ps, const void* f, int f_is_complex,
+                 void* flm, void* work, int batch, int precision, bool accumulate,
+                 hipStream_t s) {
+  int st;
+  cplx* ftm = static_cast<cplx*>(work);
   const bool resample = p->fwd_kind != 0;
   // resampled forward: phi-spectrum of the input rows first, then the
   // resampled MWSS-at-2L ftm after it in the workspace
   cplx* spec = ftm;
   if (resample) ftm = spec + (int64_t)batch * p->fwd_in_rows * p->Mc;
   if ((st = ring_analysis_stage(p, f, f_is_complex, resample ? spec : ftm, batch, s))) return st;
-  const int64_t ftm_stride = (int64_t)p->fwd_rows * p->Mc;
   if (resample) {
-    ResampleArgs ra{};
-    ra.in = spec;
-    ra.in_map_stride = (int64_t)p->fwd_in_rows * p->Mc;
-    ra.out = ftm;
-    ra.out_map_stride = ftm_stride;
-    ra.ncol = p->Mc;
-    ra.m_lo = p->m_lo;
-    ra.nm = p->nm;
-    ra.L = L;
-    ra.spin = p->d.spin;
-    ra.mw = p->fwd_kind == 1;
-    ra.nyq_split = real ? 1 : 0;
-    ra.weights = p->fwd_weights;
-    ra.phi_factor = ra.mw ? (2.0 * L) / (2.0 * L - 1.0) : 1.0;
-    ra.lds = p->rs_info[3];
-    for (int i = 0; i < 3; ++i) {
-      ra.conv[i] = p->rs_info[i];
-      ra.chirp_off[i] = p->rs_chirp[i];
-    }
-    ra.pool = p->chirp;
-    ProfScope ps(S2H_KERNEL_RESAMPLE, s);
+    const ResampleArgs ra = resample_args(p, spec, ftm, ps);
+    ProfScope prof(S2H_KERNEL_RESAMPLE, s);
     if ((st = hip_status(launch_resample(ra, batch, s)))) return st;
   }
-  return lat_forward_stage(p, ftm, flm, batch, precision, accumulate, s);
+  return lat_forward_stage(p, ftm, flm, batch, precision, accumulate, s, false, ps);
+}
+
+}  // namespace
+
+extern "C" int s2h_forward_ex(const s2h_plan* p, const void* f, int f_is_complex, void* flm,
+                              void* work, int batch, int precision, int flags,
+                              s2h_stream stream) {
+  if (!p || !f || !flm || batch < 0 || (batch > 0 && !work)) return S2H_ERR_INVALID;
+  if (!valid_precision(precision)) return S2H_ERR_INVALID;
+  if (flags & ~S2H_FWD_ACCUMULATE) return S2H_ERR_INVALID;
+  if (batch == 0) return S2H_OK;
+  int st = check_device(p);
+  if (st) return st;
+  if (p->d.norm == S2H_NORM_NONE) return S2H_ERR_INVALID;
+  return forward_impl(p, nullptr, f, f_is_complex, flm, work, batch, precision,
+                      (flags & S2H_FWD_ACCUMULATE) != 0, static_cast<hipStream_t>(stream));
 }
 
 extern "C" int s2h_inverse(const s2h_plan* p, const void* flm, void* f, void* work, int batch,
@@ -366,25 +394,17 @@ extern "C" int s2h_inverse(const s2h_plan* p, const void* flm, void* f, void* wo
   return s2h_inverse_ex(p, flm, f, work, batch, S2H_PREC_FP64, 0, nullptr, stream);
 }
 
-extern "C" int s2h_inverse_ex(const s2h_plan* p, const void* flm, void* f, void* work, int batch,
-                              int precision, int flags, const void* sub_from,
-                              s2h_stream stream) {
-  if (!p || !flm || !f || batch < 0 || (batch > 0 && !work)) return S2H_ERR_INVALID;
-  if (flags & ~(S2H_INV_REAL_OUTPUT | S2H_INV_SUBTRACT)) return S2H_ERR_INVALID;
-  const bool real_out = (flags & S2H_INV_REAL_OUTPUT) != 0;
-  if (real_out && !p->d.reality) return S2H_ERR_INVALID;
-  if ((flags & S2H_INV_SUBTRACT) && !sub_from) return S2H_ERR_INVALID;
-  if (!valid_precision(precision)) return S2H_ERR_INVALID;
-  if (batch == 0) return S2H_OK;
-  int st = check_device(p);
-  if (st) return st;
-  if (p->d.norm == S2H_NORM_NONE) return S2H_ERR_INVALID;
-  hipStream_t s = static_cast<hipStream_t>(stream);
+namespace {
+
+// The inverse pipeline: latitudinal synthesis (+ poles) and the ring iFFTs;
+// with ps, the batch is a set of n-planes (complex output).
+int inverse_impl(const s2h_plan* p, const PlaneSet* ps, const void* flm, void* f, void* work,
+                 int batch, int precision, bool real_out, const void* sub_from, hipStream_t s) {
+  int st;
   const int L = p->d.L;
   cplx* ftm = static_cast<cplx*>(work);
   const int64_t ftm_stride = (int64_t)p->inv_rows * p->Mc;
-  if ((st = lat_inverse_stage(p, flm, ftm, batch, precision, s))) return st;
-
+  if ((st = lat_inverse_stage(p, flm, ftm, batch, precision, s, false, ps))) return st;
   RingInvArgs r{};
   r.in = ftm;
   r.in_map_stride = ftm_stride;
@@ -400,9 +420,29 @@ extern "C" int s2h_inverse_ex(const s2h_plan* p, const void* flm, void* f, void*
   r.map_stride = p->npix;
   r.chirp_pool = p->chirp;
   r.L = L;
-  r.sub_from = (flags & S2H_INV_SUBTRACT) ? sub_from : nullptr;
-  ProfScope ps(S2H_KERNEL_RING_FFT_INVERSE, s);
+  r.sub_from = sub_from;
+  ProfScope prof(S2H_KERNEL_RING_FFT_INVERSE, s);
   return hip_status(launch_ring_fft_inverse(r, p->inv_set, batch, s));
+}
+
+}  // namespace
+
+extern "C" int s2h_inverse_ex(const s2h_plan* p, const void* flm, void* f, void* work, int batch,
+                              int precision, int flags, const void* sub_from,
+                              s2h_stream stream) {
+  if (!p || !flm || !f || batch < 0 || (batch > 0 && !work)) return S2H_ERR_INVALID;
+  if (flags & ~(S2H_INV_REAL_OUTPUT | S2H_INV_SUBTRACT)) return S2H_ERR_INVALID;
+  const bool real_out = (flags & S2H_INV_REAL_OUTPUT) != 0;
+  if (real_out && !p->d.reality) return S2H_ERR_INVALID;
+  if ((flags & S2H_INV_SUBTRACT) && !sub_from) return S2H_ERR_INVALID;
+  if (!valid_precision(precision)) return S2H_ERR_INVALID;
+  if (batch == 0) return S2H_OK;
+  int st = check_device(p);
+  if (st) return st;
+  if (p->d.norm == S2H_NORM_NONE) return S2H_ERR_INVALID;
+  return inverse_impl(p, nullptr, flm, f, work, batch, precision, real_out,
+                      (flags & S2H_INV_SUBTRACT) ? sub_from : nullptr,
+                      static_cast<hipStream_t>(stream));
 }
 
 // ---------------------------------------------------------------- ring spectra
@@ -531,20 +571,11 @@ extern "C" int s2h_ring_dot(const s2h_plan* p, const void* Y, const double* num,
                                     alpha, static_cast<hipStream_t>(stream)));
 }
 
-// Adjoint of s2h_forward (the VJP of a complex-linear map, torch convention):
-// grad_f = F^H g.  F = sign N_l Lat_fwd . (e^{-im phi0} w_t) . DFT_ext (+ poles),
-// so F^H = DFT_ext^H . (w_t e^{+im phi0}) . Lat_inv with the forward plan's
-// tables: the ring iFFT with folding and weights after the latitudinal
-// synthesis on the forward grid.  With reality the fill's adjoint folds the
-// m < 0 cotangents onto m > 0 and the result is real.
-extern "C" int s2h_forward_adjoint(const s2h_plan* p, const void* g, void* grad_f, void* work,
-                                   int batch, s2h_stream stream) {
-  if (!p || !g || !grad_f || batch < 0 || (batch > 0 && !work)) return S2H_ERR_INVALID;
-  if (batch == 0) return S2H_OK;
-  int st = check_device(p);
-  if (st) return st;
-  if (p->d.norm == S2H_NORM_NONE) return S2H_ERR_INVALID;
-  hipStream_t s = static_cast<hipStream_t>(stream);
+namespace {
+
+int forward_adjoint_impl(const s2h_plan* p, const PlaneSet* ps, const void* g, void* grad_f,
+                         void* work, int batch, hipStream_t s) {
+  int st;
   const int L = p->d.L;
   const bool real = p->d.reality != 0;
   const int64_t fs = flm_stride(p);
@@ -566,7 +597,8 @@ extern "C" int s2h_forward_adjoint(const s2h_plan* p, const void* g, void* grad_
   a.ftm_ncol = p->Mc;
   a.ftm_m_first = p->m_lo;
   a.lmin_out = std::max(std::abs(p->d.spin), p->d.L_lower);
-  if ((st = hip_status(launch_lat_inverse(a, p->fwd_paired, s)))) return st;
+  if (ps) a.planes = ps->fwd;
+  if ((st = hip_status(launch_lat_inverse(a, ps ? false : p->fwd_paired, s)))) return st;
   if (p->fwd_row_south >= 0 || p->fwd_row_north >= 0) {
     // adjoint of the forward pole terms: the pole rows of the synthesis
     PoleArgs pa{};
@@ -577,8 +609,10 @@ extern "C" int s2h_forward_adjoint(const s2h_plan* p, const void* g, void* grad_
     pa.ftm_ncol = p->Mc;
     pa.ftm_m_first = p->m_lo;
     pa.L = L;
-    pa.spin = p->d.spin;
-    pa.lmin = a.lmin_out;
+    pa.spin = ps ? ps->spin0 : p->d.spin;
+    pa.spin_step = ps ? ps->spin_step : 0;
+    pa.lmin = ps ? p->d.L_lower : a.lmin_out;
+    pa.lmin_spin = ps ? 1 : 0;
     pa.row_south = p->fwd_row_south;
     pa.row_north = p->fwd_row_north;
     pa.norm = p->fwd_norm;
@@ -587,26 +621,7 @@ extern "C" int s2h_forward_adjoint(const s2h_plan* p, const void* g, void* grad_
   }
   int64_t in_stride = ftm_stride;
   if (resample) {
-    ResampleArgs ra{};
-    ra.in = spec;
-    ra.in_map_stride = (int64_t)p->fwd_in_rows * p->Mc;
-    ra.out = ftm;
-    ra.out_map_stride = ftm_stride;
-    ra.ncol = p->Mc;
-    ra.m_lo = p->m_lo;
-    ra.nm = p->nm;
-    ra.L = L;
-    ra.spin = p->d.spin;
-    ra.mw = p->fwd_kind == 1;
-    ra.nyq_split = real ? 1 : 0;
-    ra.weights = p->fwd_weights;
-    ra.phi_factor = ra.mw ? (2.0 * L) / (2.0 * L - 1.0) : 1.0;
-    ra.lds = p->rs_info[3];
-    for (int i = 0; i < 3; ++i) {
-      ra.conv[i] = p->rs_info[i];
-      ra.chirp_off[i] = p->rs_chirp[i];
-    }
-    ra.pool = p->chirp;
+    const ResampleArgs ra = resample_args(p, spec, ftm, ps);
     if ((st = hip_status(launch_resample_adjoint(ra, batch, s)))) return st;
     in_stride = ra.in_map_stride;
   }
@@ -629,18 +644,9 @@ extern "C" int s2h_forward_adjoint(const s2h_plan* p, const void* g, void* grad_
   return hip_status(launch_ring_fft_inverse(r, p->fwd_set, batch, s));
 }
 
-// Adjoint of s2h_inverse: grad_flm = I^H g = sign N_l Lat_fwd . (e^{-im phi0})
-// . DFT_ext on the inverse grid, plus the pole rows' adjoint.  With reality
-// the inverse reads only m >= 0 and completes Hermitian-symmetrically, so the
-// m > 0 cotangents double and m < 0 stay zero.
-extern "C" int s2h_inverse_adjoint(const s2h_plan* p, const void* g, int g_is_complex,
-                                   void* grad_flm, void* work, int batch, s2h_stream stream) {
-  if (!p || !g || !grad_flm || batch < 0 || (batch > 0 && !work)) return S2H_ERR_INVALID;
-  if (batch == 0) return S2H_OK;
-  int st = check_device(p);
-  if (st) return st;
-  if (p->d.norm == S2H_NORM_NONE) return S2H_ERR_INVALID;
-  hipStream_t s = static_cast<hipStream_t>(stream);
+int inverse_adjoint_impl(const s2h_plan* p, const PlaneSet* ps, const void* g, int g_is_complex,
+                         void* grad_flm, void* work, int batch, hipStream_t s) {
+  int st;
   const int L = p->d.L;
   const bool real = p->d.reality != 0;
   const int64_t fs = flm_stride(p);
@@ -670,7 +676,8 @@ extern "C" int s2h_inverse_adjoint(const s2h_plan* p, const void* g, int g_is_co
   a.ftm_ncol = p->Mc;
   a.ftm_m_first = p->m_lo;
   a.lmin_out = p->d.L_lower;
-  if ((st = hip_status(launch_lat_forward(a, p->inv_paired, s)))) return st;
+  if (ps) a.planes = ps->inv;
+  if ((st = hip_status(launch_lat_forward(a, ps ? false : p->inv_paired, s)))) return st;
   if (p->inv_row_south >= 0 || p->inv_row_north >= 0) {
     PoleArgs pa{};
     pa.flm = static_cast<cplx*>(grad_flm);
@@ -680,7 +687,8 @@ extern "C" int s2h_inverse_adjoint(const s2h_plan* p, const void* g, int g_is_co
     pa.ftm_ncol = p->Mc;
     pa.ftm_m_first = p->m_lo;
     pa.L = L;
-    pa.spin = p->d.spin;
+    pa.spin = ps ? ps->spin0 : p->d.spin;
+    pa.spin_step = ps ? ps->spin_step : 0;
     pa.lmin = p->d.L_lower;
     pa.row_south = p->inv_row_south;
     pa.row_north = p->inv_row_north;
@@ -693,6 +701,40 @@ extern "C" int s2h_inverse_adjoint(const s2h_plan* p, const void* g, int g_is_co
       return st;
   }
   return S2H_OK;
+}
+
+}  // namespace
+
+// Adjoint of s2h_forward (the VJP of a complex-linear map, torch convention):
+// grad_f = F^H g.  F = sign N_l Lat_fwd . (e^{-im phi0} w_t) . DFT_ext (+ poles),
+// so F^H = DFT_ext^H . (w_t e^{+im phi0}) . Lat_inv with the forward plan's
+// tables: the ring iFFT with folding and weights after the latitudinal
+// synthesis on the forward grid.  With reality the fill's adjoint folds the
+// m < 0 cotangents onto m > 0 and the result is real.
+extern "C" int s2h_forward_adjoint(const s2h_plan* p, const void* g, void* grad_f, void* work,
+                                   int batch, s2h_stream stream) {
+  if (!p || !g || !grad_f || batch < 0 || (batch > 0 && !work)) return S2H_ERR_INVALID;
+  if (batch == 0) return S2H_OK;
+  int st = check_device(p);
+  if (st) return st;
+  if (p->d.norm == S2H_NORM_NONE) return S2H_ERR_INVALID;
+  return forward_adjoint_impl(p, nullptr, g, grad_f, work, batch,
+                              static_cast<hipStream_t>(stream));
+}
+
+// Adjoint of s2h_inverse: grad_flm = I^H g = sign N_l Lat_fwd . (e^{-im phi0})
+// . DFT_ext on the inverse grid, plus the pole rows' adjoint.  With reality
+// the inverse reads only m >= 0 and completes Hermitian-symmetrically, so the
+// m > 0 cotangents double and m < 0 stay zero.
+extern "C" int s2h_inverse_adjoint(const s2h_plan* p, const void* g, int g_is_complex,
+                                   void* grad_flm, void* work, int batch, s2h_stream stream) {
+  if (!p || !g || !grad_flm || batch < 0 || (batch > 0 && !work)) return S2H_ERR_INVALID;
+  if (batch == 0) return S2H_OK;
+  int st = check_device(p);
+  if (st) return st;
+  if (p->d.norm == S2H_NORM_NONE) return S2H_ERR_INVALID;
+  return inverse_adjoint_impl(p, nullptr, g, g_is_complex, grad_flm, work, batch,
+                              static_cast<hipStream_t>(stream));
 }
 
 extern "C" int s2h_lat_forward(const s2h_plan* p, const void* ftm, void* flm, int batch,
@@ -883,7 +925,7 @@ extern "C" int healpix_fft_hip_last_status(void) { return g_last_xla_status; }
 
 extern "C" int s2h_gamma_dft(int N, int64_t npix, int mode, double scale, const void* in,
                              void* out, s2h_stream stream) {
-  if (N < 1 || N > 8 || npix < 0 || mode < 0 || mode > 3 || !in || !out || in == out)
+  if (N < 1 || N > 8 || npix < 0 || mode < 0 || mode > 5 || !in || !out || in == out)
     return S2H_ERR_INVALID;
   return hip_status(
       launch_gamma_dft(N, npix, mode, scale, in, out, static_cast<hipStream_t>(stream)));
@@ -988,4 +1030,173 @@ extern "C" int s2h_probe_mfma_f64_rate(int blocks, int iters, double* out, s2h_s
 extern "C" int s2h_probe_mfma_f32(const float* a, const float* b, float* out, s2h_stream stream) {
   if (!a || !b || !out) return S2H_ERR_INVALID;
   return hip_status(launch_mfma_f32_probe(a, b, out, static_cast<hipStream_t>(stream)));
+}
+
+// ---------------------------------------------------------------- Wigner
+// SO(3) transforms with the n-planes as one batch per stage (the reference
+// loops, or vmaps, a spin transform over n: s2fft/transforms/wigner.py:110-196,
+// 442-533, 536-645).  Planes are indexed k = N - 1 + n.
+namespace {
+
+int64_t wig_flm_stride(const s2h_wigner_plan* w) { return (int64_t)w->d.L * (2 * w->d.L - 1); }
+
+// The planes k0 .. k0 + count - 1 as one multi-plane batch (count >= 2), or
+// a plain transform of plane k0's own plan (count == 1).
+const s2h_plan* plane_plan(const s2h_wigner_plan* w, int k0, int count, PlaneSet* ps) {
+  if (count == 1) return w->plans[k0];
+  ps->fwd = w->fwd_planes + k0;
+  ps->inv = w->inv_planes + k0;
+  ps->spin0 = (w->d.N - 1) - k0;  // spin of plane k is -(k - (N - 1))
+  ps->spin_step = -1;
+  return w->geom;
+}
+
+// Workspace layout: [fban: G npix complex][flmn copy: G L (2L-1) complex]
+// [one real plane: npix doubles, padded to complex][spin work]
+struct WigWork {
+  cplx* fban;
+  cplx* gcopy;
+  double* real_plane;
+  void* spin;
+};
+
+WigWork wig_work(const s2h_wigner_plan* w, void* work) {
+  WigWork ww;
+  ww.fban = static_cast<cplx*>(work);
+  ww.gcopy = ww.fban + (int64_t)w->G * w->npix;
+  ww.real_plane = reinterpret_cast<double*>(ww.gcopy + (int64_t)w->G * wig_flm_stride(w));
+  ww.spin = reinterpret_cast<cplx*>(ww.real_plane) + (w->npix + 1) / 2;
+  return ww;
+}
+
+int wig_check(const s2h_wigner_plan* w, const void* a, const void* b, const void* work) {
+  if (!w || !a || !b || !work) return S2H_ERR_INVALID;
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess) return S2H_ERR_HIP;
+  return dev == w->device ? S2H_OK : S2H_ERR_INVALID;
+}
+
+}  // namespace
+
+extern "C" size_t s2h_wigner_workspace_bytes(const s2h_wigner_plan* w) {
+  if (!w) return 0;
+  const s2h_plan* p = w->plans[w->d.N - 1];
+  return ((size_t)w->G * ((size_t)w->npix + (size_t)wig_flm_stride(w)) +
+          (size_t)(w->npix + 1) / 2) * sizeof(cplx) +
+         s2h_workspace_bytes(p, w->G);
+}
+
+extern "C" int s2h_wigner_forward(const s2h_wigner_plan* w, const void* f, int f_is_complex,
+                                  void* flmn, void* work, s2h_stream stream) {
+  int st = wig_check(w, f, flmn, work);
+  if (st) return st;
+  if (w->d.reality && f_is_complex) return S2H_ERR_INVALID;  // a real signal: float64 f
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  const int N = w->d.N, G = w->G;
+  const WigWork ww = wig_work(w, work);
+  const int64_t fs = wig_flm_stride(w);
+  cplx* out = static_cast<cplx*>(flmn);
+  // fban[k] = (2 pi / G) sum_g f[g] e^{-2 pi i n g / G} (wigner.py:495-502)
+  if ((st = hip_status(launch_gamma_dft(N, w->npix, f_is_complex ? 0 : 2, 2.0 * M_PI / G, f,
+                                        ww.fban, s))))
+    return st;
+  PlaneSet ps;
+  if (!w->d.reality) {
+    const s2h_plan* p = plane_plan(w, 0, G, &ps);
+    return forward_impl(p, G > 1 ? &ps : nullptr, ww.fban, 1, out, ww.spin, G, S2H_PREC_FP64,
+                        false, s);
+  }
+  // reality: the n = 0 plane as a real transform, the n >= 1 planes as one
+  // batch, the n < 0 planes by symmetry (wigner.py:506-526)
+  if ((st = forward_impl(w->real0, nullptr, ww.fban + (int64_t)(N - 1) * w->npix, 1,
+                         out + (N - 1) * fs, ww.spin, 1, S2H_PREC_FP64, false, s)))
+    return st;
+  if (N > 1) {
+    const s2h_plan* p = plane_plan(w, N, N - 1, &ps);
+    if ((st = forward_impl(p, N - 1 > 1 ? &ps : nullptr, ww.fban + (int64_t)N * w->npix, 1,
+                           out + (int64_t)N * fs, ww.spin, N - 1, S2H_PREC_FP64, false, s)))
+      return st;
+  }
+  return hip_status(launch_wigner_fill(out, w->d.L, N, s));
+}
+
+extern "C" int s2h_wigner_inverse(const s2h_wigner_plan* w, const void* flmn, void* f,
+                                  void* work, s2h_stream stream) {
+  int st = wig_check(w, flmn, f, work);
+  if (st) return st;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  const int N = w->d.N, G = w->G;
+  const WigWork ww = wig_work(w, work);
+  const cplx* in = static_cast<const cplx*>(flmn);
+  // planes n >= 0 only for a real signal (wigner.py:179-195)
+  const int k0 = w->d.reality ? N - 1 : 0;
+  const int count = G - k0;
+  PlaneSet ps;
+  const s2h_plan* p = plane_plan(w, k0, count, &ps);
+  if ((st = inverse_impl(p, count > 1 ? &ps : nullptr, in + k0 * wig_flm_stride(w), ww.fban,
+                         ww.spin, count, S2H_PREC_FP64, false, nullptr, s)))
+    return st;
+  return hip_status(
+      launch_gamma_dft(N, w->npix, w->d.reality ? 3 : 1, 1.0, ww.fban, f, s));
+}
+
+// grad_f = F^H g: the planes' spin adjoints, then the adjoint of the gamma DFT.
+extern "C" int s2h_wigner_forward_adjoint(const s2h_wigner_plan* w, const void* g, void* grad_f,
+                                          void* work, s2h_stream stream) {
+  int st = wig_check(w, g, grad_f, work);
+  if (st) return st;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  const int N = w->d.N, G = w->G;
+  const WigWork ww = wig_work(w, work);
+  const int64_t fs = wig_flm_stride(w);
+  PlaneSet ps;
+  if (!w->d.reality) {
+    const s2h_plan* p = plane_plan(w, 0, G, &ps);
+    if ((st = forward_adjoint_impl(p, G > 1 ? &ps : nullptr, g, ww.fban, ww.spin, G, s)))
+      return st;
+    return hip_status(launch_gamma_dft(N, w->npix, 1, 2.0 * M_PI / G, ww.fban, grad_f, s));
+  }
+  // reality: the n < 0 planes' cotangents fold back onto n > 0 (adjoint of the
+  // symmetric fill), the n = 0 plane goes through the real plan's adjoint
+  if (hipMemcpyAsync(ww.gcopy, g, (size_t)G * fs * sizeof(cplx), hipMemcpyDeviceToDevice, s) !=
+      hipSuccess)
+    return S2H_ERR_HIP;
+  if ((st = hip_status(launch_wigner_fill_adjoint(ww.gcopy, w->d.L, N, s)))) return st;
+  // the real plan's adjoint writes float64, widened into plane n = 0 of fban
+  if ((st = forward_adjoint_impl(w->real0, nullptr, ww.gcopy + (N - 1) * fs, ww.real_plane,
+                                 ww.spin, 1, s)))
+    return st;
+  if ((st = hip_status(launch_real_to_complex(ww.real_plane, ww.fban, w->npix, s)))) return st;
+  if (N > 1) {
+    const s2h_plan* p = plane_plan(w, N, N - 1, &ps);
+    if ((st = forward_adjoint_impl(p, N - 1 > 1 ? &ps : nullptr, ww.gcopy + (int64_t)N * fs,
+                                   ww.fban + w->npix, ww.spin, N - 1, s)))
+      return st;
+  }
+  return hip_status(launch_gamma_dft(N, w->npix, 4, 2.0 * M_PI / G, ww.fban, grad_f, s));
+}
+
+// grad_flmn = I^H g: the adjoint of the gamma DFT, then the planes' spin
+// adjoints (zero for the n < 0 planes of a real signal, which the inverse
+// does not read).
+extern "C" int s2h_wigner_inverse_adjoint(const s2h_wigner_plan* w, const void* g,
+                                          int g_is_complex, void* grad_flmn, void* work,
+                                          s2h_stream stream) {
+  int st = wig_check(w, g, grad_flmn, work);
+  if (st) return st;
+  if (w->d.reality && g_is_complex) return S2H_ERR_INVALID;  // real output: real cotangent
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  const int N = w->d.N, G = w->G;
+  const WigWork ww = wig_work(w, work);
+  const int64_t fs = wig_flm_stride(w);
+  cplx* out = static_cast<cplx*>(grad_flmn);
+  const int mode = w->d.reality ? 5 : (g_is_complex ? 0 : 2);
+  if ((st = hip_status(launch_gamma_dft(N, w->npix, mode, 1.0, g, ww.fban, s)))) return st;
+  const int k0 = w->d.reality ? N - 1 : 0;
+  const int count = G - k0;
+  if (k0 > 0 && (st = hip_status(launch_zero(out, 2 * (int64_t)k0 * fs, s)))) return st;
+  PlaneSet ps;
+  const s2h_plan* p = plane_plan(w, k0, count, &ps);
+  return inverse_adjoint_impl(p, count > 1 ? &ps : nullptr, ww.fban, 1, out + k0 * fs, ww.spin,
+                              count, s);
 }

@@ -37,7 +37,11 @@ __global__ void __launch_bounds__(kThreads) chirp_setup_kernel(const int* length
 // Direct DFT over gamma (G = 2N-1 <= 15 points) for every (beta, alpha)
 // sample.  Modes: 0 forward from complex f, 2 forward from real f (both give
 // fban[k], k = n + N-1, n in [-(N-1), N-1]), 1 inverse to complex f from all
-// 2N-1 rows, 3 inverse to real f from the rows n >= 0 only (Hermitian).
+// 2N-1 rows, 3 inverse to real f from the rows n >= 0 only (Hermitian),
+// 4 Re of the inverse sum over the rows n >= 0 only (the adjoint of mode 2
+// restricted to n >= 0: Re sum_{n >= 0} x_n e^{+2 pi i n g / G}), 5 the
+// adjoint of mode 3: from real f, the N rows n >= 0 of the forward sum, the
+// rows n >= 1 doubled.
 __global__ void __launch_bounds__(kThreads) gamma_dft_kernel(int N, int64_t npix, int mode,
                                                              double scale, const void* in_,
                                                              void* out_) {
@@ -48,7 +52,7 @@ __global__ void __launch_bounds__(kThreads) gamma_dft_kernel(int N, int64_t npix
   if (mode == 0 || mode == 1) {
     const cplx* in = static_cast<const cplx*>(in_);
     for (int g = 0; g < G; ++g) x[g] = in[(int64_t)g * npix + p];
-  } else if (mode == 2) {
+  } else if (mode == 2 || mode == 5) {
     const double* in = static_cast<const double*>(in_);
     for (int g = 0; g < G; ++g) x[g] = {in[(int64_t)g * npix + p], 0.0};
   } else {
@@ -63,6 +67,14 @@ __global__ void __launch_bounds__(kThreads) gamma_dft_kernel(int N, int64_t npix
       for (int g = 0; g < G; ++g) acc = cadd(acc, cmul(x[g], expi_pi(-2 * (int64_t)n * g, G)));
       out[(int64_t)k * npix + p] = {acc.x * scale, acc.y * scale};
     }
+  } else if (mode == 5) {
+    cplx* out = static_cast<cplx*>(out_);
+    for (int n = 0; n < N; ++n) {
+      cplx acc = {0.0, 0.0};
+      for (int g = 0; g < G; ++g) acc = cadd(acc, cmul(x[g], expi_pi(-2 * (int64_t)n * g, G)));
+      const double w = (n ? 2.0 : 1.0) * scale;
+      out[(int64_t)n * npix + p] = {acc.x * w, acc.y * w};
+    }
   } else if (mode == 1) {
     cplx* out = static_cast<cplx*>(out_);
     for (int g = 0; g < G; ++g) {
@@ -75,9 +87,10 @@ __global__ void __launch_bounds__(kThreads) gamma_dft_kernel(int N, int64_t npix
     }
   } else {
     double* out = static_cast<double*>(out_);
+    const double wn = mode == 3 ? 2.0 : 1.0;
     for (int g = 0; g < G; ++g) {
       double acc = x[0].x;
-      for (int n = 1; n < N; ++n) acc += 2.0 * cmul(x[n], expi_pi(2 * (int64_t)n * g, G)).x;
+      for (int n = 1; n < N; ++n) acc += wn * cmul(x[n], expi_pi(2 * (int64_t)n * g, G)).x;
       out[(int64_t)g * npix + p] = acc * scale;
     }
   }
@@ -95,6 +108,24 @@ __global__ void __launch_bounds__(kThreads) wigner_fill_kernel(cplx* flmn, int L
   const cplx v = flmn[(N - 1 + n) * plane + (int64_t)l * (2 * L - 1) + (L - 1 - m)];
   const double s = ((m + n) & 1) ? -1.0 : 1.0;
   flmn[(N - 1 - n) * plane + (int64_t)l * (2 * L - 1) + c] = {s * v.x, -s * v.y};
+}
+
+// Adjoint of wigner_fill_kernel for a real-valued loss: the cotangent of
+// each filled entry flmn[N-1-n][l][L-1+m] flows back to its source
+// flmn[N-1+n][l][L-1-m] as (-1)^(m+n) conj(.): g[N-1+n][l][L-1+m] +=
+// (-1)^(m+n) conj(g[N-1-n][l][L-1-m]) for n >= 1 (in place).
+__global__ void __launch_bounds__(kThreads) wigner_fill_adjoint_kernel(cplx* g, int L, int N) {
+  const int n = blockIdx.z + 1;
+  const int l = blockIdx.y;
+  const int c = blockIdx.x * kThreads + threadIdx.x;
+  if (c >= 2 * L - 1) return;
+  const int m = c - (L - 1);
+  const int64_t plane = (int64_t)L * (2 * L - 1);
+  const cplx v = g[(N - 1 - n) * plane + (int64_t)l * (2 * L - 1) + (L - 1 - m)];
+  const double s = ((m + n) & 1) ? -1.0 : 1.0;
+  cplx* o = g + (N - 1 + n) * plane + (int64_t)l * (2 * L - 1) + c;
+  o->x += s * v.x;
+  o->y -= s * v.y;
 }
 
 // ------------------------------------------------------------ launchers
@@ -131,4 +162,27 @@ hipError_t allow_large_lds() {
                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 
+}  // namespace s2h
+
+namespace s2h {
+__global__ void __launch_bounds__(kThreads) real_to_complex_kernel(const double* in, cplx* out,
+                                                                   int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) out[i] = {in[i], 0.0};
+}
+
+hipError_t launch_real_to_complex(const double* in, cplx* out, int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(real_to_complex_kernel, dim3((unsigned)((n + kThreads - 1) / kThreads)),
+                     dim3(kThreads), 0, s, in, out, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_wigner_fill_adjoint(cplx* g, int L, int N, hipStream_t s) {
+  if (N < 2) return hipSuccess;
+  hipLaunchKernelGGL(wigner_fill_adjoint_kernel,
+                     dim3((2 * L - 1 + kThreads - 1) / kThreads, L, N - 1), dim3(kThreads), 0, s,
+                     g, L, N);
+  return hipGetLastError();
+}
 }  // namespace s2h

@@ -68,6 +68,15 @@ struct RingAliasArgs {
   double* part;             // [map][ring job] partial sums: the dot (dot mode) or |R|^2
 };
 
+// Recursion tables of one map of a multi-plane launch (the n-planes of a
+// Wigner transform: one spin -n per map, every other table shared).
+struct LatPlane {
+  const MStart* mstart;     // indexed by m - m_lo
+  const Coef* coef;         // [|m|][l]
+  int lmin_out;             // first l written / read
+  int pad;
+};
+
 struct LatArgs {
   const LatRing* rings;
   int nrings;               // entries of rings (pairs count as one)
@@ -94,6 +103,8 @@ struct LatArgs {
   int mirror;               // forward with fill, real maps: also write
                             // flm[l, -m] = (-1)^m conj(flm[l, m]) (no conjugate fill pass)
   int flm_packed;           // flm in the packed m-major layout (real maps, m >= 0 only)
+  const LatPlane* planes;   // non-null: map b uses planes[b] in place of mstart, coef and
+                            // lmin_out (VALU kernels, unpaired rings)
 };
 
 // Harmonic-side addressing.  Reference layout: flm[b][l][L-1+m].  Packed
@@ -140,6 +151,8 @@ hipError_t launch_chirp_setup(const int* lengths, const int* conv_lens, const in
 hipError_t launch_gamma_dft(int N, int64_t npix, int mode, double scale, const void* in,
                             void* out, hipStream_t s);
 hipError_t launch_wigner_fill(cplx* flmn, int L, int N, hipStream_t s);
+hipError_t launch_wigner_fill_adjoint(cplx* g, int L, int N, hipStream_t s);
+hipError_t launch_real_to_complex(const double* in, cplx* out, int64_t n, hipStream_t s);
 hipError_t allow_large_lds();
 
 hipError_t launch_lat_inverse(const LatArgs& a, bool paired, hipStream_t s);
@@ -201,7 +214,12 @@ struct PoleArgs {
   int row_north;            // -1 if absent
   const double* norm;       // per-l normalisation (times the global sign)
   int batch;
+  int spin_step;            // map b has spin + spin_step b (multi-plane launches)
+  int lmin_spin;            // lmin of map b is max(lmin, |its spin|)
 };
+__host__ __device__ __forceinline__ int pole_spin(const PoleArgs& a, int64_t b) {
+  return a.spin + a.spin_step * (int)b;
+}
 hipError_t launch_pole_inverse(const PoleArgs& a, hipStream_t s);
 hipError_t launch_pole_forward(const PoleArgs& a, hipStream_t s);
 hipError_t launch_zero(void* p, int64_t ndoubles, hipStream_t s);
@@ -268,6 +286,7 @@ struct ResampleArgs {
   int conv[3];              // Bluestein lengths of the DFTs 2L-1, 2L, 4L (0 = mixed)
   int64_t chirp_off[3];
   const cplx* pool;
+  int spin_step;            // map b has spin + spin_step b (multi-plane launches)
 };
 hipError_t launch_resample(const ResampleArgs& a, int batch, hipStream_t s);
 hipError_t allow_large_lds_resample();

@@ -36,11 +36,14 @@ __global__ void __launch_bounds__(kThreads) lat_inverse_kernel(LatArgs a) {
   const int p = blockIdx.y * kThreads + threadIdx.x;
   if (p >= a.nrings) return;
   const int b0 = blockIdx.z * NB;
-  const MStart st = a.mstart[m - a.m_lo];
+  // multi-plane launches (NB = 1): this map's own recursion tables
+  const LatPlane* pl = a.planes ? a.planes + b0 : nullptr;
+  const MStart st = (pl ? pl->mstart : a.mstart)[m - a.m_lo];
+  const int lmin = pl ? pl->lmin_out : a.lmin_out;
   const LatRing rg = a.rings[p];
   const double x = rg.x;
   const double bsign = m < 0 ? -1.0 : 1.0;
-  const Coef* __restrict__ cf = a.coef + (int64_t)am * a.L;
+  const Coef* __restrict__ cf = (pl ? pl->coef : a.coef) + (int64_t)am * a.L;
   const cplx* __restrict__ flm = a.flm + (int64_t)b0 * a.flm_map_stride + flm_col(a, m);
 
   double d, dm1 = 0.0;
@@ -63,7 +66,7 @@ __global__ void __launch_bounds__(kThreads) lat_inverse_kernel(LatArgs a) {
 #pragma unroll
     for (int i = 0; i < kCheck; ++i) {
       const int l = blk + i;
-      if (l < L && l >= a.lmin_out) {
+      if (l < L && l >= lmin) {
         const bool odd = PAIRED && ((l + m) & 1);
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
@@ -115,9 +118,12 @@ __global__ void __launch_bounds__(kThreads) lat_forward_kernel(LatArgs a) {
   const int64_t b = blockIdx.z;
   const int g = blockIdx.y;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const MStart st = a.mstart[m - a.m_lo];
+  // multi-plane launches: this map's own recursion tables
+  const LatPlane* pl = a.planes ? a.planes + b : nullptr;
+  const MStart st = (pl ? pl->mstart : a.mstart)[m - a.m_lo];
+  const int lmin = pl ? pl->lmin_out : a.lmin_out;
   const double bsign = m < 0 ? -1.0 : 1.0;
-  const Coef* __restrict__ cf = a.coef + (int64_t)am * a.L;
+  const Coef* __restrict__ cf = (pl ? pl->coef : a.coef) + (int64_t)am * a.L;
   const int L = a.L;
   const cplx* ftm = a.ftm + b * a.ftm_map_stride + (m - a.ftm_m_first);
 
@@ -196,7 +202,7 @@ __global__ void __launch_bounds__(kThreads) lat_forward_kernel(LatArgs a) {
     __syncthreads();
     if (threadIdx.x < kChunk) {
       const int l = blk + threadIdx.x;
-      if (l < L && l >= a.lmin_out) {
+      if (l < L && l >= lmin) {
         cplx t = wsum[0][threadIdx.x];
 #pragma unroll
         for (int w = 1; w < kThreads / 64; ++w) t = cadd(t, wsum[w][threadIdx.x]);
@@ -227,6 +233,7 @@ static hipError_t lat_inv_nb(const LatArgs& a, bool paired, hipStream_t s) {
 
 hipError_t launch_lat_inverse(const LatArgs& a, bool paired, hipStream_t s) {
   if (a.nm == 0 || a.nrings == 0 || a.batch == 0) return hipSuccess;
+  if (a.planes) return lat_inv_nb<1>(a, paired, s);  // one map (plane) per block
   if (a.batch >= 8) return lat_inv_nb<8>(a, paired, s);
   if (a.batch >= 4) return lat_inv_nb<4>(a, paired, s);
   if (a.batch >= 2) return lat_inv_nb<2>(a, paired, s);

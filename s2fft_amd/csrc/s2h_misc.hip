@@ -13,16 +13,19 @@ __global__ void __launch_bounds__(kThreads) pole_inverse_kernel(PoleArgs a) {
   const bool south = blockIdx.x == 0;
   const int row = south ? a.row_south : a.row_north;
   if (row < 0) return;
+  const int spin = pole_spin(a, b);
+  const int aspin = spin < 0 ? -spin : spin;
+  const int lmin = a.lmin_spin && aspin > a.lmin ? aspin : a.lmin;
   cplx* r = a.ftm + b * a.ftm_map_stride + (int64_t)row * a.ftm_ncol;
   for (int c = threadIdx.x; c < a.ftm_ncol; c += kThreads) r[c] = {0.0, 0.0};
   __syncthreads();
   if (threadIdx.x == 0) {
-    const int m = south ? a.spin : -a.spin;
+    const int m = south ? spin : -spin;
     const cplx* f = a.flm + b * a.flm_map_stride + (a.L - 1 + m);
     cplx acc = {0.0, 0.0};
-    for (int l = a.lmin; l < a.L; ++l) {
+    for (int l = lmin; l < a.L; ++l) {
       double w = a.norm[l];
-      if (south && ((l - a.spin) & 1)) w = -w;
+      if (south && ((l - spin) & 1)) w = -w;
       const cplx v = f[(int64_t)l * (2 * a.L - 1)];
       acc.x += w * v.x;
       acc.y += w * v.y;
@@ -37,25 +40,28 @@ __global__ void __launch_bounds__(kThreads) pole_inverse_kernel(PoleArgs a) {
 __global__ void __launch_bounds__(kThreads) pole_forward_kernel(PoleArgs a) {
   const int64_t b = blockIdx.y;
   const int l = blockIdx.x * kThreads + threadIdx.x;
-  if (l < a.lmin || l >= a.L) return;
+  const int spin = pole_spin(a, b);
+  const int aspin = spin < 0 ? -spin : spin;
+  const int lmin = a.lmin_spin && aspin > a.lmin ? aspin : a.lmin;
+  if (l < lmin || l >= a.L) return;
   const cplx* base = a.ftm + b * a.ftm_map_stride;
   cplx* f = a.flm + b * a.flm_map_stride + (int64_t)l * (2 * a.L - 1);
   const double w = a.norm[l];
   if (a.row_south >= 0) {
-    const int c = a.spin - a.ftm_m_first;
+    const int c = spin - a.ftm_m_first;
     if (c >= 0 && c < a.ftm_ncol) {
       const cplx v = base[(int64_t)a.row_south * a.ftm_ncol + c];
-      const double s = ((l - a.spin) & 1) ? -w : w;
-      cplx* o = f + (a.L - 1 + a.spin);
+      const double s = ((l - spin) & 1) ? -w : w;
+      cplx* o = f + (a.L - 1 + spin);
       o->x += s * v.x;
       o->y += s * v.y;
     }
   }
   if (a.row_north >= 0) {
-    const int c = -a.spin - a.ftm_m_first;
+    const int c = -spin - a.ftm_m_first;
     if (c >= 0 && c < a.ftm_ncol) {
       const cplx v = base[(int64_t)a.row_north * a.ftm_ncol + c];
-      cplx* o = f + (a.L - 1 - a.spin);
+      cplx* o = f + (a.L - 1 - spin);
       o->x += w * v.x;
       o->y += w * v.y;
     }
@@ -427,7 +433,8 @@ __global__ void __launch_bounds__(kThreads) resample_kernel(ResampleArgs a) {
   const int64_t b = blockIdx.y;
   double* re = smem;
   double* im = smem + a.lds;
-  const double msgn = (((m + a.spin) % 2) + 2) % 2 ? -1.0 : 1.0;
+  const int spin_b = a.spin + a.spin_step * (int)b;
+  const double msgn = (((m + spin_b) % 2) + 2) % 2 ? -1.0 : 1.0;
   const cplx* g = a.in + b * a.in_map_stride + col;
 
   int nrow;  // rows of the MWSS column at L now held in LDS
@@ -634,7 +641,8 @@ __global__ void __launch_bounds__(kThreads) resample_adjoint_kernel(ResampleArgs
   const int64_t b = blockIdx.y;
   double* re = smem;
   double* im = smem + a.lds;
-  const double msgn = (((m + a.spin) % 2) + 2) % 2 ? -1.0 : 1.0;
+  const int spin_b = a.spin + a.spin_step * (int)b;
+  const double msgn = (((m + spin_b) % 2) + 2) % 2 ? -1.0 : 1.0;
   constexpr int kPer = kMaxConvLen / kThreads;
 
   // rows 0..2L of the upsampled grid, weighted, zero-extended to 4L

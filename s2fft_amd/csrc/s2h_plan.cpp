@@ -677,3 +677,70 @@ extern "C" int s2h_plan_destroy(s2h_plan* p) {
   delete p;
   return S2H_OK;
 }
+
+// ---------------------------------------------------------------- Wigner
+// One spin plan per n (spin -n, Wigner normalisation, sign (-1)^n: the
+// reference's per-n precomputes of price_mcewen.py:258-374 with the
+// normalisation of wigner.py:179-195 folded in), and their recursion tables
+// gathered into per-plane arrays for multi-plane launches.
+extern "C" int s2h_wigner_plan_destroy(s2h_wigner_plan* w) {
+  if (!w) return S2H_OK;
+  for (s2h_plan* p : w->plans) s2h_plan_destroy(p);
+  s2h_plan_destroy(w->real0);
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  (void)hipSetDevice(w->device);
+  for (void* ptr : w->allocations) (void)hipFree(ptr);
+  (void)hipSetDevice(cur);
+  delete w;
+  return S2H_OK;
+}
+
+extern "C" int s2h_wigner_plan_create(s2h_wigner_plan** out, const s2h_wigner_desc* desc) {
+  if (!out || !desc) return S2H_ERR_INVALID;
+  *out = nullptr;
+  const s2h_wigner_desc& d = *desc;
+  if (d.N < 1) return S2H_ERR_INVALID;
+  if (d.N >= 8) return S2H_ERR_SPIN;  // wigner.py:87-88, 419-420
+  if (d.N - 1 >= d.L) return S2H_ERR_INVALID;
+  s2h_wigner_plan* w = new (std::nothrow) s2h_wigner_plan();
+  if (!w) return S2H_ERR_NOMEM;
+  w->d = d;
+  w->G = 2 * d.N - 1;
+  if (hipGetDevice(&w->device) != hipSuccess) {
+    delete w;
+    return S2H_ERR_HIP;
+  }
+  int st = S2H_OK;
+  for (int n = -(d.N - 1); n < d.N && st == S2H_OK; ++n) {
+    s2h_desc sd{d.sampling, d.L, d.nside, -n, d.L_lower, 0, S2H_NORM_WIGNER, (n & 1) ? -1 : 1};
+    s2h_plan* p = nullptr;
+    st = s2h_plan_create(&p, &sd);
+    if (st == S2H_OK) w->plans.push_back(p);
+  }
+  if (st == S2H_OK && d.reality) {
+    s2h_desc sd{d.sampling, d.L, d.nside, 0, d.L_lower, 1, S2H_NORM_WIGNER, 1};
+    st = s2h_plan_create(&w->real0, &sd);
+  }
+  if (st == S2H_OK) {
+    w->npix = w->plans[d.N - 1]->npix;
+    if (d.N > 1) w->geom = w->plans[d.N];  // n = 1: spin -1, unpaired rings
+    std::vector<LatPlane> fp, ip;
+    for (int k = 0; k < w->G; ++k) {
+      const s2h_plan* p = w->plans[k];
+      const int an = std::abs(k - (d.N - 1));
+      fp.push_back(LatPlane{p->fwd_mstart, p->fwd_coef, std::max(an, d.L_lower), 0});
+      ip.push_back(LatPlane{p->inv_mstart, p->inv_coef, d.L_lower, 0});
+    }
+    s2h_plan holder;  // upload() records allocations on a plan
+    st = upload(&holder, fp, &w->fwd_planes);
+    if (st == S2H_OK) st = upload(&holder, ip, &w->inv_planes);
+    w->allocations = holder.allocations;
+  }
+  if (st != S2H_OK) {
+    s2h_wigner_plan_destroy(w);
+    return st;
+  }
+  *out = w;
+  return S2H_OK;
+}

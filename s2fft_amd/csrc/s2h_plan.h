@@ -49,6 +49,18 @@ struct s2h_plan {
 };
 
 namespace s2h {
+// A multi-plane launch: map b of a batch is the n-plane with its own
+// recursion tables (fwd[b] / inv[b], device arrays) and spin spin0 +
+// spin_step b; every other table comes from the geometry plan, whose rings
+// must be unpaired (a spin != 0 plan).  The Wigner transforms run all their
+// n-planes as one batch this way.
+struct PlaneSet {
+  const LatPlane* fwd = nullptr;
+  const LatPlane* inv = nullptr;
+  int spin0 = 0;
+  int spin_step = 0;
+};
+
 // Ring-job tables for the reference-layout HEALPix FFTs (cached per device).
 struct HpTables {
   JobSet set;
@@ -60,3 +72,20 @@ struct HpTables {
 int healpix_tables_prepare(int nside, int L);
 int healpix_tables(int nside, int L, HpTables* out);
 }  // namespace s2h
+
+// Wigner (SO(3)) transform plan: one spin plan per n (spin -n, Wigner
+// normalisation, sign (-1)^n), with their recursion tables gathered into
+// per-plane arrays so that the n-planes run as one batch (s2h_capi.cpp).
+struct s2h_wigner_plan {
+  s2h_wigner_desc d{};
+  int device = 0;
+  int G = 0;                       // 2N - 1 gamma samples / n-planes
+  int64_t npix = 0;                // (beta, alpha) samples per gamma
+  std::vector<s2h_plan*> plans;    // plans[N - 1 + n], complex, n in [-(N-1), N)
+  s2h_plan* real0 = nullptr;       // reality: the n = 0 plane's forward (spin 0, real)
+  s2h_plan* geom = nullptr;        // an unpaired plan for multi-plane launches (N > 1)
+  // per-plane tables, n = -(N-1) .. N-1 (index N - 1 + n)
+  s2h::LatPlane* fwd_planes = nullptr;
+  s2h::LatPlane* inv_planes = nullptr;
+  std::vector<void*> allocations;
+};
