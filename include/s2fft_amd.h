@@ -25,6 +25,11 @@
  *                                  descriptor of lib/include/s2fft.h:29-59
  *   s2h_build_healpix_fft_descriptor
  *                                  lib/src/extensions.cc:115-149
+ *   s2h_compute_all_slices         s2fft/recursions/price_mcewen.py:377-643
+ *   s2h_precompute_kernel / _forward / _inverse
+ *                                  s2fft/precompute_transforms/construct.py:
+ *                                  16-161, spherical.py:87-149,314-380
+ *   s2h_wigner_*                   s2fft/transforms/wigner.py:13-107,346-439
  *   s2h_gamma_dft                  the FFT over gamma of
  *                                  s2fft/transforms/wigner.py:183-195,495-502
  *   s2h_wigner_reality_fill        s2fft/transforms/wigner.py:523-526
@@ -248,6 +253,36 @@ int s2h_wigner_forward_adjoint(const s2h_wigner_plan *plan, const void *g, void 
                                void *work, s2h_stream stream);
 int s2h_wigner_inverse_adjoint(const s2h_wigner_plan *plan, const void *g, int g_is_complex,
                                void *grad_flmn, void *work, s2h_stream stream);
+
+/* Stored Wigner-d tables.
+ * s2h_compute_all_slices replaces compute_all_slices
+ * (s2fft/recursions/price_mcewen.py:377-643): for a plan created with
+ * S2H_NORM_NONE (reality 0; its sampling is irrelevant), beta: ntheta device
+ * doubles, dl: (2L-1, ntheta, L) float64 with dl[L-1+m][t][l] =
+ * d^l_{m,-spin}(beta_t), zero for l < max(|m|, |spin|).
+ * The precompute path (s2fft/precompute_transforms/construct.py:16-161 and
+ * spherical.py:87-149, 314-380): s2h_precompute_kernel writes the reference's
+ * kernel K[t][l][m - m_start] = scale d^l_{m,-spin}(theta_t) x the plan's
+ * normalisation (x the forward quadrature weight, x the HEALPix ring phase
+ * e^{-+ i m phi0}) on the forward grid (MW/MWSS: the MWSS grid at 2L, poles
+ * included) or the inverse grid, float64 or complex128 (HEALPix),
+ * s2h_precompute_kernel_bytes(plan, forward) bytes; m from 0 with reality,
+ * else from -(L-1).  s2h_precompute_forward: the ring FFTs (+ resampling)
+ * without weights or phases, then flm = scale sum_t K ftm, conjugate fill
+ * with reality (the reference's scale is (-1)^spin).  s2h_precompute_inverse:
+ * ftm = scale sum_l K flm, then the ring iFFTs without phases.  A kernel the
+ * reference built (same layout) may be passed.  work: s2h_workspace_bytes. */
+int s2h_compute_all_slices(const s2h_plan *plan, const double *beta, int ntheta, double *dl,
+                           s2h_stream stream);
+size_t s2h_precompute_kernel_bytes(const s2h_plan *plan, int forward);
+int s2h_precompute_kernel(const s2h_plan *plan, int forward, double scale, void *kernel,
+                          s2h_stream stream);
+int s2h_precompute_forward(const s2h_plan *plan, const void *kernel, int kernel_is_complex,
+                           double scale, const void *f, int f_is_complex, void *flm, void *work,
+                           int batch, s2h_stream stream);
+int s2h_precompute_inverse(const s2h_plan *plan, const void *kernel, int kernel_is_complex,
+                           double scale, const void *flm, void *f, void *work, int batch,
+                           s2h_stream stream);
 
 /* The transforms split at the ring spectra (no counterpart in the reference;
  * for refinement in spectral space).  For plans whose forward and inverse

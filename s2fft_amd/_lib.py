@@ -50,7 +50,8 @@ EXPORTED = (
     "s2h_build_healpix_fft_descriptor",
     "s2h_wigner_plan_create", "s2h_wigner_plan_destroy", "s2h_wigner_workspace_bytes",
     "s2h_wigner_forward", "s2h_wigner_inverse", "s2h_wigner_forward_adjoint",
-    "s2h_wigner_inverse_adjoint",
+    "s2h_wigner_inverse_adjoint", "s2h_compute_all_slices", "s2h_precompute_kernel_bytes",
+    "s2h_precompute_kernel", "s2h_precompute_forward", "s2h_precompute_inverse",
     "s2h_profile_enable", "s2h_profile_query", "s2h_wigner_reality_fill",
     "s2h_batch_sumsq_workspace_bytes", "s2h_batch_sumsq",
     "s2h_ring_spectra_bytes", "s2h_ring_analysis", "s2h_forward_rings", "s2h_inverse_rings",
@@ -110,6 +111,12 @@ def _declare(lib):
         "s2h_build_healpix_fft_descriptor": (i32, [i32, i32, i32, i32, i32, i32,
                                                    ctypes.POINTER(HealpixFftDescriptor)]),
         "healpix_fft_hip": (None, [vp, ctypes.POINTER(vp), ctypes.c_char_p, sz]),
+        "s2h_compute_all_slices": (i32, [vp, vp, i32, vp, vp]),
+        "s2h_precompute_kernel_bytes": (sz, [vp, i32]),
+        "s2h_precompute_kernel": (i32, [vp, i32, ctypes.c_double, vp, vp]),
+        "s2h_precompute_forward": (i32, [vp, vp, i32, ctypes.c_double, vp, i32, vp, vp, i32,
+                                         vp]),
+        "s2h_precompute_inverse": (i32, [vp, vp, i32, ctypes.c_double, vp, vp, vp, i32, vp]),
         "s2h_wigner_plan_create": (i32, [ctypes.POINTER(vp), vp]),
         "s2h_wigner_plan_destroy": (i32, [vp]),
         "s2h_wigner_workspace_bytes": (sz, [vp]),

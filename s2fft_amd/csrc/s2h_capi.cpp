@@ -152,7 +152,7 @@ bool valid_precision(int precision) {
 // The ring FFTs of f (weights and ring phases fused) into `out`: the plan's
 // forward grid, or its input rows when the forward resamples.
 int ring_analysis_stage(const s2h_plan* p, const void* f, int f_is_complex, cplx* out, int batch,
-                        hipStream_t s) {
+                        hipStream_t s, bool raw = false) {
   const bool real = p->d.reality != 0;
   const bool resample = p->fwd_kind != 0;
   RingFwdArgs r{};
@@ -173,8 +173,10 @@ int ring_analysis_stage(const s2h_plan* p, const void* f, int f_is_complex, cplx
   r.out_map_stride = (int64_t)(resample ? p->fwd_in_rows : p->fwd_rows) * p->Mc;
   r.ncol = p->Mc;
   r.m_first = p->m_lo;
-  r.apply_weight = resample ? 0 : 1;
-  r.apply_phase = p->d.sampling == S2H_HEALPIX;
+  // raw: no quadrature weight and no ring phase (a stored precompute kernel
+  // carries both)
+  r.apply_weight = resample || raw ? 0 : 1;
+  r.apply_phase = !raw && p->d.sampling == S2H_HEALPIX;
   r.zero_neg_2L = 0;
   r.L = p->d.L;
   r.chirp_pool = p->chirp;
@@ -1199,4 +1201,146 @@ extern "C" int s2h_wigner_inverse_adjoint(const s2h_wigner_plan* w, const void* 
   const s2h_plan* p = plane_plan(w, k0, count, &ps);
   return inverse_adjoint_impl(p, count > 1 ? &ps : nullptr, ww.fban, 1, out + k0 * fs, ww.spin,
                               count, s);
+}
+
+// ---------------------------------------------------------------- d-tables
+// compute_all_slices (s2fft/recursions/price_mcewen.py:377-643) and the
+// precompute path (s2fft/precompute_transforms/construct.py:16-161,
+// spherical.py:87-149, 314-380) on the device.
+extern "C" int s2h_compute_all_slices(const s2h_plan* p, const double* beta, int ntheta,
+                                      double* dl, s2h_stream stream) {
+  if (!p || ntheta < 0 || (ntheta > 0 && (!beta || !dl))) return S2H_ERR_INVALID;
+  if (p->d.norm != S2H_NORM_NONE || p->d.reality) return S2H_ERR_INVALID;
+  if (ntheta == 0) return S2H_OK;
+  int st = check_device(p);
+  if (st) return st;
+  DTableArgs a{};
+  a.mstart = p->inv_mstart;
+  a.coef = p->inv_coef;
+  a.theta = beta;
+  a.T = ntheta;
+  a.L = p->d.L;
+  a.m_lo = p->m_lo;
+  a.nm = p->nm;
+  a.scale = p->sign;  // undo the plan's global sign: bare d
+  a.out = dl;
+  a.sm = (int64_t)ntheta * p->d.L;
+  a.st = p->d.L;
+  a.sl = 1;
+  return hip_status(launch_dtable(a, static_cast<hipStream_t>(stream)));
+}
+
+extern "C" size_t s2h_precompute_kernel_bytes(const s2h_plan* p, int forward) {
+  if (!p || p->d.norm == S2H_NORM_NONE) return 0;
+  const size_t T = (size_t)(forward ? p->fwd_rows : p->inv_rows);
+  const size_t el = p->d.sampling == S2H_HEALPIX ? sizeof(cplx) : sizeof(double);
+  return T * (size_t)p->d.L * (size_t)p->nm * el;
+}
+
+extern "C" int s2h_precompute_kernel(const s2h_plan* p, int forward, double scale, void* kernel,
+                                     s2h_stream stream) {
+  if (!p || !kernel || p->d.norm == S2H_NORM_NONE) return S2H_ERR_INVALID;
+  int st = check_device(p);
+  if (st) return st;
+  const bool hp = p->d.sampling == S2H_HEALPIX;
+  DTableArgs a{};
+  a.mstart = forward ? p->fwd_mstart : p->inv_mstart;
+  a.coef = forward ? p->fwd_coef : p->inv_coef;
+  a.theta = forward ? p->fwd_theta : p->inv_theta;
+  a.T = forward ? p->fwd_rows : p->inv_rows;
+  a.L = p->d.L;
+  a.m_lo = p->m_lo;
+  a.nm = p->nm;
+  a.wt = forward ? p->fwd_row_w : nullptr;
+  a.phi0 = hp ? p->row_phi0 : nullptr;
+  a.phase_sign = forward ? -1.0 : 1.0;
+  a.scale = scale * p->sign;  // the plan's tables carry its global sign
+  a.cplx = hp ? 1 : 0;
+  a.out = kernel;
+  a.sm = 1;
+  a.st = (int64_t)p->d.L * p->nm;
+  a.sl = p->nm;
+  return hip_status(launch_dtable(a, static_cast<hipStream_t>(stream)));
+}
+
+extern "C" int s2h_precompute_forward(const s2h_plan* p, const void* kernel, int kernel_is_complex,
+                                      double scale, const void* f, int f_is_complex, void* flm,
+                                      void* work, int batch, s2h_stream stream) {
+  if (!p || !kernel || !f || !flm || batch < 0 || (batch > 0 && !work)) return S2H_ERR_INVALID;
+  if (p->d.norm == S2H_NORM_NONE) return S2H_ERR_INVALID;
+  if (batch == 0) return S2H_OK;
+  int st = check_device(p);
+  if (st) return st;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  cplx* ftm = static_cast<cplx*>(work);
+  const bool resample = p->fwd_kind != 0;
+  cplx* spec = ftm;
+  if (resample) ftm = spec + (int64_t)batch * p->fwd_in_rows * p->Mc;
+  if ((st = ring_analysis_stage(p, f, f_is_complex, resample ? spec : ftm, batch, s, true)))
+    return st;
+  if (resample) {
+    ResampleArgs ra = resample_args(p, spec, ftm, nullptr);
+    ra.weights = nullptr;  // the kernel carries the quadrature weights
+    if ((st = hip_status(launch_resample(ra, batch, s)))) return st;
+  }
+  PcArgs a{};
+  a.kernel = kernel;
+  a.kernel_complex = kernel_is_complex ? 1 : 0;
+  a.T = p->fwd_rows;
+  a.L = p->d.L;
+  a.m_lo = p->m_lo;
+  a.nm = p->nm;
+  a.y = ftm;
+  a.y_map_stride = (int64_t)p->fwd_rows * p->Mc;
+  a.flm = static_cast<cplx*>(flm);
+  a.flm_map_stride = flm_stride(p);
+  a.scale = scale;
+  a.batch = batch;
+  if ((st = hip_status(launch_pc_forward(a, s)))) return st;
+  if (p->d.reality)
+    return hip_status(launch_reality_fill(static_cast<cplx*>(flm), flm_stride(p), p->d.L, batch, s));
+  return S2H_OK;
+}
+
+extern "C" int s2h_precompute_inverse(const s2h_plan* p, const void* kernel, int kernel_is_complex,
+                                      double scale, const void* flm, void* f, void* work,
+                                      int batch, s2h_stream stream) {
+  if (!p || !kernel || !f || !flm || batch < 0 || (batch > 0 && !work)) return S2H_ERR_INVALID;
+  if (p->d.norm == S2H_NORM_NONE) return S2H_ERR_INVALID;
+  if (batch == 0) return S2H_OK;
+  int st = check_device(p);
+  if (st) return st;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  const int L = p->d.L;
+  cplx* ftm = static_cast<cplx*>(work);
+  PcArgs a{};
+  a.kernel = kernel;
+  a.kernel_complex = kernel_is_complex ? 1 : 0;
+  a.T = p->inv_rows;
+  a.L = L;
+  a.m_lo = p->m_lo;
+  a.nm = p->nm;
+  a.y = ftm;
+  a.y_map_stride = (int64_t)p->inv_rows * p->Mc;
+  a.flm = const_cast<cplx*>(static_cast<const cplx*>(flm));
+  a.flm_map_stride = flm_stride(p);
+  a.scale = scale;
+  a.batch = batch;
+  if ((st = hip_status(launch_pc_inverse(a, s)))) return st;
+  RingInvArgs r{};
+  r.in = ftm;
+  r.in_map_stride = a.y_map_stride;
+  r.ncol = p->Mc;
+  r.m_first = p->m_lo;
+  r.m_lo = -(L - 1);
+  r.m_hi = L - 1;
+  r.herm_mode = p->d.reality ? 1 : 0;
+  r.apply_phase = 0;  // the kernel carries the ring phases
+  r.apply_weight = 0;
+  r.f = f;
+  r.f_complex = p->inv_out_complex ? 1 : 0;
+  r.map_stride = p->npix;
+  r.chirp_pool = p->chirp;
+  r.L = L;
+  return hip_status(launch_ring_fft_inverse(r, p->inv_set, batch, s));
 }

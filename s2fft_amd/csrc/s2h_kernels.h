@@ -198,6 +198,42 @@ hipError_t launch_mlp_adam(float* w, const float* grad, float* m1, float* m2, in
 hipError_t launch_mfma_bf16_probe(const uint16_t* a, const uint16_t* b, float* out,
                                   hipStream_t s);
 
+// Stored d-tables (s2h_precompute.hip): out[mi * sm + t * st + l * sl] =
+// scale wt[t] lw[l] d^l_{m,n}(theta_t) (times e^{i phase_sign m phi0[t]} as
+// complex128 when cplx), m = m_lo + mi, from a plan's recursion tables.
+struct DTableArgs {
+  const MStart* mstart;     // indexed by mi
+  const Coef* coef;         // [|m|][l]
+  const double* theta;      // T colatitudes
+  int T, L, m_lo, nm;
+  const double* wt;         // per-theta factor, or null
+  const double* lw;         // per-l factor, or null
+  const double* phi0;       // per-theta azimuth (cplx), or null
+  double phase_sign;
+  double scale;
+  int cplx;
+  void* out;
+  int64_t sm, st, sl;
+};
+hipError_t launch_dtable(const DTableArgs& a, hipStream_t s);
+
+// Contractions with a stored kernel K[t][l][mi] (float64 or complex128):
+// forward flm[b][l][L-1+m] = scale sum_t K Y[b][t][mi]; inverse
+// Y[b][t][mi] = scale sum_l K flm[b][l][L-1+m]; m = m_lo + mi.
+struct PcArgs {
+  const void* kernel;
+  int kernel_complex;
+  int T, L, m_lo, nm;
+  cplx* y;                  // [b][t][mi], y_map_stride per map
+  int64_t y_map_stride;
+  cplx* flm;                // [b][l][L-1+m], flm_map_stride per map
+  int64_t flm_map_stride;
+  double scale;
+  int batch;
+};
+hipError_t launch_pc_forward(const PcArgs& a, hipStream_t s);
+hipError_t launch_pc_inverse(const PcArgs& a, hipStream_t s);
+
 // Small fix-up kernels (s2h_misc.hip).
 struct PoleArgs {
   // inverse: set ftm rows of the poles; forward: add pole terms into flm
@@ -280,7 +316,7 @@ struct ResampleArgs {
   int L, spin;
   int mw;                   // 1: MW input (L rows), 0: MWSS input (L+1 rows)
   int nyq_split;            // real signals: split the theta-Nyquist bin (see kernel)
-  const double* weights;    // 2L+1 quadrature weights of MWSS at 2L
+  const double* weights;    // 2L+1 quadrature weights of MWSS at 2L (null: none)
   double phi_factor;        // 2L/(2L-1) for MW, 1 for MWSS
   int lds;                  // LDS elements (complex) per workgroup
   int conv[3];              // Bluestein lengths of the DFTs 2L-1, 2L, 4L (0 = mixed)

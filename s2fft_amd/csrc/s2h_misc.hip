@@ -555,7 +555,7 @@ __global__ void __launch_bounds__(kThreads) resample_kernel(ResampleArgs a) {
   col_dft(re, im, 4 * L, a.conv[2], a.chirp_off[2], a.pool, +1);
   cplx* out = a.out + b * a.out_map_stride + col;
   for (int t = threadIdx.x; t <= 2 * L; t += kThreads) {
-    const double w = a.weights[t] * a.phi_factor;
+    const double w = (a.weights ? a.weights[t] : 1.0) * a.phi_factor;
     out[(int64_t)t * a.ncol] = {re[t] * w, im[t] * w};
   }
 }

@@ -628,6 +628,22 @@ static int plan_build(s2h_plan* p) {
   if (!wf.empty() && p->fwd_kind != 0) {
     if ((st = upload(p, wf, &p->fwd_weights))) return st;
   }
+  {
+    std::vector<double> thf_rows;
+    if (p->fwd_kind == 0) {
+      thf_rows = thi;
+    } else {
+      for (int t = 0; t <= 2 * L; ++t) thf_rows.push_back(2.0 * t * M_PI / (4.0 * L));
+    }
+    if ((st = upload(p, thf_rows, &p->fwd_theta))) return st;
+    if ((st = upload(p, thi, &p->inv_theta))) return st;
+    if ((st = upload(p, wf, &p->fwd_row_w))) return st;
+    if (hp) {
+      std::vector<double> ph;
+      for (int t = 0; t < p->inv_rows; ++t) ph.push_back(M_PI * healpix_phi0_over_pi(t, nside));
+      if ((st = upload(p, ph, &p->row_phi0))) return st;
+    }
+  }
   if ((st = cb.build(&p->chirp, &p->allocations))) return st;
   return S2H_OK;
 }

@@ -45,6 +45,14 @@ struct s2h_plan {
 
   s2h::cplx* chirp = nullptr;
   int64_t npix = 0;   // pixels per map (forward input / inverse output)
+
+  // per-row tables of the forward grid (fwd_rows) and the inverse grid
+  // (inv_rows), for the precompute kernels: colatitudes, the forward
+  // quadrature weight of each forward row, the HEALPix first-pixel azimuth
+  double* fwd_theta = nullptr;
+  double* inv_theta = nullptr;
+  double* fwd_row_w = nullptr;
+  double* row_phi0 = nullptr;  // HEALPix only (forward and inverse grids agree)
   std::vector<void*> allocations;
 };
 

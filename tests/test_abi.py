@@ -76,14 +76,20 @@ def test_api_argument_errors_before_any_device_work():
         s2fft_amd.forward(np.zeros(48), 4, sampling="healpix", method="hip")
 
 
-def test_precompute_api_errors_and_kernel_handles():
-    from s2fft_amd.precompute_transforms import construct, spherical as pspin
-    with pytest.raises(ValueError):
+def test_precompute_api_errors_before_any_device_work():
+    """The reference's argument errors (precompute_transforms/spherical.py:62-63,
+    construct.py:65-69, 372-375) fire before anything touches a GPU."""
+    from s2fft_amd.precompute_transforms import construct, spherical as pspin, wigner as pwig
+    with pytest.raises(ValueError, match="not recognised"):
         pspin.forward(np.zeros((8, 15)), 8, method="numpy")
-    k = construct.spin_spherical_kernel(8, 2, reality=True, sampling="mw")
-    assert k.reality is False and k.spin == 2 and k.L == 8
-    with pytest.raises(ValueError):
-        pspin._check_kernel(k, 9, 2, "mw", None, False, True)
+    with pytest.raises(ValueError, match="not recognised"):
+        pwig.inverse(np.zeros((3, 8, 15)), 8, 2, method="jax")
+    with pytest.raises(ValueError, match="unstable"):
+        construct.spin_spherical_kernel(8, 7, recursion="price-mcewen")
+    with pytest.raises(ValueError, match="not recognised"):
+        construct.spin_spherical_kernel(8, 0, recursion="bogus")
+    with pytest.raises(ValueError, match="not valid"):
+        construct.wigner_kernel(8, 2, sampling="gl", mode="fft")
 
 
 def test_cg_entry_points_reject_bad_arguments_without_gpu():

@@ -1,26 +1,79 @@
-"""Precompute-path API (s2fft.precompute_transforms) with method="hip" (needs a GPU).
+"""Stored d-tables and the precompute-path API with method="hip" (needs a GPU).
 
-The reference's precompute transforms compute the same spin-spherical and
-Wigner transforms as its on-the-fly path, from a stored kernel; here they are
-checked against the oracle at the FP64 tolerance of the plain kernels
-(reference tests/test_spherical_precompute.py:57-58 use 1e-12).
+* compute_all_slices (s2fft/recursions/price_mcewen.py:377-643): the HIP
+  l-recursion table against the oracle's m-recursion table (1e-12) and at the
+  poles against Wigner's explicit sum.
+* construct.spin_spherical_kernel / wigner_kernel (construct.py:16-161,
+  332-468): the device-built kernels against the oracle's restatement, which
+  tests/test_oracle_pins.py pins to the explicit sum.
+* precompute_transforms.spherical / wigner: the transforms with a stored
+  kernel against the oracle's on-the-fly transforms (the reference compares
+  the two paths at 1e-12, tests/test_spherical_precompute.py:57-58), with a
+  reference-layout NumPy kernel, and with a scaled kernel (the kernel is
+  really applied).
 """
 
 import numpy as np
 import pytest
+import torch
 
+from oracle import pins, pm as opm, precompute as opc
 from oracle import signals, transforms as otr
+from oracle import geometry as ogeo
 from tests.conftest import max_rel
 
-from s2fft_amd.precompute_transforms import construct, spherical as pspin
+from s2fft_amd.precompute_transforms import construct, spherical as pspin, wigner as pwig
+from s2fft_amd.recursions import price_mcewen
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 
-@pytest.mark.parametrize("sampling,nside,L,spin,reality", [
-    ("mw", None, 12, 0, True), ("mwss", None, 10, 2, False), ("gl", None, 9, -1, False),
-    ("dh", None, 8, 0, False), ("healpix", 8, 16, 0, True)])
+class _nullctx:
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        return False
+
+
+@pytest.mark.parametrize("L,spin", [(8, 0), (8, 1), (9, -2), (64, 0), (64, 3)])
+def test_compute_all_slices_matches_oracle(L, spin):
+    beta = np.linspace(0.05, np.pi - 0.05, 37)
+    got = price_mcewen.compute_all_slices(beta, L, spin)
+    assert got.shape == (2 * L - 1, len(beta), L)
+    ref = opc.compute_all_slices(beta, L, spin)
+    err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+    # the reference's m-recursion loses digits next to the poles as |spin| grows
+    assert err < (1e-12 if abs(spin) < 3 else 1e-10), err
+
+
+@pytest.mark.parametrize("spin", [0, 2])
+def test_compute_all_slices_at_the_poles(spin):
+    L = 6
+    beta = np.array([0.0, np.pi])
+    got = price_mcewen.compute_all_slices(torch.from_numpy(beta).cuda(), L, spin).cpu().numpy()
+    for m in range(-(L - 1), L):
+        for l in range(max(abs(m), abs(spin)), L):
+            for t in range(2):
+                assert abs(got[L - 1 + m, t, l] - pins.wigner_d(l, m, -spin, beta[t])) < 1e-14
+
+
+KCASES = [("mw", None, 12, 0, True), ("mwss", None, 10, 2, False), ("gl", None, 9, -1, False),
+          ("dh", None, 8, 0, False), ("healpix", 8, 16, 0, True), ("healpix", 4, 8, 0, False),
+          ("mw", None, 9, 1, False)]
+
+
+@pytest.mark.parametrize("forward", [False, True])
+@pytest.mark.parametrize("sampling,nside,L,spin,reality", KCASES)
+def test_spin_kernel_matches_oracle(sampling, nside, L, spin, reality, forward):
+    got = construct.spin_spherical_kernel(L, spin, reality, sampling, nside, forward).cpu().numpy()
+    ref = opc.spin_spherical_kernel(L, spin, reality, sampling, nside, forward)
+    assert got.shape == ref.shape and got.dtype == ref.dtype
+    assert np.max(np.abs(got - ref)) < TOL * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("sampling,nside,L,spin,reality", KCASES)
 def test_precompute_spherical_matches_oracle(rng, sampling, nside, L, spin, reality):
     flm = signals.generate_flm(rng, L, 0, spin, reality)
     ker_i = construct.spin_spherical_kernel(L, spin, reality, sampling, nside, forward=False)
@@ -32,6 +85,21 @@ def test_precompute_spherical_matches_oracle(rng, sampling, nside, L, spin, real
     got = pspin.forward(fin, L, spin, ker_f, sampling, reality, "hip", nside)
     ref = otr.forward(fin, L, spin, nside, sampling, reality)
     assert max_rel(got, ref) < TOL
+    # the kernel is applied: a reference-layout NumPy kernel gives the same,
+    # a scaled one scales the result
+    np_ker = opc.spin_spherical_kernel(L, spin, reality, sampling, nside, True)
+    assert max_rel(pspin.forward(fin, L, spin, np_ker, sampling, reality, "hip", nside), ref) < TOL
+    twice = pspin.forward(fin, L, spin, 2 * ker_f, sampling, reality, "hip", nside)
+    assert max_rel(twice, 2 * ref) < TOL
+
+
+def test_precompute_forward_iterative_healpix(rng):
+    nside, L = 8, 16
+    flm = signals.generate_flm(rng, L, reality=True)
+    f = np.real(otr.inverse(flm, L, 0, nside, "healpix", True))
+    got = pspin.forward(f, L, 0, None, "healpix", True, "hip", nside, iter=3)
+    ref = otr.forward_iterative(f, L, 0, nside, "healpix", True, n_iter=3)
+    assert max_rel(got, ref) < TOL
 
 
 def test_precompute_reality_spin_falls_back_to_complex(rng):
@@ -40,3 +108,37 @@ def test_precompute_reality_spin_falls_back_to_complex(rng):
     with pytest.warns(UserWarning):
         f = pspin.inverse(flm, L, spin, None, "mw", True, "hip")
     assert max_rel(f, otr.inverse(flm, L, spin, sampling="mw")) < TOL
+
+
+def test_precompute_kernel_shape_is_checked(rng):
+    L = 6
+    flm = signals.generate_flm(rng, L)
+    bad = np.zeros((L, L, 2 * L - 1))          # MW inverse kernels have L rows: this has L
+    with pytest.raises(ValueError):
+        pspin.inverse(flm, L, 0, bad[:, :, :-1], "mw", False, "hip")
+
+
+WCASES = [("mw", None, 6, 2, False), ("mw", None, 6, 3, True), ("mwss", None, 6, 2, False),
+          ("dh", None, 5, 2, True), ("gl", None, 5, 2, False), ("healpix", 4, 8, 2, False)]
+
+
+@pytest.mark.parametrize("sampling,nside,L,N,reality", WCASES)
+def test_precompute_wigner_matches_oracle(rng, sampling, nside, L, N, reality):
+    exact = sampling == "healpix" and N > 1
+    ctx = opm.exact_d() if exact else _nullctx()
+    for fwd in (False, True):
+        got = construct.wigner_kernel(L, N, reality, sampling, nside, fwd).cpu().numpy()
+        ref = opc.wigner_kernel(L, N, reality, sampling, nside, fwd, exact=exact)
+        assert got.shape == ref.shape
+        assert np.max(np.abs(got - ref)) < TOL * np.max(np.abs(ref))
+    flmn = signals.generate_flmn(rng, L, N, reality=reality)
+    with ctx:
+        f_ref = otr.wigner_inverse(flmn, L, N, nside, sampling, reality)
+        fl_ref = otr.wigner_forward(f_ref, L, N, nside, sampling, reality)
+    f = pwig.inverse(flmn, L, N, None, sampling, reality, "hip", nside)
+    assert max_rel(f, f_ref) < TOL
+    got = pwig.forward(f_ref, L, N, None, sampling, reality, "hip", nside)
+    assert max_rel(got, fl_ref) < TOL
+    ker = construct.wigner_kernel(L, N, reality, sampling, nside, True)
+    assert max_rel(pwig.forward(f_ref, L, N, 3 * ker, sampling, reality, "hip", nside),
+                   3 * fl_ref) < TOL

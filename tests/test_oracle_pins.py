@@ -372,3 +372,56 @@ def test_inverse_rings_is_a_ring_subset_of_inverse(sampling, nside, L, rings, re
         want = full[rings].reshape(-1)
     got = T.inverse_rings(flm, L, rings, spin, nside, sampling, reality)
     assert max_rel(got, want) < 1e-14
+
+
+# ------------------------------------------------------------ precompute path
+from oracle import precompute as opc  # noqa: E402
+
+
+@pytest.mark.parametrize("spin", [0, 1, -2])
+def test_compute_all_slices_matches_explicit_wigner_d(spin):
+    L = 7
+    beta = np.array([0.2, 1.0, np.pi / 2, 2.5, 3.0])
+    dl = opc.compute_all_slices(beta, L, spin)
+    assert dl.shape == (2 * L - 1, len(beta), L)
+    for m in range(-(L - 1), L):
+        for l in range(L):
+            exact = [pins.wigner_d(l, m, -spin, b) if l >= max(abs(m), abs(spin)) else 0.0
+                     for b in beta]
+            assert np.max(np.abs(dl[L - 1 + m, :, l] - exact)) < 1e-12, (m, l)
+
+
+@pytest.mark.parametrize("sampling,nside,L,spin,reality", [
+    ("mw", None, 6, 0, True), ("mwss", None, 6, 1, False), ("gl", None, 5, 0, False),
+    ("dh", None, 5, -1, False), ("healpix", 2, 5, 0, True), ("healpix", 2, 5, 0, False)])
+@pytest.mark.parametrize("forward", [False, True])
+def test_spin_kernel_and_transforms_match_the_on_the_fly_oracle(sampling, nside, L, spin,
+                                                                reality, forward):
+    """The kernel from the m-recursion equals the explicit-sum one, and the
+    precompute transforms equal the on-the-fly transforms (the reference's
+    precompute tests compare the two paths, test_spherical_precompute.py:57-58)."""
+    k = opc.spin_spherical_kernel(L, spin, reality, sampling, nside, forward)
+    ke = opc.spin_spherical_kernel(L, spin, reality, sampling, nside, forward, exact=True)
+    assert np.max(np.abs(k - ke)) < 1e-12 * np.max(np.abs(ke))
+    flm = signals.generate_flm(np.random.default_rng(9), L, 0, spin, reality)
+    f = T.inverse(flm, L, spin, nside, sampling, reality)
+    if not forward:
+        got = opc.inverse_transform(flm, ke, L, sampling, reality, spin, nside)
+        assert max_rel(got, f) < 1e-12
+    else:
+        fin = np.real(f) if reality else f
+        got = opc.forward_transform(fin, ke, L, sampling, reality, spin, nside)
+        assert max_rel(got, T.forward(fin, L, spin, nside, sampling, reality)) < 1e-12
+
+
+@pytest.mark.parametrize("sampling,nside,L", [("mw", None, 5), ("healpix", 2, 5)])
+@pytest.mark.parametrize("forward", [False, True])
+def test_wigner_kernel_m_recursion_matches_explicit_sum(sampling, nside, L, forward):
+    k = opc.wigner_kernel(L, 2, False, sampling, nside, forward)
+    ke = opc.wigner_kernel(L, 2, False, sampling, nside, forward, exact=True)
+    assert k.shape == ke.shape
+    # HEALPix n != 0: the m-recursion's NaN-dropped rings (cos theta = 1/3 at nside 2)
+    if sampling == "healpix":
+        ok = ~np.isclose(np.abs(np.cos(geo.thetas(L, "healpix", nside))), 1 / 3)
+        k, ke = k[:, ok], ke[:, ok]
+    assert np.max(np.abs(k - ke)) < 1e-12 * np.max(np.abs(ke))
