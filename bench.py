@@ -10,25 +10,24 @@ signal-generator distribution).  These pipelines are timed on the same maps:
              FP64 kernels, residual on the pixels (two ring FFTs per
              iteration): the "plain FP64 GPU" baseline of the metric;
 * plain spectral -- the same FP64 iteration with the residual on the ring
-             spectra (no ring FFT inside the loop), to separate that
-             algorithmic gain from the precision gain;
-* mixed CG + learned -- the headline: conjugate gradients on F I x = F f
-             (F I is Hermitian positive definite for the HEALPix quadrature)
-             with the residual held in FP64 on the ring spectra, FP64
-             recursion, split-BF16 / BF16 matrix-core projections and
-             syntheses, one FP64 synthesis for the first residual, and the
-             learned MLP preconditioner on the first residual (its direction
-             kept per map only where the FP64 ring residual drops);
-* mixed CG -- the same without the network;
-* mixed Jacobi + learned -- Jacobi steps with the learned preconditioner on
-             the first two, BF16 matrix cores and an FP64 residual refresh.
+             spectra (no ring FFT inside the loop);
+* fp64 CG -- conjugate gradients on F I x = F f, every transform FP64: the
+             algorithmic gain of CG alone;
+* mixed CG + learned -- the headline (`value`): CG with the residual held in
+             FP64 on the ring spectra, FP64 recursion, split-BF16 / BF16
+             matrix-core projections and syntheses, one FP64 synthesis, and
+             the learned MLP preconditioner on the first residual (its
+             direction kept per map only where the FP64 ring residual drops);
+* mixed CG -- the same without the network (the precision gain over fp64 CG);
+* mixed Jacobi + learned -- Jacobi steps with the learned preconditioner.
 
-Iteration counts are fixed before timing by running each pipeline once.
-`value` is the mixed CG + learned pipeline's maps/s; the others and the ratios
-sit in `config` (`config.legs`).  The roofline is reported for the plain pipeline's dominant kernel
-(timed live with HIP events on its launch stream) and, under
-`config.mixed_roofline`, for the mixed pipeline's.  The CPU baseline is the
-NumPy oracle on a bounded sample of the same workload.
+Step counts are fixed before timing by running each pipeline once against
+the seeded coefficients, and confirmed on a second set of 16 maps (seeds
+316..331): `config.legs[*].confirm_max_rel_err`.  Kernel times come from a
+separate untimed pass (HIP events on the launch stream), never from the timed
+steps.  A single-map leg runs the nside-512, L=1024 forward with iter=3 on the
+FP64 kernels (`config.single_map`).  The CPU baseline is a torch-CPU twin of
+the oracle's latitudinal recursion on all host threads, on a bounded sample.
 """
 
 import argparse
@@ -44,7 +43,7 @@ FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 matrix
 BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch at this workload from rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE, scaled by a 1 GiB device copy's readings (x2.000 and x1.000):
+# WRITE_SIZE, scaled by a 1 GiB device copy's readings:
 # profiles/counters_healpix_nside1024_L2048_b16.md
 TRAFFIC_BYTES = {"lat_forward": 6.86e9, "lat_inverse": 5.72e9}
 
@@ -59,27 +58,28 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-7)
     ap.add_argument("--max-iter", type=int, default=40)
     ap.add_argument("--seed0", type=int, default=300)
+    ap.add_argument("--confirm-seed0", type=int, default=316,
+                    help="second seed set that confirms every leg's step count")
     ap.add_argument("--mixed-precision", default="bf16x1", choices=["bf16x1", "bf16", "mixed"])
     ap.add_argument("--mixed-inv-precision", default="bf16", choices=["bf16", "mixed"])
     ap.add_argument("--refresh-every", type=int, default=3)
-    ap.add_argument("--init-precision", default="bf16", choices=["fp64", "bf16", "mixed"],
-                    help="mixed leg: arithmetic of the synthesis forming the first residual "
-                         "(its error is removed by the FP64 refresh)")
-    ap.add_argument("--learned-steps", type=int, default=2,
-                    help="Jacobi leg: learned steps")
+    ap.add_argument("--init-precision", default="bf16", choices=["fp64", "bf16", "mixed"])
+    ap.add_argument("--learned-steps", type=int, default=2, help="Jacobi leg: learned steps")
     ap.add_argument("--cg-proj", default="bf16,bf16x1",
                     help="CG legs: projection arithmetic of the residuals r_0, r_1, ... "
                          "(the last repeats)")
     ap.add_argument("--cg-init", default="bf16", choices=["fp64", "bf16", "bf16x1", "mixed"])
     ap.add_argument("--cg-learned-steps", type=int, default=1)
-    ap.add_argument("--no-jacobi-leg", action="store_true",
-                    help="skip the mixed Jacobi (+ learned) leg")
+    ap.add_argument("--no-jacobi-leg", action="store_true")
+    ap.add_argument("--no-fp64-cg", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--pixel-residual", action="store_true",
-                    help="mixed leg: residuals on the pixels (two ring FFTs per step) instead of "
-                         "on the ring spectra")
+    ap.add_argument("--no-confirm", action="store_true")
+    ap.add_argument("--no-single-map", action="store_true")
+    ap.add_argument("--single-map-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="target duration of the CPU baseline sample")
     ap.add_argument("--kernels-once", action="store_true",
                     help="run each transform kind once at the bench workload, plus a 1 GiB "
                          "device copy, and exit (for rocprofv3 --pmc counter passes)")
@@ -90,51 +90,89 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(nside, L, n_transforms, rings=512, msteps=256):
-    """Oracle (NumPy, one core) on a bounded sample of the same workload.
+def host_cpu():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
-    Sample: the oracle's Price-McEwen latitudinal recursion for `rings` of
-    the 4*nside-1 rings and the first `msteps` of its L m-steps (every step
-    costs the same: it updates full (rings x L) arrays), plus one full
-    ring FFT of a map.  Scaled to one transform, times the transforms of a
-    plain refined forward.
+
+def cpu_baseline(nside, L, n_transforms, seconds):
+    """Torch-CPU twin of the oracle's latitudinal m-recursion (the reference's
+    JAX array program) on every host thread, on a bounded sample.
+
+    Sample: `rings` equatorial-band rings of the 4 nside - 1 and the first
+    `msteps` of the L m-steps (every step updates full (rings x L) arrays, so
+    the time scales with both), timed as the minimum of 3 repeats, plus one
+    full ring FFT of a map (torch.fft per ring); scaled to one transform, times
+    the transforms of the plain refined forward.
     """
     import numpy as np
+    import torch
 
-    from oracle import ffts, geometry, pm
+    from oracle import geometry, torch_twin
 
+    threads = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit():
+        threads = min(threads, int(env))
+    torch.set_num_threads(threads)
     ntheta = 4 * nside - 1
-    beta = geometry.thetas(L, "healpix", nside)[:rings]
+    rings = 512
+    beta = geometry.thetas(L, "healpix", nside)[nside:nside + rings]
     rng = np.random.default_rng(0)
     ftm = rng.standard_normal((rings, 2 * L)) + 1j * rng.standard_normal((rings, 2 * L))
     t0 = time.perf_counter()
-    gen = pm.dslices(beta, L, 0, True, 0, None, "healpix", nside, True)
-    acc = 0.0
-    for k, (m, D) in enumerate(gen):
-        acc += float(np.nansum(D[:, -1:] * ftm[:, L + m : L + m + 1]).real)
-        if k + 1 >= msteps:
-            break
-    t_lat = time.perf_counter() - t0
-    f = rng.standard_normal(12 * nside * nside)
+    pre = torch_twin.precompute(beta, L)
+    t_pre = time.perf_counter() - t0
     t0 = time.perf_counter()
-    ffts.healpix_fft(f, L, nside, True)
+    torch_twin.forward_latitudinal_sample(ftm, pre, L, 8)
+    probe = (time.perf_counter() - t0) / 8
+    msteps = int(max(8, min(L, seconds / 3 / max(probe, 1e-6))))
+    best = float("inf")
+    for _ in range(3):
+        t0 = time.perf_counter()
+        torch_twin.forward_latitudinal_sample(ftm, pre, L, msteps)
+        best = min(best, time.perf_counter() - t0)
+    # the reference recomputes its precomputes in every call (precomps=None)
+    t_lat = (best * (L / msteps) + t_pre) * (ntheta / rings)
+    # ring FFT of one map (real rfft per ring, torch)
+    f = torch.from_numpy(rng.standard_normal(12 * nside * nside))
+    lens = [4 * (t + 1) if t < nside - 1 else 4 * nside if t <= 3 * nside - 1
+            else 4 * (4 * nside - t - 1) for t in range(ntheta)]
+    t0 = time.perf_counter()
+    off = 0
+    for n in lens:
+        torch.fft.rfft(f[off:off + n])
+        off += n
     t_fft = time.perf_counter() - t0
-    per_transform = t_lat * (ntheta / rings) * (L / msteps) + t_fft
-    per_map = per_transform * n_transforms
+    per_map = (t_lat + t_fft) * n_transforms
     return {
         "value": 1.0 / per_map,
         "unit": "maps/s",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": (f"oracle PM recursion on {rings}/{ntheta} rings x {msteps}/{L} m-steps "
-                   f"({t_lat:.1f} s) + one ring FFT ({t_fft:.1f} s), scaled to "
-                   f"{n_transforms} transforms per map"),
+        "host_cpu": host_cpu(),
+        "sample": (f"torch-CPU twin of the latitudinal m-recursion on {rings}/{ntheta} rings x "
+                   f"{msteps}/{L} m-steps, min of 3 repeats ({best:.2f} s), its precomputes "
+                   f"for those rings ({t_pre:.2f} s), plus one ring FFT "
+                   f"({t_fft:.2f} s), scaled to {n_transforms} transforms per map "
+                   f"on {threads} threads"),
     }
+
+
+KERNEL_NAMES = ("lat_forward", "lat_inverse", "lat_forward_fp32", "lat_inverse_fp32",
+                "lat_forward_bf16", "lat_inverse_bf16", "ring_fft_forward", "ring_fft_inverse",
+                "ring_residual")
 
 
 def kernel_times(_lib):
     """(total ms, launches) per kernel kind; the latitudinal kinds by arithmetic."""
-    names = {"lat_forward": _lib.KERNEL_LAT_FORWARD, "lat_inverse": _lib.KERNEL_LAT_INVERSE,
+    codes = {"lat_forward": _lib.KERNEL_LAT_FORWARD, "lat_inverse": _lib.KERNEL_LAT_INVERSE,
              "lat_forward_fp32": _lib.KERNEL_LAT_FORWARD_FP32,
              "lat_inverse_fp32": _lib.KERNEL_LAT_INVERSE_FP32,
              "lat_forward_bf16": _lib.KERNEL_LAT_FORWARD_BF16,
@@ -142,14 +180,12 @@ def kernel_times(_lib):
              "ring_fft_forward": _lib.KERNEL_RING_FFT_FORWARD,
              "ring_fft_inverse": _lib.KERNEL_RING_FFT_INVERSE,
              "ring_residual": _lib.KERNEL_RING_RESIDUAL}
-    out = {k: _lib.profile_query(v) for k, v in names.items()}
+    out = {k: _lib.profile_query(v) for k, v in codes.items()}
     return {k: v for k, v in out.items() if v[1] > 0}
 
 
 def mfma_f64_ceiling(_lib, torch, blocks=4096, iters=250):
-    """Sustained V_MFMA_F64_16X16X4F64 TFLOP/s of this device (s2h_probe_mfma_f64_rate:
-    8 independent accumulators per wave, no memory traffic), timed with HIP
-    events on the launch stream."""
+    """Sustained V_MFMA_F64_16X16X4F64 TFLOP/s of this device (s2h_probe_mfma_f64_rate)."""
     out = torch.empty(blocks * 256, dtype=torch.float64, device="cuda")
     s = torch.cuda.current_stream()
     _lib.check(_lib.lib().s2h_probe_mfma_f64_rate(blocks, 10, out.data_ptr(), s.cuda_stream))
@@ -162,13 +198,17 @@ def mfma_f64_ceiling(_lib, torch, blocks=4096, iters=250):
     return blocks * 4 * iters * 8 * 2048.0 / (ms * 1e-3) / 1e12
 
 
+def lat_flop(nside, L, B):
+    """Algorithmic flops of one latitudinal launch: ceil(ntheta/2) ring pairs x
+    P = L(L+1)/2 coefficients x (5 + 4 B)."""
+    return math.ceil((4 * nside - 1) / 2) * (L * (L + 1) // 2) * (5 + 4 * B)
+
+
 def roofline(kernels, nside, L, B):
     """Roofline entry of the dominant kernel kind (by total time).
 
-    Latitudinal kernels: algorithmic flops per launch = ceil(ntheta/2) ring
-    pairs x P = L(L+1)/2 coefficients x (5 + 4 B) (5 for the recursion step,
-    4 per map for one complex x real multiply-add), priced against the peak
-    of the launch's arithmetic (FP64, FP32 or dense BF16 matrix).
+    Latitudinal kernels: algorithmic flops per launch (lat_flop), priced
+    against the peak of the launch's arithmetic (FP64, FP32 or dense BF16).
     Ring FFTs: algorithmic bytes per launch = B maps x (8 B per pixel + 16 B
     per ftm element, ntheta x L).
     """
@@ -176,7 +216,7 @@ def roofline(kernels, nside, L, B):
     avg_ms = ms / max(n, 1)
     ntheta = 4 * nside - 1
     if dom.startswith("lat"):
-        flop = math.ceil(ntheta / 2) * (L * (L + 1) // 2) * (5 + 4 * B)
+        flop = lat_flop(nside, L, B)
         achieved = flop / (avg_ms * 1e-3) / 1e12
         arith = "bf16" if dom.endswith("bf16") else "fp32" if dom.endswith("fp32") else "fp64"
         peak = {"bf16": BF16_PEAK_TFLOPS, "fp32": FP32_PEAK_TFLOPS}.get(arith, FP64_PEAK_TFLOPS)
@@ -191,13 +231,7 @@ def roofline(kernels, nside, L, B):
 
 
 def kernels_once(nside, B):
-    """One launch of each transform kind at the bench workload, for counters.
-
-    FP64, FP32 ("mixed"), split-BF16 and single-BF16 forward and inverse, the
-    learned preconditioner, the ring-spectra pieces, one learned CG refinement
-    (3 steps) and a plain 1 GiB device copy, whose FETCH_SIZE / WRITE_SIZE
-    readings calibrate the HBM byte counts.
-    """
+    """One launch of each transform kind at the bench workload, for counters."""
     import torch
 
     from s2fft_amd.learned.precond import MLPPreconditioner
@@ -230,6 +264,48 @@ def kernels_once(nside, B):
     print("bench.py --kernels-once: done", flush=True)
 
 
+def single_map_leg(args, torch, _lib, sph, generate_flm):
+    """nside 512, L 1024, one real map (seed 1): forward with iter=3 on the
+    FP64 kernels (one map: the VALU latitudinal kernels), maps/s and the
+    kernels against their rooflines."""
+    import numpy as np
+
+    nside, L = 512, 1024
+    flm = torch.from_numpy(generate_flm(np.random.default_rng(1), L, reality=True)).cuda()
+    f = sph.inverse_hip(flm[None], L, 0, nside, "healpix", True).real.contiguous()
+    run = lambda: sph.forward_iterative_hip(f, L, 0, nside, "healpix", True, 0, 3)
+    out = None
+    for _ in range(max(args.warmup, 1)):
+        out = run()
+    torch.cuda.synchronize()
+    err = float((out[0] - flm).abs().max() / flm.abs().max())
+    t0 = time.perf_counter()
+    for _ in range(args.single_map_steps):
+        run()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    _lib.profile_enable(True)
+    run()
+    torch.cuda.synchronize()
+    kt = kernel_times(_lib)
+    _lib.profile_enable(False)
+    res = {"workload": "HEALPix nside=512 L=1024 real spin-0 forward, iter=3, one map, FP64",
+           "maps_per_s": args.single_map_steps / el, "ms_per_map": 1e3 * el / args.single_map_steps,
+           "max_rel_err_vs_flm": err,
+           "kernel_ms": {k: {"total_ms": v[0], "launches": v[1]} for k, v in kt.items()}}
+    flop = lat_flop(nside, L, 1)
+    for k in ("lat_forward", "lat_inverse"):
+        if k in kt:
+            ms = kt[k][0] / kt[k][1]
+            res[k + "_fp64_tflops"] = flop / (ms * 1e-3) / 1e12
+            res[k + "_frac_of_fp64_peak"] = res[k + "_fp64_tflops"] / FP64_PEAK_TFLOPS
+    if "ring_fft_forward" in kt:
+        ms = kt["ring_fft_forward"][0] / kt["ring_fft_forward"][1]
+        nbytes = 8 * 12 * nside * nside + 16 * (4 * nside - 1) * L
+        res["ring_fft_forward_gbs"] = nbytes / (ms * 1e-3) / 1e9
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -253,66 +329,89 @@ def main():
         kernels_once(nside, B)
         return
 
-    log(f"[bench] generating {B} maps nside={nside} L={L}")
-    flm_np = np.stack([generate_flm(np.random.default_rng(args.seed0 + i), L, reality=True)
-                       for i in range(B)])
-    flm_true = torch.from_numpy(flm_np).to(dev)
-    del flm_np
     plan = sph.get_plan("healpix", L, nside, 0, 0, True)
-    f = sph.inverse_hip(flm_true, L, 0, nside, "healpix", True, plan=plan).real.contiguous()
-    torch.cuda.synchronize()
 
-    def max_rel(x):
-        return float((x - flm_true).abs().max() / flm_true.abs().max())
+    def make_maps(seed0):
+        flm_np = np.stack([generate_flm(np.random.default_rng(seed0 + i), L, reality=True)
+                           for i in range(B)])
+        flm_t = torch.from_numpy(flm_np).to(dev)
+        del flm_np
+        fm = sph.inverse_hip(flm_t, L, 0, nside, "healpix", True, plan=plan).real.contiguous()
+        torch.cuda.synchronize()
+        return flm_t, fm
+
+    log(f"[bench] generating {B} maps nside={nside} L={L}")
+    flm_true, f = make_maps(args.seed0)
+    confirm = None
+    if not args.no_confirm:
+        log(f"[bench] generating the confirmation set (seeds {args.confirm_seed0}..)")
+        confirm = make_maps(args.confirm_seed0)
+
+    def max_rel(x, ref):
+        return float((x - ref).abs().max() / ref.abs().max())
 
     def timed(step, label):
+        """W warmups, K timed steps (no profiling), then one untimed profiled
+        step for the kernel times (per step)."""
         out = None
         for i in range(args.warmup):
             out = step()
             torch.cuda.synchronize()
             log(f"[bench] {label} warmup {i + 1}/{args.warmup} done")
-        err = max_rel(out) if out is not None else float("nan")
+        err = max_rel(out, flm_true) if out is not None else float("nan")
         del out
-        _lib.profile_enable(True)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):
             out = step()
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
+        if not args.warmup:
+            err = max_rel(out, flm_true)
+        del out
+        _lib.profile_enable(True)
+        step()
+        torch.cuda.synchronize()
         kt = kernel_times(_lib)
         _lib.profile_enable(False)
-        if not args.warmup:
-            err = max_rel(out)
-        del out
         log(f"[bench] {label}: {args.steps} steps in {elapsed:.3f} s, err {err:.2e}")
         return elapsed, err, kt
 
+    def confirm_err(run):
+        if confirm is None:
+            return None
+        flm2, f2 = confirm
+        return max_rel(run(f2), flm2)
+
+    legs = {}
     # ---------------------------------------------------------------- plain
     plain = None
     if not args.no_plain:
         log("[bench] plain FP64: calibrating the Jacobi count")
         flm = sph.forward_hip(f, L, 0, nside, "healpix", True, plan=plan)
-        errs = [max_rel(flm)]
+        errs = [max_rel(flm, flm_true)]
         n_plain = 0
         while errs[-1] > args.tol and n_plain < args.max_iter:
             rec = sph.inverse_hip(flm, L, 0, nside, "healpix", True, plan=plan)
             flm = flm + sph.forward_hip(f - rec, L, 0, nside, "healpix", True, plan=plan)
             n_plain += 1
-            errs.append(max_rel(flm))
+            errs.append(max_rel(flm, flm_true))
             log(f"[bench]   iter {n_plain}: max-rel err {errs[-1]:.3e}")
-        del flm
+        del flm, rec
         torch.cuda.empty_cache()
-        el, err, kt = timed(lambda: refine.refine_fp64(f, L, 0, nside, "healpix", True, n_plain),
-                            "plain")
+        pix = lambda fm: refine.refine_fp64(fm, L, 0, nside, "healpix", True, n_plain)
+        el, err, kt = timed(lambda: pix(f), "plain")
         plain = {"elapsed": el, "err": err, "kernels": kt, "iters": n_plain, "calib": errs,
-                 "roofline": roofline(kt, nside, L, B)}
-        # the same Jacobi iteration with its residuals on the ring spectra
-        el, err, kt = timed(lambda: refine.refine_fp64(f, L, 0, nside, "healpix", True, n_plain,
-                                                       spectral=True), "plain spectral")
-        plain["spectral"] = {"elapsed": el, "err": err, "kernels": kt}
+                 "roofline": roofline(kt, nside, L, B), "confirm": confirm_err(pix)}
+        spec = lambda fm: refine.refine_fp64(fm, L, 0, nside, "healpix", True, n_plain,
+                                             spectral=True)
+        el, err, kt = timed(lambda: spec(f), "plain spectral")
+        legs["plain_fp64_spectral"] = {"elapsed": el, "err": err, "kernels": kt,
+                                       "iters": n_plain, "calib": errs, "stats": {},
+                                       "roofline": roofline(kt, nside, L, B),
+                                       "confirm": confirm_err(spec)}
 
-    # ---------------------------------------------------------------- mixed
+    # ---------------------------------------------------------------- CG legs
     precond, learned = None, False
     if not args.no_learned:
         from s2fft_amd.learned import precond as lp
@@ -321,50 +420,51 @@ def main():
             learned = True
         else:
             log("[bench] no trained preconditioner weights: the mixed legs run without it")
-    spectral = not args.pixel_residual
     cg_proj = tuple(args.cg_proj.split(","))
 
     def calibrate(label, run):
         hist, stats = [], {}
-        run(args.max_iter, hist, stats)
+        run(f, args.max_iter, hist, stats)
         ok = [i for i, e in enumerate(hist) if e <= args.tol]
         n = ok[0] if ok else args.max_iter
         log(f"[bench]   {label} errors {['%.1e' % e for e in hist[:n + 2]]} -> {n} steps")
         torch.cuda.empty_cache()
         return n, hist, stats
 
-    def cg_run(pc, lsteps):
-        def run(n, hist=None, stats=None):
-            return refine.refine_cg(f, L, 0, nside, "healpix", True, n, plan=plan, precond=pc,
-                                    learned_steps=lsteps, init_precision=args.cg_init,
-                                    proj_precision=cg_proj, synth_precision="bf16",
-                                    history=hist, flm_true=flm_true if hist is not None else None,
-                                    stats=stats)
+    def cg_run(pc, lsteps, fp64=False):
+        def run(fm, n, hist=None, stats=None):
+            kw = dict(init_precision="fp64", proj_precision=("fp64",), synth_precision="fp64",
+                      last_synth_precision="fp64") if fp64 else \
+                dict(init_precision=args.cg_init, proj_precision=cg_proj, synth_precision="bf16")
+            return refine.refine_cg(fm, L, 0, nside, "healpix", True, n, plan=plan, precond=pc,
+                                    learned_steps=lsteps, history=hist,
+                                    flm_true=flm_true if hist is not None else None,
+                                    stats=stats, **kw)
         return run
 
-    def jacobi_run(n, hist=None, stats=None):
+    def jacobi_run(fm, n, hist=None, stats=None):
         return refine.refine_mixed(
-            f, L, 0, nside, "healpix", True, n, plan=plan, precond=precond,
+            fm, L, 0, nside, "healpix", True, n, plan=plan, precond=precond,
             refresh_every=args.refresh_every, history=hist,
             flm_true=flm_true if hist is not None else None, stats=stats,
             precision=args.mixed_precision, inv_precision=args.mixed_inv_precision,
-            learned_steps=args.learned_steps, spectral=spectral,
-            init_precision=args.init_precision)
+            learned_steps=args.learned_steps, spectral=True, init_precision=args.init_precision)
 
-    legs = {}
     specs = [("mixed_cg_learned", cg_run(precond, args.cg_learned_steps))] if learned else []
     specs.append(("mixed_cg", cg_run(None, 0)))
+    if not args.no_fp64_cg:
+        specs.append(("fp64_cg", cg_run(None, 0, fp64=True)))
     if not args.no_jacobi_leg:
         specs.append(("mixed_jacobi_learned" if learned else "mixed_jacobi", jacobi_run))
     for label, run in specs:
         log(f"[bench] {label}: calibrating the step count")
         n, hist, stats = calibrate(label, run)
-        el, err, kt = timed(lambda: run(n), label)
+        el, err, kt = timed(lambda: run(f, n), label)
         legs[label] = {"elapsed": el, "err": err, "kernels": kt, "iters": n, "calib": hist[:n + 1],
-                       "stats": stats, "roofline": roofline(kt, nside, L, B)}
+                       "stats": stats, "roofline": roofline(kt, nside, L, B),
+                       "confirm": confirm_err(lambda fm: run(fm, n))}
     head_label = "mixed_cg_learned" if learned else "mixed_cg"
     mixed = legs[head_label]
-    n_mixed = mixed["iters"]
 
     ceiling = mfma_f64_ceiling(_lib, torch)
     log(f"[bench] FP64 matrix-core rate probe: {ceiling:.1f} TFLOP/s")
@@ -376,6 +476,21 @@ def main():
     value = B * args.steps / mixed["elapsed"]
     plain_value = B * args.steps / plain["elapsed"] if plain else None
     head = plain["roofline"] if plain else mixed["roofline"]
+
+    def leg_json(v):
+        d = {"maps_per_s": B * args.steps / v["elapsed"],
+             "ms_per_step": 1e3 * v["elapsed"] / args.steps, "steps": v["iters"],
+             "max_rel_err": v["err"], "confirm_max_rel_err": v.get("confirm"),
+             "calibration_errs": v["calib"],
+             "learned_accepted": v["stats"].get("accepted"),
+             "learned_judged": v["stats"].get("steps"),
+             "kernel_ms_per_step": {kk: {"total_ms": vv[0], "launches": vv[1]}
+                                    for kk, vv in v["kernels"].items()},
+             "roofline": v["roofline"]}
+        if plain_value:
+            d["speedup_vs_plain_fp64"] = d["maps_per_s"] / plain_value
+        return d
+
     result = {
         "metric": METRIC,
         "value": value,
@@ -401,46 +516,35 @@ def main():
                           f"residual(s), kept per map only where the FP64 ring residual drops"
                           if learned else "")),
             "nside": nside, "L": L, "batch": B, "learned": learned,
-            "mixed_steps": n_mixed, "mixed_max_rel_err": mixed["err"],
-            "mixed_calibration_errs": mixed["calib"],
-            "learned_steps_accepted": mixed["stats"].get("accepted"),
-            "learned_steps_judged": mixed["stats"].get("steps"),
-            "mixed_kernel_ms": {k: {"total_ms": v[0], "launches": v[1]}
-                                for k, v in mixed["kernels"].items()},
+            "mixed_steps": mixed["iters"], "mixed_max_rel_err": mixed["err"],
+            "mixed_confirm_max_rel_err": mixed.get("confirm"),
+            "confirm_seeds": (f"{args.confirm_seed0}..{args.confirm_seed0 + B - 1}"
+                              if confirm is not None else None),
             "mixed_roofline": mixed["roofline"],
-            "legs": {k: {"maps_per_s": B * args.steps / v["elapsed"],
-                         "ms_per_step": 1e3 * v["elapsed"] / args.steps, "steps": v["iters"],
-                         "max_rel_err": v["err"], "calibration_errs": v["calib"],
-                         "learned_accepted": v["stats"].get("accepted"),
-                         "learned_judged": v["stats"].get("steps"),
-                         "kernel_ms": {kk: {"total_ms": vv[0], "launches": vv[1]}
-                                       for kk, vv in v["kernels"].items()}}
-                     for k, v in legs.items()},
+            "legs": {k: leg_json(v) for k, v in legs.items()},
         },
         "roofline": head,
     }
     if plain:
-        spec_value = B * args.steps / plain["spectral"]["elapsed"]
         result["config"].update({
             "plain_fp64_maps_per_s": plain_value,
             "speedup_vs_plain_fp64": value / plain_value,
-            "plain_fp64_spectral_maps_per_s": spec_value,
-            "speedup_vs_plain_fp64_spectral": value / spec_value,
-            "plain_fp64_spectral_max_rel_err": plain["spectral"]["err"],
-            "plain_fp64_spectral_kernel_ms": {k: {"total_ms": v[0], "launches": v[1]}
-                                              for k, v in plain["spectral"]["kernels"].items()},
             "plain_ms_per_step": 1e3 * plain["elapsed"] / args.steps,
             "plain_refine_iters": plain["iters"], "plain_max_rel_err": plain["err"],
+            "plain_confirm_max_rel_err": plain["confirm"],
             "plain_calibration_errs": plain["calib"],
-            "plain_kernel_ms": {k: {"total_ms": v[0], "launches": v[1]}
-                                for k, v in plain["kernels"].items()},
+            "plain_kernel_ms_per_step": {k: {"total_ms": v[0], "launches": v[1]}
+                                         for k, v in plain["kernels"].items()},
         })
-        for k, v in result["config"]["legs"].items():
-            v["speedup_vs_plain_fp64"] = v["maps_per_s"] / plain_value
+    del f, flm_true, confirm
+    torch.cuda.empty_cache()
+    if not args.no_single_map:
+        log("[bench] single-map leg: nside 512, iter=3")
+        result["config"]["single_map"] = single_map_leg(args, torch, _lib, sph, generate_flm)
     if not args.no_cpu_baseline:
-        log("[bench] cpu baseline (oracle sample)")
-        n_tr = 1 + 2 * (plain["iters"] if plain else n_mixed)
-        result["cpu_baseline"] = cpu_baseline(nside, L, n_tr)
+        log("[bench] cpu baseline (torch-CPU twin of the oracle recursion)")
+        n_tr = 1 + 2 * (plain["iters"] if plain else mixed["iters"])
+        result["cpu_baseline"] = cpu_baseline(nside, L, n_tr, args.cpu_seconds)
     print(json.dumps(result), flush=True)
 
 

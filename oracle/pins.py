@@ -126,29 +126,48 @@ def lat_inverse_longdouble(flm, beta, L):
     """ftm[t, m] = sum_l flm[l, L-1+m] d^l_{m0}(beta_t), m = 0..L-1, in long double.
 
     The three-term recurrence in l of d^l_{m0} (Bonnet-type, normalised
-    Wigner d) from d^m_{m0} = (-1)^m sqrt((2m)!/(m!^2)) (cos(b/2) sin(b/2))^m;
-    the 80-bit exponent range holds the start values unscaled for L <= ~600.
+    Wigner d) from d^m_{m0} = (-1)^m sqrt((2m)!/(m!^2)) (cos(b/2) sin(b/2))^m,
+    every m at once (l outer): the start values are formed in log space, so
+    the 80-bit exponent range (1e+-4932) holds them up to L of several
+    thousand; values below it (the highest m at the most polar rings) are
+    zero, and stay below it for every l < L.
     Returns complex128 of shape (len(beta), L).
     """
     ld = np.longdouble
     th = np.asarray(beta, dtype=np.float64).astype(ld)
-    x, c2, s2 = np.cos(th), np.cos(th / 2), np.sin(th / 2)
-    out = np.zeros((len(th), L), dtype=np.clongdouble)
-    for m in range(L):
-        K = ld(1)
-        for i in range(1, m + 1):
-            K *= ld(m + i) / ld(i)
-        d = (-1) ** m * np.sqrt(K) * (c2 * s2) ** m
-        dm1 = np.zeros_like(d)
-        acc = np.zeros(len(th), dtype=np.clongdouble)
-        for l in range(m, L):
-            acc += d * flm[l, L - 1 + m]
-            if l == 0:
-                dn = x * d
-            else:
-                D = ld(l) * np.sqrt(ld((l + 1) ** 2 - m * m) * ld((l + 1) ** 2))
-                dn = (ld(2 * l + 1) * l * (l + 1) / D) * x * d \
-                    - (ld(l + 1) * np.sqrt(ld(l * l - m * m) * ld(l * l)) / D) * dm1
-            dm1, d = d, dn
-        out[:, m] = acc
-    return out.astype(np.complex128)
+    x = np.cos(th)
+    cs = np.cos(th / 2) * np.sin(th / 2)
+    nt = len(th)
+    mv = np.arange(L).astype(ld)
+    # log sqrt((2m)!/(m!^2)) by K_m = K_{m-1} (2m)(2m-1)/m^2
+    inc = np.zeros(L, dtype=ld)
+    inc[1:] = np.log((2 * mv[1:]) * (2 * mv[1:] - 1) / (mv[1:] * mv[1:]))
+    half_logk = np.cumsum(inc) / 2
+    sign = np.where(np.arange(L) % 2 == 1, ld(-1), ld(1))
+    with np.errstate(under="ignore", divide="ignore"):
+        d0 = sign[None, :] * np.exp(half_logk[None, :] + mv[None, :] * np.log(cs)[:, None])
+    fr = np.asarray(flm[:, L - 1:].real, dtype=ld)
+    fi = np.asarray(flm[:, L - 1:].imag, dtype=ld)
+    dcur = np.zeros((nt, L), dtype=ld)
+    dprev = np.zeros((nt, L), dtype=ld)
+    accr = np.zeros((nt, L), dtype=ld)
+    acci = np.zeros((nt, L), dtype=ld)
+    for l in range(L):
+        dcur[:, l] = d0[:, l]
+        dprev[:, l] = 0
+        k = l + 1
+        accr[:, :k] += dcur[:, :k] * fr[l, :k]
+        acci[:, :k] += dcur[:, :k] * fi[l, :k]
+        if l == L - 1:
+            break
+        if l == 0:
+            dn = x[:, None] * dcur[:, :1]
+        else:
+            m = mv[:k]
+            D = ld(l) * np.sqrt((ld(l + 1) ** 2 - m * m) * ld(l + 1) ** 2)
+            A = ld(2 * l + 1) * l * (l + 1) / D
+            B = ld(l + 1) * np.sqrt((ld(l * l) - m * m) * ld(l * l)) / D
+            dn = A * x[:, None] * dcur[:, :k] - B * dprev[:, :k]
+        dprev[:, :k] = dcur[:, :k]
+        dcur[:, :k] = dn
+    return (accr.astype(np.float64) + 1j * acci.astype(np.float64)).astype(np.complex128)

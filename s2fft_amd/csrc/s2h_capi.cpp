@@ -210,9 +210,13 @@ int lat_forward_stage(const s2h_plan* p, const cplx* ftm, void* flm, int batch, 
   a.accumulate = accumulate ? 1 : 0;
   if (ps) a.planes = ps->fwd;
   const bool poles = p->fwd_row_south >= 0 || p->fwd_row_north >= 0;
-  const bool fill = !ps && bf16 && !poles && lat_forward_bf16_groups(a, split) == 1;
+  // the FP64 matrix-core projection (one workgroup per m) and the BF16 ones
+  // with one ring group per m write every entry of their m columns
+  const bool mfma64 = !ps && !bf16 && !mixed && batch >= kMfmaMinBatch;
+  const bool fill =
+      mfma64 || (!ps && bf16 && !poles && lat_forward_bf16_groups(a, split) == 1);
   a.fill = fill ? 1 : 0;
-  a.mirror = fill && p->d.reality && !packed ? 1 : 0;
+  a.mirror = fill && !poles && p->d.reality && !packed ? 1 : 0;
   if (!accumulate && !fill && (st = hip_status(launch_zero(flm, 2 * fs * batch, s)))) return st;
   {
     ProfScope prof(bf16 ? S2H_KERNEL_LAT_FORWARD_BF16
@@ -1129,17 +1133,30 @@ extern "C" int s2h_wigner_inverse(const s2h_wigner_plan* w, const void* flmn, vo
   const hipStream_t s = static_cast<hipStream_t>(stream);
   const int N = w->d.N, G = w->G;
   const WigWork ww = wig_work(w, work);
+  const int64_t fs = wig_flm_stride(w);
   const cplx* in = static_cast<const cplx*>(flmn);
-  // planes n >= 0 only for a real signal (wigner.py:179-195)
-  const int k0 = w->d.reality ? N - 1 : 0;
-  const int count = G - k0;
   PlaneSet ps;
-  const s2h_plan* p = plane_plan(w, k0, count, &ps);
-  if ((st = inverse_impl(p, count > 1 ? &ps : nullptr, in + k0 * wig_flm_stride(w), ww.fban,
-                         ww.spin, count, S2H_PREC_FP64, false, nullptr, s)))
+  if (!w->d.reality) {
+    const s2h_plan* p = plane_plan(w, 0, G, &ps);
+    if ((st = inverse_impl(p, G > 1 ? &ps : nullptr, in, ww.fban, ww.spin, G, S2H_PREC_FP64,
+                           false, nullptr, s)))
+      return st;
+    return hip_status(launch_gamma_dft(N, w->npix, 1, 1.0, ww.fban, f, s));
+  }
+  // a real signal uses the planes n >= 0 (wigner.py:179-195): n = 0 as a real
+  // transform (its m >= 0 coefficients, Hermitian completion), the n >= 1
+  // planes as one batch
+  if ((st = inverse_impl(w->real0, nullptr, in + (N - 1) * fs, ww.real_plane, ww.spin, 1,
+                         S2H_PREC_FP64, true, nullptr, s)))
     return st;
-  return hip_status(
-      launch_gamma_dft(N, w->npix, w->d.reality ? 3 : 1, 1.0, ww.fban, f, s));
+  if ((st = hip_status(launch_real_to_complex(ww.real_plane, ww.fban, w->npix, s)))) return st;
+  if (N > 1) {
+    const s2h_plan* p = plane_plan(w, N, N - 1, &ps);
+    if ((st = inverse_impl(p, N - 1 > 1 ? &ps : nullptr, in + N * fs, ww.fban + w->npix, ww.spin,
+                           N - 1, S2H_PREC_FP64, false, nullptr, s)))
+      return st;
+  }
+  return hip_status(launch_gamma_dft(N, w->npix, 3, 1.0, ww.fban, f, s));
 }
 
 // grad_f = F^H g: the planes' spin adjoints, then the adjoint of the gamma DFT.
@@ -1180,7 +1197,7 @@ extern "C" int s2h_wigner_forward_adjoint(const s2h_wigner_plan* w, const void* 
 
 // grad_flmn = I^H g: the adjoint of the gamma DFT, then the planes' spin
 // adjoints (zero for the n < 0 planes of a real signal, which the inverse
-// does not read).
+// does not read; its n = 0 plane through the real plan's adjoint).
 extern "C" int s2h_wigner_inverse_adjoint(const s2h_wigner_plan* w, const void* g,
                                           int g_is_complex, void* grad_flmn, void* work,
                                           s2h_stream stream) {
@@ -1194,15 +1211,21 @@ extern "C" int s2h_wigner_inverse_adjoint(const s2h_wigner_plan* w, const void* 
   cplx* out = static_cast<cplx*>(grad_flmn);
   const int mode = w->d.reality ? 5 : (g_is_complex ? 0 : 2);
   if ((st = hip_status(launch_gamma_dft(N, w->npix, mode, 1.0, g, ww.fban, s)))) return st;
-  const int k0 = w->d.reality ? N - 1 : 0;
-  const int count = G - k0;
-  if (k0 > 0 && (st = hip_status(launch_zero(out, 2 * (int64_t)k0 * fs, s)))) return st;
   PlaneSet ps;
-  const s2h_plan* p = plane_plan(w, k0, count, &ps);
-  return inverse_adjoint_impl(p, count > 1 ? &ps : nullptr, ww.fban, 1, out + k0 * fs, ww.spin,
-                              count, s);
+  if (!w->d.reality) {
+    const s2h_plan* p = plane_plan(w, 0, G, &ps);
+    return inverse_adjoint_impl(p, G > 1 ? &ps : nullptr, ww.fban, 1, out, ww.spin, G, s);
+  }
+  if (N > 1 && (st = hip_status(launch_zero(out, 2 * (int64_t)(N - 1) * fs, s)))) return st;
+  // plane n = 0 (row 0 of fban, real-valued): the real plan reads its real part
+  if ((st = inverse_adjoint_impl(w->real0, nullptr, ww.fban, 1, out + (N - 1) * fs, ww.spin, 1,
+                                 s)))
+    return st;
+  if (N == 1) return S2H_OK;
+  const s2h_plan* p = plane_plan(w, N, N - 1, &ps);
+  return inverse_adjoint_impl(p, N - 1 > 1 ? &ps : nullptr, ww.fban + w->npix, 1, out + N * fs,
+                              ww.spin, N - 1, s);
 }
-
 // ---------------------------------------------------------------- d-tables
 // compute_all_slices (s2fft/recursions/price_mcewen.py:377-643) and the
 // precompute path (s2fft/precompute_transforms/construct.py:16-161,

@@ -99,9 +99,11 @@ def test_healpix_fft_ex_ortho_and_none(rng, norm):
     assert max_rel(ftm.cpu().numpy(), ref) < 1e-12
     back = torch.empty(npix, dtype=torch.complex128, device="cuda")
     _lib.check(lib.s2h_healpix_ifft_ex(nside, L, 0, norm, 1, ftm.data_ptr(), back.data_ptr(), 1, s))
-    # ORTHO round trip is the identity (both directions 1/sqrt(n)); NONE scales by n
+    # the inverse folds the 2L columns back onto each ring (healpix_ffts.py:19-82);
+    # ORTHO scales each ring by 1/sqrt(n) again, NONE not at all
     _, n = offts.ring_offsets(nside)
-    want = f if norm == _lib.FFT_NORM_ORTHO else f * np.repeat(n, n)
+    want = offts.healpix_ifft(ftm.cpu().numpy(), L, nside) * np.repeat(
+        _ring_scale(nside, L, mode), n)
     assert max_rel(back.cpu().numpy(), want) < 1e-12
 
 

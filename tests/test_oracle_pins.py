@@ -425,3 +425,22 @@ def test_wigner_kernel_m_recursion_matches_explicit_sum(sampling, nside, L, forw
         ok = ~np.isclose(np.abs(np.cos(geo.thetas(L, "healpix", nside))), 1 / 3)
         k, ke = k[:, ok], ke[:, ok]
     assert np.max(np.abs(k - ke)) < 1e-12 * np.max(np.abs(ke))
+
+
+def test_torch_twin_matches_the_oracle_recursion():
+    """The CPU baseline's torch twin computes what the oracle's latitudinal
+    step computes (the first m-steps of the positive half)."""
+    torch = pytest.importorskip("torch")
+    from oracle import torch_twin
+
+    nside, L = 16, 32
+    beta = geo.thetas(L, "healpix", nside)
+    rng = np.random.default_rng(3)
+    ftm = rng.standard_normal((len(beta), 2 * L)) + 1j * rng.standard_normal((len(beta), 2 * L))
+    pre = torch_twin.precompute(beta, L)
+    ref_pre = pm.precomputes(L, 0, "healpix", nside, True, 0, beta)
+    assert np.max(np.abs(pre["lren"].numpy() - ref_pre[0][1])) < 1e-12 * np.max(np.abs(ref_pre[0][1]))
+    got = torch_twin.forward_latitudinal_sample(ftm, pre, L, L).numpy()
+    flm = pm.forward_latitudinal_step(ftm, beta, L, 0, nside, "healpix", True)
+    ref = np.stack([flm[:, L - 1 + m] for m in range(L - 1, -1, -1)])
+    assert max_rel(got, ref) < 1e-13
