@@ -75,6 +75,14 @@ __device__ __forceinline__ void split_pair(float a, float b, bf16x8& hi, bf16x8&
   lo[e + 1] = l[1];
 }
 
+// The value of lane i ^ 1 (DPP quad permutation [1, 0, 3, 2]).
+__device__ __forceinline__ double swap_pair_lanes(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)u, 0xB1, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), 0xB1, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
 __device__ __forceinline__ f32x4 mma(const bf16x8& a, const bf16x8& b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
@@ -88,30 +96,31 @@ __device__ __forceinline__ f32x4 mma3(const bf16x8& ah, const bf16x8& al, const 
 }
 
 constexpr int kFW = 8;          // forward: waves per workgroup
-constexpr int kFRow = 72;       // forward d tile row stride (BF16): 64 pairs + 8 pad, so the
-                                // 16-byte B-operand reads of 16 rows hit distinct banks
 constexpr int kFT = kFW * 64;
 constexpr int kFTiles = 2;      // 8-map tiles per workgroup (16 maps)
 
 }  // namespace
 
 // ------------------------------------------------------------------ forward
-// NS: 64-pair sets per wave (their products accumulate into the same MFMA
-// accumulators, so a workgroup covers kFW * NS * 64 ring pairs).  SPLIT: 3 =
-// hi*hi + hi*lo + lo*hi, 1 = hi*hi only (BF16 operands, ~4e-3 relative: the
-// correction transform of a refinement whose residual is FP64).
-template <bool PAIRED, int NS, int SPLIT>
+// NP: adjacent ring pairs per lane.  A wave covers 64 NP consecutive pairs
+// (lane i: pairs NP i .. NP i + NP - 1), a workgroup kFW * 64 NP.  At each
+// recursion step a lane holds the d-values of its NP pairs at one l, which
+// are adjacent in the tile row of that l: one packed LDS store per step
+// (8 bytes for NP = 4, 4 bytes for NP = 2; the lo parts a second one).
+// SPLIT: 3 = hi*hi + hi*lo + lo*hi, 1 = hi*hi only (BF16 operands, ~4e-3
+// relative: the correction transform of a refinement whose residual is FP64).
+template <bool PAIRED, int NP, int SPLIT>
 __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
   constexpr bool LO = SPLIT == 3;
-  // NI sets step together, sharing each coefficient load and giving the FP64
-  // recursion NI independent chains.
-  constexpr int NI = LO ? 2 : 4;
-  static_assert(NS % NI == 0, "set groups");
-  constexpr int kTileE = 32 * kFRow;               // one [l_local 32][pair 64 + pad] BF16 tile
+  static_assert(NP == 2 || NP == 4, "pairs per lane");
+  constexpr int KB = 2 * NP;                      // 32-pair K blocks per wave
+  constexpr int kRow = 64 * NP + 8;               // tile row stride (BF16): 8 pad, so the
+                                                  // 16-byte B reads of 8 rows hit distinct banks
+  constexpr int kTileE = 32 * kRow;               // [l_local 32 (even 16 | odd 16)][pair]
   constexpr int kTileB = kTileE * 2;
-  constexpr int kWaveB = (LO ? 2 * NI : NI) * kTileB;
+  constexpr int kWaveB = (LO ? 2 : 1) * kTileB;
   static_assert(kWaveB >= kFTiles * 2 * 256 * 4, "result tile fits the wave's d tiles");
-  // per wave: NI hi d tiles (+ NI lo tiles); after the chunk's products the
+  // per wave: the hi d tile (+ the lo tile); after the chunk's products the
   // same bytes hold the wave's results [tile][parity][row 16][col 16] (FP32)
   __shared__ __attribute__((aligned(16))) unsigned char wbuf[kFW][kWaveB];
   __shared__ lat::ChunkCoef ccs[2][lat::kChunk];  // recursion coefficients, double-buffered
@@ -124,59 +133,66 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
   const MStart st = a.mstart[m - a.m_lo];
   const double bsign = m < 0 ? -1.0 : 1.0;
   const Coef* __restrict__ cf = a.ucoef + (int64_t)am * L;  // unit-c: d_l = sigma_l e_l
-  __bf16* dh0 = reinterpret_cast<__bf16*>(wbuf[wave]);  // tile j at dh0 + j * kTileE
-  __bf16* dl = reinterpret_cast<__bf16*>(wbuf[wave] + NI * kTileB);
+  __bf16* dh = reinterpret_cast<__bf16*>(wbuf[wave]);
+  __bf16* dl = reinterpret_cast<__bf16*>(wbuf[wave] + kTileB);
   float* ctw = reinterpret_cast<float*>(wbuf[wave]);
+  const int pbase = (blockIdx.y * kFW + wave) * 64 * NP;
 
-  // recursion state of this lane's pair in each set
-  double x[NS], d[NS], dm1[NS];
-  int ls[NS];
-  // A operands: row = (map row/2 of the tile, re/im row%2), K = pairs
-  // 32 kb + 8 kg + e of the set
-  bf16x8 seh[NS][kFTiles][2], soh[NS][kFTiles][2];
-  bf16x8 sel[LO ? NS : 1][kFTiles][2], sol[LO ? NS : 1][kFTiles][2];
+  // recursion state of this lane's pairs
+  double x[NP], d[NP], dm1[NP];
+  int ls[NP];
 #pragma unroll
-  for (int s = 0; s < NS; ++s) {
-    const int base = ((blockIdx.y * kFW + wave) * NS + s) * 64;
-    const int p = base + lane;
-    dm1[s] = 0.0;
-    x[s] = 0.0;
-    d[s] = 0.0;
-    ls[s] = 0;
+  for (int j = 0; j < NP; ++j) {
+    const int p = pbase + NP * lane + j;
+    dm1[j] = 0.0;
+    x[j] = 0.0;
+    d[j] = 0.0;
+    ls[j] = 0;
     if (p < a.nrings) {
       const LatRing rg = a.rings[p];
-      x[s] = rg.x;
-      start_value(st, rg.c2, rg.s2, d[s], ls[s]);
+      x[j] = rg.x;
+      start_value(st, rg.c2, rg.s2, d[j], ls[j]);
     }
+  }
+  // A operands: row = (map row/2 of the tile, re/im row%2), K = pairs
+  // 32 kb + 8 kg + e of the wave.  Each ring value is loaded once: of the two
+  // lanes of a map (rows 2r: re, 2r + 1: im) the first loads ring ta, the
+  // second ring tb, and they swap the component the other needs.
+  bf16x8 seh[KB][kFTiles], soh[KB][kFTiles];
+  bf16x8 sel[LO ? KB : 1][kFTiles], sol[LO ? KB : 1][kFTiles];
+  const int comp = row & 1;
 #pragma unroll
-    for (int t = 0; t < kFTiles; ++t) {
-      const int amap = b0 + 8 * t + (row >> 1);
-      const bool have = amap < a.batch;
-      const cplx* ftm = a.ftm + (int64_t)amap * a.ftm_map_stride + (m - a.ftm_m_first);
+  for (int t = 0; t < kFTiles; ++t) {
+    const int amap = b0 + 8 * t + (row >> 1);
+    const bool have = amap < a.batch;
+    const cplx* ftm = a.ftm + (int64_t)amap * a.ftm_map_stride + (m - a.ftm_m_first);
 #pragma unroll
-      for (int kb = 0; kb < 2; ++kb) {
+    for (int kb = 0; kb < KB; ++kb) {
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          const int q = base + 32 * kb + 8 * kg + e;
-          double ve = 0.0, vo = 0.0;
-          if (have && q < a.nrings) {
-            const LatRing rq = a.rings[q];
-            const cplx fa = ftm[(int64_t)rq.ta * a.ftm_ncol];
-            cplx fb = {0.0, 0.0};
-            if (PAIRED && rq.tb >= 0 && rq.tb != rq.ta) fb = ftm[(int64_t)rq.tb * a.ftm_ncol];
-            const double ca = (row & 1) ? fa.y : fa.x;
-            const double cb = (row & 1) ? fb.y : fb.x;
-            ve = ca + cb;
-            vo = PAIRED ? ca - cb : ca;
-          }
-          __bf16 h, l;
-          split2(ve, h, l);
-          seh[s][t][kb][e] = h;
-          if constexpr (LO) sel[s][t][kb][e] = l;
-          split2(vo, h, l);
-          soh[s][t][kb][e] = h;
-          if constexpr (LO) sol[s][t][kb][e] = l;
+      for (int e = 0; e < 8; ++e) {
+        const int q = pbase + 32 * kb + 8 * kg + e;
+        cplx v = {0.0, 0.0};
+        if (have && q < a.nrings) {
+          const int2 tt = *reinterpret_cast<const int2*>(&a.rings[q].ta);
+          const int tr = comp ? tt.y : tt.x;
+          if (comp == 0 || (PAIRED && tt.y >= 0 && tt.y != tt.x))
+            v = ftm[(int64_t)tr * a.ftm_ncol];
         }
+        // lane 2r keeps fa.x and receives fb.x; lane 2r + 1 keeps fb.y and
+        // receives fa.y (a zero fb where the ring is unpaired)
+        const double give = comp ? v.x : v.y;
+        const double got = swap_pair_lanes(give);
+        const double ca = comp ? got : v.x;
+        const double cb = comp ? v.y : got;
+        const double ve = ca + cb;
+        const double vo = PAIRED ? ca - cb : ca;
+        __bf16 h, l;
+        split2(ve, h, l);
+        seh[kb][t][e] = h;
+        if constexpr (LO) sel[kb][t][e] = l;
+        split2(vo, h, l);
+        soh[kb][t][e] = h;
+        if constexpr (LO) sol[kb][t][e] = l;
       }
     }
   }
@@ -211,110 +227,107 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
       acc[t][0] = f32x4{0.f, 0.f, 0.f, 0.f};
       acc[t][1] = f32x4{0.f, 0.f, 0.f, 0.f};
     }
+    // a pair contributes only while its scale index is 0 (it changes only in
+    // the rescale after the chunk): one mask per two pairs, applied to their
+    // packed BF16 values
+    unsigned mk[NP / 2];
 #pragma unroll
-    for (int sg = 0; sg < NS; sg += NI) {
-      // 32 unit-c steps e_{l+1} = (a' x - b') e_l - e_{l-1} of sets sg ..
-      // sg + NI - 1; the values past the band-limit are zero (zero
-      // coefficients from l = L - 1 on).  The coefficients of each GS-step
-      // group are read from LDS while the previous group steps; steps i, i + 1
-      // share one paired BF16 conversion.
-      constexpr int GS = LO ? 4 : 2;
-      double ca[GS], cb[GS], na[GS], nb[GS];
+    for (int h = 0; h < NP / 2; ++h)
+      mk[h] = (ls[2 * h] == 0 ? 0x0000ffffu : 0u) | (ls[2 * h + 1] == 0 ? 0xffff0000u : 0u);
+    // 32 unit-c steps e_{l+1} = (a' x - b') e_l - e_{l-1}; the values past
+    // the band-limit are zero (zero coefficients from l = L - 1 on).  Each
+    // GS-step group's coefficients are read from LDS while the previous group
+    // steps.  Step i goes to tile row (i odd ? 16 : 0) + i / 2.
+    constexpr int GS = LO ? 4 : 2;
+    double ca[GS], cb[GS], na[GS], nb[GS];
+#pragma unroll
+    for (int e = 0; e < GS; ++e) {
+      ca[e] = cs[e].a;
+      cb[e] = cs[e].nb;
+    }
+#pragma unroll
+    for (int g = 0; g < 32 / GS; ++g) {
+      if (g + 1 < 32 / GS) {
+#pragma unroll
+        for (int e = 0; e < GS; ++e) {
+          na[e] = cs[GS * (g + 1) + e].a;
+          nb[e] = cs[GS * (g + 1) + e].nb;
+        }
+      }
 #pragma unroll
       for (int e = 0; e < GS; ++e) {
-        ca[e] = cs[e].a;
-        cb[e] = cs[e].nb;
-      }
+        const int i = GS * g + e;
+        const int trow = (i & 1) * 16 + (i >> 1);
+        float fv[NP];
 #pragma unroll
-      for (int g = 0; g < 32 / GS; ++g) {
-        if (g + 1 < 32 / GS) {
+        for (int j = 0; j < NP; ++j) {
+          fv[j] = to_f32(d[j]);
+          const double dn = fma(fma(ca[e], x[j], cb[e]), d[j], -dm1[j]);
+          dm1[j] = d[j];
+          d[j] = dn;
+        }
+        unsigned hw[NP / 2], lw[NP / 2];
 #pragma unroll
-          for (int e = 0; e < GS; ++e) {
-            na[e] = cs[GS * (g + 1) + e].a;
-            nb[e] = cs[GS * (g + 1) + e].nb;
+        for (int h = 0; h < NP / 2; ++h) {
+          const bf16x2 hh = __builtin_convertvector(f32x2{fv[2 * h], fv[2 * h + 1]}, bf16x2);
+          const unsigned bits = __builtin_bit_cast(unsigned, hh);
+          hw[h] = bits & mk[h];
+          if constexpr (LO) {
+            const float h0 = __builtin_bit_cast(float, bits << 16);
+            const float h1 = __builtin_bit_cast(float, bits & 0xffff0000u);
+            const bf16x2 lo = __builtin_convertvector(f32x2{fv[2 * h] - h0, fv[2 * h + 1] - h1},
+                                                      bf16x2);
+            lw[h] = __builtin_bit_cast(unsigned, lo) & mk[h];
           }
         }
-        float fv[NI][GS];
-#pragma unroll
-        for (int e = 0; e < GS; ++e) {
-#pragma unroll
-          for (int j = 0; j < NI; ++j) {
-            const int s = sg + j;
-            const float f = to_f32(d[s]);
-            fv[j][e] = ls[s] == 0 ? f : 0.0f;
-            const double dn = fma(fma(ca[e], x[s], cb[e]), d[s], -dm1[s]);
-            dm1[s] = d[s];
-            d[s] = dn;
-          }
-        }
-        // even / odd l (relative to l0) in separate 16-column halves
-#pragma unroll
-        for (int j = 0; j < NI; ++j) {
-#pragma unroll
-          for (int e = 0; e < GS; e += 2) {
-            const int i0 = GS * g + e;  // even: column i0 / 2; i0 + 1: column 16 + i0 / 2
-            __bf16* th = dh0 + j * kTileE + (i0 >> 1) * kFRow + lane;
-            const bf16x2 h = __builtin_convertvector(f32x2{fv[j][e], fv[j][e + 1]}, bf16x2);
-            th[0] = h[0];
-            th[16 * kFRow] = h[1];
-            if constexpr (LO) {
-              const unsigned bits = __builtin_bit_cast(unsigned, h);
-              const float h0 = __builtin_bit_cast(float, bits << 16);
-              const float h1 = __builtin_bit_cast(float, bits & 0xffff0000u);
-              const bf16x2 lo =
-                  __builtin_convertvector(f32x2{fv[j][e] - h0, fv[j][e + 1] - h1}, bf16x2);
-              __bf16* tl = dl + j * kTileE + (i0 >> 1) * kFRow + lane;
-              tl[0] = lo[0];
-              tl[16 * kFRow] = lo[1];
-            }
-          }
-        }
-#pragma unroll
-        for (int e = 0; e < GS; ++e) {
-          ca[e] = na[e];
-          cb[e] = nb[e];
+        const int off = trow * kRow + NP * lane;
+        if constexpr (NP == 4) {
+          *reinterpret_cast<uint2*>(dh + off) = uint2{hw[0], hw[1]};
+          if constexpr (LO) *reinterpret_cast<uint2*>(dl + off) = uint2{lw[0], lw[1]};
+        } else {
+          *reinterpret_cast<unsigned*>(dh + off) = hw[0];
+          if constexpr (LO) *reinterpret_cast<unsigned*>(dl + off) = lw[0];
         }
       }
 #pragma unroll
-      for (int j = 0; j < NI; ++j) lat::rescale(d[sg + j], dm1[sg + j], ls[sg + j]);
-      lat::wave_sync();
-#pragma unroll
-      for (int j = 0; j < NI; ++j) {
-        const int s = sg + j;
-        const __bf16* dh = dh0 + j * kTileE;
-        const __bf16* dlj = dl + j * kTileE;
-#pragma unroll
-        for (int par = 0; par < 2; ++par) {
-#pragma unroll
-          for (int kb = 0; kb < 2; ++kb) {
-            // B[k][j]: d of pair 32 kb + 8 kg + e at column j = row (parity par)
-            const int off = (par * 16 + row) * kFRow + 32 * kb + 8 * kg;
-            const bf16x8 bh = *reinterpret_cast<const bf16x8*>(dh + off);
-            if constexpr (LO) {
-              const bf16x8 bl = *reinterpret_cast<const bf16x8*>(dlj + off);
-#pragma unroll
-              for (int t = 0; t < kFTiles; ++t) {
-                if (par == 0)
-                  acc[t][0] = mma3(seh[s][t][kb], sel[s][t][kb], bh, bl, acc[t][0]);
-                else
-                  acc[t][1] = mma3(soh[s][t][kb], sol[s][t][kb], bh, bl, acc[t][1]);
-              }
-            } else {
-#pragma unroll
-              for (int t = 0; t < kFTiles; ++t) {
-                if (par == 0)
-                  acc[t][0] = mma(seh[s][t][kb], bh, acc[t][0]);
-                else
-                  acc[t][1] = mma(soh[s][t][kb], bh, acc[t][1]);
-              }
-            }
-          }
-        }
+      for (int e = 0; e < GS; ++e) {
+        ca[e] = na[e];
+        cb[e] = nb[e];
       }
-      lat::wave_sync();
     }
+#pragma unroll
+    for (int j = 0; j < NP; ++j) lat::rescale(d[j], dm1[j], ls[j]);
+    lat::wave_sync();
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+#pragma unroll
+      for (int par = 0; par < 2; ++par) {
+        // B[k][j]: d of pair 32 kb + 8 kg + e at column j = row (parity par)
+        const int off = (par * 16 + row) * kRow + 32 * kb + 8 * kg;
+        const bf16x8 bh = *reinterpret_cast<const bf16x8*>(dh + off);
+        if constexpr (LO) {
+          const bf16x8 bl = *reinterpret_cast<const bf16x8*>(dl + off);
+#pragma unroll
+          for (int t = 0; t < kFTiles; ++t) {
+            if (par == 0)
+              acc[t][0] = mma3(seh[kb][t], sel[kb][t], bh, bl, acc[t][0]);
+            else
+              acc[t][1] = mma3(soh[kb][t], sol[kb][t], bh, bl, acc[t][1]);
+          }
+        } else {
+#pragma unroll
+          for (int t = 0; t < kFTiles; ++t) {
+            if (par == 0)
+              acc[t][0] = mma(seh[kb][t], bh, acc[t][0]);
+            else
+              acc[t][1] = mma(soh[kb][t], bh, acc[t][1]);
+          }
+        }
+      }
+    }
+    lat::wave_sync();
     // C[i][j]: lane j + 16 (i/4), register i%4 -> this wave's result tile,
-    // over its own d tiles (their last reads are ordered before by wave_sync)
+    // over its own d tile (its last reads are ordered before by wave_sync)
 #pragma unroll
     for (int t = 0; t < kFTiles; ++t)
 #pragma unroll
@@ -586,16 +599,16 @@ __global__ void __launch_bounds__(kIT, 3) lat_inverse_bf16_kernel(LatArgs a) {
 }
 
 // ------------------------------------------------------------------ launchers
-template <int NS, int SPLIT>
+template <int NP, int SPLIT>
 static hipError_t lat_fwd_bf16(const LatArgs& a0, bool paired, hipStream_t s) {
   LatArgs a = a0;
-  const int per_wg = kFW * NS * 64;
+  const int per_wg = kFW * NP * 64;
   a.groups = (a.nrings + per_wg - 1) / per_wg;
   dim3 grid(a.nm, a.groups, (a.batch + 8 * kFTiles - 1) / (8 * kFTiles));
   if (paired)
-    hipLaunchKernelGGL((lat_forward_bf16_kernel<true, NS, SPLIT>), grid, dim3(kFT), 0, s, a);
+    hipLaunchKernelGGL((lat_forward_bf16_kernel<true, NP, SPLIT>), grid, dim3(kFT), 0, s, a);
   else
-    hipLaunchKernelGGL((lat_forward_bf16_kernel<false, NS, SPLIT>), grid, dim3(kFT), 0, s, a);
+    hipLaunchKernelGGL((lat_forward_bf16_kernel<false, NP, SPLIT>), grid, dim3(kFT), 0, s, a);
   return hipGetLastError();
 }
 
