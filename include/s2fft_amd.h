@@ -56,7 +56,8 @@ enum s2h_status {
   S2H_ERR_HIP = -3,         /* a HIP runtime call failed */
   S2H_ERR_NOMEM = -4,       /* device or host allocation failed at plan creation */
   S2H_ERR_SPIN = -5,        /* |spin| >= 8 (the reference raises Warning) */
-  S2H_ERR_BANDLIMIT = -6    /* L < 2 nside for HEALPix (reference AssertionError) */
+  S2H_ERR_BANDLIMIT = -6,   /* L < 2 nside for HEALPix (reference AssertionError) */
+  S2H_ERR_SCRATCH = -7      /* s2h_forward_noalloc: workspace beyond its fixed scratch */
 };
 
 /* Normalisation folded into a plan's Wigner-d recursion. */
@@ -419,6 +420,21 @@ int s2h_mlp_grad(const float *params, const float *X, const float *Y, const floa
 /* One Adam step (step >= 1) on the FP32 master parameters. */
 int s2h_mlp_adam(float *params, const float *grad, float *m1, float *m2, int step, float lr,
                  float beta1, float beta2, float eps, s2h_stream stream);
+
+/* s2h_forward_noalloc: s2h_forward (FP64) with its workspace in a fixed
+ * device array of S2H_NOALLOC_SCRATCH_BYTES per device, held from the first
+ * call until the process ends, so that the call allocates nothing (no
+ * hipMalloc, no hipFree, no allocating library routine).  No counterpart in
+ * the reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Returns
+ * S2H_ERR_SCRATCH when s2h_workspace_bytes(plan, batch) exceeds the scratch
+ * (real HEALPix transforms: 4 maps at nside 256, 1 at nside 512).  Every call on a
+ * device shares its scratch: calls that overlap on one device race.
+ * s2h_noalloc_scratch_bytes returns the scratch size. */
+#define S2H_NOALLOC_SCRATCH_BYTES (64u << 20)
+size_t s2h_noalloc_scratch_bytes(void);
+int s2h_forward_noalloc(const s2h_plan *plan, const void *f, int f_is_complex, void *flm,
+                        int batch, s2h_stream stream);
 
 /* Diagnostics (no counterpart in the reference): one V_MFMA_F64_16X16X4F64
  * with a[lane], b[lane] (64 doubles each) as the A and B operands and a zero

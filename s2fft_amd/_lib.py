@@ -24,6 +24,7 @@ S2H_ERR_HIP = -3
 S2H_ERR_NOMEM = -4
 S2H_ERR_SPIN = -5
 S2H_ERR_BANDLIMIT = -6
+S2H_ERR_SCRATCH = -7
 
 
 class Desc(ctypes.Structure):
@@ -62,6 +63,7 @@ EXPORTED = (
     "s2h_probe_mfma_f32", "s2h_forward_ex", "s2h_inverse_ex",
     "s2h_mlp_param_count", "s2h_mlp_precondition", "s2h_mlp_features",
     "s2h_mlp_grad_workspace_bytes", "s2h_mlp_grad", "s2h_mlp_adam", "s2h_probe_mfma_bf16",
+    "s2h_noalloc_scratch_bytes", "s2h_forward_noalloc",
 )
 
 KERNEL_LAT_FORWARD, KERNEL_LAT_INVERSE, KERNEL_RING_FFT_FORWARD = 0, 1, 2
@@ -144,6 +146,8 @@ def _declare(lib):
         "s2h_cg_update": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, i64, i32, vp]),
         "s2h_cg_finish": (i32, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), i32, i64, i32, vp,
                                 i32, vp]),
+        "s2h_noalloc_scratch_bytes": (sz, []),
+        "s2h_forward_noalloc": (i32, [vp, vp, i32, vp, i32, vp]),
         "s2h_profile_enable": (i32, [i32]),
         "s2h_profile_query": (i32, [i32, ctypes.POINTER(ctypes.c_double),
                                     ctypes.POINTER(ctypes.c_int)]),

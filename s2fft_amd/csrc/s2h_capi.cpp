@@ -119,6 +119,9 @@ extern "C" const char* s2h_status_string(int s) {
     case S2H_ERR_NOMEM: return "out of memory";
     case S2H_ERR_SPIN: return "Recursive transform may provide lower precision beyond spin ~ 8";
     case S2H_ERR_BANDLIMIT: return "HEALPix requires L >= 2 nside";
+    case S2H_ERR_SCRATCH:
+      return "workspace exceeds the fixed scratch of s2h_forward_noalloc "
+             "(S2H_NOALLOC_SCRATCH_BYTES); use s2h_forward with a workspace";
     default: return "unknown status";
   }
 }

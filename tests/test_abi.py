@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_status_strings():
     lib = _lib.lib()
-    for code in range(-6, 1):
+    for code in range(-7, 1):
         assert lib.s2h_status_string(code).decode() != "unknown status"
     assert lib.s2h_status_string(-99).decode() == "unknown status"
     assert lib.s2h_version() >= 1
