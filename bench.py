@@ -17,7 +17,8 @@ signal-generator distribution).  These pipelines are timed on the same maps:
              FP64 on the ring spectra, FP64 recursion, split-BF16 / BF16
              matrix-core projections and syntheses, one FP64 synthesis, and
              the learned MLP preconditioner on the first residual (its
-             direction kept per map only where the FP64 ring residual drops);
+             direction synthesised with the exact FP64 inverse and kept per
+             map only where that FP64 ring residual drops);
 * mixed CG -- the same without the network (the precision gain over fp64 CG);
 * mixed Jacobi + learned -- Jacobi steps with the learned preconditioner.
 
@@ -513,7 +514,8 @@ def main():
                        f"projections of the residuals, split-BF16 matrix-core syntheses of the "
                        f"directions, one FP64 synthesis"
                        + (f"; learned MLP preconditioner on the first {args.cg_learned_steps} "
-                          f"residual(s), kept per map only where the FP64 ring residual drops"
+                          f"residual(s), its direction synthesised with the exact FP64 inverse "
+                          f"and kept per map only where that FP64 ring residual drops"
                           if learned else "")),
             "nside": nside, "L": L, "batch": B, "learned": learned,
             "mixed_steps": mixed["iters"], "mixed_max_rel_err": mixed["err"],

@@ -15,8 +15,9 @@ The update delta is a correction whose error is eps |delta|, so the error
 contracts by rho + eps per step (rho ~ 0.13 for HEALPix at L = 2 nside);
 the incremental residual carries an error below eps sum_k |I delta_k|, which
 is about eps rho / (1 - rho) of |f| because the first residual is exact.
-P is the identity (Jacobi) or a learned per-(l, m) preconditioner that is
-accepted only where the FP64-tracked residual decreases.
+P is the identity (Jacobi) or a learned per-(l, m) preconditioner whose step
+is kept only where the residual, measured with the exact FP64 synthesis of
+the step, decreases.
 
 With ``spectral`` (plans with one ring grid: HEALPix) the residual lives on
 the ring spectra instead of the pixels: r = G - A(Lat_inv(flm)) with
@@ -45,7 +46,7 @@ def refine_fp64(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=No
 def refine_mixed(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=None,
                  precond=None, refresh_every=0, history=None, flm_true=None, stats=None,
                  precision="mixed", inv_precision=None, learned_steps=None, spectral=False,
-                 init_precision="fp64"):
+                 init_precision="fp64", judge_precision="fp64"):
     """Mixed-precision refinement with an FP64 residual (see module docstring).
 
     precond: None (Jacobi) or an object with ``apply(h, L, reality, lmin)``
@@ -63,6 +64,8 @@ def refine_mixed(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=N
     a plan with one ring grid.
     init_precision: arithmetic of the synthesis that forms the first residual
     (FP64 by default; a reduced one leaves its error in r until a refresh).
+    judge_precision: arithmetic of the synthesis that measures a learned
+    step's residual (the exact FP64 inverse by default).
     """
     s = sampling.lower()
     plan = plan or sph.get_plan(s, L, nside, spin, L_lower, reality, device=f.device)
@@ -105,7 +108,7 @@ def refine_mixed(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=N
         refresh = bool(refresh_every and (k + 1) % refresh_every == 0)
         if precond is not None and k < learned_steps:
             delta, r, rn = _judged_step(precond, h, r, rn, f, flm, kw, stats, synth, ip, refresh,
-                                        packed)
+                                        packed, judge_precision)
             flm += delta
         else:
             flm += h
@@ -118,7 +121,7 @@ def refine_mixed(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=N
 def refine_cg(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=None, precond=None,
               learned_steps=0, init_precision="bf16", proj_precision=("bf16", "bf16x1"),
               synth_precision="bf16", last_synth_precision="bf16x1", history=None,
-              flm_true=None, stats=None):
+              flm_true=None, stats=None, judge_precision="fp64"):
     """Conjugate-gradient refinement of the forward transform on the ring spectra.
 
     F I is Hermitian positive definite on the coefficients (the HEALPix and
@@ -147,8 +150,10 @@ def refine_cg(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=None
     synthesis only sets its step length (no residual is formed from it), so
     it runs at last_synth_precision.  precond: the
     learned preconditioner for the first ``learned_steps`` residuals; a
-    learned direction is kept for a map only where that step lowers the norm
-    of its FP64 ring residual R, else the map redoes the step along its CG
+    learned direction is synthesised at judge_precision (the exact FP64
+    inverse by default), so the residual R that takes its step is the FP64
+    residual of the new iterate, and it is kept for a map only where that
+    step lowers the norm of R, else the map redoes the step along its CG
     direction (stats counts accepted / judged).  Needs a plan with one ring
     grid (HEALPix, GL).  Returns flm (B, L, 2L-1).
     """
@@ -199,8 +204,9 @@ def refine_cg(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=None
     for k in range(n_iter):
         p = ps[-1]
         last = k + 1 == n_iter
-        Y = sph.inverse_rings_hip(p, plan, last_synth_precision if last else synth_precision,
-                                  packed)
+        prec = judge_precision if judged(k) else \
+            last_synth_precision if last else synth_precision
+        Y = sph.inverse_rings_hip(p, plan, prec, packed)
         pq, alpha = sph.ring_dot_hip(Y, plan, num=rz)
         if last:
             if eager:
@@ -269,27 +275,30 @@ def _redo_plain(bad, R, Y, p, alpha, rz, r, plan, synth_precision, packed, pL, n
     alpha[bad], rz[bad], n_R[bad] = ab, rzb, nb
 
 
-def _judged_step(precond, h, r, rn, f, flm, kw, stats, synth, ip, exact, packed=False):
+def _judged_step(precond, h, r, rn, f, flm, kw, stats, synth, ip, exact, packed=False,
+                 judge_precision="fp64"):
     """Learned step per map where it lowers the FP64-held residual, else Jacobi.
 
-    The candidate delta = P(h) moves the residual to r - I~(delta) (or, on a
-    refresh step, to the exact f - I(flm + delta)); a map keeps it only if
-    that residual's 2-norm is below the current one (rn: the current squared
-    norms, or None).  Rejected maps take the Jacobi step delta = h instead
-    (one more synthesis, for those maps only).
+    The candidate delta = P(h) moves the residual to r - I(delta), with I the
+    synthesis at judge_precision (the exact FP64 inverse by default; on a
+    refresh step the whole residual f - I(flm + delta) is re-formed in FP64);
+    a map keeps it only if that residual's 2-norm is below the current one
+    (rn: the current squared norms, or None).  Rejected maps take the Jacobi
+    step delta = h instead (one more synthesis at the reduced precision ip,
+    for those maps only).
     Returns (delta, new residual, its squared norms where known, else None).
     """
     L = kw["L"]
     lmin = max(abs(kw["spin"]), kw["L_lower"])
     delta = precond.apply(h, L, kw["reality"], lmin, packed=packed)
 
-    def residual(d, rows):
+    def residual(d, rows, prec):
         if exact:
             return synth((flm[rows] + d).contiguous(), sub=f[rows])
-        return synth(d.contiguous(), ip, sub=r[rows])
+        return synth(d.contiguous(), prec, sub=r[rows])
 
     every = slice(None)
-    r_new = residual(delta, every)
+    r_new = residual(delta, every, judge_precision)
     new_n = _sumsq(r_new)
     ok = new_n < (_sumsq(r) if rn is None else rn)
     if stats is not None:
@@ -299,7 +308,7 @@ def _judged_step(precond, h, r, rn, f, flm, kw, stats, synth, ip, exact, packed=
         return delta, r_new, new_n
     bad = (~ok).nonzero().flatten()
     hj = h[bad].contiguous()
-    rj = residual(hj, bad)
+    rj = residual(hj, bad, ip)
     delta = delta.clone()
     r_new = r_new.clone()
     delta[bad] = hj

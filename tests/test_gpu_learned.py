@@ -4,6 +4,11 @@ Trains the network briefly on the GPU (plain-transform data at nside 32) and
 compares the mixed refinement with and without it on held-out maps at
 nside 64: the Jacobi-judged learned steps must reach 1e-7 in no more steps
 than Jacobi, never raise a map's residual, and the counts are printed.
+
+The shipped weights (trained at nside 256 on 4096 maps) must give a strict
+gain on held-out maps at nside 256: fewer Jacobi steps to 1e-7, and lower
+conjugate-gradient errors at equal step counts, every learned step judged
+with the exact FP64 synthesis.
 """
 
 import pytest
@@ -38,3 +43,63 @@ def test_learned_gain_over_jacobi(small_model, precision):
     print("learned errs", ["%.1e" % e for e in lrn["errs"]])
     assert jac["steps_to_tol"] is not None and lrn["steps_to_tol"] is not None
     assert lrn["steps_to_tol"] <= jac["steps_to_tol"]
+
+
+# ---------------------------------------------------------------- shipped weights
+def _held_out(nside, n_maps, seed):
+    from s2fft_amd.transforms import spherical as sph
+    from s2fft_amd.utils.signal_generator import generate_flm
+    import numpy as np
+
+    L = 2 * nside
+    flm = torch.from_numpy(np.stack([generate_flm(np.random.default_rng(seed + i), L,
+                                                  reality=True)
+                                     for i in range(n_maps)])).cuda()
+    plan = sph.get_plan("healpix", L, nside, 0, 0, True)
+    f = sph.inverse_hip(flm, L, 0, nside, "healpix", True, plan=plan).real.contiguous()
+    return L, plan, flm, f
+
+
+def test_shipped_weights_strict_gain_jacobi_nside256():
+    """The shipped network (trained at nside 256) against plain Jacobi on
+    held-out maps at nside 256: the mixed refinement with its steps judged by
+    the exact FP64 synthesis reaches 1e-7 in strictly fewer steps."""
+    from s2fft_amd.transforms import refine
+
+    nside = 256
+    L, plan, flm, f = _held_out(nside, 8, 9100)
+    pc = lp.MLPPreconditioner.load(device=f.device)
+    res = {}
+    for name, p in (("jacobi", None), ("learned", pc)):
+        hist, stats = [], {}
+        refine.refine_mixed(f, L, 0, nside, "healpix", True, 10, plan=plan, precond=p,
+                            refresh_every=3, history=hist, flm_true=flm, stats=stats,
+                            precision="bf16", spectral=True)
+        ok = [i for i, e in enumerate(hist) if e <= 1e-7]
+        res[name] = (ok[0] if ok else None, hist, stats)
+        print(name, "steps", res[name][0], ["%.1e" % e for e in hist], stats, flush=True)
+    nj, nl = res["jacobi"][0], res["learned"][0]
+    assert nj is not None and nl is not None and nl < nj
+    assert res["learned"][2]["accepted"] > 0
+
+
+def test_shipped_weights_strict_gain_cg_nside256():
+    """Conjugate gradients with the shipped network on the first residual
+    against the same CG without it (held-out maps, nside 256): strictly lower
+    error after the learned step and at the end, at equal step counts."""
+    from s2fft_amd.transforms import refine
+
+    nside = 256
+    L, plan, flm, f = _held_out(nside, 8, 9200)
+    pc = lp.MLPPreconditioner.load(device=f.device)
+    hists = {}
+    for name, p in (("cg", None), ("cg_learned", pc)):
+        hist, stats = [], {}
+        refine.refine_cg(f, L, 0, nside, "healpix", True, 3, plan=plan, precond=p,
+                         learned_steps=1 if p is not None else 0, history=hist, flm_true=flm,
+                         stats=stats)
+        hists[name] = hist
+        print(name, ["%.2e" % e for e in hist], stats, flush=True)
+    a, b = hists["cg"], hists["cg_learned"]
+    assert b[0] == a[0]           # same start
+    assert b[1] < a[1] and b[2] < a[2]
