@@ -43,10 +43,15 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
 FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 matrix
 BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch at this workload from rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE, scaled by a 1 GiB device copy's readings:
-# profiles/counters_healpix_nside1024_L2048_b16.md
-TRAFFIC_BYTES = {"lat_forward": 6.86e9, "lat_inverse": 5.72e9}
+# HBM bytes per launch at this workload (DRAM reads + writes) from rocprofv3 --pmc
+# TCC_EA0_RDREQ_DRAM_32B_sum and TCC_EA0_WRREQ_WRITE_DRAM_32B_sum, 32 bytes each
+# (request-size resolved: a 1 GiB device copy reads and writes 1.074e9 B by them):
+# profiles/counters_healpix_nside1024_L2048_b16.md.  lat_forward: the FP64
+# projection into the reference flm layout (the plain leg); lat_inverse: the FP64
+# synthesis; the BF16 kinds: the packed single-BF16 projection and split-BF16
+# synthesis of the CG loop.
+TRAFFIC_BYTES = {"lat_forward": 16.42e9, "lat_inverse": 5.96e9,
+                 "lat_forward_bf16": 5.38e9, "lat_inverse_bf16": 5.05e9}
 
 
 def parse():
@@ -258,6 +263,11 @@ def kernels_once(nside, B):
     sph.forward_rings_hip(R, L, plan, "bf16x1")
     sph.inverse_hip(flm, L, 0, nside, "healpix", True, plan=plan, real_output=True)
     refine.refine_cg(f, L, 0, nside, "healpix", True, 3, plan=plan, precond=pc, learned_steps=1)
+    # the single-map workload: nside 512, one real map, forward with iter=3
+    # (the VALU latitudinal kernels)
+    n1 = 512
+    f1 = torch.randn((1, 12 * n1 * n1), generator=g, dtype=torch.float64, device=dev)
+    sph.forward_iterative_hip(f1, 2 * n1, 0, n1, "healpix", True, 0, 3)
     src = torch.empty(1 << 27, dtype=torch.float64, device=dev)  # 1 GiB
     src.fill_(1.0)
     torch.empty_like(src).copy_(src)
