@@ -4,20 +4,27 @@ Test infrastructure only -- see ``oracle/__init__.py``.
 
 No reference counterpart: the reference refines with plain Jacobi steps
 (s2fft/utils/iterative_refinement.py:9-46).  The network is
-    z1 = W1q x + b1,  a1 = gelu(z1),  z2 = W2q bf16(a1) + b2,
-    a2 = gelu(z2),    y  = W3q a2 + b3,          Wq = bf16(W)
-with the tanh form of GELU, evaluated here in float64 (the BF16 roundings
-are applied exactly), and its gradient by hand-derived backpropagation with
-the roundings treated as the identity.  The gradient is pinned in the tests
-by central finite differences of ``loss`` (with the roundings switched off),
-and the feature construction by a direct loop.
+    z1 = W1h h(x) + b1,  a1 = h(gelu(z1)),  z2 = W2h a1 + b2,
+    a2 = h(gelu(z2)),    y  = W3h a2 + b3,   Wh = h(W), h = round to fp16
+with GELU the tanh form whose tanh factor is replaced by a degree-6
+polynomial (``gelu``), evaluated here in float64 (the FP16 roundings
+applied exactly), and its gradient by hand-derived
+backpropagation with the roundings treated as the identity.  The gradient
+is pinned in the tests by central finite differences of ``loss`` (with the
+roundings switched off), the polynomial against the tanh form, and the
+feature construction by a direct loop.
 """
 
 import numpy as np
 
 H = 128
 N_PARAMS = H * 4 + H + H * H + H + 2 * H + 2
-_C = np.sqrt(2.0 / np.pi) * 2.0   # 2 sqrt(2/pi)
+
+# gelu(x) = (x + |x| T(min(|x|, A) - A/2)) / 2, T a polynomial standing for
+# tanh(sqrt(2/pi) (a + 0.044715 a^3)) on [0, A]; coefficients exact in FP16
+GELU_A = 3.75
+GELU_C = np.array([0.939453125, 0.1392822265625, -0.1318359375, 0.05487060546875,
+                   -0.0016756057739257812, -0.00531768798828125, 0.0010290145874023438])
 
 
 def bf16(x):
@@ -54,28 +61,43 @@ def init_params(rng):
     return pack(W1, np.zeros(H), W2, np.zeros(H), W3, np.zeros(2)).astype(np.float32)
 
 
+def f16(x):
+    """Round to the nearest IEEE half (ties to even), as float64."""
+    return np.asarray(x, dtype=np.float64).astype(np.float16).astype(np.float64)
+
+
+def gelu_tanh(x):
+    """The tanh form 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3)))."""
+    return 0.5 * x * (1.0 + np.tanh(np.sqrt(2.0 / np.pi) * (x + 0.044715 * x ** 3)))
+
+
 def gelu(x):
-    u = _C * (x + 0.044715 * x ** 3)
-    return x / (1.0 + np.exp(-u))
+    x = np.asarray(x, dtype=np.float64)
+    a = np.abs(x)
+    u = np.minimum(a, GELU_A) - GELU_A / 2
+    return 0.5 * (x + a * np.polynomial.polynomial.polyval(u, GELU_C))
 
 
 def gelu_grad(x):
-    u = _C * (x + 0.044715 * x ** 3)
-    sg = 1.0 / (1.0 + np.exp(-u))
-    du = _C * (1.0 + 3 * 0.044715 * x * x)
-    return sg + x * sg * (1 - sg) * du
+    x = np.asarray(x, dtype=np.float64)
+    a = np.abs(x)
+    u = np.minimum(a, GELU_A) - GELU_A / 2
+    t = np.polynomial.polynomial.polyval(u, GELU_C)
+    dt = np.polynomial.polynomial.polyval(u, GELU_C[1:] * np.arange(1, len(GELU_C)))
+    t = np.where(a < GELU_A, t + a * dt, t)
+    return 0.5 * (1.0 + np.where(x < 0, -t, t))
 
 
 def forward(p, X, quantize=True):
     """Returns y (n, 2) and the intermediates."""
     W1, b1, W2, b2, W3, b3 = (np.asarray(t, dtype=np.float64) for t in unpack(p))
-    q = (lambda t: bf16(t).astype(np.float64)) if quantize else (lambda t: t)
-    W1q, W2q, W3q = q(W1), q(W2), q(W3)
-    X = np.asarray(X, dtype=np.float64)
+    qh = f16 if quantize else (lambda t: t)
+    W1q, W2q, W3q = qh(W1), qh(W2), qh(W3)
+    X = qh(np.asarray(X, dtype=np.float64))
     z1 = X @ W1q.T + b1
-    a1 = q(gelu(z1))
+    a1 = qh(gelu(z1))
     z2 = a1 @ W2q.T + b2
-    a2 = gelu(z2)
+    a2 = qh(gelu(z2))
     y = a2 @ W3q.T + b3
     return y, (X, z1, a1, z2, a2, W1q, W2q, W3q)
 

@@ -26,6 +26,14 @@ def sources():
     return sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.cpp"))
 
 
+# Per-source flags.  The network's inference kernel reads every matrix-core
+# result with vector instructions (GELU): results in VGPRs instead of AGPRs
+# spare one move per element.
+EXTRA_FLAGS = {
+    "s2h_mlp.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+}
+
+
 def build(verbose=False):
     """Compile every HIP/C++ source into one shared library (always rebuilds)."""
     objdir = HERE / "_build"
@@ -42,7 +50,7 @@ def build(verbose=False):
     procs = []
     for src in sources():
         obj = objdir / (src.name + ".o")
-        cmd = [hipcc(), *flags, "-c", str(src), "-o", str(obj)]
+        cmd = [hipcc(), *flags, *EXTRA_FLAGS.get(src.name, []), "-c", str(src), "-o", str(obj)]
         if src.suffix == ".cpp":
             cmd.insert(1, "-x")
             cmd.insert(2, "hip")

@@ -1,17 +1,21 @@
 // Learned preconditioner for the iterative forward transform: a per-(l, m)
 // MLP  (re, im, l/L, m/L) -> correction (re, im),  two hidden layers of 128
-// with GELU (tanh form), BF16 weights and FP32 accumulation.
+// with GELU, FP16 weights, features and activations, FP32 accumulation:
 //
-//   z1 = W1q x + b1,  a1 = gelu(z1),  z2 = W2q bf16(a1) + b2,
-//   a2 = gelu(z2),    y  = W3q a2 + b3,           Wq = bf16(W)
+//   z1 = W1h h(x) + b1,  a1 = h(gelu(z1)),  z2 = W2h a1 + b2,
+//   a2 = h(gelu(z2)),    y  = W3h a2 + b3,   Wh = h(W), h = round to FP16
+//
+// gelu is the tanh form x/2 (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3))) with the
+// tanh factor replaced by a polynomial (gelu below), so that inference runs it
+// on packed FP16 arithmetic without transcendentals.
 //
 // x = (h_re / s, h_im / s, l / L, m / L) for the harmonic residual h of a map
 // whose RMS over its coefficients is s; the correction is s * y.  Inference
-// runs layer 2 on the BF16 matrix cores (V_MFMA_F32_16X16X32_BF16); training
-// (forward + backward, FP32 on the vector ALUs) accumulates per-workgroup
-// gradient partials that a second kernel reduces in a fixed order, and Adam
-// updates the FP32 master weights.  The backward treats the BF16 roundings
-// as the identity (straight-through).
+// runs all three layers on the FP16 matrix cores; training (forward +
+// backward, FP32 on the vector ALUs) accumulates per-workgroup gradient
+// partials that a second kernel reduces in a fixed order, and Adam updates
+// the FP32 master weights.  The backward treats the roundings as the identity
+// (straight-through).
 #include "s2h_internal.h"
 #include "s2h_kernels.h"
 
@@ -33,24 +37,75 @@ static_assert(oB3 + 2 == kMlpParams, "parameter layout");
 __device__ __forceinline__ float bf16_round(float v) {
   return (float)(__bf16)v;
 }
-
-// GELU, tanh form: 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3)))
-//                = x / (1 + exp(-2 sqrt(2/pi) (x + 0.044715 x^3)))
-// evaluated as x * rcp(1 + exp2(t)), t = -2 sqrt(2/pi) log2(e) (x + 0.044715 x^3):
-// seven vector instructions, two of them transcendental (the hardware exp2 and
-// reciprocal, about 1 ulp each).  Saturates correctly: exp2 -> inf gives 0.
-__device__ __forceinline__ float gelu(float x) {
-  constexpr float c1 = -2.302208198144325f;         // -2 sqrt(2/pi) log2(e)
-  constexpr float c3 = c1 * 0.044715f;
-  const float t = x * fmaf(c3, x * x, c1);
-  return x * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(t));
+__device__ __forceinline__ float f16_round(float v) {
+  return (float)(_Float16)v;
 }
-// d gelu / dx for the same form: sig + x sig (1 - sig) u'
+
+// GELU.  The tanh form equals (x + |x| T(|x|)) / 2, T(a) = tanh(sqrt(2/pi)
+// (a + 0.044715 a^3)).  T is replaced by a degree-6 polynomial in
+// u = min(|x|, 3.75) - 1.875 (weighted least squares towards the minimax fit
+// on [0, 3.75], coefficients exact in FP16): within 2.7e-4 max(1, |x|) of the
+// tanh form for every x.
+constexpr float kGeluA = 3.75f, kGeluU = 1.875f;
+__device__ constexpr float kGeluC[7] = {0.939453125f,          0.1392822265625f,
+                                        -0.1318359375f,        0.05487060546875f,
+                                        -0.0016756057739257812f, -0.00531768798828125f,
+                                        0.0010290145874023438f};
+
+__device__ __forceinline__ float gelu(float x) {
+  const float a = fabsf(x);
+  const float u = fminf(a, kGeluA) - kGeluU;
+  float p = kGeluC[6];
+#pragma unroll
+  for (int i = 5; i >= 0; --i) p = fmaf(p, u, kGeluC[i]);
+  return 0.5f * fmaf(a, p, x);
+}
+// d gelu / dx = (1 + sign(x) (T + |x| T' [|x| < 3.75])) / 2
 __device__ __forceinline__ float gelu_grad(float x) {
-  const float u = 1.5957691216057308f * fmaf(0.044715f * x, x * x, x);
-  const float sg = 1.0f / (1.0f + __expf(-u));
-  const float du = 1.5957691216057308f * fmaf(3.0f * 0.044715f, x * x, 1.0f);
-  return fmaf(x * sg * (1.0f - sg), du, sg);
+  const float a = fabsf(x);
+  const float u = fminf(a, kGeluA) - kGeluU;
+  float p = kGeluC[6], dp = 0.f;
+#pragma unroll
+  for (int i = 5; i >= 0; --i) {
+    dp = fmaf(dp, u, p);
+    p = fmaf(p, u, kGeluC[i]);
+  }
+  const float t = a < kGeluA ? fmaf(a, dp, p) : p;
+  return 0.5f * (1.0f + (x < 0.f ? -t : t));
+}
+
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// 2 gelu(x) on packed FP16 (gelu2x_operand): per pair one conversion, the
+// magnitude, the clamp and shift, seven FMAs; no transcendentals.  The factor
+// 1/2 is left to the next layer's weights.
+// The MFMA operand of eight hidden units held as two accumulator tiles:
+// elements 0-3 from t0, 4-7 from t1, each through 2 gelu.  The four pairs
+// advance through the polynomial in lockstep, so that no step waits on the
+// one before it.
+__device__ __forceinline__ f16x8 gelu2x_operand(const f32x4& t0, const f32x4& t1) {
+  h2 x[4] = {h2{(_Float16)t0[0], (_Float16)t0[1]}, h2{(_Float16)t0[2], (_Float16)t0[3]},
+             h2{(_Float16)t1[0], (_Float16)t1[1]}, h2{(_Float16)t1[2], (_Float16)t1[3]}};
+  h2 a[4], u[4], p[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) a[q] = __builtin_elementwise_abs(x[q]);
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    u[q] = __builtin_elementwise_min(a[q], (h2)(_Float16)kGeluA) - (h2)(_Float16)kGeluU;
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    p[q] = __builtin_elementwise_fma((h2)(_Float16)kGeluC[6], u[q], (h2)(_Float16)kGeluC[5]);
+#pragma unroll
+  for (int i = 4; i >= 0; --i) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) p[q] = __builtin_elementwise_fma(p[q], u[q], (h2)(_Float16)kGeluC[i]);
+  }
+  h2 g[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) g[q] = __builtin_elementwise_fma(a[q], p[q], x[q]);
+  return f16x8{g[0][0], g[0][1], g[1][0], g[1][1], g[2][0], g[2][1], g[3][0], g[3][1]};
 }
 
 // packed (l, m) enumeration: reality -> m in [0, l], P = L(L+1)/2;
@@ -69,6 +124,20 @@ __device__ __forceinline__ void lm_of(int64_t p, int reality, int& l, int& m) {
     l = ll;
     m = (int)(p - (int64_t)ll * ll - ll);
   }
+}
+
+// (l, m) of packed row p < 2^31 in 32-bit arithmetic: the FP32 root of
+// moff(m) = p (moff as packed_moff) lands within a few m of the answer and
+// the two searches settle it.
+__device__ __forceinline__ void packed_lm_i32(int p, int L, int& l, int& m) {
+  const float b = 2.0f * (float)L + 1.0f;
+  const float d = fmaxf(fmaf(b, b, -8.0f * (float)p), 0.0f);
+  int mm = (int)((b - __builtin_amdgcn_sqrtf(d)) * 0.5f);
+  mm = min(max(mm, 0), L - 1);
+  while (mm > 0 && mm * L - (mm * (mm - 1)) / 2 > p) --mm;
+  while (mm + 1 < L && (mm + 1) * L - ((mm + 1) * mm) / 2 <= p) ++mm;
+  m = mm;
+  l = p - (mm * L - (mm * (mm - 1)) / 2) + mm;
 }
 
 // row p -> (l, m) and the element index of row p in h / out
@@ -108,53 +177,75 @@ __global__ void __launch_bounds__(kThreads) mlp_sumsq_kernel(MlpRowArgs a, doubl
 }
 
 // ---------------------------------------------------------------- inference
-// One wave handles 16 rows per pass, all three layers in registers:
-//  * layer 1 as 8 V_MFMA_F32_16X16X4F32, one per 16 hidden units (A = W1q,
-//    16 units x 4 features; B = the 16 rows' features, one per lane: lane
-//    r + 16 f holds feature f of row r), bias preloaded in the accumulator.
-//    The result holds unit 16 nt + 4 (lane/16) + i of row lane%16 in
-//    register i of tile nt.
-//  * layer 2 as 8 column tiles x 4 k-blocks of V_MFMA_F32_16X16X32_BF16.  Its
-//    A operand (A[i][k]: lane i + 16 (k/8 % 4), element k % 8 per 32-k
-//    block) is taken straight from layer 1's registers, which fixes the
-//    order of the hidden units along k: k = 32 kb + 8 q + e is unit
-//    32 kb + 16 (e/4) + 4 q + e%4.  W2q is staged in LDS in the B-operand
-//    layout with its columns in that order.
-//  * layer 3 as a 16-lane shuffle reduction.
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
+// One wave handles TILES tiles of 16 rows per pass, all three layers on the
+// FP16 matrix cores with the activations in registers (operand layouts of
+// V_MFMA_F32_16X16X{16,32}_F16: A[i][k] in lane i + 16 (k/E % 4), element
+// k % E, B[k][j] in lane j + 16 (k/E % 4), element k % E, E = 4 or 8 halves
+// per lane; result C[i][j] in lane j + 16 (i/4), register i % 4):
+//  * layer 1 as 8 V_MFMA_F32_16X16X16_F16, one per 16 hidden units: A = W1h
+//    (lanes 0-15 hold the unit's 4 weights, the rest zero), B = the rows'
+//    4 features (lane r < 16 holds row r's), bias preloaded in the
+//    accumulator.  The result holds unit 16 nt + 4 (lane/16) + i of row
+//    lane%16 in register i of tile nt.
+//  * layer 2 as 8 output tiles x 4 k-blocks of V_MFMA_F32_16X16X32_F16 with
+//    the weights as A and the activations as B.  B is taken straight from
+//    layer 1's registers (through 2 gelu), which fixes the order of the
+//    hidden units along k: k = 32 kb + 8 q + e is unit
+//    32 kb + 16 (e/4) + 4 q + e%4.  W2h / 2 is staged in LDS in the A layout
+//    with its columns in that order.  The result again holds unit
+//    16 ct + 4 (lane/16) + i of row lane%16 in register i of tile ct.
+//  * layer 3 the same way as 4 MFMAs with W3h / 2 padded to 16 outputs as A:
+//    lanes 0-15 end with the row's two outputs in registers 0 and 1.
+// Each weight fragment read from LDS serves every tile of the pass.
+typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kInfWaves = 4;
+constexpr int kInfTiles = 1;  // 16-row tiles per wave per pass
 
+template <int TILES>
 __global__ void __launch_bounds__(kInfWaves * 64) mlp_apply_kernel(MlpRowArgs a,
                                                                    const float* __restrict__ w,
                                                                    const double* sumsq) {
-  // B operand of column tile ct, k-block kb for lane l:
-  // W2q[16 ct + l%16][unit(32 kb + 8 (l/16) + e)], unit() as above
-  __shared__ bf16x8 wb[8][4][64];
-  // A operand of layer-1 tile nt for lane l: W1q[16 nt + l%16][l/16]
-  __shared__ float w1[8][64];
+  // A operand of layer-2 tile ct, k-block kb for lane l:
+  // W2h[16 ct + l%16][unit(32 kb + 8 (l/16) + e)] / 2, unit() as above
+  __shared__ f16x8 w2a[8][4][64];
+  // A operand of layer 3, k-block kb: W3h[l%16][unit(...)] / 2, zero for l%16 >= 2
+  __shared__ f16x8 w3a[4][64];
+  // A operand of layer-1 tile nt for lane l < 16: W1h[16 nt + l][0..3]
+  __shared__ f16x4 w1a[8][16];
   __shared__ __attribute__((aligned(16))) float b1[kH];
-  __shared__ float b2[kH], w3[2 * kH];
+  __shared__ __attribute__((aligned(16))) float b2[kH];
   __shared__ float b3[2];
   for (int e = threadIdx.x; e < 8 * 4 * 64; e += kInfWaves * 64) {
     const int ct = e >> 8, kb = (e >> 6) & 3, l = e & 63;
     const int j = 16 * ct + (l & 15), q4 = 4 * (l >> 4);
-    bf16x8 v;
+    f16x8 v;
 #pragma unroll
     for (int q = 0; q < 8; ++q)
-      v[q] = (__bf16)w[oW2 + j * kH + 32 * kb + 16 * (q >> 2) + q4 + (q & 3)];
-    wb[ct][kb][l] = v;
+      v[q] = (_Float16)(0.5f * w[oW2 + j * kH + 32 * kb + 16 * (q >> 2) + q4 + (q & 3)]);
+    w2a[ct][kb][l] = v;
   }
-  for (int e = threadIdx.x; e < 8 * 64; e += kInfWaves * 64) {
-    const int nt = e >> 6, l = e & 63;
-    w1[nt][l] = bf16_round(w[oW1 + (16 * nt + (l & 15)) * 4 + (l >> 4)]);
+  for (int e = threadIdx.x; e < 4 * 64; e += kInfWaves * 64) {
+    const int kb = e >> 6, l = e & 63;
+    const int o = l & 15, q4 = 4 * (l >> 4);
+    f16x8 v;
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      v[q] = o < 2 ? (_Float16)(0.5f * w[oW3 + o * kH + 32 * kb + 16 * (q >> 2) + q4 + (q & 3)])
+                   : (_Float16)0.0f;
+    w3a[kb][l] = v;
+  }
+  for (int e = threadIdx.x; e < 8 * 16; e += kInfWaves * 64) {
+    const int u = e;  // unit 16 nt + l
+    f16x4 v;
+#pragma unroll
+    for (int f = 0; f < 4; ++f) v[f] = (_Float16)w[oW1 + 4 * u + f];
+    w1a[u >> 4][u & 15] = v;
   }
   for (int e = threadIdx.x; e < kH; e += kInfWaves * 64) {
     b1[e] = w[oB1 + e];
     b2[e] = w[oB2 + e];
   }
-  for (int e = threadIdx.x; e < 2 * kH; e += kInfWaves * 64) w3[e] = bf16_round(w[oW3 + e]);
   if (threadIdx.x < 2) b3[threadIdx.x] = w[oB3 + threadIdx.x];
   __syncthreads();
 
@@ -167,97 +258,107 @@ __global__ void __launch_bounds__(kInfWaves * 64) mlp_apply_kernel(MlpRowArgs a,
   const cplx* __restrict__ h = a.h + (int64_t)b * a.map_stride;
   cplx* __restrict__ out = a.out + (int64_t)b * a.map_stride;
   const int row = lane & 15, kq = lane >> 4;
+  const f16x4 zero4 = {(_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f};
 
   const int64_t ntile = (a.P + 15) / 16;
-  for (int64_t tile = (int64_t)blockIdx.x * kInfWaves + wave; tile < ntile;
-       tile += (int64_t)gridDim.x * kInfWaves) {
-    // The weights are re-read from LDS every tile: kept in registers across
-    // tiles (the compiler's choice without this barrier) they take ~400
-    // registers and leave one wave per SIMD to hide every latency.
-    asm volatile("" ::: "memory");
-    // (l, m) of row `row`; packed: from the tile's first row, stepping over
-    // the m-blocks (block m holds l = m .. L - 1)
+  const int64_t npass = (ntile + TILES - 1) / TILES;
+  const int64_t stride = (int64_t)gridDim.x * kInfWaves;
+  // Row `row` of tile `tile`: (l, m), its element index and (lanes 0-15) its
+  // residual, loaded one pass ahead so that the load's latency overlaps the
+  // pass before.  Packed: (l, m) from the tile's first row, stepping over the
+  // m-blocks (block m holds l = m .. L - 1).
+  auto locate = [&](int64_t tile, int& l, int& m, int64_t& idx, cplx& v) {
     const int64_t p = tile * 16 + row;
-    int l = 0, m = 0;
-    int64_t idx = p;
+    l = 0;
+    m = 0;
+    idx = p;
+    v = cplx{0.0, 0.0};
+    if (p >= a.P) return;
     if (a.packed) {
       int l0, m0;
-      packed_lm(tile * 16, a.L, l0, m0);
+      packed_lm_i32((int)(tile * 16), a.L, l0, m0);
       l = l0 + row;
       m = m0;
       while (l >= a.L && m + 1 < a.L) {
         l -= a.L - m - 1;
         ++m;
       }
-    } else if (p < a.P) {
+    } else {
       idx = row_index(a, p, l, m);
     }
-    // feature kq of row `row`: (h_re / s, h_im / s, l / L, m / L)
-    float xf = 0.f;
-    if (p < a.P && l >= a.lmin) {
-      const cplx v = h[idx];
-      const float fl = (float)(kq == 2 ? l : m) * inv_L;
-      xf = kq == 0 ? (float)v.x * inv_sig : kq == 1 ? (float)v.y * inv_sig : fl;
+    if (kq == 0 && l >= a.lmin) v = h[idx];
+  };
+  int lr[TILES], mr[TILES];
+  int64_t ir[TILES];
+  cplx vr[TILES];
+  int64_t pass = (int64_t)blockIdx.x * kInfWaves + wave;
+#pragma unroll
+  for (int s = 0; s < TILES; ++s) locate(pass * TILES + s, lr[s], mr[s], ir[s], vr[s]);
+  for (; pass < npass; pass += stride) {
+    // The weights are re-read from LDS every pass: kept in registers across
+    // passes (the compiler's choice without this barrier) they take ~200
+    // registers and leave one wave per SIMD to hide every latency.
+    asm volatile("" ::: "memory");
+    int ln[TILES], mn[TILES];
+    int64_t in[TILES];
+    cplx vn[TILES];
+#pragma unroll
+    for (int s = 0; s < TILES; ++s) locate((pass + stride) * TILES + s, ln[s], mn[s], in[s], vn[s]);
+    f16x8 av[TILES][4];
+#pragma unroll
+    for (int s = 0; s < TILES; ++s) {
+      // lanes 0-15: the features (h_re / s, h_im / s, l / L, m / L) of row `row`
+      f16x4 xf = zero4;
+      if (kq == 0 && (pass * TILES + s) * 16 + row < a.P && lr[s] >= a.lmin)
+        xf = f16x4{(_Float16)((float)vr[s].x * inv_sig), (_Float16)((float)vr[s].y * inv_sig),
+                   (_Float16)((float)lr[s] * inv_L), (_Float16)((float)mr[s] * inv_L)};
+      // layer 1: z of unit 16 nt + 4 kq + i, row `row`, in register i of z1[nt]
+      f32x4 z1[8];
+#pragma unroll
+      for (int nt = 0; nt < 8; ++nt) {
+        z1[nt] = *reinterpret_cast<const f32x4*>(&b1[16 * nt + 4 * kq]);
+        const f16x4 wv = kq == 0 ? w1a[nt][row] : zero4;
+        z1[nt] = __builtin_amdgcn_mfma_f32_16x16x16f16(wv, xf, z1[nt], 0, 0, 0);
+      }
+#pragma unroll
+      for (int kb = 0; kb < 4; ++kb) av[s][kb] = gelu2x_operand(z1[2 * kb], z1[2 * kb + 1]);
     }
-    // layer 1: z of unit 16 nt + 4 kq + i, row `row`, in register i of z1[nt]
-    f32x4 z1[8];
+    // layer 2: unit 16 ct + 4 kq + i of row `row` in register i of acc[s][ct]
+    // (each tile's bias read from LDS by itself: one read instead of four moves)
+    f32x4 acc[TILES][8];
 #pragma unroll
-    for (int nt = 0; nt < 8; ++nt) {
-      z1[nt] = *reinterpret_cast<const f32x4*>(&b1[16 * nt + 4 * kq]);
-      z1[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(w1[nt][lane], xf, z1[nt], 0, 0, 0);
-    }
-    f32x4 acc[8];
-#pragma unroll
-    for (int ct = 0; ct < 8; ++ct) acc[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int kb = 0; kb < 4; ++kb) {
-      bf16x8 av;
-#pragma unroll
-      for (int q = 0; q < 8; ++q) av[q] = (__bf16)gelu(z1[2 * kb + (q >> 2)][q & 3]);
+    for (int s = 0; s < TILES; ++s) {
 #pragma unroll
       for (int ct = 0; ct < 8; ++ct)
-        acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, wb[ct][kb][lane], acc[ct], 0, 0, 0);
+        acc[s][ct] = *reinterpret_cast<const volatile f32x4*>(&b2[16 * ct + 4 * kq]);
     }
-    // result: register r of lane holds row 4 kq + r, column 16 ct + row
-    float y0[4] = {0.f, 0.f, 0.f, 0.f}, y1[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int ct = 0; ct < 8; ++ct) {
-      const int j = 16 * ct + row;
+    for (int kb = 0; kb < 4; ++kb) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const float a2 = gelu(acc[ct][r] + b2[j]);
-        y0[r] = fmaf(w3[j], a2, y0[r]);
-        y1[r] = fmaf(w3[kH + j], a2, y1[r]);
+      for (int ct = 0; ct < 8; ++ct) {
+        const f16x8 wa = w2a[ct][kb][lane];
+#pragma unroll
+        for (int s = 0; s < TILES; ++s)
+          acc[s][ct] = __builtin_amdgcn_mfma_f32_16x16x32_f16(wa, av[s][kb], acc[s][ct], 0, 0, 0);
       }
     }
+    // layer 3 and the write-out: lane `row` (kq = 0) holds the row's outputs
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int s = 0; s < TILES; ++s) {
+      f32x4 y = {b3[0], b3[1], 0.f, 0.f};
 #pragma unroll
-      for (int o = 8; o > 0; o >>= 1) {
-        y0[r] += __shfl_xor(y0[r], o);
-        y1[r] += __shfl_xor(y1[r], o);
-      }
-    }
-    // lane (kq, row = r) writes row 4 kq + r: pick register r == row; its
-    // (l, m) come from lane 4 kq + r, which computed that row's features
-    const int src = 4 * kq + (row & 3);
-    const int lq = __shfl(l, src), mq = __shfl(m, src);
-    if (row < 4) {
-      float o0 = y0[0], o1 = y1[0];
-#pragma unroll
-      for (int r = 1; r < 4; ++r)
-        if (row == r) {
-          o0 = y0[r];
-          o1 = y1[r];
-        }
-      const int64_t q = tile * 16 + 4 * kq + row;
-      if (q < a.P) {
-        const int64_t idx = a.packed ? q : (int64_t)lq * (2 * a.L - 1) + a.L - 1 + mq;
+      for (int kb = 0; kb < 4; ++kb)
+        y = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+            w3a[kb][lane], gelu2x_operand(acc[s][2 * kb], acc[s][2 * kb + 1]), y, 0, 0, 0);
+      const int64_t q = (pass * TILES + s) * 16 + row;
+      if (kq == 0 && q < a.P) {
+        const int lq = lr[s], mq = mr[s];
+        const int64_t idx = ir[s];
         if (lq < a.lmin) {
           out[idx] = cplx{0.0, 0.0};
         } else {
-          double re = sig * (double)(o0 + b3[0]);
-          double im = sig * (double)(o1 + b3[1]);
+          const double re = sig * (double)y[0];
+          double im = sig * (double)y[1];
           if (a.reality && mq == 0) im = 0.0;
           out[idx] = cplx{re, im};
           if (a.reality && !a.packed && mq > 0) {
@@ -266,6 +367,13 @@ __global__ void __launch_bounds__(kInfWaves * 64) mlp_apply_kernel(MlpRowArgs a,
           }
         }
       }
+    }
+#pragma unroll
+    for (int s = 0; s < TILES; ++s) {
+      lr[s] = ln[s];
+      mr[s] = mn[s];
+      ir[s] = in[s];
+      vr[s] = vn[s];
     }
   }
 }
@@ -311,19 +419,19 @@ __global__ void __launch_bounds__(kTT) mlp_grad_kernel(const float* __restrict__
                                                        int64_t n, float loss_scale,
                                                        float* __restrict__ partial,
                                                        double* __restrict__ loss_partial) {
-  __shared__ float W2s[kH * kW2S];  // W2q[j][k] at j * kW2S + k
+  __shared__ float W2s[kH * kW2S];  // W2h[j][k] at j * kW2S + k
   __shared__ float W1s[kH * 4], B1s[kH], B2s[kH], W3s[2 * kH], B3s[2];
   __shared__ float xs[kTR][4], a1s[kTR][kW2S], a2s[kTR][kW2S], dz2s[kTR][kW2S];
   __shared__ float douts[kTR][2];
   __shared__ double lred[kTT / 64];
   const int t = threadIdx.x;
-  for (int e = t; e < kH * kH; e += kTT) W2s[(e / kH) * kW2S + (e % kH)] = bf16_round(w[oW2 + e]);
-  for (int e = t; e < kH * 4; e += kTT) W1s[e] = bf16_round(w[oW1 + e]);
+  for (int e = t; e < kH * kH; e += kTT) W2s[(e / kH) * kW2S + (e % kH)] = f16_round(w[oW2 + e]);
+  for (int e = t; e < kH * 4; e += kTT) W1s[e] = f16_round(w[oW1 + e]);
   for (int e = t; e < kH; e += kTT) {
     B1s[e] = w[oB1 + e];
     B2s[e] = w[oB2 + e];
   }
-  for (int e = t; e < 2 * kH; e += kTT) W3s[e] = bf16_round(w[oW3 + e]);
+  for (int e = t; e < 2 * kH; e += kTT) W3s[e] = f16_round(w[oW3 + e]);
   if (t < 2) B3s[t] = w[oB3 + t];
   __syncthreads();
 
@@ -340,7 +448,7 @@ __global__ void __launch_bounds__(kTT) mlp_grad_kernel(const float* __restrict__
     const int64_t r0 = blk * kTR;
     if (t < kTR * 4) {
       const int r = t / 4, f = t % 4;
-      xs[r][f] = (r0 + r < n) ? X[(r0 + r) * 4 + f] : 0.f;
+      xs[r][f] = (r0 + r < n) ? f16_round(X[(r0 + r) * 4 + f]) : 0.f;
     }
     __syncthreads();
     // layer 1 (unit k, rows 2i + half)
@@ -351,7 +459,7 @@ __global__ void __launch_bounds__(kTT) mlp_grad_kernel(const float* __restrict__
       z1[i] = fmaf(W1s[4 * k + 3], xs[r][3],
                    fmaf(W1s[4 * k + 2], xs[r][2],
                         fmaf(W1s[4 * k + 1], xs[r][1], fmaf(W1s[4 * k], xs[r][0], B1s[k]))));
-      a1s[r][k] = bf16_round(gelu(z1[i]));
+      a1s[r][k] = f16_round(gelu(z1[i]));
     }
     __syncthreads();
     // layer 2 (unit j = k)
@@ -363,7 +471,7 @@ __global__ void __launch_bounds__(kTT) mlp_grad_kernel(const float* __restrict__
 #pragma unroll 8
       for (int q = 0; q < kH; ++q) s = fmaf(W2s[k * kW2S + q], a1s[r][q], s);
       z2[i] = s;
-      a2s[r][k] = gelu(s);
+      a2s[r][k] = f16_round(gelu(s));
     }
     __syncthreads();
     // layer 3 and the loss: threads 0..31 -> (row t/2, output t%2)
@@ -405,7 +513,7 @@ __global__ void __launch_bounds__(kTT) mlp_grad_kernel(const float* __restrict__
 #pragma unroll
       for (int i = 0; i < 64; ++i) gW2[i] = fmaf(dz2s[r][2 * i + half], a1, gW2[i]);
     }
-    // da1[r][k] = sum_j dz2[r][j] W2q[j][k]; back through gelu(z1)
+    // da1[r][k] = sum_j dz2[r][j] W2h[j][k]; back through gelu(z1)
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       const int r = 2 * i + half;
@@ -503,10 +611,12 @@ hipError_t launch_mlp_sumsq(const MlpRowArgs& a, double* sumsq, hipStream_t s) {
 hipError_t launch_mlp_apply(const MlpRowArgs& a, const float* w, const double* sumsq,
                             hipStream_t s) {
   if (a.batch == 0 || a.P == 0) return hipSuccess;
+  if (a.packed && a.P + 32 > INT32_MAX) return hipErrorInvalidValue;
   const int64_t ntile = (a.P + 15) / 16;
-  const int64_t want = (ntile + kInfWaves - 1) / kInfWaves;
+  const int64_t npass = (ntile + kInfTiles - 1) / kInfTiles;
+  const int64_t want = (npass + kInfWaves - 1) / kInfWaves;
   dim3 grid((unsigned)(want < 2048 ? want : 2048), a.batch);
-  hipLaunchKernelGGL(mlp_apply_kernel, grid, dim3(kInfWaves * 64), 0, s, a, w, sumsq);
+  hipLaunchKernelGGL(mlp_apply_kernel<kInfTiles>, grid, dim3(kInfWaves * 64), 0, s, a, w, sumsq);
   return hipGetLastError();
 }
 
@@ -538,6 +648,8 @@ hipError_t launch_mlp_adam(float* w, const float* grad, float* m1, float* m2, in
                      dim3(kThreads), 0, s, w, grad, m1, m2, step, lr, beta1, beta2, eps);
   return hipGetLastError();
 }
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 // One V_MFMA_F32_16X16X32_BF16 on caller operands (64 x 8 bf16 bit patterns
 // each), for checking the lane layouts: out[4 lane + r] = register r.

@@ -5,8 +5,10 @@
 * The hand-derived gradient against central finite differences of the loss
   (roundings switched off, float64), so a dropped term or a transposed weight
   fails.
-* GELU (tanh form) against its closed form and its derivative against finite
-  differences.
+* FP16 rounding against numpy's float16 conversion and torch's float16.
+* GELU (the polynomial stand-in for the tanh factor) within 2.7e-4 max(1, |x|)
+  of the tanh form's closed form, continuous at the clamp, and its
+  derivative against finite differences.
 * Feature rows: RMS scaling and (l, m) enumeration against a direct loop.
 """
 
@@ -28,13 +30,31 @@ def test_bf16_matches_torch():
     assert np.array_equal(got[fin], ref[fin])
 
 
-def test_gelu_closed_form_and_grad():
-    x = np.linspace(-6, 6, 241)
+def test_f16_matches_torch():
+    rng = np.random.default_rng(5)
+    x = np.concatenate([rng.standard_normal(10000) * 10.0 ** rng.integers(-6, 5, 10000),
+                        [1.0, 1.0 + 2 ** -11, 1.0 + 3 * 2 ** -11, -2.5, 6e4, 0.0]])
+    ref = torch.from_numpy(x.astype(np.float32)).to(torch.float16).to(torch.float64).numpy()
+    got = mlp.f16(x.astype(np.float32))
+    assert np.array_equal(got, ref)
+
+
+def test_gelu_against_tanh_form_and_grad():
+    x = np.linspace(-40, 40, 160001)
     ref = 0.5 * x * (1 + np.tanh(np.sqrt(2 / np.pi) * (x + 0.044715 * x ** 3)))
-    assert np.allclose(mlp.gelu(x), ref, atol=1e-14)
+    err = np.abs(mlp.gelu(x) - ref) / np.maximum(1.0, np.abs(x))
+    assert err.max() <= 2.7e-4
+    assert np.allclose(mlp.gelu_tanh(x), ref, atol=1e-14)
+    # continuous at the clamp |x| = 3.75 and odd-symmetric around x / 2
+    A = mlp.GELU_A
+    assert abs(mlp.gelu(A - 1e-12) - mlp.gelu(A + 1e-12)) < 1e-10
+    assert np.allclose(mlp.gelu(x) - mlp.gelu(-x), x, atol=1e-12)
+    # derivative against central differences away from the kinks at 0 and +-A
+    xs = np.linspace(-6, 6, 241)
+    xs = xs[(np.abs(xs) > 1e-3) & (np.abs(np.abs(xs) - A) > 1e-3)]
     h = 1e-6
-    fd = (mlp.gelu(x + h) - mlp.gelu(x - h)) / (2 * h)
-    assert np.allclose(mlp.gelu_grad(x), fd, atol=1e-8)
+    fd = (mlp.gelu(xs + h) - mlp.gelu(xs - h)) / (2 * h)
+    assert np.allclose(mlp.gelu_grad(xs), fd, atol=1e-8)
 
 
 def test_grad_matches_finite_differences():
