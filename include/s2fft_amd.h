@@ -311,6 +311,17 @@ int s2h_inverse_rings(const s2h_plan *plan, const void *flm, void *Y, int batch,
  * to and from the reference layout (unpack writes the zeros and the m < 0
  * half, (-1)^m conj). */
 enum { S2H_FLM_PACKED = 4 };
+/* S2H_SPECTRA_TILED (flags of s2h_forward_rings, s2h_inverse_rings,
+ * s2h_ring_residual_ex, s2h_ring_dot_ex): the spectra in blocks of 8 rows,
+ * element (row t, column c) of a map at ((t / 8) Mc + c) 8 + t % 8 complex128
+ * entries, so that the 8 rings of a block share a 128-byte line per column
+ * (the latitudinal kernels gather and scatter one column over every ring).
+ * Same size as the ring-major spectra: s2h_ring_spectra_bytes rounds the
+ * rows up to a multiple of 8 for both.  s2h_ring_residual_ex and
+ * s2h_ring_dot_ex take it for real maps only.  S2H_SPECTRA_G_RING_MAJOR
+ * (s2h_ring_residual_ex, with S2H_SPECTRA_TILED): G ring-major (as
+ * s2h_ring_analysis writes it), Y and R tiled; R may not be G. */
+enum { S2H_SPECTRA_TILED = 8, S2H_SPECTRA_G_RING_MAJOR = 16 };
 int s2h_flm_pack(int L, const void *flm, void *flm_packed, int batch, s2h_stream stream);
 int s2h_flm_unpack(int L, const void *flm_packed, void *flm, int batch, s2h_stream stream);
 int s2h_ring_residual(const s2h_plan *plan, const void *G, const void *Y, void *R, int batch,
@@ -348,6 +359,13 @@ int s2h_ring_residual_scaled(const s2h_plan *plan, const void *G, const void *Y,
 size_t s2h_ring_dot_workspace_bytes(const s2h_plan *plan, int batch);
 int s2h_ring_dot(const s2h_plan *plan, const void *Y, const double *num, double *pq,
                  double *alpha, void *work, int batch, s2h_stream stream);
+/* The same with layout flags (S2H_SPECTRA_TILED, S2H_SPECTRA_G_RING_MAJOR);
+ * flags 0 is s2h_ring_residual_scaled / s2h_ring_dot. */
+int s2h_ring_residual_ex(const s2h_plan *plan, const void *G, const void *Y, const double *alpha,
+                         void *R, double *sumsq, void *work, int batch, int flags,
+                         s2h_stream stream);
+int s2h_ring_dot_ex(const s2h_plan *plan, const void *Y, const double *num, double *pq,
+                    double *alpha, void *work, int batch, int flags, s2h_stream stream);
 size_t s2h_flm_dots_workspace_bytes(int nv, int batch);
 int s2h_flm_dots(const void *z, const void *const *v, int nv, int64_t n, int packed_L, int batch,
                  double *out, void *work, s2h_stream stream);

@@ -155,7 +155,7 @@ bool valid_precision(int precision) {
 // The ring FFTs of f (weights and ring phases fused) into `out`: the plan's
 // forward grid, or its input rows when the forward resamples.
 int ring_analysis_stage(const s2h_plan* p, const void* f, int f_is_complex, cplx* out, int batch,
-                        hipStream_t s, bool raw = false) {
+                        hipStream_t s, bool raw = false, int64_t out_map_stride = 0) {
   const bool real = p->d.reality != 0;
   const bool resample = p->fwd_kind != 0;
   RingFwdArgs r{};
@@ -173,7 +173,9 @@ int ring_analysis_stage(const s2h_plan* p, const void* f, int f_is_complex, cplx
   }
   r.map_stride = p->npix;
   r.out = out;
-  r.out_map_stride = (int64_t)(resample ? p->fwd_in_rows : p->fwd_rows) * p->Mc;
+  r.out_map_stride = out_map_stride > 0
+                         ? out_map_stride
+                         : (int64_t)(resample ? p->fwd_in_rows : p->fwd_rows) * p->Mc;
   r.ncol = p->Mc;
   r.m_first = p->m_lo;
   // raw: no quadrature weight and no ring phase (a stored precompute kernel
@@ -191,9 +193,16 @@ int ring_analysis_stage(const s2h_plan* p, const void* f, int f_is_complex, cplx
 // conjugate fill): flm = or += the forward transform of ftm.  The BF16
 // kernels with one ring group per m write every entry themselves (zeros and
 // the m < 0 mirror); otherwise flm is zeroed first and filled after.
+// The ring spectra of the C ABI (s2h_ring_*): map stride and layout.  The
+// internal stages default to the forward grid's rows, ring-major.
+struct SpecView {
+  int64_t map_stride;
+  int tiled;
+};
+
 int lat_forward_stage(const s2h_plan* p, const cplx* ftm, void* flm, int batch, int precision,
                       bool accumulate, hipStream_t s, bool packed = false,
-                      const PlaneSet* ps = nullptr) {
+                      const PlaneSet* ps = nullptr, const SpecView* sv = nullptr) {
   const bool mixed = precision == S2H_PREC_MIXED;
   const bool bf16 = precision == S2H_PREC_BF16 || precision == S2H_PREC_BF16X1;
   const int split = precision == S2H_PREC_BF16X1 ? 1 : 3;
@@ -206,9 +215,10 @@ int lat_forward_stage(const s2h_plan* p, const cplx* ftm, void* flm, int batch, 
   a.flm_packed = packed ? 1 : 0;
   a.flm_map_stride = fs;
   a.ftm = const_cast<cplx*>(ftm);
-  a.ftm_map_stride = ftm_stride;
+  a.ftm_map_stride = sv ? sv->map_stride : ftm_stride;
   a.ftm_ncol = p->Mc;
   a.ftm_m_first = p->m_lo;
+  a.ftm_tiled = sv ? sv->tiled : 0;
   a.lmin_out = std::max(std::abs(p->d.spin), p->d.L_lower);
   a.accumulate = accumulate ? 1 : 0;
   if (ps) a.planes = ps->fwd;
@@ -267,7 +277,8 @@ int lat_forward_stage(const s2h_plan* p, const cplx* ftm, void* flm, int batch, 
 
 // Latitudinal synthesis (+ pole rows) into inverse-grid ring spectra ftm.
 int lat_inverse_stage(const s2h_plan* p, const void* flm, cplx* ftm, int batch, int precision,
-                      hipStream_t s, bool packed = false, const PlaneSet* ps = nullptr) {
+                      hipStream_t s, bool packed = false, const PlaneSet* ps = nullptr,
+                      const SpecView* sv = nullptr) {
   const bool mixed = precision == S2H_PREC_MIXED;
   const bool bf16 = precision == S2H_PREC_BF16 || precision == S2H_PREC_BF16X1;
   const int64_t ftm_stride = (int64_t)p->inv_rows * p->Mc;
@@ -277,9 +288,10 @@ int lat_inverse_stage(const s2h_plan* p, const void* flm, cplx* ftm, int batch, 
   a.flm_packed = packed ? 1 : 0;
   if (packed) a.flm_map_stride = (int64_t)p->d.L * (p->d.L + 1) / 2;
   a.ftm = ftm;
-  a.ftm_map_stride = ftm_stride;
+  a.ftm_map_stride = sv ? sv->map_stride : ftm_stride;
   a.ftm_ncol = p->Mc;
   a.ftm_m_first = p->m_lo;
+  a.ftm_tiled = sv ? sv->tiled : 0;
   if (ps) a.planes = ps->inv;
   {
     ProfScope prof(bf16 ? S2H_KERNEL_LAT_INVERSE_BF16
@@ -458,9 +470,22 @@ extern "C" int s2h_inverse_ex(const s2h_plan* p, const void* flm, void* f, void*
 // The transforms split at the ring spectra, for refinement in spectral space:
 // F(f - I(x)) = Lat_fwd(G - A(Lat_inv(x))) with G the ring spectra of f and A
 // the ring FFT of the ring iFFT, applied without either FFT.
+namespace {
+// map stride of the C ABI's ring spectra (rows padded to whole row blocks)
+int64_t spec_stride(const s2h_plan* p) { return (int64_t)p->spec_rows * p->Mc; }
+SpecView spec_view(const s2h_plan* p, int flags) {
+  return SpecView{spec_stride(p), (flags & S2H_SPECTRA_TILED) ? 1 : 0};
+}
+// zeros into the padding rows of C-ABI spectra the call wrote
+int zero_spec_pad(const s2h_plan* p, void* spec, int tiled, int batch, hipStream_t s) {
+  return hip_status(launch_spec_pad_zero(static_cast<cplx*>(spec), spec_stride(p), p->Mc,
+                                         p->fwd_rows, p->spec_rows, tiled, batch, s));
+}
+}  // namespace
+
 extern "C" size_t s2h_ring_spectra_bytes(const s2h_plan* p, int batch) {
   if (!p || batch < 0 || !spectral_ok(p)) return 0;
-  return (size_t)batch * (size_t)p->fwd_rows * (size_t)p->Mc * sizeof(cplx);
+  return (size_t)batch * (size_t)spec_stride(p) * sizeof(cplx);
 }
 
 extern "C" int s2h_ring_analysis(const s2h_plan* p, const void* f, int f_is_complex, void* G,
@@ -470,14 +495,17 @@ extern "C" int s2h_ring_analysis(const s2h_plan* p, const void* f, int f_is_comp
   if (batch == 0) return S2H_OK;
   int st = check_device(p);
   if (st) return st;
-  return ring_analysis_stage(p, f, f_is_complex, static_cast<cplx*>(G), batch,
-                             static_cast<hipStream_t>(stream));
+  if ((st = ring_analysis_stage(p, f, f_is_complex, static_cast<cplx*>(G), batch,
+                                static_cast<hipStream_t>(stream), false, spec_stride(p))))
+    return st;
+  return zero_spec_pad(p, G, 0, batch, static_cast<hipStream_t>(stream));
 }
 
 extern "C" int s2h_forward_rings(const s2h_plan* p, const void* G, void* flm, int batch,
                                  int precision, int flags, s2h_stream stream) {
   if (!p || !G || !flm || batch < 0) return S2H_ERR_INVALID;
-  if (!valid_precision(precision) || (flags & ~(S2H_FWD_ACCUMULATE | S2H_FLM_PACKED)))
+  if (!valid_precision(precision) ||
+      (flags & ~(S2H_FWD_ACCUMULATE | S2H_FLM_PACKED | S2H_SPECTRA_TILED)))
     return S2H_ERR_INVALID;
   const bool packed = (flags & S2H_FLM_PACKED) != 0;
   if (packed && !p->d.reality) return S2H_ERR_INVALID;
@@ -487,22 +515,26 @@ extern "C" int s2h_forward_rings(const s2h_plan* p, const void* G, void* flm, in
   if (st) return st;
   hipStream_t s = static_cast<hipStream_t>(stream);
   const bool accumulate = (flags & S2H_FWD_ACCUMULATE) != 0;
+  const SpecView sv = spec_view(p, flags);
   return lat_forward_stage(p, static_cast<const cplx*>(G), flm, batch, precision, accumulate, s,
-                           packed);
+                           packed, nullptr, &sv);
 }
 
 extern "C" int s2h_inverse_rings(const s2h_plan* p, const void* flm, void* Y, int batch,
                                  int precision, int flags, s2h_stream stream) {
   if (!p || !flm || !Y || batch < 0 || !valid_precision(precision)) return S2H_ERR_INVALID;
-  if (flags & ~S2H_FLM_PACKED) return S2H_ERR_INVALID;
+  if (flags & ~(S2H_FLM_PACKED | S2H_SPECTRA_TILED)) return S2H_ERR_INVALID;
   const bool packed = (flags & S2H_FLM_PACKED) != 0;
   if (packed && !p->d.reality) return S2H_ERR_INVALID;
   if (!spectral_ok(p)) return S2H_ERR_UNSUPPORTED;
   if (batch == 0) return S2H_OK;
   int st = check_device(p);
   if (st) return st;
-  return lat_inverse_stage(p, flm, static_cast<cplx*>(Y), batch, precision,
-                           static_cast<hipStream_t>(stream), packed);
+  const SpecView sv = spec_view(p, flags);
+  if ((st = lat_inverse_stage(p, flm, static_cast<cplx*>(Y), batch, precision,
+                              static_cast<hipStream_t>(stream), packed, nullptr, &sv)))
+    return st;
+  return zero_spec_pad(p, Y, sv.tiled, batch, static_cast<hipStream_t>(stream));
 }
 
 extern "C" int s2h_flm_pack(int L, const void* flm, void* flm_p, int batch, s2h_stream stream) {
@@ -526,7 +558,7 @@ RingAliasArgs alias_args(const s2h_plan* p, const void* G, const void* Y, void* 
   a.y = static_cast<const cplx*>(Y);
   a.g = static_cast<const cplx*>(G);
   a.r = static_cast<cplx*>(R);
-  a.map_stride = (int64_t)p->fwd_rows * p->Mc;
+  a.map_stride = spec_stride(p);
   a.ncol = p->Mc;
   a.m_first = p->m_lo;
   a.herm = p->d.reality ? 1 : 0;
@@ -537,23 +569,68 @@ RingAliasArgs alias_args(const s2h_plan* p, const void* G, const void* Y, void* 
 
 }  // namespace
 
-extern "C" int s2h_ring_residual_scaled(const s2h_plan* p, const void* G, const void* Y,
-                                        const double* alpha, void* R, double* sumsq, void* work,
-                                        int batch, s2h_stream stream) {
+namespace {
+
+// Tiled alias arguments (real maps only: Hermitian spectra, m in [0, L)).
+RingAliasTiledArgs alias_tiled_args(const s2h_plan* p, const void* G, const void* Y, void* R,
+                                    bool g_tiled) {
+  RingAliasTiledArgs a{};
+  a.rowjobs = p->fwd_rowjobs;
+  a.nblocks = p->spec_rows / kSpecBlock;
+  a.belt_blocks = p->spec_belt_blocks;
+  a.n_belt = p->n_belt_blocks;
+  a.polar_blocks = p->spec_polar_blocks;
+  a.n_polar = p->n_polar_blocks;
+  a.y = static_cast<const cplx*>(Y);
+  a.g = static_cast<const cplx*>(G);
+  a.r = static_cast<cplx*>(R);
+  a.g_tiled = g_tiled ? 1 : 0;
+  a.map_stride = spec_stride(p);
+  a.ncol = p->Mc;
+  a.apply_phase = p->d.sampling == S2H_HEALPIX;
+  a.L = p->d.L;
+  return a;
+}
+
+}  // namespace
+
+extern "C" int s2h_ring_residual_ex(const s2h_plan* p, const void* G, const void* Y,
+                                    const double* alpha, void* R, double* sumsq, void* work,
+                                    int batch, int flags, s2h_stream stream) {
   if (!p || !G || !Y || !R || batch < 0 || Y == R) return S2H_ERR_INVALID;
+  if (flags & ~(S2H_SPECTRA_TILED | S2H_SPECTRA_G_RING_MAJOR)) return S2H_ERR_INVALID;
+  const bool tiled = (flags & S2H_SPECTRA_TILED) != 0;
+  const bool g_rm = (flags & S2H_SPECTRA_G_RING_MAJOR) != 0;
+  if (g_rm && (!tiled || G == R)) return S2H_ERR_INVALID;
   if (!spectral_ok(p)) return S2H_ERR_UNSUPPORTED;
+  if (tiled && (!p->d.reality || p->m_lo != 0)) return S2H_ERR_UNSUPPORTED;
   if (batch == 0) return S2H_OK;
   if (sumsq && !work) return S2H_ERR_INVALID;
   int st = check_device(p);
   if (st) return st;
-  RingAliasArgs a = alias_args(p, G, Y, R);
-  a.alpha = alpha;
   ProfScope ps(S2H_KERNEL_RING_RESIDUAL, static_cast<hipStream_t>(stream));
   const hipStream_t s = static_cast<hipStream_t>(stream);
-  if (sumsq)
-    return hip_status(launch_ring_alias_sumsq(a, p->fwd_set, batch, static_cast<double*>(work),
-                                              sumsq, s));
-  return hip_status(launch_ring_alias(a, p->fwd_set, batch, s));
+  if (tiled) {
+    RingAliasTiledArgs a = alias_tiled_args(p, G, Y, R, !g_rm);
+    a.alpha = alpha;
+    st = sumsq ? hip_status(
+                     launch_ring_alias_tiled_sumsq(a, batch, static_cast<double*>(work), sumsq, s))
+               : hip_status(launch_ring_alias_tiled(a, batch, s));
+  } else {
+    RingAliasArgs a = alias_args(p, G, Y, R);
+    a.alpha = alpha;
+    st = sumsq ? hip_status(launch_ring_alias_sumsq(a, p->fwd_set, batch,
+                                                    static_cast<double*>(work), sumsq, s))
+               : hip_status(launch_ring_alias(a, p->fwd_set, batch, s));
+  }
+  if (st || R == G) return st;
+  return zero_spec_pad(p, R, tiled ? 1 : 0, batch, s);
+}
+
+extern "C" int s2h_ring_residual_scaled(const s2h_plan* p, const void* G, const void* Y,
+                                        const double* alpha, void* R, double* sumsq, void* work,
+                                        int batch, s2h_stream stream) {
+  return s2h_ring_residual_ex(p, G, Y, alpha, R, sumsq, work, batch, 0, stream);
 }
 
 extern "C" int s2h_ring_residual(const s2h_plan* p, const void* G, const void* Y, void* R,
@@ -563,21 +640,39 @@ extern "C" int s2h_ring_residual(const s2h_plan* p, const void* G, const void* Y
 
 extern "C" size_t s2h_ring_dot_workspace_bytes(const s2h_plan* p, int batch) {
   if (!p || batch <= 0 || !spectral_ok(p)) return 0;
-  return (size_t)batch * (size_t)ring_dot_jobs(p->fwd_set) * sizeof(double);
+  // one partial per ring job (ring-major) or per row block (tiled)
+  const int parts =
+      std::max(ring_dot_jobs(p->fwd_set), ring_dot_tiled_parts(p->spec_rows / kSpecBlock));
+  return (size_t)batch * (size_t)parts * sizeof(double);
 }
 
-extern "C" int s2h_ring_dot(const s2h_plan* p, const void* Y, const double* num, double* pq,
-                            double* alpha, void* work, int batch, s2h_stream stream) {
+extern "C" int s2h_ring_dot_ex(const s2h_plan* p, const void* Y, const double* num, double* pq,
+                               double* alpha, void* work, int batch, int flags,
+                               s2h_stream stream) {
   if (!p || !Y || batch < 0 || (!pq && !alpha) || (alpha && !num)) return S2H_ERR_INVALID;
+  if (flags & ~S2H_SPECTRA_TILED) return S2H_ERR_INVALID;
+  const bool tiled = (flags & S2H_SPECTRA_TILED) != 0;
   if (!spectral_ok(p)) return S2H_ERR_UNSUPPORTED;
+  if (tiled && (!p->d.reality || p->m_lo != 0)) return S2H_ERR_UNSUPPORTED;
   if (batch == 0) return S2H_OK;
   if (!work) return S2H_ERR_INVALID;
   int st = check_device(p);
   if (st) return st;
-  const RingAliasArgs a = alias_args(p, Y, Y, nullptr);
   ProfScope ps(S2H_KERNEL_RING_RESIDUAL, static_cast<hipStream_t>(stream));
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  if (tiled) {
+    const RingAliasTiledArgs a = alias_tiled_args(p, Y, Y, nullptr, true);
+    return hip_status(
+        launch_ring_dot_tiled(a, batch, static_cast<double*>(work), num, pq, alpha, s));
+  }
+  const RingAliasArgs a = alias_args(p, Y, Y, nullptr);
   return hip_status(launch_ring_dot(a, p->fwd_set, batch, static_cast<double*>(work), num, pq,
-                                    alpha, static_cast<hipStream_t>(stream)));
+                                    alpha, s));
+}
+
+extern "C" int s2h_ring_dot(const s2h_plan* p, const void* Y, const double* num, double* pq,
+                            double* alpha, void* work, int batch, s2h_stream stream) {
+  return s2h_ring_dot_ex(p, Y, num, pq, alpha, work, batch, 0, stream);
 }
 
 namespace {

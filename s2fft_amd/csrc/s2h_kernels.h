@@ -53,6 +53,38 @@ struct RingInvArgs {
 // fold of Y's m-spectrum onto n_t bins with e^{+i m phi0} (as the inverse
 // FFT reads it), then the periodic extension times n_t w_t e^{-i m phi0} (as
 // the forward FFT writes it).  Same layouts as RingInvArgs.in / RingFwdArgs.out.
+// Ring-spectra addressing within one map.  Ring-major: element (row t,
+// column c) at t ncol + c.  Tiled: rows in blocks of 8, at
+// ((t / 8) ncol + c) 8 + t % 8, so that for every column the 8 rings of a
+// block share one 128-byte line (the latitudinal kernels gather and scatter
+// one column over all rings).
+constexpr int kSpecBlock = 8;
+__host__ __device__ __forceinline__ int64_t spec_row(int t, int ncol, int tiled) {
+  return tiled ? (((int64_t)(t >> 3) * ncol) << 3) + (t & 7) : (int64_t)t * ncol;
+}
+__host__ __device__ __forceinline__ int spec_cstride(int tiled) { return tiled ? kSpecBlock : 1; }
+
+// R = G - alpha A(Y) / the CG dot on tiled spectra of real maps (Hermitian,
+// m in [0, L)): one workgroup per block of 8 rows and map.
+struct RingAliasTiledArgs {
+  const RingJob* rowjobs;   // ring of each spectra row (nphi 0: padding row)
+  int nblocks;              // rows / 8
+  const int* belt_blocks;   // blocks whose rows all have n >= 2L - 1 (no alias)
+  int n_belt;
+  const int* polar_blocks;  // the other blocks
+  int n_polar;
+  const cplx* y;            // tiled
+  const cplx* g;            // tiled, or ring-major (g_tiled 0); may equal r when tiled
+  cplx* r;                  // tiled
+  int g_tiled;
+  int64_t map_stride;       // rows * ncol, all three arrays
+  int ncol;                 // L
+  int apply_phase;
+  int L;
+  const double* alpha;      // per-map scale (null: 1)
+  double* part;             // [map][block][slot] partial sums: the dot (dot mode) or |R|^2
+};
+
 struct RingAliasArgs {
   const RingJob* jobs;
   const cplx* y;            // spectra to alias (m in [-(L-1), L-1], Hermitian if herm)
@@ -91,11 +123,12 @@ struct LatArgs {
   // harmonic side: flm[b][l][L-1+m]
   cplx* flm;
   int64_t flm_map_stride;
-  // ring side: ftm[b][row][m - ftm_m_first]
+  // ring side: ftm[b] + spec_row(row) + (m - ftm_m_first) spec_cstride
   cplx* ftm;
   int64_t ftm_map_stride;
   int ftm_ncol;
   int ftm_m_first;
+  int ftm_tiled;            // ring spectra in blocks of 8 rings (spec_row)
   int groups;               // ring groups per m (forward); >1 accumulates atomically
   int accumulate;           // forward: add into flm instead of overwriting it
   int fill;                 // forward, one ring group per m: also write zeros below
@@ -142,6 +175,13 @@ hipError_t launch_ring_alias(const RingAliasArgs& a, const JobSet& js, int batch
 int ring_dot_jobs(const JobSet& js);
 // launch_ring_alias that also writes sumsq[b] = sum |R|^2 of map b (part as
 // for launch_ring_dot)
+hipError_t launch_ring_alias_tiled(const RingAliasTiledArgs& a, int batch, hipStream_t s);
+// partial sums per map of the tiled alias kernel (its workspace)
+int ring_dot_tiled_parts(int nblocks);
+hipError_t launch_ring_alias_tiled_sumsq(const RingAliasTiledArgs& a, int batch, double* part,
+                                         double* sumsq, hipStream_t s);
+hipError_t launch_ring_dot_tiled(const RingAliasTiledArgs& a, int batch, double* part,
+                                 const double* num, double* pq, double* alpha, hipStream_t s);
 hipError_t launch_ring_alias_sumsq(const RingAliasArgs& a, const JobSet& js, int batch,
                                    double* part, double* sumsq, hipStream_t s);
 hipError_t launch_ring_dot(const RingAliasArgs& a, const JobSet& js, int batch, double* part,
@@ -259,6 +299,8 @@ __host__ __device__ __forceinline__ int pole_spin(const PoleArgs& a, int64_t b) 
 hipError_t launch_pole_inverse(const PoleArgs& a, hipStream_t s);
 hipError_t launch_pole_forward(const PoleArgs& a, hipStream_t s);
 hipError_t launch_zero(void* p, int64_t ndoubles, hipStream_t s);
+hipError_t launch_spec_pad_zero(cplx* spec, int64_t map_stride, int ncol, int row0, int rows,
+                                int tiled, int batch, hipStream_t s);
 constexpr int kFlmDotMax = 4;      // vectors per launch_flm_dots
 constexpr int kFlmDotBlocks = 256; // slices per map of launch_flm_dots
 struct FlmDotArgs {

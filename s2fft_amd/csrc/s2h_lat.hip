@@ -157,14 +157,14 @@ __global__ void __launch_bounds__(kThreads) lat_inverse_kernel(LatArgs a) {
   // paired rings: ring b takes (-1)^(l + m) d, i.e. the parity of l - ell0,
   // with the roles swapped when ell0 + m is odd
   const bool flip = ((l0 + m) & 1) != 0;
-  cplx* out = a.ftm + (int64_t)b0 * a.ftm_map_stride + (m - a.ftm_m_first);
+  cplx* out = a.ftm + (int64_t)b0 * a.ftm_map_stride + (int64_t)(m - a.ftm_m_first) * spec_cstride(a.ftm_tiled);
 #pragma unroll
   for (int j = 0; j < NB; ++j) {
     if (j < nb) {
       cplx* o = out + (int64_t)j * a.ftm_map_stride;
       const cplx ae = flip ? acc[1][j] : acc[0][j], ao = flip ? acc[0][j] : acc[1][j];
-      o[(int64_t)rg.ta * a.ftm_ncol] = cadd(ae, ao);
-      if (PAIRED && rg.tb >= 0 && rg.tb != rg.ta) o[(int64_t)rg.tb * a.ftm_ncol] = csub(ae, ao);
+      o[spec_row(rg.ta, a.ftm_ncol, a.ftm_tiled)] = cadd(ae, ao);
+      if (PAIRED && rg.tb >= 0 && rg.tb != rg.ta) o[spec_row(rg.tb, a.ftm_ncol, a.ftm_tiled)] = csub(ae, ao);
     }
   }
 }
@@ -196,7 +196,7 @@ __global__ void __launch_bounds__(kThreads) lat_forward_kernel(LatArgs a) {
   const double bsign = m < 0 ? -1.0 : 1.0;
   const Coef* __restrict__ cf = (pl ? pl->coef : a.coef) + (int64_t)am * a.L;
   const int L = a.L;
-  const cplx* ftm = a.ftm + b * a.ftm_map_stride + (m - a.ftm_m_first);
+  const cplx* ftm = a.ftm + b * a.ftm_map_stride + (int64_t)(m - a.ftm_m_first) * spec_cstride(a.ftm_tiled);
   double* rw = red[wave];
 
   double x[R], d[R], dm1[R];
@@ -214,9 +214,9 @@ __global__ void __launch_bounds__(kThreads) lat_forward_kernel(LatArgs a) {
       const LatRing rg = a.rings[p];
       x[r] = rg.x;
       start_value(st, rg.c2, rg.s2, d[r], ls[r]);
-      const cplx fa = ftm[(int64_t)rg.ta * a.ftm_ncol];
+      const cplx fa = ftm[spec_row(rg.ta, a.ftm_ncol, a.ftm_tiled)];
       cplx fb = {0.0, 0.0};
-      if (PAIRED && rg.tb >= 0 && rg.tb != rg.ta) fb = ftm[(int64_t)rg.tb * a.ftm_ncol];
+      if (PAIRED && rg.tb >= 0 && rg.tb != rg.ta) fb = ftm[spec_row(rg.tb, a.ftm_ncol, a.ftm_tiled)];
       se[r] = cadd(fa, fb);
       so[r] = csub(fa, fb);
     }

@@ -165,7 +165,7 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
   for (int t = 0; t < kFTiles; ++t) {
     const int amap = b0 + 8 * t + (row >> 1);
     const bool have = amap < a.batch;
-    const cplx* ftm = a.ftm + (int64_t)amap * a.ftm_map_stride + (m - a.ftm_m_first);
+    const cplx* ftm = a.ftm + (int64_t)amap * a.ftm_map_stride + (int64_t)(m - a.ftm_m_first) * spec_cstride(a.ftm_tiled);
 #pragma unroll
     for (int kb = 0; kb < KB; ++kb) {
 #pragma unroll
@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
           const int2 tt = *reinterpret_cast<const int2*>(&a.rings[q].ta);
           const int tr = comp ? tt.y : tt.x;
           if (comp == 0 || (PAIRED && tt.y >= 0 && tt.y != tt.x))
-            v = ftm[(int64_t)tr * a.ftm_ncol];
+            v = ftm[spec_row(tr, a.ftm_ncol, a.ftm_tiled)];
         }
         // lane 2r keeps fa.x and receives fb.x; lane 2r + 1 keeps fb.y and
         // receives fa.y (a zero fb where the ring is unpaired)
@@ -585,15 +585,15 @@ __global__ void __launch_bounds__(kIT, 3) lat_inverse_bf16_kernel(LatArgs a) {
     const int bm = b0 + t * 8 + (row >> 1);
     if (bm >= a.batch) continue;
     double* out = reinterpret_cast<double*>(a.ftm + (int64_t)bm * a.ftm_map_stride +
-                                            (m - a.ftm_m_first)) + (row & 1);
+                                            (int64_t)(m - a.ftm_m_first) * spec_cstride(a.ftm_tiled)) + (row & 1);
 #pragma unroll
     for (int q = 0; q < 4; ++q)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int ra = ta[4 * q + r], rb = tb[4 * q + r];
         const double e = (double)acc[q][t][0][r], o = (double)acc[q][t][1][r];
-        if (ra >= 0) out[2 * (int64_t)ra * a.ftm_ncol] = e + o;
-        if (rb >= 0) out[2 * (int64_t)rb * a.ftm_ncol] = e - o;
+        if (ra >= 0) out[2 * spec_row(ra, a.ftm_ncol, a.ftm_tiled)] = e + o;
+        if (rb >= 0) out[2 * spec_row(rb, a.ftm_ncol, a.ftm_tiled)] = e - o;
       }
   }
 }

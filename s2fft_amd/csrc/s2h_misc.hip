@@ -112,6 +112,28 @@ __global__ void __launch_bounds__(kThreads) zero_kernel(double* p, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) p[i] = 0.0;
 }
 
+// Zeros of the padding rows [row0, rows) of ring spectra (either layout):
+// the spectra producers write ring rows only.
+__global__ void __launch_bounds__(kThreads) spec_pad_zero_kernel(cplx* spec, int64_t map_stride,
+                                                                 int ncol, int row0, int rows,
+                                                                 int tiled) {
+  cplx* base = spec + (int64_t)blockIdx.x * map_stride;
+  const int n = (rows - row0) * ncol;
+  const int cs = spec_cstride(tiled);
+  for (int i = threadIdx.x; i < n; i += kThreads) {
+    const int t = row0 + i / ncol, c = i % ncol;
+    base[spec_row(t, ncol, tiled) + (int64_t)c * cs] = cplx{0.0, 0.0};
+  }
+}
+
+hipError_t launch_spec_pad_zero(cplx* spec, int64_t map_stride, int ncol, int row0, int rows,
+                                int tiled, int batch, hipStream_t s) {
+  if (batch <= 0 || rows <= row0 || ncol <= 0) return hipSuccess;
+  hipLaunchKernelGGL(spec_pad_zero_kernel, dim3(batch), dim3(kThreads), 0, s, spec, map_stride,
+                     ncol, row0, rows, tiled);
+  return hipGetLastError();
+}
+
 hipError_t launch_zero(void* p, int64_t ndoubles, hipStream_t s) {
   if (ndoubles <= 0) return hipSuccess;
   int64_t blocks = (ndoubles + kThreads - 1) / kThreads;

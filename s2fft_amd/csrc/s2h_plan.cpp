@@ -604,6 +604,27 @@ static int plan_build(s2h_plan* p) {
   p->n_fwd_rings = (int)fwd_r.size();
   if ((st = make_set(p, inv_j, &p->inv_set))) return st;
   if ((st = make_set(p, fwd_j, &p->fwd_set))) return st;
+  {
+    // forward ring of each ring-spectra row, padded to whole row blocks
+    p->spec_rows = (p->fwd_rows + kSpecBlock - 1) / kSpecBlock * kSpecBlock;
+    std::vector<RingJob> rj(p->spec_rows, RingJob{});
+    for (const RingJob& j : fwd_j)
+      if (j.row >= 0 && j.row < p->spec_rows) rj[j.row] = j;
+    if ((st = upload(p, rj, &p->fwd_rowjobs))) return st;
+    std::vector<int> belt, polar;
+    for (int blk = 0; blk < p->spec_rows / kSpecBlock; ++blk) {
+      bool no_alias = true;
+      for (int q = 0; q < kSpecBlock; ++q) {
+        const int n = rj[blk * kSpecBlock + q].nphi;
+        if (n > 0 && n < 2 * L - 1) no_alias = false;
+      }
+      (no_alias ? belt : polar).push_back(blk);
+    }
+    p->n_belt_blocks = (int)belt.size();
+    p->n_polar_blocks = (int)polar.size();
+    if (!belt.empty() && (st = upload(p, belt, &p->spec_belt_blocks))) return st;
+    if (!polar.empty() && (st = upload(p, polar, &p->spec_polar_blocks))) return st;
+  }
   if ((st = upload(p, msf, &p->fwd_mstart))) return st;
   if ((st = upload(p, msi, &p->inv_mstart))) return st;
   if ((st = upload(p, cf, &p->fwd_coef))) return st;

@@ -46,6 +46,17 @@ struct s2h_plan {
   s2h::cplx* chirp = nullptr;
   int64_t npix = 0;   // pixels per map (forward input / inverse output)
 
+  // ring spectra of the C ABI: fwd_rows rounded up to a whole block of
+  // kSpecBlock rows, and the forward ring of each of those rows (nphi 0 for
+  // rows without one) for the tiled alias kernel
+  int spec_rows = 0;
+  s2h::RingJob* fwd_rowjobs = nullptr;
+  // row blocks without aliasing (every ring n >= 2L - 1) and the others
+  int* spec_belt_blocks = nullptr;
+  int n_belt_blocks = 0;
+  int* spec_polar_blocks = nullptr;
+  int n_polar_blocks = 0;
+
   // per-row tables of the forward grid (fwd_rows) and the inverse grid
   // (inv_rows), for the precompute kernels: colatitudes, the forward
   // quadrature weight of each forward row, the HEALPix first-pixel azimuth

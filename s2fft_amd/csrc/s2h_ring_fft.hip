@@ -857,6 +857,289 @@ __global__ void __launch_bounds__(256) ring_dot_final_kernel(const double* part,
   }
 }
 
+// ------------------------------------------------------------------ aliasing, tiled
+// ring_alias_kernel's cases for real maps (Hermitian spectra stored for m in
+// [0, L)) on the tiled spectra (spec_row), per row block of 8 rows:
+//  * belt blocks (every row n >= 2L - 1, no alias: R = G - w n Y) by
+//    ring_alias_belt_kernel: kAliasSplit workgroups per block, lane 8 cg + r
+//    on row r and column group v = split NG + cg, so that a wave's access to
+//    8 columns of the 8 rows is one contiguous kilobyte;
+//  * the other blocks by ring_alias_polar_kernel: RW rows of the block per
+//    workgroup, their spectra staged in LDS by whole columns (RW 16 coalesced
+//    bytes each), then per row as in ring_alias_kernel:
+//      L <= n < 2L - 1: the one alias m - n of m >= n - L + 1, whose
+//        coefficient is conj Y[n - m]: A(Y)[m] = w n (Y[m] + sgn conj Y[n - m]);
+//      n < L: residue sums T[q] = sum_j s_j c(q + j n) over m' in
+//        [-(L-1), L-1] (c(m') = conj Y[-m'] for m' < 0, s_j = -1 for odd j
+//        when sgn < 0), then A(Y)[q + j n] = s_j w n T[q];  residues of rows
+//        with n below the lanes per row split their j over groups that meet
+//        in LDS in group order,
+//    A(Y) / (w n) written over the staged Y, and R = G - w n A written back
+//    by whole columns.
+// Every per-row sum runs in a fixed order, and each workgroup's partial sum
+// (dot or |R|^2) is reduced in a fixed order into part[map][block][slot]
+// (kAliasSlots slots per block; the unused ones are zeroed): deterministic.
+constexpr int kAliasSplit = 4;
+constexpr int kAliasSlots = 8;
+constexpr int kAliasPolarTpb = 1024;
+constexpr int kAliasPolarLds = 128 * 1024;  // staged spectra per workgroup (bytes)
+constexpr int kAliasMaxCols = kAliasPolarLds / 16 / kAliasPolarTpb;  // per lane: 8
+
+// rows of a block per polar workgroup: the largest power of two <= 8 with
+// RW L 16 bytes within the LDS budget (1 for L > 8192: unsupported beyond)
+__host__ __device__ inline int alias_polar_rows(int L) {
+  int rw = kSpecBlock;
+  while (rw > 1 && (int64_t)rw * L * 16 > kAliasPolarLds) rw >>= 1;
+  return rw;
+}
+
+// fixed-order workgroup sum of dsum into part[slot]; slot 0's workgroup
+// also zeroes the slots >= nslots
+template <int TPB>
+__device__ __forceinline__ void alias_block_sum(double dsum, double* red, double* part, int slot,
+                                                int nslots) {
+  const int t = threadIdx.x;
+  for (int o = 32; o > 0; o >>= 1) dsum += __shfl_xor(dsum, o);
+  __syncthreads();
+  if ((t & 63) == 0) red[t >> 6] = dsum;
+  __syncthreads();
+  if (t == 0) {
+    double tot = 0.0;
+    for (int w = 0; w < TPB / 64; ++w) tot += red[w];
+    part[slot] = tot;
+    if (slot == 0)
+      for (int k = nslots; k < kAliasSlots; ++k) part[k] = 0.0;
+  }
+}
+
+template <int TPB, bool DOT>
+__global__ void __launch_bounds__(TPB) ring_alias_belt_kernel(RingAliasTiledArgs a) {
+  constexpr int NG = TPB / kSpecBlock;  // column groups per workgroup
+  constexpr int NV = kAliasSplit * NG;  // column groups per row block
+  constexpr int U = 4;                  // loads in flight
+  __shared__ double red[TPB / 64];
+  const int t = threadIdx.x, r = t & 7, cg = t >> 3;
+  const int blk = a.belt_blocks[blockIdx.x], b = blockIdx.y, sp = blockIdx.z;
+  const int v = sp * NG + cg;
+  const int row = kSpecBlock * blk + r;
+  const RingJob job = a.rowjobs[row];
+  const int n = job.nphi;
+  const int L = a.L;
+  const int64_t tile = (int64_t)b * a.map_stride + (int64_t)blk * kSpecBlock * a.ncol;
+  const cplx* __restrict__ y = a.y + tile + r;  // column m at y[8 m]
+  cplx* rr = a.r + tile + r;
+  const cplx* g = a.g_tiled ? a.g + tile + r
+                            : a.g + (int64_t)b * a.map_stride + (int64_t)row * a.ncol;
+  const int gs = a.g_tiled ? kSpecBlock : 1;
+  const double wn = job.weight * n * (a.alpha && !DOT ? a.alpha[b] : 1.0);
+  double dsum = 0.0;
+  if (n > 0) {
+    for (int m0 = v; m0 < L; m0 += U * NV) {
+      cplx yv[U], gv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int m = m0 + u * NV;
+        yv[u] = gv[u] = cplx{0.0, 0.0};
+        if (m < L) {
+          yv[u] = y[kSpecBlock * m];
+          if (!DOT) gv[u] = g[gs * m];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int m = m0 + u * NV;
+        if (m >= L) continue;
+        if constexpr (DOT) {
+          dsum += (m > 0 ? 2.0 * wn : wn) * (yv[u].x * yv[u].x + yv[u].y * yv[u].y);
+        } else {
+          const cplx rv = {gv[u].x - wn * yv[u].x, gv[u].y - wn * yv[u].y};
+          rr[kSpecBlock * m] = rv;
+          dsum = fma(rv.x, rv.x, fma(rv.y, rv.y, dsum));
+        }
+      }
+    }
+  }
+  if (DOT || a.part)
+    alias_block_sum<TPB>(dsum, red, a.part + ((int64_t)b * a.nblocks + blk) * kAliasSlots, sp,
+                         kAliasSplit);
+}
+
+template <bool DOT>
+__global__ void __launch_bounds__(kAliasPolarTpb) ring_alias_polar_kernel(RingAliasTiledArgs a) {
+  constexpr int TPB = kAliasPolarTpb;
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+  cplx* ys = reinterpret_cast<cplx*>(lds_raw);  // [column m][row of the workgroup]
+  __shared__ double xch[2 * TPB];                // residue groups' partial sums
+  __shared__ double red[TPB / 64];
+  const int L = a.L;
+  const int RW = alias_polar_rows(L);
+  const int H = kSpecBlock / RW;                 // workgroups per block
+  // blockIdx.x -> (block, part z): the H parts of a block on one XCD
+  // (workgroups are dealt to the 8 XCDs round-robin), so that they read each
+  // 128-byte line of the block through that XCD's L2 at about the same time
+  const int i = blockIdx.x;
+  const int z = (i >> 3) % H;
+  const int bi = ((i >> 3) / H) * 8 + (i & 7);
+  if (bi >= a.n_polar) return;
+  const int blk = a.polar_blocks[bi], b = blockIdx.y;
+  const int t = threadIdx.x;
+  const int r0 = z * RW;
+  const int64_t tile = (int64_t)b * a.map_stride + (int64_t)blk * kSpecBlock * a.ncol;
+  const int ntot = L * RW;  // <= kAliasMaxCols TPB
+  // stage Y: element (m, q) of the workgroup's rows at ys[m RW + q]; every
+  // load of the thread in flight before the first LDS store (q = t % RW is
+  // the thread's own: TPB is a multiple of RW)
+  const int qt = t % RW;
+  {
+    cplx v[kAliasMaxCols];
+#pragma unroll
+    for (int k = 0; k < kAliasMaxCols; ++k) {
+      const int idx = t + k * TPB;
+      v[k] = idx < ntot ? a.y[tile + (int64_t)(idx / RW) * kSpecBlock + r0 + qt]
+                        : cplx{0.0, 0.0};
+    }
+#pragma unroll
+    for (int k = 0; k < kAliasMaxCols; ++k) {
+      const int idx = t + k * TPB;
+      if (idx < ntot) ys[idx] = v[k];
+    }
+  }
+  __syncthreads();
+
+  const int LPR = TPB / RW;                      // lanes per row
+  const int rq = t / LPR, li = t - rq * LPR;
+  const int row = kSpecBlock * blk + r0 + rq;
+  const RingJob job = a.rowjobs[row];
+  const int n = job.nphi;
+  const double wn = job.weight * n * (a.alpha && !DOT ? a.alpha[b] : 1.0);
+  const int e = (int)rint(n * job.phi0_over_pi);
+  const double sgn = (a.apply_phase && (e & 1)) ? -1.0 : 1.0;
+  auto Y = [&](int m) { return ys[m * RW + rq]; };
+  const int kind = n == 0 ? 0 : n > 2 * L - 2 ? 1 : n >= L ? 2 : 3;  // none, no alias, pair, fold
+  double dsum = 0.0;
+
+  // pair rows: A(Y) / (w n) of columns li + k LPR, fold rows: T of residues
+  // q = li + k LPR (n >= LPR) or of the one residue of a split, in registers
+  cplx av[kAliasMaxCols];
+  cplx* tq = av;
+  const bool splitf = kind == 3 && n < LPR;
+  const int G = splitf ? LPR / n : 1;
+  const int gi = splitf ? li / n : 0;
+  const int qs = splitf ? li % n : li;
+  const bool owns = kind == 3 && gi < G;
+  FastDiv dv(kind == 3 ? n : 1);
+#pragma unroll
+  for (int k = 0; k < kAliasMaxCols; ++k) av[k] = cplx{0.0, 0.0};
+  if (kind == 2) {
+    const int m_pair = n - L + 1;
+#pragma unroll
+    for (int k = 0; k < kAliasMaxCols; ++k) {
+      const int m = li + k * LPR;
+      if (m < L) {
+        cplx v = Y(m);
+        if (m >= m_pair) {
+          const cplx yp = Y(n - m);
+          v = {fma(sgn, yp.x, v.x), fma(-sgn, yp.y, v.y)};
+        }
+        av[k] = v;
+      }
+    }
+  } else if (owns) {
+#pragma unroll
+    for (int k = 0; k < kAliasMaxCols; ++k) {
+      const int q = qs + k * (splitf ? 0 : LPR);
+      if ((splitf && k > 0) || q >= n) continue;
+      const int j0 = -dv.floor(q + L - 1);
+      const int cnt = dv.floor(L - 1 - q) - j0 + 1;
+      cplx acc = {0.0, 0.0};
+      for (int kk = gi; kk < cnt; kk += G) {
+        const int mp = q + (j0 + kk) * n;
+        cplx v = Y(mp < 0 ? -mp : mp);
+        if (mp < 0) v.y = -v.y;
+        const double sj = ((j0 + kk) & 1) && sgn < 0 ? -1.0 : 1.0;
+        acc = {fma(sj, v.x, acc.x), fma(sj, v.y, acc.y)};
+      }
+      tq[k] = acc;
+    }
+  }
+  // split residues: the groups' partial sums meet in LDS in group order
+  xch[2 * t] = tq[0].x;
+  xch[2 * t + 1] = tq[0].y;
+  __syncthreads();
+  if (splitf && owns) {
+    cplx T = {0.0, 0.0};
+    for (int gg = 0; gg < G; ++gg) {
+      const int src = rq * LPR + gg * n + qs;
+      T = {T.x + xch[2 * src], T.y + xch[2 * src + 1]};
+    }
+    tq[0] = T;
+  }
+  if constexpr (!DOT) __syncthreads();  // every read of the staged Y is done
+
+  // columns of the row's outputs: A / (w n) over the staged Y (write mode),
+  // or the dot's terms c_m w n Re(conj Y[m] A[m]) (dot mode)
+  auto out = [&](int m, cplx A) {
+    if constexpr (DOT) {
+      const cplx yv = Y(m);
+      dsum += (m > 0 ? 2.0 * wn : wn) * (yv.x * A.x + yv.y * A.y);
+    } else {
+      ys[m * RW + rq] = A;
+    }
+  };
+  if (kind == 1) {
+    if constexpr (DOT)
+      for (int m = li; m < L; m += LPR) out(m, Y(m));
+  } else if (kind == 2) {
+#pragma unroll
+    for (int k = 0; k < kAliasMaxCols; ++k) {
+      const int m = li + k * LPR;
+      if (m < L) out(m, av[k]);
+    }
+  } else if (owns) {
+#pragma unroll
+    for (int k = 0; k < kAliasMaxCols; ++k) {
+      const int q = qs + k * (splitf ? 0 : LPR);
+      if ((splitf && k > 0) || q >= n) continue;
+      const int cnt = dv.floor(L - 1 - q) + 1;
+      for (int kk = gi; kk < cnt; kk += G) {
+        const double sj = ((kk & 1) && sgn < 0) ? -1.0 : 1.0;
+        out(q + kk * n, cplx{sj * tq[k].x, sj * tq[k].y});
+      }
+    }
+  }
+
+  if constexpr (!DOT) {
+    __syncthreads();
+    // R = G - w n A by whole columns (this thread: row r0 + qt), the G loads
+    // in flight together
+    const int rw_row = kSpecBlock * blk + r0 + qt;
+    const RingJob jq = a.rowjobs[rw_row];
+    if (jq.nphi > 0) {
+      const double c = jq.weight * jq.nphi * (a.alpha ? a.alpha[b] : 1.0);
+      cplx gv[kAliasMaxCols];
+#pragma unroll
+      for (int k = 0; k < kAliasMaxCols; ++k) {
+        const int idx = t + k * TPB;
+        const int m = idx / RW;
+        gv[k] = idx >= ntot ? cplx{0.0, 0.0}
+                : a.g_tiled ? a.g[tile + (int64_t)m * kSpecBlock + r0 + qt]
+                            : a.g[(int64_t)b * a.map_stride + (int64_t)rw_row * a.ncol + m];
+      }
+#pragma unroll
+      for (int k = 0; k < kAliasMaxCols; ++k) {
+        const int idx = t + k * TPB;
+        if (idx >= ntot) continue;
+        const cplx A = ys[idx];
+        const cplx rv = {gv[k].x - c * A.x, gv[k].y - c * A.y};
+        a.r[tile + (int64_t)(idx / RW) * kSpecBlock + r0 + qt] = rv;
+        dsum = fma(rv.x, rv.x, fma(rv.y, rv.y, dsum));
+      }
+    }
+  }
+  if (DOT || a.part)
+    alias_block_sum<TPB>(dsum, red, a.part + ((int64_t)b * a.nblocks + blk) * kAliasSlots, z, H);
+}
+
 // ------------------------------------------------------------------ launchers
 namespace {
 
@@ -959,6 +1242,70 @@ hipError_t launch_ring_alias_sumsq(const RingAliasArgs& a, const JobSet& js, int
 }
 
 int ring_dot_jobs(const JobSet& js) { return js.n_pow2 + js.n_smooth + js.n_blue; }
+int ring_dot_tiled_parts(int nblocks) { return nblocks * kAliasSlots; }
+namespace {
+// both tiled alias kernels over their blocks (part: null, or the per-block slots)
+hipError_t launch_alias_tiled_pair(const RingAliasTiledArgs& a, int batch, bool dot,
+                                   hipStream_t s) {
+  if (a.n_belt > 0) {
+    const dim3 grid(a.n_belt, batch, kAliasSplit);
+    if (dot)
+      hipLaunchKernelGGL((ring_alias_belt_kernel<kTpbAlias, true>), grid, dim3(kTpbAlias), 0, s,
+                         a);
+    else
+      hipLaunchKernelGGL((ring_alias_belt_kernel<kTpbAlias, false>), grid, dim3(kTpbAlias), 0,
+                         s, a);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (a.n_polar > 0) {
+    const int RW = alias_polar_rows(a.L);
+    if ((int64_t)RW * a.L * 16 > kAliasPolarLds) return hipErrorInvalidValue;
+    const int H = kSpecBlock / RW;
+    const dim3 grid((unsigned)((a.n_polar + 7) / 8 * 8 * H), batch);
+    const size_t lds = (size_t)RW * a.L * sizeof(cplx);
+    if (dot)
+      hipLaunchKernelGGL((ring_alias_polar_kernel<true>), grid, dim3(kAliasPolarTpb), lds, s, a);
+    else
+      hipLaunchKernelGGL((ring_alias_polar_kernel<false>), grid, dim3(kAliasPolarTpb), lds, s, a);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+}  // namespace
+
+hipError_t launch_ring_alias_tiled(const RingAliasTiledArgs& a, int batch, hipStream_t s) {
+  if (batch == 0 || a.nblocks == 0) return hipSuccess;
+  RingAliasTiledArgs b = a;
+  b.part = nullptr;
+  return launch_alias_tiled_pair(b, batch, false, s);
+}
+
+hipError_t launch_ring_alias_tiled_sumsq(const RingAliasTiledArgs& a, int batch, double* part,
+                                         double* sumsq, hipStream_t s) {
+  if (batch == 0 || a.nblocks == 0) return hipSuccess;
+  RingAliasTiledArgs b = a;
+  b.part = part;
+  hipError_t e = launch_alias_tiled_pair(b, batch, false, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(ring_dot_final_kernel, dim3(batch), dim3(256), 0, s, part,
+                     a.nblocks * kAliasSlots, nullptr, sumsq, nullptr);
+  return hipGetLastError();
+}
+
+hipError_t launch_ring_dot_tiled(const RingAliasTiledArgs& a, int batch, double* part,
+                                 const double* num, double* pq, double* alpha, hipStream_t s) {
+  if (batch == 0 || a.nblocks == 0) return hipSuccess;
+  RingAliasTiledArgs b = a;
+  b.part = part;
+  b.alpha = nullptr;
+  hipError_t e = launch_alias_tiled_pair(b, batch, true, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(ring_dot_final_kernel, dim3(batch), dim3(256), 0, s, part,
+                     a.nblocks * kAliasSlots, num, pq, alpha);
+  return hipGetLastError();
+}
 
 hipError_t launch_ring_dot(const RingAliasArgs& a, const JobSet& js, int batch, double* part,
                            const double* num, double* pq, double* alpha, hipStream_t s) {
@@ -987,6 +1334,13 @@ hipError_t allow_large_lds_rings() {
                        (const void*)ring_inv_kernel<kTpbBlue, kMaxConvLen, kBlue>};
   for (const void* f : fns) {
     const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e != hipSuccess) return e;
+  }
+  const void* alias[] = {(const void*)ring_alias_polar_kernel<true>,
+                         (const void*)ring_alias_polar_kernel<false>};
+  for (const void* f : alias) {
+    const hipError_t e =
+        hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, kAliasPolarLds);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;

@@ -162,34 +162,41 @@ def refine_cg(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=None
     if not sph.spectra_supported(plan):
         raise ValueError("refine_cg needs a sampling whose forward and inverse share one ring grid")
     packed = bool(reality)
+    # real maps: R and every Y in the tiled spectra layout (the latitudinal
+    # kernels' column accesses then touch whole cache lines); G stays as the
+    # ring analysis writes it
+    tl = packed
     pL = L if packed else 0
     unpack = (lambda x: sph.flm_unpack_hip(x, L)) if packed else (lambda x: x)
     lmin = max(abs(spin), L_lower)
 
     def proj(R, k):
         prec = proj_precision[min(k, len(proj_precision) - 1)]
-        return sph.forward_rings_hip(R, L, plan, prec, packed=packed)
+        return sph.forward_rings_hip(R, L, plan, prec, packed=packed, tiled=tl)
 
     def precondition(r, k):
         if precond is not None and k < learned_steps:
             return precond.apply(r, L, reality, lmin, packed=packed)
         return r
 
-    R = sph.ring_analysis_hip(f, plan)
-    x = sph.forward_rings_hip(R, L, plan, init_precision, packed=packed)
+    G = sph.ring_analysis_hip(f, plan)
+    x = sph.forward_rings_hip(G, L, plan, init_precision, packed=packed)
     _track(history, x, flm_true, unpack)
     if n_iter <= 0:
         return unpack(x)
     def judged(k):
         return precond is not None and k < learned_steps
 
-    # R = G - A(I(x_0)) in place of G; |R|^2 per map where a judged step follows
-    Y = sph.inverse_rings_hip(x, plan, "fp64", packed)
+    # R = G - A(I(x_0)) (tiled: into a new array, else in place of G); |R|^2
+    # per map where a judged step follows
+    Y = sph.inverse_rings_hip(x, plan, "fp64", packed, tiled=tl)
     n_R = None
+    kw_r = dict(out=None, tiled=True, g_tiled=False) if tl else dict(out=G)
     if judged(0):
-        _, n_R = sph.ring_residual_hip(R, Y, plan, out=R, sumsq=True)
+        R, n_R = sph.ring_residual_hip(G, Y, plan, sumsq=True, **kw_r)
     else:
-        sph.ring_residual_hip(R, Y, plan, out=R)
+        R = sph.ring_residual_hip(G, Y, plan, **kw_r)
+    del G
     r = proj(R, 0)
     z = precondition(r, 0)
     rz = sph.flm_dots_hip(z, [r], pL)[0]
@@ -206,8 +213,8 @@ def refine_cg(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=None
         last = k + 1 == n_iter
         prec = judge_precision if judged(k) else \
             last_synth_precision if last else synth_precision
-        Y = sph.inverse_rings_hip(p, plan, prec, packed)
-        pq, alpha = sph.ring_dot_hip(Y, plan, num=rz)
+        Y = sph.inverse_rings_hip(p, plan, prec, packed, tiled=tl)
+        pq, alpha = sph.ring_dot_hip(Y, plan, num=rz, tiled=tl)
         if last:
             if eager:
                 sph.cg_update_hip(x, p, alpha)
@@ -218,9 +225,10 @@ def refine_cg(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=None
                                      unpack_L=L if packed else None)
         n_new = None
         if judged(k) or judged(k + 1):
-            _, n_new = sph.ring_residual_hip(R, Y, plan, out=R, alpha=alpha, sumsq=True)
+            _, n_new = sph.ring_residual_hip(R, Y, plan, out=R, alpha=alpha, sumsq=True,
+                                             tiled=tl)
         else:
-            sph.ring_residual_hip(R, Y, plan, out=R, alpha=alpha)
+            sph.ring_residual_hip(R, Y, plan, out=R, alpha=alpha, tiled=tl)
         if judged(k):
             ok = n_new < n_R
             if stats is not None:
@@ -228,7 +236,7 @@ def refine_cg(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=None
                 stats["steps"] = stats.get("steps", 0) + int(ok.numel())
             if not bool(ok.all()):
                 _redo_plain((~ok).nonzero().flatten(), R, Y, p, alpha, rz, rs[-1], plan,
-                            synth_precision, packed, pL, n_new)
+                            synth_precision, packed, pL, n_new, tl)
             kept = ok
         n_R = n_new
         r_new = proj(R, k + 1)
@@ -259,18 +267,19 @@ def refine_cg(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=None
     return unpack(x)
 
 
-def _redo_plain(bad, R, Y, p, alpha, rz, r, plan, synth_precision, packed, pL, n_R):
+def _redo_plain(bad, R, Y, p, alpha, rz, r, plan, synth_precision, packed, pL, n_R, tl=False):
     """Judge's fallback of refine_cg: maps ``bad`` take back their learned
     step's residual update (R += alpha A(Y)) and step along r instead; R and
     the maps' rows of p, alpha, rz and the residual norms n_R are replaced in
-    place (x is updated afterwards with the replaced alpha and p)."""
+    place (x is updated afterwards with the replaced alpha and p).  ``tl``:
+    R and Y in the tiled spectra layout."""
     Rb = R[bad].contiguous()
-    sph.ring_residual_hip(Rb, Y[bad].contiguous(), plan, out=Rb, alpha=-alpha[bad])
+    sph.ring_residual_hip(Rb, Y[bad].contiguous(), plan, out=Rb, alpha=-alpha[bad], tiled=tl)
     rb = r[bad].contiguous()
     rzb = sph.flm_dots_hip(rb, [rb], pL)[0]
-    Yb = sph.inverse_rings_hip(rb, plan, synth_precision, packed)
-    _, ab = sph.ring_dot_hip(Yb, plan, num=rzb)
-    _, nb = sph.ring_residual_hip(Rb, Yb, plan, out=Rb, alpha=ab, sumsq=True)
+    Yb = sph.inverse_rings_hip(rb, plan, synth_precision, packed, tiled=tl)
+    _, ab = sph.ring_dot_hip(Yb, plan, num=rzb, tiled=tl)
+    _, nb = sph.ring_residual_hip(Rb, Yb, plan, out=Rb, alpha=ab, sumsq=True, tiled=tl)
     R[bad], p[bad] = Rb, rb
     alpha[bad], rz[bad], n_R[bad] = ab, rzb, nb
 

@@ -255,6 +255,9 @@ def inverse_diff(flm, L, spin=0, nside=None, sampling="mw", reality=False, L_low
 # G = ring_analysis_hip(f), forward = forward_rings_hip(G) and
 # F(f - I(x)) = forward_rings_hip(ring_residual_hip(G, inverse_rings_hip(x))),
 # where the residual applies (ring FFT . ring iFFT) without either FFT.
+# ``tiled``: spectra in blocks of 8 rows (S2H_SPECTRA_TILED), the layout in
+# which the latitudinal kernels' column gathers and scatters touch whole
+# cache lines; ring_analysis_hip writes ring-major spectra.
 def spectra_supported(plan):
     """Whether the plan's forward and inverse share one ring grid (HEALPix, GL)."""
     return int(_lib.lib().s2h_ring_spectra_bytes(plan.handle, 1)) > 0
@@ -281,11 +284,13 @@ def _flm_shape(B, L, packed):
     return (B, L * (L + 1) // 2) if packed else (B, L, 2 * L - 1)
 
 
-def forward_rings_hip(G, L, plan, precision="fp64", accumulate_into=None, packed=False):
+def forward_rings_hip(G, L, plan, precision="fp64", accumulate_into=None, packed=False,
+                      tiled=False):
     """flm (B, L, 2L-1) from ring spectra; ``accumulate_into``: flm += in place.
-    ``packed``: flm in the packed m-major layout (B, L(L+1)/2) (real maps)."""
+    ``packed``: flm in the packed m-major layout (B, L(L+1)/2) (real maps).
+    ``tiled``: G in the tiled layout."""
     B = G.shape[0]
-    flags = _lib.FLM_PACKED if packed else 0
+    flags = (_lib.FLM_PACKED if packed else 0) | (_lib.SPECTRA_TILED if tiled else 0)
     shape = _flm_shape(B, L, packed)
     if accumulate_into is not None:
         flm = accumulate_into
@@ -301,14 +306,15 @@ def forward_rings_hip(G, L, plan, precision="fp64", accumulate_into=None, packed
     return flm
 
 
-def inverse_rings_hip(flm, plan, precision="fp64", packed=False):
+def inverse_rings_hip(flm, plan, precision="fp64", packed=False, tiled=False):
     """Ring spectra (B, rows * Mc) of the latitudinal synthesis of flm
-    (``packed``: flm in the packed m-major layout)."""
+    (``packed``: flm in the packed m-major layout; ``tiled``: the spectra
+    in the tiled layout)."""
     flm = flm.contiguous()
     Y = _new_spectra(plan, flm.shape[0], flm.device)
+    flags = (_lib.FLM_PACKED if packed else 0) | (_lib.SPECTRA_TILED if tiled else 0)
     _lib.check(_lib.lib().s2h_inverse_rings(plan.handle, flm.data_ptr(), Y.data_ptr(),
-                                            flm.shape[0], _precision(precision),
-                                            _lib.FLM_PACKED if packed else 0,
+                                            flm.shape[0], _precision(precision), flags,
                                             _stream(flm.device)), "inverse rings")
     return Y
 
@@ -333,10 +339,13 @@ def flm_unpack_hip(flm_p, L):
     return out
 
 
-def ring_residual_hip(G, Y, plan, out=None, alpha=None, sumsq=False):
+def ring_residual_hip(G, Y, plan, out=None, alpha=None, sumsq=False, tiled=False,
+                      g_tiled=None):
     """R = G - A(Y), or G - alpha[b] A(Y) with a per-map float64 device tensor
     ``alpha`` (out may be G).  ``sumsq``: also return the per-map sums of
-    |R|^2 (one pass, fixed order) as (R, sumsq)."""
+    |R|^2 (one pass, fixed order) as (R, sumsq).  ``tiled``: Y and R in the
+    tiled layout (real maps), G too unless ``g_tiled`` is False (G as
+    ring_analysis_hip writes it; out may then not be G)."""
     B = G.shape[0]
     R = torch.empty_like(G) if out is None else out
     a = None
@@ -350,28 +359,32 @@ def ring_residual_hip(G, Y, plan, out=None, alpha=None, sumsq=False):
         sq = torch.empty(B, dtype=torch.float64, device=G.device)
         work = torch.empty(max(int(lib.s2h_ring_dot_workspace_bytes(plan.handle, B)), 8),
                            dtype=torch.uint8, device=G.device)
-    _lib.check(lib.s2h_ring_residual_scaled(plan.handle, G.contiguous().data_ptr(),
-                                            Y.contiguous().data_ptr(), a, R.data_ptr(),
-                                            sq.data_ptr() if sq is not None else None,
-                                            work.data_ptr() if work is not None else None,
-                                            B, _stream(G.device)), "ring residual")
+    flags = 0
+    if tiled:
+        flags = _lib.SPECTRA_TILED | (0 if g_tiled in (None, True) else _lib.SPECTRA_G_RING_MAJOR)
+    _lib.check(lib.s2h_ring_residual_ex(plan.handle, G.contiguous().data_ptr(),
+                                        Y.contiguous().data_ptr(), a, R.data_ptr(),
+                                        sq.data_ptr() if sq is not None else None,
+                                        work.data_ptr() if work is not None else None,
+                                        B, flags, _stream(G.device)), "ring residual")
     return (R, sq) if sumsq else R
 
 
-def ring_dot_hip(Y, plan, num=None):
+def ring_dot_hip(Y, plan, num=None, tiled=False):
     """pq[b] = <p, F A(Y)> for Y = inverse_rings_hip(p) (the curvature of a
-    conjugate-gradient step); with ``num`` also alpha = num / pq.
-    Returns pq, or (pq, alpha)."""
+    conjugate-gradient step); with ``num`` also alpha = num / pq.  ``tiled``:
+    Y in the tiled layout (real maps).  Returns pq, or (pq, alpha)."""
     B = Y.shape[0]
     lib = _lib.lib()
     pq = torch.empty(B, dtype=torch.float64, device=Y.device)
     alpha = torch.empty_like(pq) if num is not None else None
     work = torch.empty(max(int(lib.s2h_ring_dot_workspace_bytes(plan.handle, B)), 8),
                        dtype=torch.uint8, device=Y.device)
-    _lib.check(lib.s2h_ring_dot(plan.handle, Y.contiguous().data_ptr(),
-                                num.contiguous().data_ptr() if num is not None else None,
-                                pq.data_ptr(), alpha.data_ptr() if alpha is not None else None,
-                                work.data_ptr(), B, _stream(Y.device)), "ring dot")
+    _lib.check(lib.s2h_ring_dot_ex(plan.handle, Y.contiguous().data_ptr(),
+                                   num.contiguous().data_ptr() if num is not None else None,
+                                   pq.data_ptr(), alpha.data_ptr() if alpha is not None else None,
+                                   work.data_ptr(), B, _lib.SPECTRA_TILED if tiled else 0,
+                                   _stream(Y.device)), "ring dot")
     return pq if num is None else (pq, alpha)
 
 
