@@ -4,28 +4,19 @@ Test infrastructure only -- see ``oracle/__init__.py``.
 
 No reference counterpart: the reference refines with plain Jacobi steps
 (s2fft/utils/iterative_refinement.py:9-46).  The network is
-    z1 = W1h h(x) + b1,  a1 = h(gelu(z1)),  z2 = W2h a1 + b2,
-    a2 = h(gelu(z2)),    y  = W3h a2 + b3,   Wh = h(W), h = round to fp16
-with GELU the tanh form whose tanh factor is replaced by a degree-6
-polynomial (``gelu``), evaluated here in float64 (the FP16 roundings
-applied exactly), and its gradient by hand-derived
-backpropagation with the roundings treated as the identity.  The gradient
-is pinned in the tests by central finite differences of ``loss`` (with the
-roundings switched off), the polynomial against the tanh form, and the
-feature construction by a direct loop.
+    z1 = W1h h(x) + b1,  a1 = h(relu(z1)),  z2 = W2h a1 + b2,
+    a2 = h(relu(z2)),    y  = W3h a2 + b3,   Wh = h(W), h = round to fp16
+evaluated here in float64 (the FP16 roundings applied exactly), and its
+gradient by hand-derived backpropagation with the roundings treated as the
+identity.  The gradient is pinned in the tests by central finite
+differences of ``loss`` (with the roundings switched off), and the feature
+construction by a direct loop.
 """
 
 import numpy as np
 
 H = 128
 N_PARAMS = H * 4 + H + H * H + H + 2 * H + 2
-
-# gelu(x) = (x + |x| T(min(|x|, A) - A/2)) / 2, T a polynomial standing for
-# tanh(sqrt(2/pi) (a + 0.044715 a^3)) on [0, A]; coefficients exact in FP16
-GELU_A = 3.75
-GELU_C = np.array([0.939453125, 0.1392822265625, -0.1318359375, 0.05487060546875,
-                   -0.0016756057739257812, -0.00531768798828125, 0.0010290145874023438])
-
 
 def bf16(x):
     """Round float32 values to the nearest bfloat16 (ties to even)."""
@@ -66,26 +57,12 @@ def f16(x):
     return np.asarray(x, dtype=np.float64).astype(np.float16).astype(np.float64)
 
 
-def gelu_tanh(x):
-    """The tanh form 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3)))."""
-    return 0.5 * x * (1.0 + np.tanh(np.sqrt(2.0 / np.pi) * (x + 0.044715 * x ** 3)))
+def relu(x):
+    return np.maximum(np.asarray(x, dtype=np.float64), 0.0)
 
 
-def gelu(x):
-    x = np.asarray(x, dtype=np.float64)
-    a = np.abs(x)
-    u = np.minimum(a, GELU_A) - GELU_A / 2
-    return 0.5 * (x + a * np.polynomial.polynomial.polyval(u, GELU_C))
-
-
-def gelu_grad(x):
-    x = np.asarray(x, dtype=np.float64)
-    a = np.abs(x)
-    u = np.minimum(a, GELU_A) - GELU_A / 2
-    t = np.polynomial.polynomial.polyval(u, GELU_C)
-    dt = np.polynomial.polynomial.polyval(u, GELU_C[1:] * np.arange(1, len(GELU_C)))
-    t = np.where(a < GELU_A, t + a * dt, t)
-    return 0.5 * (1.0 + np.where(x < 0, -t, t))
+def relu_grad(x):
+    return (np.asarray(x, dtype=np.float64) > 0).astype(np.float64)
 
 
 def forward(p, X, quantize=True):
@@ -95,9 +72,9 @@ def forward(p, X, quantize=True):
     W1q, W2q, W3q = qh(W1), qh(W2), qh(W3)
     X = qh(np.asarray(X, dtype=np.float64))
     z1 = X @ W1q.T + b1
-    a1 = qh(gelu(z1))
+    a1 = qh(relu(z1))
     z2 = a1 @ W2q.T + b2
-    a2 = qh(gelu(z2))
+    a2 = qh(relu(z2))
     y = a2 @ W3q.T + b3
     return y, (X, z1, a1, z2, a2, W1q, W2q, W3q)
 
@@ -117,10 +94,10 @@ def grad(p, X, Y, w=None, loss_scale=1.0, quantize=True):
     dy = 2.0 * (y - np.asarray(Y, dtype=np.float64)) * wt[:, None] * loss_scale
     gW3 = dy.T @ a2
     gb3 = dy.sum(0)
-    dz2 = (dy @ W3q) * gelu_grad(z2)
+    dz2 = (dy @ W3q) * relu_grad(z2)
     gW2 = dz2.T @ a1
     gb2 = dz2.sum(0)
-    dz1 = (dz2 @ W2q) * gelu_grad(z1)
+    dz1 = (dz2 @ W2q) * relu_grad(z1)
     gW1 = dz1.T @ X
     gb1 = dz1.sum(0)
     return pack(gW1, gb1, gW2, gb2, gW3, gb3)

@@ -1,21 +1,17 @@
 // Learned preconditioner for the iterative forward transform: a per-(l, m)
 // MLP  (re, im, l/L, m/L) -> correction (re, im),  two hidden layers of 128
-// with GELU, FP16 weights, features and activations, FP32 accumulation:
+// with ReLU, FP16 weights, features and activations, FP32 accumulation:
 //
-//   z1 = W1h h(x) + b1,  a1 = h(gelu(z1)),  z2 = W2h a1 + b2,
-//   a2 = h(gelu(z2)),    y  = W3h a2 + b3,   Wh = h(W), h = round to FP16
-//
-// gelu is the tanh form x/2 (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3))) with the
-// tanh factor replaced by a polynomial (gelu below), so that inference runs it
-// on packed FP16 arithmetic without transcendentals.
+//   z1 = W1h h(x) + b1,  a1 = h(relu(z1)),  z2 = W2h a1 + b2,
+//   a2 = h(relu(z2)),    y  = W3h a2 + b3,   Wh = h(W), h = round to FP16
 //
 // x = (h_re / s, h_im / s, l / L, m / L) for the harmonic residual h of a map
 // whose RMS over its coefficients is s; the correction is s * y.  Inference
-// runs all three layers on the FP16 matrix cores; training (forward +
-// backward, FP32 on the vector ALUs) accumulates per-workgroup gradient
-// partials that a second kernel reduces in a fixed order, and Adam updates
-// the FP32 master weights.  The backward treats the roundings as the identity
-// (straight-through).
+// runs all three layers on the FP16 matrix cores with the activations on
+// packed FP16; training (forward + backward, FP32 on the vector ALUs)
+// accumulates per-workgroup gradient partials that a second kernel reduces
+// in a fixed order, and Adam updates the FP32 master weights.  The backward
+// treats the roundings as the identity (straight-through).
 #include "s2h_internal.h"
 #include "s2h_kernels.h"
 
@@ -41,71 +37,24 @@ __device__ __forceinline__ float f16_round(float v) {
   return (float)(_Float16)v;
 }
 
-// GELU.  The tanh form equals (x + |x| T(|x|)) / 2, T(a) = tanh(sqrt(2/pi)
-// (a + 0.044715 a^3)).  T is replaced by a degree-6 polynomial in
-// u = min(|x|, 3.75) - 1.875 (weighted least squares towards the minimax fit
-// on [0, 3.75], coefficients exact in FP16): within 2.7e-4 max(1, |x|) of the
-// tanh form for every x.
-constexpr float kGeluA = 3.75f, kGeluU = 1.875f;
-__device__ constexpr float kGeluC[7] = {0.939453125f,          0.1392822265625f,
-                                        -0.1318359375f,        0.05487060546875f,
-                                        -0.0016756057739257812f, -0.00531768798828125f,
-                                        0.0010290145874023438f};
-
-__device__ __forceinline__ float gelu(float x) {
-  const float a = fabsf(x);
-  const float u = fminf(a, kGeluA) - kGeluU;
-  float p = kGeluC[6];
-#pragma unroll
-  for (int i = 5; i >= 0; --i) p = fmaf(p, u, kGeluC[i]);
-  return 0.5f * fmaf(a, p, x);
-}
-// d gelu / dx = (1 + sign(x) (T + |x| T' [|x| < 3.75])) / 2
-__device__ __forceinline__ float gelu_grad(float x) {
-  const float a = fabsf(x);
-  const float u = fminf(a, kGeluA) - kGeluU;
-  float p = kGeluC[6], dp = 0.f;
-#pragma unroll
-  for (int i = 5; i >= 0; --i) {
-    dp = fmaf(dp, u, p);
-    p = fmaf(p, u, kGeluC[i]);
-  }
-  const float t = a < kGeluA ? fmaf(a, dp, p) : p;
-  return 0.5f * (1.0f + (x < 0.f ? -t : t));
-}
+// Activation: ReLU (derivative 1 for x > 0, else 0).
+__device__ __forceinline__ float act(float x) { return fmaxf(x, 0.0f); }
+__device__ __forceinline__ float act_grad(float x) { return x > 0.0f ? 1.0f : 0.0f; }
 
 typedef _Float16 h2 __attribute__((ext_vector_type(2)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// 2 gelu(x) on packed FP16 (gelu2x_operand): per pair one conversion, the
-// magnitude, the clamp and shift, seven FMAs; no transcendentals.  The factor
-// 1/2 is left to the next layer's weights.
 // The MFMA operand of eight hidden units held as two accumulator tiles:
-// elements 0-3 from t0, 4-7 from t1, each through 2 gelu.  The four pairs
-// advance through the polynomial in lockstep, so that no step waits on the
-// one before it.
-__device__ __forceinline__ f16x8 gelu2x_operand(const f32x4& t0, const f32x4& t1) {
-  h2 x[4] = {h2{(_Float16)t0[0], (_Float16)t0[1]}, h2{(_Float16)t0[2], (_Float16)t0[3]},
-             h2{(_Float16)t1[0], (_Float16)t1[1]}, h2{(_Float16)t1[2], (_Float16)t1[3]}};
-  h2 a[4], u[4], p[4];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) a[q] = __builtin_elementwise_abs(x[q]);
-#pragma unroll
-  for (int q = 0; q < 4; ++q)
-    u[q] = __builtin_elementwise_min(a[q], (h2)(_Float16)kGeluA) - (h2)(_Float16)kGeluU;
-#pragma unroll
-  for (int q = 0; q < 4; ++q)
-    p[q] = __builtin_elementwise_fma((h2)(_Float16)kGeluC[6], u[q], (h2)(_Float16)kGeluC[5]);
-#pragma unroll
-  for (int i = 4; i >= 0; --i) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) p[q] = __builtin_elementwise_fma(p[q], u[q], (h2)(_Float16)kGeluC[i]);
-  }
-  h2 g[4];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) g[q] = __builtin_elementwise_fma(a[q], p[q], x[q]);
-  return f16x8{g[0][0], g[0][1], g[1][0], g[1][1], g[2][0], g[2][1], g[3][0], g[3][1]};
+// elements 0-3 from t0, 4-7 from t1, through ReLU on packed FP16 (one
+// conversion and one maximum per pair; the rounding commutes with ReLU).
+__device__ __forceinline__ f16x8 act_operand(const f32x4& t0, const f32x4& t1) {
+  const h2 z = {(_Float16)0.0f, (_Float16)0.0f};
+  const h2 g0 = __builtin_elementwise_max(h2{(_Float16)t0[0], (_Float16)t0[1]}, z);
+  const h2 g1 = __builtin_elementwise_max(h2{(_Float16)t0[2], (_Float16)t0[3]}, z);
+  const h2 g2 = __builtin_elementwise_max(h2{(_Float16)t1[0], (_Float16)t1[1]}, z);
+  const h2 g3 = __builtin_elementwise_max(h2{(_Float16)t1[2], (_Float16)t1[3]}, z);
+  return f16x8{g0[0], g0[1], g1[0], g1[1], g2[0], g2[1], g3[0], g3[1]};
 }
 
 // packed (l, m) enumeration: reality -> m in [0, l], P = L(L+1)/2;
@@ -189,12 +138,12 @@ __global__ void __launch_bounds__(kThreads) mlp_sumsq_kernel(MlpRowArgs a, doubl
 //    lane%16 in register i of tile nt.
 //  * layer 2 as 8 output tiles x 4 k-blocks of V_MFMA_F32_16X16X32_F16 with
 //    the weights as A and the activations as B.  B is taken straight from
-//    layer 1's registers (through 2 gelu), which fixes the order of the
+//    layer 1's registers (through ReLU), which fixes the order of the
 //    hidden units along k: k = 32 kb + 8 q + e is unit
-//    32 kb + 16 (e/4) + 4 q + e%4.  W2h / 2 is staged in LDS in the A layout
+//    32 kb + 16 (e/4) + 4 q + e%4.  W2h is staged in LDS in the A layout
 //    with its columns in that order.  The result again holds unit
 //    16 ct + 4 (lane/16) + i of row lane%16 in register i of tile ct.
-//  * layer 3 the same way as 4 MFMAs with W3h / 2 padded to 16 outputs as A:
+//  * layer 3 the same way as 4 MFMAs with W3h padded to 16 outputs as A:
 //    lanes 0-15 end with the row's two outputs in registers 0 and 1.
 // Each weight fragment read from LDS serves every tile of the pass.
 typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
@@ -207,9 +156,9 @@ __global__ void __launch_bounds__(kInfWaves * 64) mlp_apply_kernel(MlpRowArgs a,
                                                                    const float* __restrict__ w,
                                                                    const double* sumsq) {
   // A operand of layer-2 tile ct, k-block kb for lane l:
-  // W2h[16 ct + l%16][unit(32 kb + 8 (l/16) + e)] / 2, unit() as above
+  // W2h[16 ct + l%16][unit(32 kb + 8 (l/16) + e)], unit() as above
   __shared__ f16x8 w2a[8][4][64];
-  // A operand of layer 3, k-block kb: W3h[l%16][unit(...)] / 2, zero for l%16 >= 2
+  // A operand of layer 3, k-block kb: W3h[l%16][unit(...)], zero for l%16 >= 2
   __shared__ f16x8 w3a[4][64];
   // A operand of layer-1 tile nt for lane l < 16: W1h[16 nt + l][0..3]
   __shared__ f16x4 w1a[8][16];
@@ -222,7 +171,7 @@ __global__ void __launch_bounds__(kInfWaves * 64) mlp_apply_kernel(MlpRowArgs a,
     f16x8 v;
 #pragma unroll
     for (int q = 0; q < 8; ++q)
-      v[q] = (_Float16)(0.5f * w[oW2 + j * kH + 32 * kb + 16 * (q >> 2) + q4 + (q & 3)]);
+      v[q] = (_Float16)w[oW2 + j * kH + 32 * kb + 16 * (q >> 2) + q4 + (q & 3)];
     w2a[ct][kb][l] = v;
   }
   for (int e = threadIdx.x; e < 4 * 64; e += kInfWaves * 64) {
@@ -231,7 +180,7 @@ __global__ void __launch_bounds__(kInfWaves * 64) mlp_apply_kernel(MlpRowArgs a,
     f16x8 v;
 #pragma unroll
     for (int q = 0; q < 8; ++q)
-      v[q] = o < 2 ? (_Float16)(0.5f * w[oW3 + o * kH + 32 * kb + 16 * (q >> 2) + q4 + (q & 3)])
+      v[q] = o < 2 ? (_Float16)w[oW3 + o * kH + 32 * kb + 16 * (q >> 2) + q4 + (q & 3)]
                    : (_Float16)0.0f;
     w3a[kb][l] = v;
   }
@@ -321,7 +270,7 @@ __global__ void __launch_bounds__(kInfWaves * 64) mlp_apply_kernel(MlpRowArgs a,
         z1[nt] = __builtin_amdgcn_mfma_f32_16x16x16f16(wv, xf, z1[nt], 0, 0, 0);
       }
 #pragma unroll
-      for (int kb = 0; kb < 4; ++kb) av[s][kb] = gelu2x_operand(z1[2 * kb], z1[2 * kb + 1]);
+      for (int kb = 0; kb < 4; ++kb) av[s][kb] = act_operand(z1[2 * kb], z1[2 * kb + 1]);
     }
     // layer 2: unit 16 ct + 4 kq + i of row `row` in register i of acc[s][ct]
     // (each tile's bias read from LDS by itself: one read instead of four moves)
@@ -349,7 +298,7 @@ __global__ void __launch_bounds__(kInfWaves * 64) mlp_apply_kernel(MlpRowArgs a,
 #pragma unroll
       for (int kb = 0; kb < 4; ++kb)
         y = __builtin_amdgcn_mfma_f32_16x16x32_f16(
-            w3a[kb][lane], gelu2x_operand(acc[s][2 * kb], acc[s][2 * kb + 1]), y, 0, 0, 0);
+            w3a[kb][lane], act_operand(acc[s][2 * kb], acc[s][2 * kb + 1]), y, 0, 0, 0);
       const int64_t q = (pass * TILES + s) * 16 + row;
       if (kq == 0 && q < a.P) {
         const int lq = lr[s], mq = mr[s];
@@ -459,7 +408,7 @@ __global__ void __launch_bounds__(kTT) mlp_grad_kernel(const float* __restrict__
       z1[i] = fmaf(W1s[4 * k + 3], xs[r][3],
                    fmaf(W1s[4 * k + 2], xs[r][2],
                         fmaf(W1s[4 * k + 1], xs[r][1], fmaf(W1s[4 * k], xs[r][0], B1s[k]))));
-      a1s[r][k] = f16_round(gelu(z1[i]));
+      a1s[r][k] = f16_round(act(z1[i]));
     }
     __syncthreads();
     // layer 2 (unit j = k)
@@ -471,7 +420,7 @@ __global__ void __launch_bounds__(kTT) mlp_grad_kernel(const float* __restrict__
 #pragma unroll 8
       for (int q = 0; q < kH; ++q) s = fmaf(W2s[k * kW2S + q], a1s[r][q], s);
       z2[i] = s;
-      a2s[r][k] = f16_round(gelu(s));
+      a2s[r][k] = f16_round(act(s));
     }
     __syncthreads();
     // layer 3 and the loss: threads 0..31 -> (row t/2, output t%2)
@@ -489,7 +438,7 @@ __global__ void __launch_bounds__(kTT) mlp_grad_kernel(const float* __restrict__
       douts[r][o] = d;
     }
     __syncthreads();
-    // backward through layer 3 and gelu(z2)
+    // backward through layer 3 and relu(z2)
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       const int r = 2 * i + half;
@@ -497,7 +446,7 @@ __global__ void __launch_bounds__(kTT) mlp_grad_kernel(const float* __restrict__
       const float a2 = a2s[r][k];
       gW3[0] = fmaf(d0, a2, gW3[0]);
       gW3[1] = fmaf(d1, a2, gW3[1]);
-      const float dz = (d0 * W3s[k] + d1 * W3s[kH + k]) * gelu_grad(z2[i]);
+      const float dz = (d0 * W3s[k] + d1 * W3s[kH + k]) * act_grad(z2[i]);
       dz2s[r][k] = dz;
       gB2 += dz;
     }
@@ -513,14 +462,14 @@ __global__ void __launch_bounds__(kTT) mlp_grad_kernel(const float* __restrict__
 #pragma unroll
       for (int i = 0; i < 64; ++i) gW2[i] = fmaf(dz2s[r][2 * i + half], a1, gW2[i]);
     }
-    // da1[r][k] = sum_j dz2[r][j] W2h[j][k]; back through gelu(z1)
+    // da1[r][k] = sum_j dz2[r][j] W2h[j][k]; back through relu(z1)
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       const int r = 2 * i + half;
       float s = 0.f;
 #pragma unroll 8
       for (int j = 0; j < kH; ++j) s = fmaf(dz2s[r][j], W2s[j * kW2S + k], s);
-      const float dz1 = s * gelu_grad(z1[i]);
+      const float dz1 = s * act_grad(z1[i]);
       gB1 += dz1;
 #pragma unroll
       for (int f = 0; f < 4; ++f) gW1[f] = fmaf(dz1, xs[r][f], gW1[f]);

@@ -6,9 +6,7 @@
   (roundings switched off, float64), so a dropped term or a transposed weight
   fails.
 * FP16 rounding against numpy's float16 conversion and torch's float16.
-* GELU (the polynomial stand-in for the tanh factor) within 2.7e-4 max(1, |x|)
-  of the tanh form's closed form, continuous at the clamp, and its
-  derivative against finite differences.
+* ReLU and its derivative against finite differences.
 * Feature rows: RMS scaling and (l, m) enumeration against a direct loop.
 """
 
@@ -39,22 +37,13 @@ def test_f16_matches_torch():
     assert np.array_equal(got, ref)
 
 
-def test_gelu_against_tanh_form_and_grad():
-    x = np.linspace(-40, 40, 160001)
-    ref = 0.5 * x * (1 + np.tanh(np.sqrt(2 / np.pi) * (x + 0.044715 * x ** 3)))
-    err = np.abs(mlp.gelu(x) - ref) / np.maximum(1.0, np.abs(x))
-    assert err.max() <= 2.7e-4
-    assert np.allclose(mlp.gelu_tanh(x), ref, atol=1e-14)
-    # continuous at the clamp |x| = 3.75 and odd-symmetric around x / 2
-    A = mlp.GELU_A
-    assert abs(mlp.gelu(A - 1e-12) - mlp.gelu(A + 1e-12)) < 1e-10
-    assert np.allclose(mlp.gelu(x) - mlp.gelu(-x), x, atol=1e-12)
-    # derivative against central differences away from the kinks at 0 and +-A
-    xs = np.linspace(-6, 6, 241)
-    xs = xs[(np.abs(xs) > 1e-3) & (np.abs(np.abs(xs) - A) > 1e-3)]
+def test_relu_and_grad():
+    x = np.linspace(-6, 6, 241)
+    assert np.array_equal(mlp.relu(x), np.where(x > 0, x, 0.0))
+    xs = x[np.abs(x) > 1e-3]
     h = 1e-6
-    fd = (mlp.gelu(xs + h) - mlp.gelu(xs - h)) / (2 * h)
-    assert np.allclose(mlp.gelu_grad(xs), fd, atol=1e-8)
+    fd = (mlp.relu(xs + h) - mlp.relu(xs - h)) / (2 * h)
+    assert np.allclose(mlp.relu_grad(xs), fd, atol=1e-8)
 
 
 def test_grad_matches_finite_differences():
