@@ -366,6 +366,15 @@ int s2h_ring_residual_ex(const s2h_plan *plan, const void *G, const void *Y, con
                          s2h_stream stream);
 int s2h_ring_dot_ex(const s2h_plan *plan, const void *Y, const double *num, double *pq,
                     double *alpha, void *work, int batch, int flags, s2h_stream stream);
+/* s2h_inverse_rings_dot: Y = s2h_inverse_rings(flm) and the dot of
+ * s2h_ring_dot_ex on it in one call, flags S2H_SPECTRA_TILED (required) and
+ * S2H_FLM_PACKED, real maps.  The matrix-core syntheses add up the dot over
+ * the rows without aliasing as they store them, so only the aliased rows are
+ * read again.  work: s2h_inverse_rings_dot_workspace_bytes(plan, batch). */
+size_t s2h_inverse_rings_dot_workspace_bytes(const s2h_plan *plan, int batch);
+int s2h_inverse_rings_dot(const s2h_plan *plan, const void *flm, void *Y, int batch, int precision,
+                          int flags, const double *num, double *pq, double *alpha, void *work,
+                          s2h_stream stream);
 size_t s2h_flm_dots_workspace_bytes(int nv, int batch);
 int s2h_flm_dots(const void *z, const void *const *v, int nv, int64_t n, int packed_L, int batch,
                  double *out, void *work, s2h_stream stream);

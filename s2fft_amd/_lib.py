@@ -58,7 +58,8 @@ EXPORTED = (
     "s2h_ring_spectra_bytes", "s2h_ring_analysis", "s2h_forward_rings", "s2h_inverse_rings",
     "s2h_ring_residual", "s2h_flm_pack", "s2h_flm_unpack",
     "s2h_ring_residual_scaled", "s2h_ring_dot_workspace_bytes", "s2h_ring_dot",
-    "s2h_ring_residual_ex", "s2h_ring_dot_ex",
+    "s2h_ring_residual_ex", "s2h_ring_dot_ex", "s2h_inverse_rings_dot_workspace_bytes",
+    "s2h_inverse_rings_dot",
     "s2h_flm_dots_workspace_bytes", "s2h_flm_dots", "s2h_cg_update", "s2h_cg_finish",
     "s2h_forward_adjoint", "s2h_inverse_adjoint", "s2h_probe_mfma_f64", "s2h_probe_mfma_f64_rate",
     "s2h_probe_mfma_f32", "s2h_forward_ex", "s2h_inverse_ex",
@@ -146,6 +147,8 @@ def _declare(lib):
         "s2h_ring_dot": (i32, [vp, vp, vp, vp, vp, vp, i32, vp]),
         "s2h_ring_residual_ex": (i32, [vp, vp, vp, vp, vp, vp, vp, i32, i32, vp]),
         "s2h_ring_dot_ex": (i32, [vp, vp, vp, vp, vp, vp, i32, i32, vp]),
+        "s2h_inverse_rings_dot_workspace_bytes": (sz, [vp, i32]),
+        "s2h_inverse_rings_dot": (i32, [vp, vp, vp, i32, i32, i32, vp, vp, vp, vp, vp]),
         "s2h_flm_dots_workspace_bytes": (sz, [i32, i32]),
         "s2h_flm_dots": (i32, [vp, ctypes.POINTER(vp), i32, i64, i32, i32, vp, vp, vp]),
         "s2h_cg_update": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, i64, i32, vp]),

@@ -278,7 +278,8 @@ int lat_forward_stage(const s2h_plan* p, const cplx* ftm, void* flm, int batch, 
 // Latitudinal synthesis (+ pole rows) into inverse-grid ring spectra ftm.
 int lat_inverse_stage(const s2h_plan* p, const void* flm, cplx* ftm, int batch, int precision,
                       hipStream_t s, bool packed = false, const PlaneSet* ps = nullptr,
-                      const SpecView* sv = nullptr) {
+                      const SpecView* sv = nullptr, double* dot_part = nullptr,
+                      int64_t dot_part_stride = 0) {
   const bool mixed = precision == S2H_PREC_MIXED;
   const bool bf16 = precision == S2H_PREC_BF16 || precision == S2H_PREC_BF16X1;
   const int64_t ftm_stride = (int64_t)p->inv_rows * p->Mc;
@@ -293,6 +294,13 @@ int lat_inverse_stage(const s2h_plan* p, const void* flm, cplx* ftm, int batch, 
   a.ftm_m_first = p->m_lo;
   a.ftm_tiled = sv ? sv->tiled : 0;
   if (ps) a.planes = ps->inv;
+  if (dot_part) {
+    a.dot_wn = p->spec_belt_wn;
+    a.dot_row_lo = p->spec_belt_row_lo;
+    a.dot_row_hi = p->spec_belt_row_hi;
+    a.dot_part = dot_part;
+    a.dot_part_stride = dot_part_stride;
+  }
   {
     ProfScope prof(bf16 ? S2H_KERNEL_LAT_INVERSE_BF16
                         : mixed ? S2H_KERNEL_LAT_INVERSE_FP32 : S2H_KERNEL_LAT_INVERSE,
@@ -673,6 +681,55 @@ extern "C" int s2h_ring_dot_ex(const s2h_plan* p, const void* Y, const double* n
 extern "C" int s2h_ring_dot(const s2h_plan* p, const void* Y, const double* num, double* pq,
                             double* alpha, void* work, int batch, s2h_stream stream) {
   return s2h_ring_dot_ex(p, Y, num, pq, alpha, work, batch, 0, stream);
+}
+
+namespace {
+// Whether the synthesis of this call runs on the matrix-core kernels, whose
+// epilogue adds up the dot over the rows without aliasing; their grid.x.
+int synth_dot_workgroups(const s2h_plan* p, int batch, int precision) {
+  const bool bf16 = precision == S2H_PREC_BF16 || precision == S2H_PREC_BF16X1;
+  const bool mma = precision == S2H_PREC_MIXED || batch >= kMfmaMinBatch;
+  if ((!bf16 && !mma) || p->spec_belt_wn == 0.0) return 0;
+  return p->nm * ((p->n_inv_rings + 255) / 256);
+}
+}  // namespace
+
+extern "C" size_t s2h_inverse_rings_dot_workspace_bytes(const s2h_plan* p, int batch) {
+  if (!p || batch <= 0 || !spectral_ok(p)) return 0;
+  const int64_t head = (int64_t)p->nm * ((p->n_inv_rings + 255) / 256);
+  const int64_t parts =
+      std::max<int64_t>(head + (int64_t)p->n_polar_blocks * ring_dot_tiled_parts(1),
+                        (int64_t)ring_dot_tiled_parts(p->spec_rows / kSpecBlock));
+  return (size_t)batch * (size_t)parts * sizeof(double);
+}
+
+extern "C" int s2h_inverse_rings_dot(const s2h_plan* p, const void* flm, void* Y, int batch,
+                                     int precision, int flags, const double* num, double* pq,
+                                     double* alpha, void* work, s2h_stream stream) {
+  if (!p || !flm || !Y || batch < 0 || !valid_precision(precision)) return S2H_ERR_INVALID;
+  if (flags & ~(S2H_FLM_PACKED | S2H_SPECTRA_TILED)) return S2H_ERR_INVALID;
+  if (!(flags & S2H_SPECTRA_TILED) || (!pq && !alpha) || (alpha && !num)) return S2H_ERR_INVALID;
+  const bool packed = (flags & S2H_FLM_PACKED) != 0;
+  if (!spectral_ok(p)) return S2H_ERR_UNSUPPORTED;
+  if (!p->d.reality || p->m_lo != 0) return S2H_ERR_UNSUPPORTED;
+  if (batch == 0) return S2H_OK;
+  if (!work) return S2H_ERR_INVALID;
+  int st = check_device(p);
+  if (st) return st;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  const SpecView sv = spec_view(p, flags);
+  double* part = static_cast<double*>(work);
+  const int head = synth_dot_workgroups(p, batch, precision);
+  const int64_t stride = head + (int64_t)p->n_polar_blocks * ring_dot_tiled_parts(1);
+  if ((st = lat_inverse_stage(p, flm, static_cast<cplx*>(Y), batch, precision, s, packed, nullptr,
+                              &sv, head > 0 ? part : nullptr, stride)))
+    return st;
+  if ((st = zero_spec_pad(p, Y, 1, batch, s))) return st;
+  ProfScope ps(S2H_KERNEL_RING_RESIDUAL, s);
+  const RingAliasTiledArgs a = alias_tiled_args(p, Y, Y, nullptr, true);
+  if (head == 0)  // vector-ALU synthesis: the whole dot over the spectra
+    return hip_status(launch_ring_dot_tiled(a, batch, part, num, pq, alpha, s));
+  return hip_status(launch_ring_dot_tiled_polar(a, batch, part, head, num, pq, alpha, s));
 }
 
 namespace {

@@ -82,7 +82,13 @@ struct RingAliasTiledArgs {
   int apply_phase;
   int L;
   const double* alpha;      // per-map scale (null: 1)
-  double* part;             // [map][block][slot] partial sums: the dot (dot mode) or |R|^2
+  double* part;             // partial sums (the dot in dot mode, else |R|^2): map b's
+                            // at part + b part_map_stride, kAliasSlots per row block,
+                            // the polar blocks' from polar_part_off, the belt
+                            // blocks' from belt_part_off (in list order)
+  int64_t part_map_stride;
+  int polar_part_off;
+  int belt_part_off;
 };
 
 struct RingAliasArgs {
@@ -138,6 +144,13 @@ struct LatArgs {
   int flm_packed;           // flm in the packed m-major layout (real maps, m >= 0 only)
   const LatPlane* planes;   // non-null: map b uses planes[b] in place of mstart, coef and
                             // lmin_out (VALU kernels, unpaired rings)
+  // matrix-core syntheses of real maps: with dot_part, the part of the CG dot
+  // <Y, A(Y)> over the rows [dot_row_lo, dot_row_hi), which have no aliasing
+  // and the one w n = dot_wn, into dot_part[map * dot_part_stride + blockIdx.x]
+  double dot_wn;
+  int dot_row_lo, dot_row_hi;
+  double* dot_part;
+  int64_t dot_part_stride;
 };
 
 // Harmonic-side addressing.  Reference layout: flm[b][l][L-1+m].  Packed
@@ -182,6 +195,12 @@ hipError_t launch_ring_alias_tiled_sumsq(const RingAliasTiledArgs& a, int batch,
                                          double* sumsq, hipStream_t s);
 hipError_t launch_ring_dot_tiled(const RingAliasTiledArgs& a, int batch, double* part,
                                  const double* num, double* pq, double* alpha, hipStream_t s);
+// The dot's polar row blocks only, after n_head partial sums per map already
+// in part (the synthesis-fused part over the other rows): pq and alpha over
+// both, part laid out per map as [n_head][polar blocks x slots].
+hipError_t launch_ring_dot_tiled_polar(const RingAliasTiledArgs& a, int batch, double* part,
+                                       int n_head, const double* num, double* pq, double* alpha,
+                                       hipStream_t s);
 hipError_t launch_ring_alias_sumsq(const RingAliasArgs& a, const JobSet& js, int batch,
                                    double* part, double* sumsq, hipStream_t s);
 hipError_t launch_ring_dot(const RingAliasArgs& a, const JobSet& js, int batch, double* part,

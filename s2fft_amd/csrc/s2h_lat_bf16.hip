@@ -580,6 +580,8 @@ __global__ void __launch_bounds__(kIT, 3) lat_inverse_bf16_kernel(LatArgs a) {
       ta[4 * q + r] = ra;
       tb[4 * q + r] = rb;
     }
+  const bool dot = a.dot_part != nullptr;
+  double dacc[2] = {0.0, 0.0};
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
     const int bm = b0 + t * 8 + (row >> 1);
@@ -592,10 +594,18 @@ __global__ void __launch_bounds__(kIT, 3) lat_inverse_bf16_kernel(LatArgs a) {
       for (int r = 0; r < 4; ++r) {
         const int ra = ta[4 * q + r], rb = tb[4 * q + r];
         const double e = (double)acc[q][t][0][r], o = (double)acc[q][t][1][r];
-        if (ra >= 0) out[2 * spec_row(ra, a.ftm_ncol, a.ftm_tiled)] = e + o;
-        if (rb >= 0) out[2 * spec_row(rb, a.ftm_ncol, a.ftm_tiled)] = e - o;
+        if (ra >= 0) {
+          out[2 * spec_row(ra, a.ftm_ncol, a.ftm_tiled)] = e + o;
+          if (dot && ra >= a.dot_row_lo && ra < a.dot_row_hi) dacc[t] = fma(e + o, e + o, dacc[t]);
+        }
+        if (rb >= 0) {
+          out[2 * spec_row(rb, a.ftm_ncol, a.ftm_tiled)] = e - o;
+          if (dot && rb >= a.dot_row_lo && rb < a.dot_row_hi) dacc[t] = fma(e - o, e - o, dacc[t]);
+        }
       }
   }
+  // the d tiles are no longer read (the loop ended on a barrier)
+  if (dot) lat::synth_dot_partials<kIW>(a, m, b0, dacc, reinterpret_cast<double*>(dth));
 }
 
 // ------------------------------------------------------------------ launchers

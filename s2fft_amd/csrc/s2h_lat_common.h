@@ -8,6 +8,7 @@
 #pragma once
 
 #include "s2h_internal.h"
+#include "s2h_kernels.h"
 
 namespace s2h {
 namespace lat {
@@ -155,6 +156,34 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The part of the conjugate-gradient dot <Y, A(Y)> over the ring rows
+// [a.dot_row_lo, a.dot_row_hi), fused into a synthesis epilogue: those rows
+// have no aliasing and one w n = a.dot_wn, so A(Y) = w n Y and a stored value
+// v adds c_m w n v^2 (c_m = 2 for m > 0 of real maps).  dacc[t]: the lane's
+// sum of v^2 for its map tile t (map 8 t + (lane % 16) / 2 of the workgroup,
+// both components).  The workgroup adds the lanes up in a fixed order
+// (component, lane group, wave) into a.dot_part[map * a.dot_part_stride +
+// blockIdx.x].  red: 2 NW 64 doubles of LDS no longer read by any thread;
+// every thread calls this.
+template <int NW>
+__device__ void synth_dot_partials(const LatArgs& a, int m, int b0, const double (&dacc)[2],
+                                   double* red) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  red[(wave * 64 + lane) * 2] = dacc[0];
+  red[(wave * 64 + lane) * 2 + 1] = dacc[1];
+  __syncthreads();
+  if (threadIdx.x < 16) {
+    const int mi = threadIdx.x, t = mi >> 3, j = mi & 7;
+    double s = 0.0;
+    for (int w = 0; w < NW; ++w)
+      for (int kg = 0; kg < 4; ++kg)
+        for (int c = 0; c < 2; ++c) s += red[(w * 64 + 16 * kg + 2 * j + c) * 2 + t];
+    if (b0 + mi < a.batch)
+      a.dot_part[(int64_t)(b0 + mi) * a.dot_part_stride + blockIdx.x] =
+          (m > 0 ? 2.0 : 1.0) * a.dot_wn * s;
+  }
 }
 
 typedef double v4d __attribute__((ext_vector_type(4)));

@@ -620,6 +620,23 @@ static int plan_build(s2h_plan* p) {
       }
       (no_alias ? belt : polar).push_back(blk);
     }
+    // the belt rows for the synthesis-fused part of the CG dot: one w n over
+    // a contiguous run of belt blocks (else left to the alias kernels)
+    p->spec_belt_wn = 0.0;
+    if (!belt.empty() && belt.back() - belt.front() + 1 == (int)belt.size()) {
+      const RingJob& j0 = rj[belt.front() * kSpecBlock];
+      bool uniform = j0.nphi > 0;
+      for (int blk : belt)
+        for (int q = 0; q < kSpecBlock; ++q) {
+          const RingJob& j = rj[blk * kSpecBlock + q];
+          uniform = uniform && j.nphi == j0.nphi && j.weight == j0.weight;
+        }
+      if (uniform) {
+        p->spec_belt_wn = j0.weight * j0.nphi;
+        p->spec_belt_row_lo = belt.front() * kSpecBlock;
+        p->spec_belt_row_hi = (belt.back() + 1) * kSpecBlock;
+      }
+    }
     p->n_belt_blocks = (int)belt.size();
     p->n_polar_blocks = (int)polar.size();
     if (!belt.empty() && (st = upload(p, belt, &p->spec_belt_blocks))) return st;

@@ -56,6 +56,10 @@ struct s2h_plan {
   int n_belt_blocks = 0;
   int* spec_polar_blocks = nullptr;
   int n_polar_blocks = 0;
+  // the belt blocks' rows [spec_belt_row_lo, spec_belt_row_hi) when they are
+  // contiguous and share one w n (HEALPix: spec_belt_wn), else spec_belt_wn 0
+  double spec_belt_wn = 0.0;
+  int spec_belt_row_lo = 0, spec_belt_row_hi = 0;
 
   // per-row tables of the forward grid (fwd_rows) and the inverse grid
   // (inv_rows), for the precompute kernels: colatitudes, the forward

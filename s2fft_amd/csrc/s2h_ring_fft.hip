@@ -960,8 +960,10 @@ __global__ void __launch_bounds__(TPB) ring_alias_belt_kernel(RingAliasTiledArgs
     }
   }
   if (DOT || a.part)
-    alias_block_sum<TPB>(dsum, red, a.part + ((int64_t)b * a.nblocks + blk) * kAliasSlots, sp,
-                         kAliasSplit);
+    alias_block_sum<TPB>(dsum, red,
+                         a.part + (int64_t)b * a.part_map_stride + a.belt_part_off +
+                             (int64_t)blockIdx.x * kAliasSlots,
+                         sp, kAliasSplit);
 }
 
 template <bool DOT>
@@ -1137,7 +1139,10 @@ __global__ void __launch_bounds__(kAliasPolarTpb) ring_alias_polar_kernel(RingAl
     }
   }
   if (DOT || a.part)
-    alias_block_sum<TPB>(dsum, red, a.part + ((int64_t)b * a.nblocks + blk) * kAliasSlots, z, H);
+    alias_block_sum<TPB>(dsum, red,
+                         a.part + (int64_t)b * a.part_map_stride + a.polar_part_off +
+                             (int64_t)bi * kAliasSlots,
+                         z, H);
 }
 
 // ------------------------------------------------------------------ launchers
@@ -1273,6 +1278,13 @@ hipError_t launch_alias_tiled_pair(const RingAliasTiledArgs& a, int batch, bool 
   }
   return hipSuccess;
 }
+
+// every row block's slots, polar blocks first
+void full_part_layout(RingAliasTiledArgs& b) {
+  b.part_map_stride = (int64_t)b.nblocks * kAliasSlots;
+  b.polar_part_off = 0;
+  b.belt_part_off = b.n_polar * kAliasSlots;
+}
 }  // namespace
 
 hipError_t launch_ring_alias_tiled(const RingAliasTiledArgs& a, int batch, hipStream_t s) {
@@ -1287,6 +1299,7 @@ hipError_t launch_ring_alias_tiled_sumsq(const RingAliasTiledArgs& a, int batch,
   if (batch == 0 || a.nblocks == 0) return hipSuccess;
   RingAliasTiledArgs b = a;
   b.part = part;
+  full_part_layout(b);
   hipError_t e = launch_alias_tiled_pair(b, batch, false, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(ring_dot_final_kernel, dim3(batch), dim3(256), 0, s, part,
@@ -1300,10 +1313,29 @@ hipError_t launch_ring_dot_tiled(const RingAliasTiledArgs& a, int batch, double*
   RingAliasTiledArgs b = a;
   b.part = part;
   b.alpha = nullptr;
+  full_part_layout(b);
   hipError_t e = launch_alias_tiled_pair(b, batch, true, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(ring_dot_final_kernel, dim3(batch), dim3(256), 0, s, part,
                      a.nblocks * kAliasSlots, num, pq, alpha);
+  return hipGetLastError();
+}
+
+hipError_t launch_ring_dot_tiled_polar(const RingAliasTiledArgs& a, int batch, double* part,
+                                       int n_head, const double* num, double* pq, double* alpha,
+                                       hipStream_t s) {
+  if (batch == 0) return hipSuccess;
+  RingAliasTiledArgs b = a;
+  b.part = part;
+  b.alpha = nullptr;
+  b.n_belt = 0;
+  b.part_map_stride = n_head + (int64_t)b.n_polar * kAliasSlots;
+  b.polar_part_off = n_head;
+  b.belt_part_off = 0;
+  hipError_t e = launch_alias_tiled_pair(b, batch, true, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(ring_dot_final_kernel, dim3(batch), dim3(256), 0, s, part,
+                     (int)b.part_map_stride, num, pq, alpha);
   return hipGetLastError();
 }
 

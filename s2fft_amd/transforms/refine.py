@@ -213,8 +213,11 @@ def refine_cg(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=None
         last = k + 1 == n_iter
         prec = judge_precision if judged(k) else \
             last_synth_precision if last else synth_precision
-        Y = sph.inverse_rings_hip(p, plan, prec, packed, tiled=tl)
-        pq, alpha = sph.ring_dot_hip(Y, plan, num=rz, tiled=tl)
+        if tl:
+            Y, pq, alpha = sph.inverse_rings_dot_hip(p, plan, prec, packed, num=rz)
+        else:
+            Y = sph.inverse_rings_hip(p, plan, prec, packed)
+            pq, alpha = sph.ring_dot_hip(Y, plan, num=rz)
         if last:
             if eager:
                 sph.cg_update_hip(x, p, alpha)
@@ -277,8 +280,11 @@ def _redo_plain(bad, R, Y, p, alpha, rz, r, plan, synth_precision, packed, pL, n
     sph.ring_residual_hip(Rb, Y[bad].contiguous(), plan, out=Rb, alpha=-alpha[bad], tiled=tl)
     rb = r[bad].contiguous()
     rzb = sph.flm_dots_hip(rb, [rb], pL)[0]
-    Yb = sph.inverse_rings_hip(rb, plan, synth_precision, packed, tiled=tl)
-    _, ab = sph.ring_dot_hip(Yb, plan, num=rzb, tiled=tl)
+    if tl:
+        Yb, _, ab = sph.inverse_rings_dot_hip(rb, plan, synth_precision, packed, num=rzb)
+    else:
+        Yb = sph.inverse_rings_hip(rb, plan, synth_precision, packed)
+        _, ab = sph.ring_dot_hip(Yb, plan, num=rzb)
     _, nb = sph.ring_residual_hip(Rb, Yb, plan, out=Rb, alpha=ab, sumsq=True, tiled=tl)
     R[bad], p[bad] = Rb, rb
     alpha[bad], rz[bad], n_R[bad] = ab, rzb, nb

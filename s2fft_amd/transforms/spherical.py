@@ -388,6 +388,30 @@ def ring_dot_hip(Y, plan, num=None, tiled=False):
     return pq if num is None else (pq, alpha)
 
 
+def inverse_rings_dot_hip(flm, plan, precision="fp64", packed=False, num=None):
+    """Y = inverse_rings_hip(flm, tiled=True) and ring_dot_hip(Y, num=num,
+    tiled=True) in one call (real maps): the matrix-core syntheses add up the
+    dot over the rows without aliasing as they store them.  Returns (Y, pq)
+    or, with ``num``, (Y, pq, alpha)."""
+    flm = flm.contiguous()
+    B = flm.shape[0]
+    lib = _lib.lib()
+    Y = _new_spectra(plan, B, flm.device)
+    pq = torch.empty(B, dtype=torch.float64, device=flm.device)
+    alpha = torch.empty_like(pq) if num is not None else None
+    work = torch.empty(max(int(lib.s2h_inverse_rings_dot_workspace_bytes(plan.handle, B)), 8),
+                       dtype=torch.uint8, device=flm.device)
+    flags = _lib.SPECTRA_TILED | (_lib.FLM_PACKED if packed else 0)
+    _lib.check(lib.s2h_inverse_rings_dot(plan.handle, flm.data_ptr(), Y.data_ptr(), B,
+                                         _precision(precision), flags,
+                                         num.contiguous().data_ptr() if num is not None else None,
+                                         pq.data_ptr(),
+                                         alpha.data_ptr() if alpha is not None else None,
+                                         work.data_ptr(), _stream(flm.device)),
+               "inverse rings + dot")
+    return (Y, pq) if num is None else (Y, pq, alpha)
+
+
 def flm_dots_hip(z, vs, packed_L=0):
     """(len(vs), B) per-map inner products sum w Re(conj(z) v) of coefficient
     tensors (B, ...) complex128; ``packed_L`` > 0: the packed layout of real

@@ -125,6 +125,26 @@ def test_ring_dot_tiled(nside, B):
     assert torch.allclose(at, ar, rtol=1e-12, atol=0)
 
 
+@pytest.mark.parametrize("nside", [16, 64])
+@pytest.mark.parametrize("B", [1, 5, 16])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "bf16", "bf16x1"])
+def test_inverse_rings_dot_fused(nside, B, precision):
+    """The synthesis with the dot fused over the rows without aliasing: the
+    same spectra bit for bit, and the dot of the separate pass to 1e-12."""
+    L = 2 * nside
+    plan = sph.get_plan("healpix", L, nside, 0, 0, True)
+    x = _flm(B, L, 130 + B)
+    num = torch.linspace(1.0, 2.0, B, device="cuda", dtype=torch.float64)
+    Yt = sph.inverse_rings_hip(x, plan, precision, packed=True, tiled=True)
+    pr, ar = sph.ring_dot_hip(Yt, plan, num=num, tiled=True)
+    Yf, pf, af = sph.inverse_rings_dot_hip(x, plan, precision, packed=True, num=num)
+    assert torch.equal(_valid(Yf, nside, L), _valid(Yt, nside, L))
+    assert torch.allclose(pf, pr, rtol=1e-12, atol=0)
+    assert torch.allclose(af, ar, rtol=1e-12, atol=0)
+    Y2, p2 = sph.inverse_rings_dot_hip(x, plan, precision, packed=True)
+    assert torch.equal(p2, pf)
+
+
 def test_tiled_refuses_complex_maps():
     nside, L = 16, 32
     plan = sph.get_plan("healpix", L, nside, 0, 0, False)
