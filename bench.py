@@ -48,10 +48,10 @@ HBM_PEAK_GBS = 8000.0
 # (request-size resolved: a 1 GiB device copy reads and writes 1.074e9 B by them):
 # profiles/counters_healpix_nside1024_L2048_b16.md.  lat_forward: the FP64
 # projection into the reference flm layout (the plain leg); lat_inverse: the FP64
-# synthesis; the BF16 kinds: the packed single-BF16 projection and split-BF16
-# synthesis of the CG loop.
-TRAFFIC_BYTES = {"lat_forward": 16.42e9, "lat_inverse": 5.96e9,
-                 "lat_forward_bf16": 5.38e9, "lat_inverse_bf16": 5.05e9}
+# synthesis of the CG loop into tiled spectra; the BF16 kinds: the single-BF16
+# projection and the split-BF16 synthesis of the CG loop (tiled spectra).
+TRAFFIC_BYTES = {"lat_forward": 15.71e9, "lat_inverse": 3.16e9,
+                 "lat_forward_bf16": 3.25e9, "lat_inverse_bf16": 4.88e9}
 
 
 def parse():
@@ -486,7 +486,8 @@ def main():
             rf["frac_of_measured_fp64_mfma"] = rf["achieved"] / ceiling
     value = B * args.steps / mixed["elapsed"]
     plain_value = B * args.steps / plain["elapsed"] if plain else None
-    head = plain["roofline"] if plain else mixed["roofline"]
+    # the dominant kernel of the measured (headline) leg
+    head = mixed["roofline"]
 
     def leg_json(v):
         d = {"maps_per_s": B * args.steps / v["elapsed"],
@@ -522,7 +523,8 @@ def main():
                        f"FP64 on the ring spectra (no ring FFT in the loop); FP64 recursion, "
                        f"{args.cg_init} first projection, {'/'.join(cg_proj)} matrix-core "
                        f"projections of the residuals, split-BF16 matrix-core syntheses of the "
-                       f"directions, one FP64 synthesis"
+                       f"directions (the last single BF16), one FP64 synthesis; spectra in "
+                       f"blocks of 8 rings, the CG dot fused into the syntheses"
                        + (f"; learned MLP preconditioner on the first {args.cg_learned_steps} "
                           f"residual(s), its direction synthesised with the exact FP64 inverse "
                           f"and kept per map only where that FP64 ring residual drops"
@@ -539,6 +541,7 @@ def main():
     }
     if plain:
         result["config"].update({
+            "plain_roofline": plain["roofline"],
             "plain_fp64_maps_per_s": plain_value,
             "speedup_vs_plain_fp64": value / plain_value,
             "plain_ms_per_step": 1e3 * plain["elapsed"] / args.steps,
