@@ -321,39 +321,51 @@ hipError_t launch_flm_dots(const FlmDotArgs& a, int batch, hipStream_t s) {
 }
 
 // x += alpha p (x optional); p_new = z - g1 p - g2 p_prev (p_new may be z;
-// p_prev, g2 optional; p_new null: only x)
+// p_prev, g2 optional; p_new null: only x).  A thread takes kCgU elements
+// kThreads apart per pass and issues all their loads before the first store.
+constexpr int kCgU = 4;
+
 __global__ void __launch_bounds__(kThreads) cg_update_kernel(CgUpdateArgs a) {
   const int b = blockIdx.y;
   const int64_t base = (int64_t)b * a.stride;
   const double al = a.x ? a.alpha[b] : 0.0;
   const double g1 = a.p_new ? a.g1[b] : 0.0;
   const double g2 = a.p_prev ? a.g2[b] : 0.0;
-  for (int64_t j = (int64_t)blockIdx.x * kThreads + threadIdx.x; j < a.n;
-       j += (int64_t)gridDim.x * kThreads) {
-    const cplx pv = a.p[base + j];
-    if (a.x) {
-      cplx xv = a.x[base + j];
-      xv.x = fma(al, pv.x, xv.x);
-      xv.y = fma(al, pv.y, xv.y);
-      a.x[base + j] = xv;
+  const cplx zero = {0.0, 0.0};
+  for (int64_t j0 = (int64_t)blockIdx.x * (kThreads * kCgU) + threadIdx.x; j0 < a.n;
+       j0 += (int64_t)gridDim.x * (kThreads * kCgU)) {
+    cplx pv[kCgU], xv[kCgU], zv[kCgU], qv[kCgU];
+#pragma unroll
+    for (int u = 0; u < kCgU; ++u) {
+      const int64_t j = j0 + u * kThreads;
+      const bool in = j < a.n;
+      pv[u] = in ? a.p[base + j] : zero;
+      xv[u] = in && a.x ? a.x[base + j] : zero;
+      zv[u] = in && a.p_new ? a.z[base + j] : zero;
+      qv[u] = in && a.p_prev ? a.p_prev[base + j] : zero;
     }
-    if (a.p_new) {
-      const cplx zv = a.z[base + j];
-      cplx nv = {fma(-g1, pv.x, zv.x), fma(-g1, pv.y, zv.y)};
-      if (a.p_prev) {
-        const cplx qv = a.p_prev[base + j];
-        nv.x = fma(-g2, qv.x, nv.x);
-        nv.y = fma(-g2, qv.y, nv.y);
+#pragma unroll
+    for (int u = 0; u < kCgU; ++u) {
+      const int64_t j = j0 + u * kThreads;
+      if (j >= a.n) continue;
+      if (a.x)
+        a.x[base + j] = cplx{fma(al, pv[u].x, xv[u].x), fma(al, pv[u].y, xv[u].y)};
+      if (a.p_new) {
+        cplx nv = {fma(-g1, pv[u].x, zv[u].x), fma(-g1, pv[u].y, zv[u].y)};
+        if (a.p_prev) {
+          nv.x = fma(-g2, qv[u].x, nv.x);
+          nv.y = fma(-g2, qv[u].y, nv.y);
+        }
+        a.p_new[base + j] = nv;
       }
-      a.p_new[base + j] = nv;
     }
   }
 }
 
 hipError_t launch_cg_update(const CgUpdateArgs& a, int batch, hipStream_t s) {
   if (batch <= 0 || a.n <= 0) return hipSuccess;
-  const int64_t want = (a.n + kThreads * 4 - 1) / (kThreads * 4);
-  const unsigned gx = (unsigned)(want < 1024 ? want : 1024);
+  const int64_t want = (a.n + kThreads * kCgU - 1) / (kThreads * kCgU);
+  const unsigned gx = (unsigned)(want < 2048 ? want : 2048);
   hipLaunchKernelGGL(cg_update_kernel, dim3(gx, batch), dim3(kThreads), 0, s, a);
   return hipGetLastError();
 }

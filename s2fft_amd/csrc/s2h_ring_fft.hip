@@ -881,15 +881,20 @@ __global__ void __launch_bounds__(256) ring_dot_final_kernel(const double* part,
 // (kAliasSlots slots per block; the unused ones are zeroed): deterministic.
 constexpr int kAliasSplit = 4;
 constexpr int kAliasSlots = 8;
+// polar workgroups: TPB threads staging up to LDSB bytes of spectra (8
+// columns per lane): 1024 threads and 128 KB (L <= 8192); the dot up to
+// L = 4096 as two workgroups of 512 threads and 64 KB per CU, so that one
+// stages while the other sums
 constexpr int kAliasPolarTpb = 1024;
-constexpr int kAliasPolarLds = 128 * 1024;  // staged spectra per workgroup (bytes)
-constexpr int kAliasMaxCols = kAliasPolarLds / 16 / kAliasPolarTpb;  // per lane: 8
+constexpr int kAliasPolarLds = 128 * 1024;
+constexpr int kAliasDotTpb = 512;
+constexpr int kAliasDotLds = 64 * 1024;
 
 // rows of a block per polar workgroup: the largest power of two <= 8 with
-// RW L 16 bytes within the LDS budget (1 for L > 8192: unsupported beyond)
-__host__ __device__ inline int alias_polar_rows(int L) {
+// RW L 16 bytes within the LDS budget ldsb
+__host__ __device__ inline int alias_polar_rows(int L, int ldsb) {
   int rw = kSpecBlock;
-  while (rw > 1 && (int64_t)rw * L * 16 > kAliasPolarLds) rw >>= 1;
+  while (rw > 1 && (int64_t)rw * L * 16 > ldsb) rw >>= 1;
   return rw;
 }
 
@@ -966,15 +971,15 @@ __global__ void __launch_bounds__(TPB) ring_alias_belt_kernel(RingAliasTiledArgs
                          sp, kAliasSplit);
 }
 
-template <bool DOT>
-__global__ void __launch_bounds__(kAliasPolarTpb) ring_alias_polar_kernel(RingAliasTiledArgs a) {
-  constexpr int TPB = kAliasPolarTpb;
+template <bool DOT, int TPB, int LDSB>
+__global__ void __launch_bounds__(TPB) ring_alias_polar_kernel(RingAliasTiledArgs a) {
+  constexpr int kAliasMaxCols = LDSB / 16 / TPB;  // staged columns per lane
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
   cplx* ys = reinterpret_cast<cplx*>(lds_raw);  // [column m][row of the workgroup]
   __shared__ double xch[2 * TPB];                // residue groups' partial sums
   __shared__ double red[TPB / 64];
   const int L = a.L;
-  const int RW = alias_polar_rows(L);
+  const int RW = alias_polar_rows(L, LDSB);
   const int H = kSpecBlock / RW;                 // workgroups per block
   // blockIdx.x -> (block, part z): the H parts of a block on one XCD
   // (workgroups are dealt to the 8 XCDs round-robin), so that they read each
@@ -1038,12 +1043,14 @@ __global__ void __launch_bounds__(kAliasPolarTpb) ring_alias_polar_kernel(RingAl
     for (int k = 0; k < kAliasMaxCols; ++k) {
       const int m = li + k * LPR;
       if (m < L) {
-        cplx v = Y(m);
+        const cplx y0 = Y(m);
+        cplx v = y0;
         if (m >= m_pair) {
           const cplx yp = Y(n - m);
           v = {fma(sgn, yp.x, v.x), fma(-sgn, yp.y, v.y)};
         }
         av[k] = v;
+        if constexpr (DOT) dsum += (m > 0 ? 2.0 * wn : wn) * (y0.x * v.x + y0.y * v.y);
       }
     }
   } else if (owns) {
@@ -1076,7 +1083,22 @@ __global__ void __launch_bounds__(kAliasPolarTpb) ring_alias_polar_kernel(RingAl
     }
     tq[0] = T;
   }
-  if constexpr (!DOT) __syncthreads();  // every read of the staged Y is done
+  if constexpr (DOT) {
+    // a fold row's dot from its residues alone: the sum over m >= 0 of
+    // c_m Re(conj Y[m] A[m]) is the sum over all m' in (-L, L) of
+    // conj(c(m')) s_j T[q] (A(-m) = conj A(m) for a real ring), i.e.
+    // sum_q T[q] conj(sum_j s_j c(q + j n)) = sum_q |T[q]|^2
+    if (owns && (!splitf || gi == 0)) {
+#pragma unroll
+      for (int k = 0; k < kAliasMaxCols; ++k) {
+        const int q = qs + k * (splitf ? 0 : LPR);
+        if ((splitf && k > 0) || q >= n) continue;
+        dsum = fma(wn, tq[k].x * tq[k].x + tq[k].y * tq[k].y, dsum);
+      }
+    }
+  } else {
+    __syncthreads();  // every read of the staged Y is done
+  }
 
   // columns of the row's outputs: A / (w n) over the staged Y (write mode),
   // or the dot's terms c_m w n Re(conj Y[m] A[m]) (dot mode)
@@ -1091,6 +1113,8 @@ __global__ void __launch_bounds__(kAliasPolarTpb) ring_alias_polar_kernel(RingAl
   if (kind == 1) {
     if constexpr (DOT)
       for (int m = li; m < L; m += LPR) out(m, Y(m));
+  } else if (DOT) {
+    // pair and fold rows: summed above
   } else if (kind == 2) {
 #pragma unroll
     for (int k = 0; k < kAliasMaxCols; ++k) {
@@ -1264,15 +1288,20 @@ hipError_t launch_alias_tiled_pair(const RingAliasTiledArgs& a, int batch, bool 
     if (e != hipSuccess) return e;
   }
   if (a.n_polar > 0) {
-    const int RW = alias_polar_rows(a.L);
-    if ((int64_t)RW * a.L * 16 > kAliasPolarLds) return hipErrorInvalidValue;
+    if ((int64_t)a.L * 16 > kAliasPolarLds) return hipErrorInvalidValue;
+    const bool small = dot && (int64_t)a.L * 16 <= kAliasDotLds;
+    const int RW = alias_polar_rows(a.L, small ? kAliasDotLds : kAliasPolarLds);
     const int H = kSpecBlock / RW;
     const dim3 grid((unsigned)((a.n_polar + 7) / 8 * 8 * H), batch);
     const size_t lds = (size_t)RW * a.L * sizeof(cplx);
-    if (dot)
-      hipLaunchKernelGGL((ring_alias_polar_kernel<true>), grid, dim3(kAliasPolarTpb), lds, s, a);
+    constexpr int T = kAliasPolarTpb, TD = kAliasDotTpb;
+    constexpr int B = kAliasPolarLds, BD = kAliasDotLds;
+    if (small)
+      hipLaunchKernelGGL((ring_alias_polar_kernel<true, TD, BD>), grid, dim3(TD), lds, s, a);
+    else if (dot)
+      hipLaunchKernelGGL((ring_alias_polar_kernel<true, T, B>), grid, dim3(T), lds, s, a);
     else
-      hipLaunchKernelGGL((ring_alias_polar_kernel<false>), grid, dim3(kAliasPolarTpb), lds, s, a);
+      hipLaunchKernelGGL((ring_alias_polar_kernel<false, T, B>), grid, dim3(T), lds, s, a);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
@@ -1368,11 +1397,15 @@ hipError_t allow_large_lds_rings() {
     const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     if (e != hipSuccess) return e;
   }
-  const void* alias[] = {(const void*)ring_alias_polar_kernel<true>,
-                         (const void*)ring_alias_polar_kernel<false>};
-  for (const void* f : alias) {
+  constexpr int T = kAliasPolarTpb, TD = kAliasDotTpb;
+  constexpr int B = kAliasPolarLds, BD = kAliasDotLds;
+  const void* alias[] = {(const void*)ring_alias_polar_kernel<true, T, B>,
+                         (const void*)ring_alias_polar_kernel<false, T, B>,
+                         (const void*)ring_alias_polar_kernel<true, TD, BD>};
+  const int lds[] = {B, B, BD};
+  for (int i = 0; i < 3; ++i) {
     const hipError_t e =
-        hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, kAliasPolarLds);
+        hipFuncSetAttribute(alias[i], hipFuncAttributeMaxDynamicSharedMemorySize, lds[i]);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
