@@ -418,9 +418,10 @@ int s2h_profile_query(int kernel, double *total_ms, int *count);
  * counterpart in the reference, whose refinement is plain Jacobi,
  * s2fft/utils/iterative_refinement.py:9-46).  A per-(l, m) MLP
  *   (re, im, l/L, m/L) -> correction (re, im)
- * with two hidden layers of 128 (GELU, tanh form), BF16-rounded weights and
- * FP32 accumulation; (re, im) is the harmonic residual divided by its RMS
- * over the map's coefficients, and the correction is scaled back by it.
+ * with two hidden layers of 128 (ReLU), FP16-rounded weights, features and
+ * activations and FP32 accumulation; (re, im) is the harmonic residual
+ * divided by its RMS over the map's coefficients (summed in a fixed order:
+ * the same output on every run), and the correction is scaled back by it.
  * params: s2h_mlp_param_count() FP32 values, layout
  *   W1[128][4] b1[128] W2[128][128] b2[128] W3[2][128] b3[2]. */
 int s2h_mlp_param_count(void);
@@ -428,12 +429,14 @@ int s2h_mlp_param_count(void);
  * whose m < 0 entries are filled by conjugate symmetry; else m in [-l, l]),
  * zero for l < lmin.  h, delta: batch x (L, 2L-1) complex128, distinct;
  * reality = 2: real maps with h and delta in the S2H_FLM_PACKED layout
- * (batch x L(L+1)/2).  work: >= 8 * batch bytes. */
+ * (batch x L(L+1)/2).  work: s2h_mlp_workspace_bytes(batch) bytes. */
+size_t s2h_mlp_workspace_bytes(int batch);
 int s2h_mlp_precondition(const float *params, const void *h, void *delta, int L, int reality,
                          int lmin, int batch, void *work, s2h_stream stream);
 /* Training rows from a residual h and the error e it should predict:
  * X[b*P + p] = (h/s, l/L, m/L) (4 floats), Y = e/s (2 floats), with P the
- * packed rows per map; rows with l < lmin are zero.  work: >= 8 * batch. */
+ * packed rows per map; rows with l < lmin are zero.
+ * work: s2h_mlp_workspace_bytes(batch) bytes. */
 int s2h_mlp_features(const void *h, const void *e, int L, int reality, int lmin, int batch,
                      float *X, float *Y, void *work, s2h_stream stream);
 /* Gradient of sum_rows w_r |y(X_r) - Y_r|^2 * loss_scale over n rows

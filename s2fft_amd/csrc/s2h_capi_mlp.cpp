@@ -30,6 +30,10 @@ int row_args(const void* h, void* out, int L, int reality, int lmin, int batch, 
 
 extern "C" int s2h_mlp_param_count(void) { return kMlpParams; }
 
+extern "C" size_t s2h_mlp_workspace_bytes(int batch) {
+  return batch > 0 ? (size_t)batch * kMlpSumsqParts * sizeof(double) : 0;
+}
+
 extern "C" int s2h_mlp_precondition(const float* params, const void* h, void* delta, int L,
                                     int reality, int lmin, int batch, void* work,
                                     s2h_stream stream) {

@@ -244,7 +244,9 @@ struct MlpRowArgs {
 };
 hipError_t launch_flm_pack(int L, const cplx* src, cplx* dst, int batch, hipStream_t s);
 hipError_t launch_flm_unpack(int L, const cplx* src, cplx* dst, int batch, hipStream_t s);
-hipError_t launch_mlp_sumsq(const MlpRowArgs& a, double* sumsq, hipStream_t s);
+// partial sums of the RMS pass per map (work of s2h_mlp_precondition / _features)
+constexpr int kMlpSumsqParts = 1024;
+hipError_t launch_mlp_sumsq(const MlpRowArgs& a, double* part, hipStream_t s);
 hipError_t launch_mlp_apply(const MlpRowArgs& a, const float* w, const double* sumsq,
                             hipStream_t s);
 hipError_t launch_mlp_features(const MlpRowArgs& a, const cplx* e, const double* sumsq, float* X,

@@ -102,8 +102,11 @@ __device__ __forceinline__ int64_t row_index(const MlpRowArgs& a, int64_t p, int
 }  // namespace
 
 // ---------------------------------------------------------------- statistics
-// sumsq[b] += sum |h[b, l, m]|^2 over the packed coefficients with l >= lmin
-__global__ void __launch_bounds__(kThreads) mlp_sumsq_kernel(MlpRowArgs a, double* sumsq) {
+// Per map, kMlpSumsqParts partial sums of |h|^2 over the rows with l >= lmin
+// (workgroup w: rows w kThreads + j kMlpSumsqParts kThreads + thread), into
+// part[b kMlpSumsqParts + w]; mlp_sigma adds them up in a fixed order, so the
+// RMS, and with it the network's output, is the same on every run.
+__global__ void __launch_bounds__(kThreads) mlp_sumsq_kernel(MlpRowArgs a, double* part) {
   const int b = blockIdx.y;
   __shared__ double red[kThreads / 64];
   double acc = 0.0;
@@ -121,8 +124,23 @@ __global__ void __launch_bounds__(kThreads) mlp_sumsq_kernel(MlpRowArgs a, doubl
   if (threadIdx.x == 0) {
     double t = 0.0;
     for (int w = 0; w < kThreads / 64; ++w) t += red[w];
-    atomicAdd(sumsq + b, t);
+    part[(int64_t)b * kMlpSumsqParts + blockIdx.x] = t;
   }
+}
+
+// RMS of map b's residual from its partial sums (fixed order: kMlpSumsqParts
+// / 64 adjacent ones per lane of the calling wave, then a butterfly); every
+// lane of the wave gets it
+__device__ __forceinline__ double mlp_sigma(const double* part, int b, int64_t count) {
+  constexpr int kPer = kMlpSumsqParts / 64;
+  static_assert(kPer * 64 == kMlpSumsqParts, "whole partial sums per lane");
+  const int lane = threadIdx.x & 63;
+  const double* p = part + (int64_t)b * kMlpSumsqParts + kPer * lane;
+  double t = 0.0;
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) t += p[i];
+  for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+  return t > 0.0 ? sqrt(t / (double)count) : 1.0;
 }
 
 // ---------------------------------------------------------------- inference
@@ -196,12 +214,16 @@ __global__ void __launch_bounds__(kInfWaves * 64) mlp_apply_kernel(MlpRowArgs a,
     b2[e] = w[oB2 + e];
   }
   if (threadIdx.x < 2) b3[threadIdx.x] = w[oB3 + threadIdx.x];
+  __shared__ double sig_s;
+  if (threadIdx.x < 64) {
+    const double sg = mlp_sigma(sumsq, blockIdx.y, a.count);
+    if (threadIdx.x == 0) sig_s = sg;
+  }
   __syncthreads();
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int b = blockIdx.y;
-  const double cnt = (double)a.count;
-  const double sig = sumsq[b] > 0.0 ? sqrt(sumsq[b] / cnt) : 1.0;
+  const double sig = sig_s;
   const float inv_sig = (float)(1.0 / sig);
   const float inv_L = 1.0f / (float)a.L;
   const cplx* __restrict__ h = a.h + (int64_t)b * a.map_stride;
@@ -334,7 +356,13 @@ __global__ void __launch_bounds__(kThreads) mlp_features_kernel(MlpRowArgs a, co
                                                                 const double* sumsq, float* X,
                                                                 float* Y) {
   const int b = blockIdx.y;
-  const double sig = sumsq[b] > 0.0 ? sqrt(sumsq[b] / (double)a.count) : 1.0;
+  __shared__ double sig_s;
+  if (threadIdx.x < 64) {
+    const double sg = mlp_sigma(sumsq, b, a.count);
+    if (threadIdx.x == 0) sig_s = sg;
+  }
+  __syncthreads();
+  const double sig = sig_s;
   for (int64_t p = (int64_t)blockIdx.x * kThreads + threadIdx.x; p < a.P;
        p += (int64_t)gridDim.x * kThreads) {
     int l, m;
@@ -547,13 +575,11 @@ __global__ void __launch_bounds__(kThreads) mlp_adam_kernel(float* w, const floa
 }
 
 // ---------------------------------------------------------------- launchers
-hipError_t launch_mlp_sumsq(const MlpRowArgs& a, double* sumsq, hipStream_t s) {
+hipError_t launch_mlp_sumsq(const MlpRowArgs& a, double* part, hipStream_t s) {
   if (a.batch == 0 || a.P == 0) return hipSuccess;
-  hipError_t e = launch_zero(sumsq, a.batch, s);
-  if (e != hipSuccess) return e;
-  const int64_t want = (a.P + kThreads - 1) / kThreads;
-  dim3 grid((unsigned)(want < 1024 ? want : 1024), a.batch);
-  hipLaunchKernelGGL(mlp_sumsq_kernel, grid, dim3(kThreads), 0, s, a, sumsq);
+  // every partial is written (a workgroup past the rows writes 0)
+  hipLaunchKernelGGL(mlp_sumsq_kernel, dim3(kMlpSumsqParts, a.batch), dim3(kThreads), 0, s, a,
+                     part);
   return hipGetLastError();
 }
 
@@ -564,7 +590,9 @@ hipError_t launch_mlp_apply(const MlpRowArgs& a, const float* w, const double* s
   const int64_t ntile = (a.P + 15) / 16;
   const int64_t npass = (ntile + kInfTiles - 1) / kInfTiles;
   const int64_t want = (npass + kInfWaves - 1) / kInfWaves;
-  dim3 grid((unsigned)(want < 2048 ? want : 2048), a.batch);
+  // each workgroup stages the weights once (16K conversions); 256 per map
+  // (64 passes per wave at L = 2048) keep that a small part of its work
+  dim3 grid((unsigned)(want < 256 ? want : 256), a.batch);
   hipLaunchKernelGGL(mlp_apply_kernel<kInfTiles>, grid, dim3(kInfWaves * 64), 0, s, a, w, sumsq);
   return hipGetLastError();
 }

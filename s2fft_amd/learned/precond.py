@@ -76,7 +76,7 @@ class MLPPreconditioner:
         h = h.contiguous()
         B = h.shape[0]
         out = torch.empty_like(h)
-        work = torch.empty(max(B, 1), dtype=torch.float64, device=h.device)
+        work = _mlp_work(B, h.device)
         mode = 2 if packed else int(bool(reality))
         _lib.check(_lib.lib().s2h_mlp_precondition(
             self.params.data_ptr(), h.data_ptr(), out.data_ptr(), L, mode, lmin,
@@ -87,6 +87,12 @@ class MLPPreconditioner:
         return self.apply(h, L if L is not None else h.shape[1], reality, lmin)
 
 
+def _mlp_work(B, device):
+    """Workspace of the RMS pass (fixed-order partial sums per map)."""
+    n = _lib.lib().s2h_mlp_workspace_bytes(max(B, 1)) // 8
+    return torch.empty(n, dtype=torch.float64, device=device)
+
+
 def features(h, e, L, reality, lmin=0):
     """Training rows X (B*P, 4), Y (B*P, 2) from residual h and error e."""
     h, e = h.contiguous(), e.contiguous()
@@ -94,7 +100,7 @@ def features(h, e, L, reality, lmin=0):
     P = L * (L + 1) // 2 if reality else L * L
     X = torch.empty((B * P, 4), dtype=torch.float32, device=h.device)
     Y = torch.empty((B * P, 2), dtype=torch.float32, device=h.device)
-    work = torch.empty(max(B, 1), dtype=torch.float64, device=h.device)
+    work = _mlp_work(B, h.device)
     _lib.check(_lib.lib().s2h_mlp_features(h.data_ptr(), e.data_ptr(), L, int(bool(reality)),
                                            lmin, B, X.data_ptr(), Y.data_ptr(),
                                            work.data_ptr(), _stream(h)), "mlp features")

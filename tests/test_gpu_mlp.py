@@ -56,7 +56,8 @@ def _params(seed):
     return p.astype(np.float32)
 
 
-@pytest.mark.parametrize("L,reality,lmin,B", [(12, True, 0, 2), (9, False, 2, 3), (33, True, 1, 1)])
+@pytest.mark.parametrize("L,reality,lmin,B", [(12, True, 0, 2), (9, False, 2, 3), (33, True, 1, 1),
+                                               (400, True, 3, 1)])
 def test_precondition_matches_oracle(L, reality, lmin, B):
     rng = np.random.default_rng(L)
     p = _params(L)
@@ -69,6 +70,19 @@ def test_precondition_matches_oracle(L, reality, lmin, B):
         assert np.abs(got[b] - ref).max() <= 2e-3 * scale
         if reality:
             assert np.all(got[b][:, L - 1].imag == 0)
+
+
+def test_precondition_deterministic():
+    """The residual's RMS is summed in a fixed order: equal outputs, bit for
+    bit, on every call (L = 512 packed: more rows than the kernels' grids)."""
+    L, B = 512, 4
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    h = torch.randn((B, L * (L + 1) // 2), generator=g, dtype=torch.complex128, device="cuda")
+    model = lp.MLPPreconditioner.load(device=torch.device("cuda"))
+    first = model.apply(h, L, True, 0, packed=True).clone()
+    for _ in range(3):
+        assert torch.equal(model.apply(h, L, True, 0, packed=True), first)
 
 
 @pytest.mark.parametrize("reality", [True, False])
