@@ -15,8 +15,18 @@ construction by a direct loop.
 
 import numpy as np
 
-H = 128
+H = 64
 N_PARAMS = H * 4 + H + H * H + H + 2 * H + 2
+
+
+def segments():
+    """(start, end) of W1, b1, W2, b2, W3, b3 in the packed parameters."""
+    sizes = [("W1", 4 * H), ("b1", H), ("W2", H * H), ("b2", H), ("W3", 2 * H), ("b3", 2)]
+    out, o = {}, 0
+    for name, n in sizes:
+        out[name] = (o, o + n)
+        o += n
+    return out
 
 def bf16(x):
     """Round float32 values to the nearest bfloat16 (ties to even)."""

@@ -12,7 +12,7 @@ constexpr int kThreads = 256;        // workgroup size of every kernel here
 constexpr int kMaxRingLen = 4096;    // longest ring FFT held in LDS (Stockham)
 constexpr int kMaxConvLen = 8192;    // longest Bluestein convolution held in LDS
 constexpr int kMaxL = 4096;           // ring phase tables cover |m| < kMaxL
-constexpr int kMlpHidden = 128;       // learned preconditioner: hidden width
+constexpr int kMlpHidden = 64;        // learned preconditioner: hidden width
 constexpr int kMlpParams = kMlpHidden * 4 + kMlpHidden + kMlpHidden * kMlpHidden + kMlpHidden +
                            2 * kMlpHidden + 2;  // W1 b1 W2 b2 W3 b3
 

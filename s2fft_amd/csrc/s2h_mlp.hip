@@ -1,5 +1,5 @@
 // Learned preconditioner for the iterative forward transform: a per-(l, m)
-// MLP  (re, im, l/L, m/L) -> correction (re, im),  two hidden layers of 128
+// MLP  (re, im, l/L, m/L) -> correction (re, im),  two hidden layers of kH
 // with ReLU, FP16 weights, features and activations, FP32 accumulation:
 //
 //   z1 = W1h h(x) + b1,  a1 = h(relu(z1)),  z2 = W2h a1 + b2,
@@ -19,14 +19,14 @@ namespace s2h {
 
 namespace {
 
-constexpr int kH = kMlpHidden;  // 128
+constexpr int kH = kMlpHidden;
 
 // master parameter offsets (floats)
-constexpr int oW1 = 0;               // [128][4]
-constexpr int oB1 = oW1 + kH * 4;    // [128]
-constexpr int oW2 = oB1 + kH;        // [128 out][128 in]
-constexpr int oB2 = oW2 + kH * kH;   // [128]
-constexpr int oW3 = oB2 + kH;        // [2][128]
+constexpr int oW1 = 0;               // [kH][4]
+constexpr int oB1 = oW1 + kH * 4;    // [kH]
+constexpr int oW2 = oB1 + kH;        // [kH out][kH in]
+constexpr int oB2 = oW2 + kH * kH;   // [kH]
+constexpr int oW3 = oB2 + kH;        // [2][kH]
 constexpr int oB3 = oW3 + 2 * kH;    // [2]
 static_assert(oB3 + 2 == kMlpParams, "parameter layout");
 
@@ -149,25 +149,28 @@ __device__ __forceinline__ double mlp_sigma(const double* part, int b, int64_t c
 // V_MFMA_F32_16X16X{16,32}_F16: A[i][k] in lane i + 16 (k/E % 4), element
 // k % E, B[k][j] in lane j + 16 (k/E % 4), element k % E, E = 4 or 8 halves
 // per lane; result C[i][j] in lane j + 16 (i/4), register i % 4):
-//  * layer 1 as 8 V_MFMA_F32_16X16X16_F16, one per 16 hidden units: A = W1h
+//  * layer 1 as kH/16 V_MFMA_F32_16X16X16_F16, one per 16 hidden units: A = W1h
 //    (lanes 0-15 hold the unit's 4 weights, the rest zero), B = the rows'
 //    4 features (lane r < 16 holds row r's), bias preloaded in the
 //    accumulator.  The result holds unit 16 nt + 4 (lane/16) + i of row
 //    lane%16 in register i of tile nt.
-//  * layer 2 as 8 output tiles x 4 k-blocks of V_MFMA_F32_16X16X32_F16 with
+//  * layer 2 as kH/16 output tiles x kH/32 k-blocks of V_MFMA_F32_16X16X32_F16 with
 //    the weights as A and the activations as B.  B is taken straight from
 //    layer 1's registers (through ReLU), which fixes the order of the
 //    hidden units along k: k = 32 kb + 8 q + e is unit
 //    32 kb + 16 (e/4) + 4 q + e%4.  W2h is staged in LDS in the A layout
 //    with its columns in that order.  The result again holds unit
 //    16 ct + 4 (lane/16) + i of row lane%16 in register i of tile ct.
-//  * layer 3 the same way as 4 MFMAs with W3h padded to 16 outputs as A:
+//  * layer 3 the same way as kH/32 MFMAs with W3h padded to 16 outputs as A:
 //    lanes 0-15 end with the row's two outputs in registers 0 and 1.
 // Each weight fragment read from LDS serves every tile of the pass.
 typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kInfWaves = 4;
 constexpr int kInfTiles = 1;  // 16-row tiles per wave per pass
+constexpr int kNT = kH / 16;  // 16-unit tiles of a hidden layer
+constexpr int kKB = kH / 32;  // 32-unit k-blocks of a hidden layer
+static_assert(kKB * 32 == kH && kKB >= 1, "hidden width: a multiple of 32");
 
 template <int TILES>
 __global__ void __launch_bounds__(kInfWaves * 64) mlp_apply_kernel(MlpRowArgs a,
@@ -175,16 +178,16 @@ __global__ void __launch_bounds__(kInfWaves * 64) mlp_apply_kernel(MlpRowArgs a,
                                                                    const double* sumsq) {
   // A operand of layer-2 tile ct, k-block kb for lane l:
   // W2h[16 ct + l%16][unit(32 kb + 8 (l/16) + e)], unit() as above
-  __shared__ f16x8 w2a[8][4][64];
+  __shared__ f16x8 w2a[kNT][kKB][64];
   // A operand of layer 3, k-block kb: W3h[l%16][unit(...)], zero for l%16 >= 2
-  __shared__ f16x8 w3a[4][64];
+  __shared__ f16x8 w3a[kKB][64];
   // A operand of layer-1 tile nt for lane l < 16: W1h[16 nt + l][0..3]
-  __shared__ f16x4 w1a[8][16];
+  __shared__ f16x4 w1a[kNT][16];
   __shared__ __attribute__((aligned(16))) float b1[kH];
   __shared__ __attribute__((aligned(16))) float b2[kH];
   __shared__ float b3[2];
-  for (int e = threadIdx.x; e < 8 * 4 * 64; e += kInfWaves * 64) {
-    const int ct = e >> 8, kb = (e >> 6) & 3, l = e & 63;
+  for (int e = threadIdx.x; e < kNT * kKB * 64; e += kInfWaves * 64) {
+    const int ct = e / (kKB * 64), kb = (e >> 6) % kKB, l = e & 63;
     const int j = 16 * ct + (l & 15), q4 = 4 * (l >> 4);
     f16x8 v;
 #pragma unroll
@@ -192,7 +195,7 @@ __global__ void __launch_bounds__(kInfWaves * 64) mlp_apply_kernel(MlpRowArgs a,
       v[q] = (_Float16)w[oW2 + j * kH + 32 * kb + 16 * (q >> 2) + q4 + (q & 3)];
     w2a[ct][kb][l] = v;
   }
-  for (int e = threadIdx.x; e < 4 * 64; e += kInfWaves * 64) {
+  for (int e = threadIdx.x; e < kKB * 64; e += kInfWaves * 64) {
     const int kb = e >> 6, l = e & 63;
     const int o = l & 15, q4 = 4 * (l >> 4);
     f16x8 v;
@@ -202,7 +205,7 @@ __global__ void __launch_bounds__(kInfWaves * 64) mlp_apply_kernel(MlpRowArgs a,
                    : (_Float16)0.0f;
     w3a[kb][l] = v;
   }
-  for (int e = threadIdx.x; e < 8 * 16; e += kInfWaves * 64) {
+  for (int e = threadIdx.x; e < kNT * 16; e += kInfWaves * 64) {
     const int u = e;  // unit 16 nt + l
     f16x4 v;
 #pragma unroll
@@ -275,7 +278,7 @@ __global__ void __launch_bounds__(kInfWaves * 64) mlp_apply_kernel(MlpRowArgs a,
     cplx vn[TILES];
 #pragma unroll
     for (int s = 0; s < TILES; ++s) locate((pass + stride) * TILES + s, ln[s], mn[s], in[s], vn[s]);
-    f16x8 av[TILES][4];
+    f16x8 av[TILES][kKB];
 #pragma unroll
     for (int s = 0; s < TILES; ++s) {
       // lanes 0-15: the features (h_re / s, h_im / s, l / L, m / L) of row `row`
@@ -284,29 +287,29 @@ __global__ void __launch_bounds__(kInfWaves * 64) mlp_apply_kernel(MlpRowArgs a,
         xf = f16x4{(_Float16)((float)vr[s].x * inv_sig), (_Float16)((float)vr[s].y * inv_sig),
                    (_Float16)((float)lr[s] * inv_L), (_Float16)((float)mr[s] * inv_L)};
       // layer 1: z of unit 16 nt + 4 kq + i, row `row`, in register i of z1[nt]
-      f32x4 z1[8];
+      f32x4 z1[kNT];
 #pragma unroll
-      for (int nt = 0; nt < 8; ++nt) {
+      for (int nt = 0; nt < kNT; ++nt) {
         z1[nt] = *reinterpret_cast<const f32x4*>(&b1[16 * nt + 4 * kq]);
         const f16x4 wv = kq == 0 ? w1a[nt][row] : zero4;
         z1[nt] = __builtin_amdgcn_mfma_f32_16x16x16f16(wv, xf, z1[nt], 0, 0, 0);
       }
 #pragma unroll
-      for (int kb = 0; kb < 4; ++kb) av[s][kb] = act_operand(z1[2 * kb], z1[2 * kb + 1]);
+      for (int kb = 0; kb < kKB; ++kb) av[s][kb] = act_operand(z1[2 * kb], z1[2 * kb + 1]);
     }
     // layer 2: unit 16 ct + 4 kq + i of row `row` in register i of acc[s][ct]
     // (each tile's bias read from LDS by itself: one read instead of four moves)
-    f32x4 acc[TILES][8];
+    f32x4 acc[TILES][kNT];
 #pragma unroll
     for (int s = 0; s < TILES; ++s) {
 #pragma unroll
-      for (int ct = 0; ct < 8; ++ct)
+      for (int ct = 0; ct < kNT; ++ct)
         acc[s][ct] = *reinterpret_cast<const volatile f32x4*>(&b2[16 * ct + 4 * kq]);
     }
 #pragma unroll
-    for (int kb = 0; kb < 4; ++kb) {
+    for (int kb = 0; kb < kKB; ++kb) {
 #pragma unroll
-      for (int ct = 0; ct < 8; ++ct) {
+      for (int ct = 0; ct < kNT; ++ct) {
         const f16x8 wa = w2a[ct][kb][lane];
 #pragma unroll
         for (int s = 0; s < TILES; ++s)
@@ -318,7 +321,7 @@ __global__ void __launch_bounds__(kInfWaves * 64) mlp_apply_kernel(MlpRowArgs a,
     for (int s = 0; s < TILES; ++s) {
       f32x4 y = {b3[0], b3[1], 0.f, 0.f};
 #pragma unroll
-      for (int kb = 0; kb < 4; ++kb)
+      for (int kb = 0; kb < kKB; ++kb)
         y = __builtin_amdgcn_mfma_f32_16x16x32_f16(
             w3a[kb][lane], act_operand(acc[s][2 * kb], acc[s][2 * kb + 1]), y, 0, 0, 0);
       const int64_t q = (pass * TILES + s) * 16 + row;
@@ -382,12 +385,16 @@ __global__ void __launch_bounds__(kThreads) mlp_features_kernel(MlpRowArgs a, co
 }
 
 // ---------------------------------------------------------------- training
-// 256 threads, rows in blocks of 16.  Thread t owns hidden unit k = t % 128
-// for rows r = 2i + t/128 (i < 8) in layers 1 and 2, and the 64 entries
-// (j, k) = (2i + t/128, t % 128), i < 64, of dW2.
+// 256 threads, rows in blocks of 16.  Thread t owns hidden unit k = t % kH
+// for rows r = kG i + g (g = t / kH, i < kTR / kG) in layers 1 and 2, and the
+// kH kH / 256 entries (j, k) = (kG i + g, k) of dW2.
 constexpr int kTR = 16;            // rows per block
 constexpr int kTT = 256;           // threads
 constexpr int kW2S = kH + 1;       // padded row stride of W2 in LDS
+constexpr int kG = kTT / kH;       // thread groups per hidden unit
+constexpr int kRG = kTR / kG;      // rows per thread in a block
+constexpr int kNW2 = kH * kH / kTT;  // dW2 entries per thread
+static_assert(kG * kH == kTT && kRG * kG == kTR, "training layout");
 
 __global__ void __launch_bounds__(kTT) mlp_grad_kernel(const float* __restrict__ w,
                                                        const float* __restrict__ X,
@@ -412,10 +419,10 @@ __global__ void __launch_bounds__(kTT) mlp_grad_kernel(const float* __restrict__
   if (t < 2) B3s[t] = w[oB3 + t];
   __syncthreads();
 
-  const int k = t % kH, half = t / kH;
-  float gW2[64];
+  const int k = t % kH, grp = t / kH;
+  float gW2[kNW2];
 #pragma unroll
-  for (int i = 0; i < 64; ++i) gW2[i] = 0.f;
+  for (int i = 0; i < kNW2; ++i) gW2[i] = 0.f;
   float gW1[4] = {0.f, 0.f, 0.f, 0.f}, gB1 = 0.f, gB2 = 0.f, gW3[2] = {0.f, 0.f};
   float gB3[2] = {0.f, 0.f};
   double lsum = 0.0;
@@ -428,11 +435,11 @@ __global__ void __launch_bounds__(kTT) mlp_grad_kernel(const float* __restrict__
       xs[r][f] = (r0 + r < n) ? f16_round(X[(r0 + r) * 4 + f]) : 0.f;
     }
     __syncthreads();
-    // layer 1 (unit k, rows 2i + half)
-    float z1[8];
+    // layer 1 (unit k, rows kG i + grp)
+    float z1[kRG];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int r = 2 * i + half;
+    for (int i = 0; i < kRG; ++i) {
+      const int r = kG * i + grp;
       z1[i] = fmaf(W1s[4 * k + 3], xs[r][3],
                    fmaf(W1s[4 * k + 2], xs[r][2],
                         fmaf(W1s[4 * k + 1], xs[r][1], fmaf(W1s[4 * k], xs[r][0], B1s[k]))));
@@ -440,10 +447,10 @@ __global__ void __launch_bounds__(kTT) mlp_grad_kernel(const float* __restrict__
     }
     __syncthreads();
     // layer 2 (unit j = k)
-    float z2[8];
+    float z2[kRG];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int r = 2 * i + half;
+    for (int i = 0; i < kRG; ++i) {
+      const int r = kG * i + grp;
       float s = B2s[k];
 #pragma unroll 8
       for (int q = 0; q < kH; ++q) s = fmaf(W2s[k * kW2S + q], a1s[r][q], s);
@@ -468,8 +475,8 @@ __global__ void __launch_bounds__(kTT) mlp_grad_kernel(const float* __restrict__
     __syncthreads();
     // backward through layer 3 and relu(z2)
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int r = 2 * i + half;
+    for (int i = 0; i < kRG; ++i) {
+      const int r = kG * i + grp;
       const float d0 = douts[r][0], d1 = douts[r][1];
       const float a2 = a2s[r][k];
       gW3[0] = fmaf(d0, a2, gW3[0]);
@@ -483,17 +490,17 @@ __global__ void __launch_bounds__(kTT) mlp_grad_kernel(const float* __restrict__
       for (int r = 0; r < kTR; ++r) gB3[t] += douts[r][t];
     }
     __syncthreads();
-    // dW2[j][k] += dz2[r][j] a1q[r][k] for the owned (j, k) = (2i + half, k)
+    // dW2[j][k] += dz2[r][j] a1q[r][k] for the owned (j, k) = (kG i + grp, k)
 #pragma unroll 1
     for (int r = 0; r < kTR; ++r) {
       const float a1 = a1s[r][k];
 #pragma unroll
-      for (int i = 0; i < 64; ++i) gW2[i] = fmaf(dz2s[r][2 * i + half], a1, gW2[i]);
+      for (int i = 0; i < kNW2; ++i) gW2[i] = fmaf(dz2s[r][kG * i + grp], a1, gW2[i]);
     }
     // da1[r][k] = sum_j dz2[r][j] W2h[j][k]; back through relu(z1)
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int r = 2 * i + half;
+    for (int i = 0; i < kRG; ++i) {
+      const int r = kG * i + grp;
       float s = 0.f;
 #pragma unroll 8
       for (int j = 0; j < kH; ++j) s = fmaf(dz2s[r][j], W2s[j * kW2S + k], s);
@@ -505,29 +512,34 @@ __global__ void __launch_bounds__(kTT) mlp_grad_kernel(const float* __restrict__
     __syncthreads();
   }
 
-  // per-workgroup partials: the two halves of a unit combine through LDS
+  // per-workgroup partials: the kG groups of a unit combine through LDS, in
+  // group order
   float* pw = partial + (int64_t)blockIdx.x * kMlpParams;
 #pragma unroll
-  for (int i = 0; i < 64; ++i) pw[oW2 + (2 * i + half) * kH + k] = gW2[i];
-  __shared__ float comb[kH][8];
-  if (half == 1) {
-    comb[k][0] = gW1[0];
-    comb[k][1] = gW1[1];
-    comb[k][2] = gW1[2];
-    comb[k][3] = gW1[3];
-    comb[k][4] = gB1;
-    comb[k][5] = gB2;
-    comb[k][6] = gW3[0];
-    comb[k][7] = gW3[1];
-  }
+  for (int i = 0; i < kNW2; ++i) pw[oW2 + (kG * i + grp) * kH + k] = gW2[i];
+  __shared__ float comb[kG][kH][8];
+  comb[grp][k][0] = gW1[0];
+  comb[grp][k][1] = gW1[1];
+  comb[grp][k][2] = gW1[2];
+  comb[grp][k][3] = gW1[3];
+  comb[grp][k][4] = gB1;
+  comb[grp][k][5] = gB2;
+  comb[grp][k][6] = gW3[0];
+  comb[grp][k][7] = gW3[1];
   __syncthreads();
-  if (half == 0) {
+  if (grp == 0) {
+    float c[8];
 #pragma unroll
-    for (int f = 0; f < 4; ++f) pw[oW1 + 4 * k + f] = gW1[f] + comb[k][f];
-    pw[oB1 + k] = gB1 + comb[k][4];
-    pw[oB2 + k] = gB2 + comb[k][5];
-    pw[oW3 + k] = gW3[0] + comb[k][6];
-    pw[oW3 + kH + k] = gW3[1] + comb[k][7];
+    for (int f = 0; f < 8; ++f) {
+      c[f] = comb[0][k][f];
+      for (int g = 1; g < kG; ++g) c[f] += comb[g][k][f];
+    }
+#pragma unroll
+    for (int f = 0; f < 4; ++f) pw[oW1 + 4 * k + f] = c[f];
+    pw[oB1 + k] = c[4];
+    pw[oB2 + k] = c[5];
+    pw[oW3 + k] = c[6];
+    pw[oW3 + kH + k] = c[7];
   }
   if (t < 2) pw[oB3 + t] = gB3[t];
   for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o);

@@ -5,8 +5,8 @@ RMS s) and the position (l/L, m/L) to a correction of flm:
 
     delta[l, m] = s * MLP(h[l, m] / s, l / L, m / L)
 
-Width 128, two hidden layers, GELU (tanh form), BF16 weights, FP32
-accumulation (s2h_mlp_* in the C ABI).  Training data come from the plain
+Two hidden layers of 64 with ReLU; FP16 weights, features and activations,
+FP32 accumulation (s2h_mlp_* in the C ABI).  Training data come from the plain
 FP64 transform: seeded band-limited coefficients flm*, maps f = I(flm*),
 Jacobi iterates x_k, residuals h_k = F(f - I(x_k)) and errors e_k = flm* - x_k;
 the network learns e_k from h_k.  Training runs on the GPU with the
@@ -35,9 +35,14 @@ def param_count():
     return int(_lib.lib().s2h_mlp_param_count())
 
 
+def hidden_width():
+    """H of the library's network: param_count() = H^2 + 8 H + 2."""
+    return int(round(math.sqrt(param_count() + 14))) - 4
+
+
 def init_params(seed=0, device="cuda"):
     """Kaiming-uniform-like initialisation (biases zero, last layer small)."""
-    H = 128
+    H = hidden_width()
     g = np.random.default_rng(seed)
     W1 = g.uniform(-1, 1, (H, 4)) / 2.0
     W2 = g.uniform(-1, 1, (H, H)) * math.sqrt(3.0 / H)

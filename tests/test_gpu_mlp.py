@@ -51,7 +51,8 @@ def _params(seed):
     rng = np.random.default_rng(seed)
     p = omlp.init_params(rng)
     p[omlp.H * 4:omlp.H * 5] = rng.standard_normal(omlp.H) * 0.1
-    p[17024:17152] = rng.standard_normal(omlp.H) * 0.1
+    lo, hi = omlp.segments()["b2"]
+    p[lo:hi] = rng.standard_normal(omlp.H) * 0.1
     p[-2:] = [0.05, -0.03]
     return p.astype(np.float32)
 
@@ -112,8 +113,7 @@ def test_grad_matches_oracle(n, nwg):
                       torch.from_numpy(w).cuda(), 1.0 / n, nwg=nwg)
     g = g.cpu().numpy().astype(np.float64)
     ref = omlp.grad(p, X, Y, w, 1.0 / n)
-    for lo, hi in [(0, 512), (512, 640), (640, 17024), (17024, 17152), (17152, 17408),
-                   (17408, 17410)]:
+    for lo, hi in omlp.segments().values():
         s = np.abs(ref[lo:hi]).max()
         assert np.abs(g[lo:hi] - ref[lo:hi]).max() <= 1e-3 * s, (lo, hi)
     assert abs(float(loss.item()) - omlp.loss(p, X, Y, w)) <= 1e-4 * omlp.loss(p, X, Y, w)

@@ -55,12 +55,10 @@ def test_grad_matches_finite_differences():
     Y = rng.standard_normal((7, 2))
     w = rng.uniform(0.5, 1.5, 7)
     g = mlp.grad(p, X, Y, w, loss_scale=0.7, quantize=False)
-    idx = np.concatenate([rng.choice(512, 6, replace=False),                  # W1
-                          512 + rng.choice(128, 3, replace=False),            # b1
-                          640 + rng.choice(16384, 8, replace=False),          # W2
-                          17024 + rng.choice(128, 3, replace=False),          # b2
-                          17152 + rng.choice(256, 4, replace=False),          # W3
-                          [17408, 17409]])                                    # b3
+    seg = mlp.segments()
+    picks = {"W1": 6, "b1": 3, "W2": 8, "b2": 3, "W3": 4, "b3": 2}
+    idx = np.concatenate([lo + rng.choice(hi - lo, picks[k], replace=False)
+                          for k, (lo, hi) in seg.items()])
     h = 1e-6
     for i in idx:
         pp, pm = p.copy(), p.copy()
