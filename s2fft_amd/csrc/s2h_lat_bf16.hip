@@ -109,7 +109,8 @@ constexpr int kFTiles = 2;      // 8-map tiles per workgroup (16 maps)
 // (8 bytes for NP = 4, 4 bytes for NP = 2; the lo parts a second one).
 // SPLIT: 3 = hi*hi + hi*lo + lo*hi, 1 = hi*hi only (BF16 operands, ~4e-3
 // relative: the correction transform of a refinement whose residual is FP64).
-template <bool PAIRED, int NP, int SPLIT>
+// TILED: the spectra layout (spec_row) as a compile-time choice.
+template <bool PAIRED, int NP, int SPLIT, bool TILED>
 __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
   constexpr bool LO = SPLIT == 3;
   static_assert(NP == 2 || NP == 4, "pairs per lane");
@@ -165,7 +166,7 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
   for (int t = 0; t < kFTiles; ++t) {
     const int amap = b0 + 8 * t + (row >> 1);
     const bool have = amap < a.batch;
-    const cplx* ftm = a.ftm + (int64_t)amap * a.ftm_map_stride + (int64_t)(m - a.ftm_m_first) * spec_cstride(a.ftm_tiled);
+    const cplx* ftm = a.ftm + (int64_t)amap * a.ftm_map_stride + (int64_t)(m - a.ftm_m_first) * spec_cstride(TILED);
 #pragma unroll
     for (int kb = 0; kb < KB; ++kb) {
 #pragma unroll
@@ -176,7 +177,7 @@ __global__ void __launch_bounds__(kFT) lat_forward_bf16_kernel(LatArgs a) {
           const int2 tt = *reinterpret_cast<const int2*>(&a.rings[q].ta);
           const int tr = comp ? tt.y : tt.x;
           if (comp == 0 || (PAIRED && tt.y >= 0 && tt.y != tt.x))
-            v = ftm[spec_row(tr, a.ftm_ncol, a.ftm_tiled)];
+            v = ftm[spec_row(tr, a.ftm_ncol, TILED)];
         }
         // lane 2r keeps fa.x and receives fb.x; lane 2r + 1 keeps fb.y and
         // receives fa.y (a zero fb where the ring is unpaired)
@@ -383,7 +384,7 @@ constexpr int kIMaps = 16;      // maps per workgroup (two column tiles)
 // product (~4e-3 relative) and no lo tiles.  Both run 3 waves per SIMD (168
 // registers; LDS 27 or 53 KB per workgroup): the split one at 2 waves took
 // 3.57 instead of 3.43 ms at L 2048, 16 maps.
-template <bool PAIRED, bool LO>
+template <bool PAIRED, bool LO, bool TILED>
 __global__ void __launch_bounds__(kIT, 3) lat_inverse_bf16_kernel(LatArgs a) {
   constexpr int kLoW = LO ? kIW : 1, kLoE = LO ? 64 * kIRow : 8, kLoB = LO ? 32 * kIRow : 8;
   // d tile per wave: [pair 64][l_local 32] hi / lo (A layout)
@@ -587,7 +588,7 @@ __global__ void __launch_bounds__(kIT, 3) lat_inverse_bf16_kernel(LatArgs a) {
     const int bm = b0 + t * 8 + (row >> 1);
     if (bm >= a.batch) continue;
     double* out = reinterpret_cast<double*>(a.ftm + (int64_t)bm * a.ftm_map_stride +
-                                            (int64_t)(m - a.ftm_m_first) * spec_cstride(a.ftm_tiled)) + (row & 1);
+                                            (int64_t)(m - a.ftm_m_first) * spec_cstride(TILED)) + (row & 1);
 #pragma unroll
     for (int q = 0; q < 4; ++q)
 #pragma unroll
@@ -595,11 +596,11 @@ __global__ void __launch_bounds__(kIT, 3) lat_inverse_bf16_kernel(LatArgs a) {
         const int ra = ta[4 * q + r], rb = tb[4 * q + r];
         const double e = (double)acc[q][t][0][r], o = (double)acc[q][t][1][r];
         if (ra >= 0) {
-          out[2 * spec_row(ra, a.ftm_ncol, a.ftm_tiled)] = e + o;
+          out[2 * spec_row(ra, a.ftm_ncol, TILED)] = e + o;
           if (dot && ra >= a.dot_row_lo && ra < a.dot_row_hi) dacc[t] = fma(e + o, e + o, dacc[t]);
         }
         if (rb >= 0) {
-          out[2 * spec_row(rb, a.ftm_ncol, a.ftm_tiled)] = e - o;
+          out[2 * spec_row(rb, a.ftm_ncol, TILED)] = e - o;
           if (dot && rb >= a.dot_row_lo && rb < a.dot_row_hi) dacc[t] = fma(e - o, e - o, dacc[t]);
         }
       }
@@ -615,10 +616,15 @@ static hipError_t lat_fwd_bf16(const LatArgs& a0, bool paired, hipStream_t s) {
   const int per_wg = kFW * NP * 64;
   a.groups = (a.nrings + per_wg - 1) / per_wg;
   dim3 grid(a.nm, a.groups, (a.batch + 8 * kFTiles - 1) / (8 * kFTiles));
-  if (paired)
-    hipLaunchKernelGGL((lat_forward_bf16_kernel<true, NP, SPLIT>), grid, dim3(kFT), 0, s, a);
+  if (paired && a.ftm_tiled)
+    hipLaunchKernelGGL((lat_forward_bf16_kernel<true, NP, SPLIT, true>), grid, dim3(kFT), 0, s, a);
+  else if (paired)
+    hipLaunchKernelGGL((lat_forward_bf16_kernel<true, NP, SPLIT, false>), grid, dim3(kFT), 0, s, a);
+  else if (a.ftm_tiled)
+    hipLaunchKernelGGL((lat_forward_bf16_kernel<false, NP, SPLIT, true>), grid, dim3(kFT), 0, s, a);
   else
-    hipLaunchKernelGGL((lat_forward_bf16_kernel<false, NP, SPLIT>), grid, dim3(kFT), 0, s, a);
+    hipLaunchKernelGGL((lat_forward_bf16_kernel<false, NP, SPLIT, false>), grid, dim3(kFT), 0, s,
+                       a);
   return hipGetLastError();
 }
 
@@ -635,10 +641,14 @@ hipError_t launch_lat_forward_bf16(const LatArgs& a, bool paired, int split, hip
 template <bool LO>
 static hipError_t lat_inv_bf16(const LatArgs& a, bool paired, hipStream_t s) {
   dim3 grid(a.nm * ((a.nrings + kIT - 1) / kIT), 1, (a.batch + kIMaps - 1) / kIMaps);
-  if (paired)
-    hipLaunchKernelGGL((lat_inverse_bf16_kernel<true, LO>), grid, dim3(kIT), 0, s, a);
+  if (paired && a.ftm_tiled)
+    hipLaunchKernelGGL((lat_inverse_bf16_kernel<true, LO, true>), grid, dim3(kIT), 0, s, a);
+  else if (paired)
+    hipLaunchKernelGGL((lat_inverse_bf16_kernel<true, LO, false>), grid, dim3(kIT), 0, s, a);
+  else if (a.ftm_tiled)
+    hipLaunchKernelGGL((lat_inverse_bf16_kernel<false, LO, true>), grid, dim3(kIT), 0, s, a);
   else
-    hipLaunchKernelGGL((lat_inverse_bf16_kernel<false, LO>), grid, dim3(kIT), 0, s, a);
+    hipLaunchKernelGGL((lat_inverse_bf16_kernel<false, LO, false>), grid, dim3(kIT), 0, s, a);
   return hipGetLastError();
 }
 
