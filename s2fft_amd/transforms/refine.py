@@ -248,8 +248,10 @@ def refine_cg(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=None
         # p_{k-1} enters only when it was a learned direction: with z = r the
         # two-term recurrence already keeps it conjugate, and the extra term
         # would add the reduced projections' error of r_{k-1} (at single BF16
-        # it costs an order of magnitude by the third step).
-        back2 = len(rs) > 1 and kept_prev is not None and bool(kept_prev.any())
+        # it costs an order of magnitude by the third step).  Maps whose learned
+        # direction was rejected get g2 = 0 (no host synchronisation to test
+        # whether any was kept).
+        back2 = len(rs) > 1 and kept_prev is not None
         d = sph.flm_dots_hip(z, [rs[-1], r_new] + (rs[-2:-1] if back2 else []), pL)
         g1 = (d[0] - d[1]) / rz
         g2 = (d[2] - d[0]) / rz_prev * kept_prev if back2 else None
