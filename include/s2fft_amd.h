@@ -433,6 +433,13 @@ int s2h_mlp_param_count(void);
 size_t s2h_mlp_workspace_bytes(int batch);
 int s2h_mlp_precondition(const float *params, const void *h, void *delta, int L, int reality,
                          int lmin, int batch, void *work, s2h_stream stream);
+/* The same, and dot[b] (device) = <delta, h> per map: sum over the stored
+ * rows of c Re(conj(delta) h), c = 2 for the m > 0 rows of real maps (their
+ * m < 0 rows) and 1 otherwise; the value s2h_flm_dots gives, summed in
+ * another fixed order.  For the conjugate-gradient step that starts from a
+ * preconditioned residual.  work: s2h_mlp_workspace_bytes(batch). */
+int s2h_mlp_precondition_dot(const float *params, const void *h, void *delta, int L, int reality,
+                             int lmin, int batch, double *dot, void *work, s2h_stream stream);
 /* Training rows from a residual h and the error e it should predict:
  * X[b*P + p] = (h/s, l/L, m/L) (4 floats), Y = e/s (2 floats), with P the
  * packed rows per map; rows with l < lmin are zero.

@@ -64,6 +64,7 @@ EXPORTED = (
     "s2h_forward_adjoint", "s2h_inverse_adjoint", "s2h_probe_mfma_f64", "s2h_probe_mfma_f64_rate",
     "s2h_probe_mfma_f32", "s2h_forward_ex", "s2h_inverse_ex",
     "s2h_mlp_param_count", "s2h_mlp_workspace_bytes", "s2h_mlp_precondition",
+    "s2h_mlp_precondition_dot",
     "s2h_mlp_features",
     "s2h_mlp_grad_workspace_bytes", "s2h_mlp_grad", "s2h_mlp_adam", "s2h_probe_mfma_bf16",
     "s2h_noalloc_scratch_bytes", "s2h_forward_noalloc",
@@ -103,6 +104,7 @@ def _declare(lib):
         "s2h_mlp_param_count": (i32, []),
         "s2h_mlp_workspace_bytes": (ctypes.c_size_t, [i32]),
         "s2h_mlp_precondition": (i32, [vp, vp, vp, i32, i32, i32, i32, vp, vp]),
+        "s2h_mlp_precondition_dot": (i32, [vp, vp, vp, i32, i32, i32, i32, vp, vp, vp]),
         "s2h_mlp_features": (i32, [vp, vp, i32, i32, i32, i32, vp, vp, vp, vp]),
         "s2h_mlp_grad_workspace_bytes": (sz, [i32]),
         "s2h_mlp_grad": (i32, [vp, vp, vp, vp, i64, ctypes.c_float, vp, vp, vp, i32, vp]),

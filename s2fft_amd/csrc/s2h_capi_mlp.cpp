@@ -31,12 +31,13 @@ int row_args(const void* h, void* out, int L, int reality, int lmin, int batch, 
 extern "C" int s2h_mlp_param_count(void) { return kMlpParams; }
 
 extern "C" size_t s2h_mlp_workspace_bytes(int batch) {
-  return batch > 0 ? (size_t)batch * kMlpSumsqParts * sizeof(double) : 0;
+  return batch > 0 ? (size_t)batch * (kMlpSumsqParts + kMlpDotParts) * sizeof(double) : 0;
 }
 
-extern "C" int s2h_mlp_precondition(const float* params, const void* h, void* delta, int L,
-                                    int reality, int lmin, int batch, void* work,
-                                    s2h_stream stream) {
+namespace {
+
+int precondition(const float* params, const void* h, void* delta, int L, int reality, int lmin,
+                 int batch, double* dot, void* work, s2h_stream stream) {
   MlpRowArgs a;
   int st = row_args(h, delta, L, reality, lmin, batch, a);
   if (st) return st;
@@ -45,10 +46,26 @@ extern "C" int s2h_mlp_precondition(const float* params, const void* h, void* de
   if (h == delta) return S2H_ERR_INVALID;
   hipStream_t s = static_cast<hipStream_t>(stream);
   double* sumsq = static_cast<double*>(work);
+  double* dot_part = sumsq + (size_t)batch * kMlpSumsqParts;
   // entries outside the rows (m > l) stay zero; the packed layout has none
   if (!a.packed && (st = hip_status(launch_zero(delta, 2 * a.map_stride * batch, s)))) return st;
   if ((st = hip_status(launch_mlp_sumsq(a, sumsq, s)))) return st;
-  return hip_status(launch_mlp_apply(a, params, sumsq, s));
+  return hip_status(launch_mlp_apply(a, params, sumsq, dot_part, dot, s));
+}
+
+}  // namespace
+
+extern "C" int s2h_mlp_precondition(const float* params, const void* h, void* delta, int L,
+                                    int reality, int lmin, int batch, void* work,
+                                    s2h_stream stream) {
+  return precondition(params, h, delta, L, reality, lmin, batch, nullptr, work, stream);
+}
+
+extern "C" int s2h_mlp_precondition_dot(const float* params, const void* h, void* delta, int L,
+                                        int reality, int lmin, int batch, double* dot,
+                                        void* work, s2h_stream stream) {
+  if (!dot && batch > 0) return S2H_ERR_INVALID;
+  return precondition(params, h, delta, L, reality, lmin, batch, dot, work, stream);
 }
 
 extern "C" int s2h_mlp_features(const void* h, const void* e, int L, int reality, int lmin,

@@ -247,8 +247,11 @@ hipError_t launch_flm_unpack(int L, const cplx* src, cplx* dst, int batch, hipSt
 // partial sums of the RMS pass per map (work of s2h_mlp_precondition / _features)
 constexpr int kMlpSumsqParts = 1024;
 hipError_t launch_mlp_sumsq(const MlpRowArgs& a, double* part, hipStream_t s);
+// partial sums of <delta, h> per map (s2h_mlp_precondition_dot)
+constexpr int kMlpDotParts = 256;
+// dot (optional, with dot_part): dot[b] = sum over the rows of c_m Re(conj(delta) h)
 hipError_t launch_mlp_apply(const MlpRowArgs& a, const float* w, const double* sumsq,
-                            hipStream_t s);
+                            double* dot_part, double* dot, hipStream_t s);
 hipError_t launch_mlp_features(const MlpRowArgs& a, const cplx* e, const double* sumsq, float* X,
                                float* Y, hipStream_t s);
 hipError_t launch_mlp_grad(const float* w, const float* X, const float* Y, const float* rw,

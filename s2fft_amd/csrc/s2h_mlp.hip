@@ -175,7 +175,8 @@ static_assert(kKB * 32 == kH && kKB >= 1, "hidden width: a multiple of 32");
 template <int TILES>
 __global__ void __launch_bounds__(kInfWaves * 64) mlp_apply_kernel(MlpRowArgs a,
                                                                    const float* __restrict__ w,
-                                                                   const double* sumsq) {
+                                                                   const double* sumsq,
+                                                                   double* dot_part) {
   // A operand of layer-2 tile ct, k-block kb for lane l:
   // W2h[16 ct + l%16][unit(32 kb + 8 (l/16) + e)], unit() as above
   __shared__ f16x8 w2a[kNT][kKB][64];
@@ -237,6 +238,9 @@ __global__ void __launch_bounds__(kInfWaves * 64) mlp_apply_kernel(MlpRowArgs a,
   const int64_t ntile = (a.P + 15) / 16;
   const int64_t npass = (ntile + TILES - 1) / TILES;
   const int64_t stride = (int64_t)gridDim.x * kInfWaves;
+  // dot_part: this lane's part of <delta, h> (c_m = 2 for the m > 0 rows of
+  // real maps: their conjugate rows), summed in a fixed order below
+  double dsum = 0.0;
   // Row `row` of tile `tile`: (l, m), its element index and (lanes 0-15) its
   // residual, loaded one pass ahead so that the load's latency overlaps the
   // pass before.  Packed: (l, m) from the tile's first row, stepping over the
@@ -335,6 +339,10 @@ __global__ void __launch_bounds__(kInfWaves * 64) mlp_apply_kernel(MlpRowArgs a,
           double im = sig * (double)y[1];
           if (a.reality && mq == 0) im = 0.0;
           out[idx] = cplx{re, im};
+          if (dot_part) {
+            const double c = a.reality && mq != 0 ? 2.0 : 1.0;
+            dsum = fma(c, fma(re, vr[s].x, im * vr[s].y), dsum);
+          }
           if (a.reality && !a.packed && mq > 0) {
             const double sg = (mq & 1) ? -1.0 : 1.0;
             out[(int64_t)lq * (2 * a.L - 1) + a.L - 1 - mq] = cplx{sg * re, -sg * im};
@@ -350,6 +358,33 @@ __global__ void __launch_bounds__(kInfWaves * 64) mlp_apply_kernel(MlpRowArgs a,
       vr[s] = vn[s];
     }
   }
+  if (dot_part) {
+    // lanes 16-63 hold 0; butterfly per wave, then the waves in order
+    __shared__ double wsum[kInfWaves];
+    for (int o = 32; o > 0; o >>= 1) dsum += __shfl_xor(dsum, o);
+    if (lane == 0) wsum[wave] = dsum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int q = 0; q < kInfWaves; ++q) t += wsum[q];
+      dot_part[(int64_t)b * kMlpDotParts + blockIdx.x] = t;
+    }
+  }
+}
+
+// dot[b] = the fixed-order sum of map b's nparts partial sums
+__global__ void __launch_bounds__(64) mlp_dot_final_kernel(const double* part, int nparts,
+                                                           double* dot) {
+  constexpr int kPer = kMlpDotParts / 64;
+  const int b = blockIdx.x, lane = threadIdx.x;
+  double t = 0.0;
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    const int k = kPer * lane + i;
+    if (k < nparts) t += part[(int64_t)b * kMlpDotParts + k];
+  }
+  for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+  if (lane == 0) dot[b] = t;
 }
 
 // ---------------------------------------------------------------- features
@@ -596,7 +631,7 @@ hipError_t launch_mlp_sumsq(const MlpRowArgs& a, double* part, hipStream_t s) {
 }
 
 hipError_t launch_mlp_apply(const MlpRowArgs& a, const float* w, const double* sumsq,
-                            hipStream_t s) {
+                            double* dot_part, double* dot, hipStream_t s) {
   if (a.batch == 0 || a.P == 0) return hipSuccess;
   if (a.packed && a.P + 32 > INT32_MAX) return hipErrorInvalidValue;
   const int64_t ntile = (a.P + 15) / 16;
@@ -604,8 +639,14 @@ hipError_t launch_mlp_apply(const MlpRowArgs& a, const float* w, const double* s
   const int64_t want = (npass + kInfWaves - 1) / kInfWaves;
   // each workgroup stages the weights once (16K conversions); 256 per map
   // (64 passes per wave at L = 2048) keep that a small part of its work
+  static_assert(kMlpDotParts >= 256 && kMlpDotParts % 64 == 0, "dot partial sums");
   dim3 grid((unsigned)(want < 256 ? want : 256), a.batch);
-  hipLaunchKernelGGL(mlp_apply_kernel<kInfTiles>, grid, dim3(kInfWaves * 64), 0, s, a, w, sumsq);
+  hipLaunchKernelGGL(mlp_apply_kernel<kInfTiles>, grid, dim3(kInfWaves * 64), 0, s, a, w, sumsq,
+                     dot ? dot_part : nullptr);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || !dot) return e;
+  hipLaunchKernelGGL(mlp_dot_final_kernel, dim3(a.batch), dim3(64), 0, s, dot_part,
+                     (int)grid.x, dot);
   return hipGetLastError();
 }
 

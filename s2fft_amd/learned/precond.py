@@ -88,6 +88,21 @@ class MLPPreconditioner:
             B, work.data_ptr(), _stream(h)), "mlp precondition")
         return out
 
+    def apply_dot(self, h, L, reality, lmin=0, packed=False):
+        """(delta, <delta, h> per map) in one pass: the dot is the one
+        ``flm_dots_hip(delta, [h], ...)`` gives (c = 2 for the m > 0 rows of
+        real maps), summed in the network's own fixed order."""
+        h = h.contiguous()
+        B = h.shape[0]
+        out = torch.empty_like(h)
+        dot = torch.empty(max(B, 1), dtype=torch.float64, device=h.device)
+        work = _mlp_work(B, h.device)
+        mode = 2 if packed else int(bool(reality))
+        _lib.check(_lib.lib().s2h_mlp_precondition_dot(
+            self.params.data_ptr(), h.data_ptr(), out.data_ptr(), L, mode, lmin,
+            B, dot.data_ptr(), work.data_ptr(), _stream(h)), "mlp precondition")
+        return out, dot[:B]
+
     def __call__(self, h, r=None, flm=None, L=None, reality=True, lmin=0):
         return self.apply(h, L if L is not None else h.shape[1], reality, lmin)
 

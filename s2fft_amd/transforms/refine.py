@@ -198,8 +198,12 @@ def refine_cg(f, L, spin, nside, sampling, reality, n_iter, L_lower=0, plan=None
         R = sph.ring_residual_hip(G, Y, plan, **kw_r)
     del G
     r = proj(R, 0)
-    z = precondition(r, 0)
-    rz = sph.flm_dots_hip(z, [r], pL)[0]
+    if judged(0) and hasattr(precond, "apply_dot"):
+        # the network's pass also forms <z_0, r_0>
+        z, rz = precond.apply_dot(r, L, reality, lmin, packed=packed)
+    else:
+        z = precondition(r, 0)
+        rz = sph.flm_dots_hip(z, [r], pL)[0]
     # p_0 may be r_0 itself: no direction buffer is written before r_0 is dropped
     rs, ps, rz_prev = [r], [z], None
     # per map: whether direction p_{k-1} / p_k is a kept learned direction

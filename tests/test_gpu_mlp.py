@@ -73,6 +73,41 @@ def test_precondition_matches_oracle(L, reality, lmin, B):
             assert np.all(got[b][:, L - 1].imag == 0)
 
 
+@pytest.mark.parametrize("L,reality,packed,lmin,B", [(512, True, True, 0, 4), (40, True, False, 2, 3),
+                                                     (33, False, False, 1, 2)])
+def test_precondition_dot_matches_flm_dots(L, reality, packed, lmin, B):
+    """s2h_mlp_precondition_dot: the same delta bit for bit, and <delta, h>
+    equal to s2h_flm_dots' to 1e-12 relative (another fixed summation order)."""
+    from s2fft_amd.transforms import spherical as sph
+    g = torch.Generator(device="cuda")
+    g.manual_seed(L)
+    shape = (B, L * (L + 1) // 2) if packed else (B, L, 2 * L - 1)
+    h = torch.randn(shape, generator=g, dtype=torch.complex128, device="cuda")
+    if not packed:
+        h = torch.from_numpy(np.stack([omlp_full(hb.cpu().numpy(), L, reality)
+                                       for hb in h])).cuda()
+    model = lp.MLPPreconditioner.load(device=torch.device("cuda"))
+    ref = model.apply(h, L, reality, lmin, packed=packed)
+    got, dot = model.apply_dot(h, L, reality, lmin, packed=packed)
+    assert torch.equal(got, ref)
+    want = sph.flm_dots_hip(ref, [h], L if packed else 0)[0]
+    assert torch.allclose(dot, want, rtol=1e-12, atol=0)
+    assert torch.all(dot.abs() > 0)
+
+
+def omlp_full(h, L, reality):
+    """A full-layout input (L, 2L-1) that holds only valid rows: |m| <= l, and
+    for real maps the conjugate-symmetric m < 0 half."""
+    ell = np.arange(L)[:, None]
+    mm = np.arange(-(L - 1), L)[None, :]
+    h = np.where(np.abs(mm) <= ell, h, 0)
+    if reality:
+        sign = np.where(mm % 2 == 0, 1.0, -1.0)
+        h = np.where(mm == 0, h.real + 0j, h)
+        h = np.where(mm < 0, np.flip(h, axis=1).conj() * sign, h)
+    return h
+
+
 def test_precondition_deterministic():
     """The residual's RMS is summed in a fixed order: equal outputs, bit for
     bit, on every call (L = 512 packed: more rows than the kernels' grids)."""
