@@ -20,7 +20,8 @@ signal-generator distribution).  These pipelines are timed on the same maps:
              direction synthesised with the exact FP64 inverse and kept per
              map only where that FP64 ring residual drops);
 * mixed CG -- the same without the network (the precision gain over fp64 CG);
-* mixed Jacobi + learned -- Jacobi steps with the learned preconditioner.
+* mixed Jacobi + learned -- Jacobi steps with the learned preconditioner;
+* mixed Jacobi -- the same Jacobi refinement without the network.
 
 Step counts are fixed before timing by running each pipeline once against
 the seeded coefficients, and confirmed on a second set of 16 maps (seeds
@@ -453,20 +454,27 @@ def main():
                                     stats=stats, **kw)
         return run
 
-    def jacobi_run(fm, n, hist=None, stats=None):
-        return refine.refine_mixed(
-            fm, L, 0, nside, "healpix", True, n, plan=plan, precond=precond,
-            refresh_every=args.refresh_every, history=hist,
-            flm_true=flm_true if hist is not None else None, stats=stats,
-            precision=args.mixed_precision, inv_precision=args.mixed_inv_precision,
-            learned_steps=args.learned_steps, spectral=True, init_precision=args.init_precision)
+    def jacobi_run(pc):
+        def run(fm, n, hist=None, stats=None):
+            return refine.refine_mixed(
+                fm, L, 0, nside, "healpix", True, n, plan=plan, precond=pc,
+                refresh_every=args.refresh_every, history=hist,
+                flm_true=flm_true if hist is not None else None, stats=stats,
+                precision=args.mixed_precision, inv_precision=args.mixed_inv_precision,
+                learned_steps=args.learned_steps, spectral=True,
+                init_precision=args.init_precision)
+        return run
 
     specs = [("mixed_cg_learned", cg_run(precond, args.cg_learned_steps))] if learned else []
     specs.append(("mixed_cg", cg_run(None, 0)))
     if not args.no_fp64_cg:
         specs.append(("fp64_cg", cg_run(None, 0, fp64=True)))
     if not args.no_jacobi_leg:
-        specs.append(("mixed_jacobi_learned" if learned else "mixed_jacobi", jacobi_run))
+        # the same Jacobi refinement with and without the network: where the
+        # network saves steps
+        if learned:
+            specs.append(("mixed_jacobi_learned", jacobi_run(precond)))
+        specs.append(("mixed_jacobi", jacobi_run(None)))
     for label, run in specs:
         log(f"[bench] {label}: calibrating the step count")
         n, hist, stats = calibrate(label, run)
