@@ -418,12 +418,12 @@ int s2h_profile_query(int kernel, double *total_ms, int *count);
  * counterpart in the reference, whose refinement is plain Jacobi,
  * s2fft/utils/iterative_refinement.py:9-46).  A per-(l, m) MLP
  *   (re, im, l/L, m/L) -> correction (re, im)
- * with two hidden layers of 128 (ReLU), FP16-rounded weights, features and
+ * with two hidden layers of H = 64 (ReLU), FP16-rounded weights, features and
  * activations and FP32 accumulation; (re, im) is the harmonic residual
  * divided by its RMS over the map's coefficients (summed in a fixed order:
  * the same output on every run), and the correction is scaled back by it.
- * params: s2h_mlp_param_count() FP32 values, layout
- *   W1[128][4] b1[128] W2[128][128] b2[128] W3[2][128] b3[2]. */
+ * params: s2h_mlp_param_count() = H^2 + 8 H + 2 FP32 values, layout
+ *   W1[H][4] b1[H] W2[H][H] b2[H] W3[2][H] b3[2]. */
 int s2h_mlp_param_count(void);
 /* delta = correction for every packed coefficient (m in [0, l] with reality,
  * whose m < 0 entries are filled by conjugate symmetry; else m in [-l, l]),
