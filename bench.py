@@ -20,8 +20,10 @@ signal-generator distribution).  These pipelines are timed on the same maps:
              direction synthesised with the exact FP64 inverse and kept per
              map only where that FP64 ring residual drops);
 * mixed CG -- the same without the network (the precision gain over fp64 CG);
-* mixed Jacobi + learned -- Jacobi steps with the learned preconditioner;
-* mixed Jacobi -- the same Jacobi refinement without the network.
+* mixed Jacobi + learned -- Jacobi steps with the learned preconditioner on
+             the first two, the FP64 residual refreshed every 2 steps;
+* mixed Jacobi -- the same without the network, refreshed every 3 steps
+             (each Jacobi leg at its fastest refresh period at nside 1024).
 
 Step counts are fixed before timing by running each pipeline once against
 the seeded coefficients, and confirmed on a second set of 16 maps (seeds
@@ -69,7 +71,10 @@ def parse():
                     help="second seed set that confirms every leg's step count")
     ap.add_argument("--mixed-precision", default="bf16x1", choices=["bf16x1", "bf16", "mixed"])
     ap.add_argument("--mixed-inv-precision", default="bf16", choices=["bf16", "mixed"])
-    ap.add_argument("--refresh-every", type=int, default=3)
+    ap.add_argument("--refresh-every", type=int, default=3,
+                    help="Jacobi legs: FP64 residual refresh period without the network")
+    ap.add_argument("--refresh-every-learned", type=int, default=2,
+                    help="the same with the network (each leg at its fastest period)")
     ap.add_argument("--init-precision", default="bf16", choices=["fp64", "bf16", "mixed"])
     ap.add_argument("--learned-steps", type=int, default=2, help="Jacobi leg: learned steps")
     ap.add_argument("--cg-proj", default="bf16,bf16x1",
@@ -455,10 +460,12 @@ def main():
         return run
 
     def jacobi_run(pc):
+        rf = args.refresh_every_learned if pc is not None else args.refresh_every
+
         def run(fm, n, hist=None, stats=None):
             return refine.refine_mixed(
                 fm, L, 0, nside, "healpix", True, n, plan=plan, precond=pc,
-                refresh_every=args.refresh_every, history=hist,
+                refresh_every=rf, history=hist,
                 flm_true=flm_true if hist is not None else None, stats=stats,
                 precision=args.mixed_precision, inv_precision=args.mixed_inv_precision,
                 learned_steps=args.learned_steps, spectral=True,
