@@ -111,6 +111,32 @@ def test_cg_entry_points_reject_bad_arguments_without_gpu():
     assert lib.s2h_cg_update(None, None, None, None, None, None, None, None, 0, 1, None) == _lib.S2H_OK
 
 
+def test_mlp_and_tiled_entry_points_reject_bad_arguments_without_gpu():
+    lib = _lib.lib()
+    inv = _lib.S2H_ERR_INVALID
+    # RMS partial sums (1024 per map) and <delta, h> partial sums (256 per map)
+    assert lib.s2h_mlp_workspace_bytes(0) == 0
+    assert lib.s2h_mlp_workspace_bytes(3) == 3 * (1024 + 256) * 8
+    buf = ctypes.create_string_buffer(64)
+    ptr = ctypes.cast(buf, ctypes.c_void_p)
+    # a null dot output, a null input, h == delta, a bad layout code
+    assert lib.s2h_mlp_precondition_dot(ptr, ptr, ptr, 8, 2, 0, 1, None, ptr, None) == inv
+    assert lib.s2h_mlp_precondition_dot(ptr, None, ptr, 8, 2, 0, 1, ptr, ptr, None) == inv
+    assert lib.s2h_mlp_precondition(ptr, ptr, ptr, 8, 2, 0, 1, ptr, None) == inv
+    assert lib.s2h_mlp_precondition(ptr, ptr, ptr, 8, 3, 0, 1, ptr, None) == inv
+    assert lib.s2h_mlp_precondition(ptr, ptr, ptr, 8, 2, 8, 1, ptr, None) == inv  # lmin >= L
+    # no maps: nothing to do, no device work
+    assert lib.s2h_mlp_precondition_dot(ptr, ptr, ptr, 8, 2, 0, 0, None, None, None) == _lib.S2H_OK
+    # the tiled ring-spectra entry points need a plan
+    assert lib.s2h_ring_residual_ex(None, None, None, None, None, None, None, 1,
+                                    _lib.SPECTRA_TILED, None) == inv
+    assert lib.s2h_ring_dot_ex(None, None, None, None, None, None, 1, _lib.SPECTRA_TILED,
+                               None) == inv
+    assert lib.s2h_inverse_rings_dot(None, None, None, 1, 0, _lib.SPECTRA_TILED, None, None,
+                                     None, None, None) == inv
+    assert lib.s2h_inverse_rings_dot_workspace_bytes(None, 4) == 0
+
+
 def test_bench_refuses_more_than_one_gpu():
     import subprocess
     import sys
